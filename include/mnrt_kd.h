@@ -1,0 +1,180 @@
+/*
+ * mnrt_kd.h — C ABI of the MI355X (gfx950) kd-tree hot path: SAH kd-tree build for
+ * triangles and photons, closest-hit ray traversal, and the exact-kNN photon-map
+ * radiance estimate.
+ *
+ * Every entry point is plain C: device pointers, sizes, and an opaque HIP stream
+ * (hipStream_t passed as void*, NULL = the null stream).  Every entry point returns an
+ * int status (MNRT_OK = 0, negative = error; mnrt_status_string() names it).  The caller
+ * owns inputs and outputs; a tree owns the arrays the library allocated for it until
+ * mnrt_tree_release().  Nothing here holds global per-scene state: two trees can be
+ * queried concurrently on two streams (the reference binds one tree into __constant__
+ * memory and texture references, raytracing.cu:125-161, photon_knn.cu:65-81).
+ *
+ * Vectors are float4 (16 B) arrays exactly as the reference's TriangleData.d_verts[3]
+ * (KernelDefs.h:506), RayChunk.d_origins/d_dirs (RayPool.h:162-171) and
+ * PhotonData.d_positions/d_powers (KernelDefs.h:537-630); the w component of inputs is
+ * ignored unless stated.
+ *
+ * ---------------------------------------------------------------------------------
+ * Tree layout in HBM (replaces KDTreeData::d_preorderTree, KDKernelDefs.h:76-194):
+ *   nodes  : uint32 pairs in preorder; the left child of inner node i is node i+1.
+ *     inner: w0 = IEEE bits of the split position, w1 = axis<<30 | right child index
+ *     leaf : w0 = element count,                   w1 = 3<<30   | first leaf reference
+ *     (30-bit indices: up to 2^30 nodes / references; the reference masks 24 bits when
+ *      writing, kdtree.cu:1742, and 28 when reading, raytracing.cu:564.)
+ *   refs   : uint32 original element index per leaf reference, leaves in preorder,
+ *            each leaf's references ascending.
+ *   recs   : leaf-ordered element records, one per reference, so a leaf's elements are
+ *            contiguous in memory:
+ *     triangle (3 x float4): [v0.xyz, index bits] [e1 = v1-v0, 0] [e2 = v2-v0, 0]
+ *     photon   (2 x float4): [pos.xyz, index bits] [power.rgb, oct-encoded direction bits]
+ * ---------------------------------------------------------------------------------
+ */
+#ifndef MNRT_KD_H
+#define MNRT_KD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MNRT_ABI_VERSION 1
+
+enum mnrt_status_code {
+    MNRT_OK = 0,
+    MNRT_ERR_ARG = -1,      /* bad argument (null pointer, size, k out of range) */
+    MNRT_ERR_HIP = -2,      /* a HIP runtime call failed */
+    MNRT_ERR_NOMEM = -3,    /* device allocation failed */
+    MNRT_ERR_NODEV = -4,    /* no usable gfx950 device */
+    MNRT_ERR_CAPACITY = -5, /* a size exceeds the layout's index width */
+    MNRT_ERR_STATE = -6     /* tree lacks what the call needs (records, kind) */
+};
+
+/* Tree kinds. */
+#define MNRT_KIND_TRI 0u
+#define MNRT_KIND_POINT 1u
+
+/* Packed node word helpers. */
+#define MNRT_LEAF_TAG 3u
+#define MNRT_INDEX_MASK 0x3FFFFFFFu
+
+/* Deepest node level that may still be split (levels start at 0 at the root).  The
+ * reference caps traversal stacks at KD_MAX_HEIGHT 50 (KDKernelDefs.h:61) but never
+ * bounds the build (KDTreeGPU.cpp:254-281); here the build stops splitting at this
+ * level so no traversal can exceed it. */
+#define MNRT_MAX_LEVEL 48u
+
+/* Largest kNN k supported by mnrt_knn_gather (the reference default k is 100 for the
+ * global map and 200 for caustics, MNRTSettings.h:431,437). */
+#define MNRT_KNN_MAX_K 256u
+
+/* Build parameters: KDTreeGPU::SetEmptySpaceRatio/SetSmallNodeMax/SetMaxQueryRadius
+ * (KDTreeGPU.h:248-280).  Defaults: triangles 0.25 / 64 (KDTreeTri.cpp:46-54),
+ * points 0.1 / 32 (KDTreePoint.cpp:120-134). */
+typedef struct mnrt_build_params {
+    float empty_space_ratio;
+    uint32_t small_node_max; /* 1..64 (ElementMask is 64 bits, KDKernelDefs.h:55) */
+    float max_query_radius;  /* VVH cost extension for point trees (kdtree.cu:1370-1389) */
+    uint32_t has_root_box;   /* 1: use root_min/root_max; 0: tight bounds of the input */
+    float root_min[3];
+    float root_max[3];
+} mnrt_build_params;
+
+/* A device-resident tree.  Filled by the builders; a caller may also fill nodes/refs
+ * itself (owns = 0) and attach records with mnrt_tree_attach_*_records(). */
+typedef struct mnrt_tree {
+    uint32_t kind;       /* MNRT_KIND_TRI or MNRT_KIND_POINT */
+    uint32_t num_nodes;
+    uint32_t num_refs;
+    uint32_t num_elems;  /* input element count */
+    uint32_t num_leaves;
+    uint32_t max_level;  /* deepest node level present */
+    float root_min[3];
+    float root_max[3];
+    uint32_t* d_nodes;   /* 2 * num_nodes words */
+    uint32_t* d_refs;    /* num_refs */
+    float* d_recs;       /* num_refs * (12 | 8) floats, see header comment */
+    uint32_t owns_nodes; /* library allocated d_nodes/d_refs */
+    uint32_t owns_recs;  /* library allocated d_recs */
+} mnrt_tree;
+
+const char* mnrt_status_string(int status);
+int mnrt_abi_version(void);
+
+/* Selects the device and checks that it is gfx950. */
+int mnrt_device_init(int device);
+
+/* --- build ------------------------------------------------------------------------ */
+
+/* KDTreeTri(const TriangleData&) + KDTreeGPU::BuildTree() (KDTreeTri.cpp:46-91,
+ * KDTreeGPU.cpp:175-209): triangle vertices as three float4 arrays of n entries. */
+int mnrt_build_tri(const float* d_v0, const float* d_v1, const float* d_v2, uint32_t n,
+                   const mnrt_build_params* params, mnrt_tree* out, void* stream);
+
+/* KDTreePoint(d_points, n, min, max, maxQueryRadius) + BuildTree()
+ * (KDTreePoint.cpp:120-134): photon positions as a float4 array.  Photon records are
+ * attached separately because the reference builds the tree on positions only. */
+int mnrt_build_points(const float* d_pos4, uint32_t n, const mnrt_build_params* params,
+                      mnrt_tree* out, void* stream);
+
+/* Leaf-ordered records for a tree whose nodes/refs exist. */
+int mnrt_tree_attach_tri_records(mnrt_tree* tree, const float* d_v0, const float* d_v1,
+                                 const float* d_v2, void* stream);
+/* d_dir4: incoming (travel) direction of each photon; the reference stores it as
+ * spherical angles in the w components (photon_dev.h:71-116). */
+int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_pos4, const float* d_pow4,
+                                    const float* d_dir4, void* stream);
+
+/* KDTreeGPU::Destroy() (KDTreeGPU.cpp:237-245): frees what the library allocated. */
+int mnrt_tree_release(mnrt_tree* tree);
+
+/* Tree SAH cost (triangles) or VVH cost (points) of a device tree, computed on the
+ * device: sum_inner 3*A(n)/A(root) + sum_leaf count*A(n)/A(root) (kdtree.cu:51,1418-1436). */
+int mnrt_tree_cost(const mnrt_tree* tree, float max_query_radius, double* out_cost,
+                   void* stream);
+
+/* --- traversal ----------------------------------------------------------------------- */
+
+/* KernelRTTraceRays(const RayChunk&, ShadingPoints&, uint* d_outIsValid)
+ * (raytracing.cu:854-879, 1617-1635): closest hit for n rays.  eps is the ray epsilon
+ * (SceneConfig.cpp:111); hits need lambda > eps.  Outputs: triangle index or -1,
+ * lambda, barycentrics (float2), optional hit point o + d*lambda (float4), optional
+ * valid flag.  Equal lambdas resolve to the lower triangle index. */
+int mnrt_trace_closest(const mnrt_tree* tree, uint32_t n, const float* d_orig4,
+                       const float* d_dir4, float eps, int32_t* d_out_tri, float* d_out_t,
+                       float* d_out_bary2, float* d_out_pt4, uint32_t* d_out_valid,
+                       void* stream);
+
+/* --- photon-map kNN radiance --------------------------------------------------------- */
+
+/* KernelPMPhotonGather (photon_knn.cu:487-552, 833-844) with the range search of
+ * dev_RangeSearch (photon_knn.cu:113-235) replaced by the exact k nearest photons
+ * (ties by photon index) among those closer than r_max:
+ *   E = sum_{front-facing of the set} power / (pi * r2),  r2 = d_k^2 when k were found,
+ *   else r_max^2;  E = 0 when fewer than 8 were found and d_clr4 is given;
+ *   radiance[idx] += alpha * rho * (1/pi) * E * influence.
+ * Queries: position, shading normal, source ray direction (normal flipped toward it),
+ * influence (float4 each, n entries).  d_clr4: diffuse colour with alpha in w, or NULL
+ * (photon visualisation: colour 1, no 8-photon cut).  d_idx_sp: output slot per query
+ * or NULL (slot = query).  Optional outputs: d_out_r2 (n floats: the r2 used),
+ * d_out_irr4 (n float4: E), d_out_nbr (n*k photon indices, sorted by (d2, index),
+ * 0xFFFFFFFF past the count), d_out_count (n: neighbours found). */
+int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                    const float* d_pos4, const float* d_nrm4, const float* d_dir4,
+                    const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,
+                    float* d_io_rad4, float* d_out_r2, float* d_out_irr4, uint32_t* d_out_nbr,
+                    uint32_t* d_out_count, void* stream);
+
+/* KDTreePoint::ComputeQueryRadii (KDTreePoint.cpp:229-248): radius to the k-th nearest
+ * photon, capped at r_max (r_max when fewer than k are closer). */
+int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                   const float* d_pos4, float* d_out_radius, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MNRT_KD_H */

@@ -1,0 +1,218 @@
+"""Python entry points over the C ABI, with torch device tensors as buffers.
+
+Mirrors the reference call surface for the hot path:
+  KDTreeTri / KDTreePoint + BuildTree   -> build_tri / build_points  (KDTreeGPU.cpp:175)
+  KernelRTTraceRays                     -> trace_closest            (raytracing.cu:1617)
+  KernelPMPhotonGather                  -> knn_gather               (photon_knn.cu:833)
+  KDTreePoint::ComputeQueryRadii        -> knn_radii                (KDTreePoint.cpp:229)
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native as N
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class DeviceTree:
+    """A device-resident tree plus the tensors that keep caller-provided arrays alive."""
+
+    def __init__(self, kind: int):
+        self.t = N.Tree()
+        self.t.kind = kind
+        self._keep = []
+
+    @property
+    def kind(self):
+        return self.t.kind
+
+    @property
+    def num_nodes(self):
+        return self.t.num_nodes
+
+    @property
+    def num_refs(self):
+        return self.t.num_refs
+
+    @property
+    def max_level(self):
+        return self.t.max_level
+
+    @property
+    def root(self):
+        return np.array(list(self.t.root_min) + list(self.t.root_max), np.float32)
+
+    def ref(self):
+        return ctypes.byref(self.t)
+
+    def nodes_host(self) -> np.ndarray:
+        torch = _torch()
+        n = 2 * self.t.num_nodes
+        out = torch.empty(n, dtype=torch.int32, device="cuda")
+        _copy_from_ptr(out, self.t.d_nodes, n * 4)
+        return out.cpu().numpy().view(np.uint32).copy()
+
+    def refs_host(self) -> np.ndarray:
+        torch = _torch()
+        n = self.t.num_refs
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        out = torch.empty(n, dtype=torch.int32, device="cuda")
+        _copy_from_ptr(out, self.t.d_refs, n * 4)
+        return out.cpu().numpy().view(np.uint32).copy()
+
+    def cost(self, max_query_radius: float = 1.0) -> float:
+        c = ctypes.c_double(0.0)
+        N.check(N.load().mnrt_tree_cost(self.ref(), max_query_radius, ctypes.byref(c), N.stream_handle()),
+                "mnrt_tree_cost")
+        return c.value
+
+    def release(self):
+        if self.t.owns_nodes or self.t.owns_recs:
+            _torch().cuda.synchronize()
+            N.check(N.load().mnrt_tree_release(self.ref()), "mnrt_tree_release")
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if self.t.owns_nodes or self.t.owns_recs:
+                self.release()
+        except Exception:
+            pass
+
+
+def _copy_from_ptr(dst, src_ptr: int, nbytes: int):
+    torch = _torch()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    rc = hip.hipMemcpyAsync(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src_ptr), nbytes, 3,
+                            ctypes.c_void_p(N.stream_handle()))
+    if rc != 0:
+        raise N.MnrtError(f"hipMemcpyAsync failed ({rc})")
+    torch.cuda.current_stream().synchronize()
+
+
+def dev(a, dtype=None):
+    """numpy -> contiguous cuda tensor (float32 / int32 views of uint32 data)."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        return a.contiguous().cuda()
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    return torch.from_numpy(a).cuda()
+
+
+def tree_from_host(kind, nodes: np.ndarray, refs: np.ndarray, root6, num_elems: int, max_level: int) -> DeviceTree:
+    """Wraps host-built node words/references (same layout) as a device tree."""
+    T = DeviceTree(kind)
+    dn = dev(np.asarray(nodes, np.uint32))
+    dr = dev(np.asarray(refs, np.uint32) if len(refs) else np.zeros(1, np.uint32))
+    T._keep += [dn, dr]
+    t = T.t
+    t.num_nodes = len(nodes) // 2
+    t.num_refs = len(refs)
+    t.num_elems = num_elems
+    t.max_level = max_level
+    for i in range(3):
+        t.root_min[i] = float(root6[i])
+        t.root_max[i] = float(root6[3 + i])
+    t.d_nodes = dn.data_ptr()
+    t.d_refs = dr.data_ptr()
+    t.owns_nodes = 0
+    t.owns_recs = 0
+    return T
+
+
+def attach_tri_records(T: DeviceTree, v0, v1, v2):
+    v0, v1, v2 = dev(v0), dev(v1), dev(v2)
+    T._keep += [v0, v1, v2]
+    N.check(N.load().mnrt_tree_attach_tri_records(T.ref(), N.ptr(v0), N.ptr(v1), N.ptr(v2), N.stream_handle()),
+            "mnrt_tree_attach_tri_records")
+
+
+def attach_photon_records(T: DeviceTree, pos, power, dirs):
+    pos, power, dirs = dev(pos), dev(power), dev(dirs)
+    T._keep += [pos, power, dirs]
+    N.check(N.load().mnrt_tree_attach_photon_records(T.ref(), N.ptr(pos), N.ptr(power), N.ptr(dirs),
+                                                     N.stream_handle()), "mnrt_tree_attach_photon_records")
+
+
+def default_params(kind: int, root=None, max_query_radius: float = 1.0) -> N.BuildParams:
+    p = N.BuildParams()
+    p.empty_space_ratio = 0.25 if kind == N.KIND_TRI else 0.1
+    p.small_node_max = 64 if kind == N.KIND_TRI else 32
+    p.max_query_radius = max_query_radius
+    if root is not None:
+        p.has_root_box = 1
+        for i in range(3):
+            p.root_min[i] = float(root[i])
+            p.root_max[i] = float(root[3 + i])
+    return p
+
+
+def build_tri(v0, v1, v2, params: N.BuildParams | None = None) -> DeviceTree:
+    v0, v1, v2 = dev(v0), dev(v1), dev(v2)
+    T = DeviceTree(N.KIND_TRI)
+    params = params or default_params(N.KIND_TRI)
+    N.check(N.load().mnrt_build_tri(N.ptr(v0), N.ptr(v1), N.ptr(v2), v0.shape[0], ctypes.byref(params), T.ref(),
+                                    N.stream_handle()), "mnrt_build_tri")
+    return T
+
+
+def build_points(pos, params: N.BuildParams | None = None) -> DeviceTree:
+    pos = dev(pos)
+    T = DeviceTree(N.KIND_POINT)
+    params = params or default_params(N.KIND_POINT)
+    N.check(N.load().mnrt_build_points(N.ptr(pos), pos.shape[0], ctypes.byref(params), T.ref(), N.stream_handle()),
+            "mnrt_build_points")
+    return T
+
+
+def trace_closest(T: DeviceTree, orig, dirs, eps: float, want_pt=False):
+    torch = _torch()
+    orig, dirs = dev(orig), dev(dirs)
+    n = orig.shape[0]
+    tri = torch.empty(n, dtype=torch.int32, device="cuda")
+    t = torch.empty(n, dtype=torch.float32, device="cuda")
+    bary = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    pt = torch.empty((n, 4), dtype=torch.float32, device="cuda") if want_pt else None
+    N.check(N.load().mnrt_trace_closest(T.ref(), n, N.ptr(orig), N.ptr(dirs), eps, N.ptr(tri), N.ptr(t),
+                                        N.ptr(bary), N.ptr(pt), None, N.stream_handle()), "mnrt_trace_closest")
+    return tri, t, bary, pt
+
+
+def knn_gather(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, clr=None, idx_sp=None, io_rad=None,
+               want_nbr=False, want_r2=False, want_irr=False, want_count=False):
+    torch = _torch()
+    pos, nrm, dirs, infl = dev(pos), dev(nrm), dev(dirs), dev(infl)
+    clr = dev(clr) if clr is not None else None
+    idx_sp = dev(idx_sp) if idx_sp is not None else None
+    n = pos.shape[0]
+    if io_rad is None:
+        io_rad = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    nbr = torch.empty(n * k, dtype=torch.int32, device="cuda") if want_nbr else None
+    r2 = torch.empty(n, dtype=torch.float32, device="cuda") if want_r2 else None
+    irr = torch.empty((n, 4), dtype=torch.float32, device="cuda") if want_irr else None
+    cnt = torch.empty(n, dtype=torch.int32, device="cuda") if want_count else None
+    N.check(N.load().mnrt_knn_gather(P.ref(), k, r_max, n, N.ptr(pos), N.ptr(nrm), N.ptr(dirs), N.ptr(infl),
+                                     N.ptr(clr), N.ptr(idx_sp), N.ptr(io_rad), N.ptr(r2), N.ptr(irr), N.ptr(nbr),
+                                     N.ptr(cnt), N.stream_handle()), "mnrt_knn_gather")
+    return dict(rad=io_rad, nbr=nbr.view(n, k) if nbr is not None else None, r2=r2, irr=irr, count=cnt)
+
+
+def knn_radii(P: DeviceTree, k: int, r_max: float, pos):
+    torch = _torch()
+    pos = dev(pos)
+    out = torch.empty(pos.shape[0], dtype=torch.float32, device="cuda")
+    N.check(N.load().mnrt_knn_radii(P.ref(), k, r_max, pos.shape[0], N.ptr(pos), N.ptr(out), N.stream_handle()),
+            "mnrt_knn_radii")
+    return out

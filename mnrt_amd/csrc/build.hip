@@ -1,0 +1,1393 @@
+// build.hip — kd-tree construction on gfx950 for triangles (SAH) and photons (VVH).
+//
+// Follows the two-stage build of KDTreeGPU::BuildTree (KDTreeGPU.cpp:175-209):
+//
+//   large-node stage (LargeNodeStage / ProcessLargeNodes, KDTreeGPU.cpp:247-782):
+//     per level, for all active nodes at once:
+//       chunk list + tight bounds   k_chunk_*, k_tight     (kdtree_chunklist.cu:73-184, kdtree.cu:310-445)
+//       empty-space cuts + median   k_plan                 (kdtree.cu:467-682, 1986-2033)
+//       element classification      k_classify             (kdtree.cu:752-824)
+//       stream compaction           scan of L/R flags      (CompactElementData, KDTreeGPU.cpp:664-782)
+//       small/large partition       k_child_plan/_write    (UpdateSmallList, KDTreeGPU.cpp:784-941)
+//       split clipping + scatter    k_scatter              (kdtree_tri.cu:190-489)
+//     The level loop keeps all per-element work on the device; the host reads five
+//     counters once per level (the reference synchronises about ten times per level).
+//
+//   small-node stage (PreProcessSmallNodes / ProcessSmallNodes, KDTreeGPU.cpp:945-1154):
+//     one wave64 per small root builds the whole subtree in LDS: split candidates and
+//     their 64-bit element masks (kdtree.cu:1035-1251), exact SAH / VVH per node with a
+//     wave argmin over (cost, candidate index) (kdtree.cu:1288-1471), preorder emission.
+//
+//   layout: subtree sizes bottom-up, addresses top-down (KDTreeGPU::PreorderTraversal,
+//   KDTreeGPU.cpp:1156-1209), in one single-workgroup pass over the large-stage nodes.
+//
+// Deterministic rules replace the reference's random and racy choices (elements in a
+// split plane go left for even index, right for odd; equal SAH costs resolve to the
+// lowest candidate index), so the tree is identical, word for word, to the one the host
+// oracle builds.
+#include <algorithm>
+#include <vector>
+
+#include "scan.h"
+
+namespace mnrt {
+
+constexpr uint32_t kChunk = 256;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+enum : uint32_t { BN_LEAF = 3, BN_SMALL = 4, BN_PENDING = 5, BN_DEAD = 6 };
+// counters[] slots written by device kernels
+enum : uint32_t {
+    C_NCHUNKS = 0,
+    C_NEWNODES,
+    C_NSMALL_NEW,
+    C_NNEXT,
+    C_ESMALL_NEW,
+    C_ENEXT,
+    C_MAXLEVEL,
+    C_LLCOUNT,
+    C_TOTAL_NODES,
+    C_TOTAL_REFS,
+    C_LEAVES,
+    C_NLEAFCOPY,
+    C_ERR,
+    C_COUNT
+};
+
+// Build-node table: one entry per node of the large-stage tree (inner nodes, empty
+// and capped leaves, small roots), 7 uint32 columns of `cap` entries.
+struct BnView {
+    uint32_t* type;   // 0..2 inner split axis, BN_LEAF, BN_SMALL, BN_PENDING, BN_DEAD
+    uint32_t* split;  // split position bits (inner)
+    uint32_t* left;
+    uint32_t* right;
+    uint32_t* level;
+    uint32_t* aux;  // small-root index (BN_SMALL) or first copied element (BN_LEAF)
+    uint32_t* cnt;  // element count (BN_LEAF)
+};
+
+struct SmallView {  // small roots
+    uint32_t* first;
+    uint32_t* cnt;
+    uint32_t* level;
+    uint32_t* bn;
+    float4* lo;
+    float4* hi;
+};
+
+struct ElemView {  // element arrays: original index, bounds, owning active node
+    uint32_t* idx;
+    float4* lo;
+    float4* hi;
+    uint32_t* node;
+};
+
+struct NodeView {  // active large nodes
+    float4* lo;
+    float4* hi;
+    uint32_t* first;
+    uint32_t* cnt;
+    uint32_t* level;
+    uint32_t* bn;
+};
+
+struct Geometry {
+    int kind;
+    const float4* v0;
+    const float4* v1;
+    const float4* v2;
+};
+
+__device__ __forceinline__ uint32_t fenc(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fdec(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+__device__ __forceinline__ float f4c(const float4& v, uint32_t a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+__device__ __forceinline__ void f4set(float4& v, uint32_t a, float x) {
+    if (a == 0) v.x = x;
+    else if (a == 1) v.y = x;
+    else v.z = x;
+}
+// -0 -> +0 so min/max and stored split words never depend on the sign of zero
+__device__ __forceinline__ float canon(float x) { return x + 0.0f; }
+
+__device__ __forceinline__ bool plane_goes_left(uint32_t idx) { return (idx & 1u) == 0u; }
+
+// Large-node classification (kernel_MarkLeftRightElements, kdtree.cu:752-824) with the
+// even/odd rule for elements in the split plane.
+__device__ __forceinline__ void classify(int kind, uint32_t idx, const float4& lo, const float4& hi,
+                                         uint32_t axis, float s, bool& L, bool& R) {
+    if (kind == MNRT_KIND_TRI) {
+        float mn = f4c(lo, axis), mx = f4c(hi, axis);
+        bool planar_on = (mn == s && mn == mx);
+        L = mn < s || (planar_on && plane_goes_left(idx));
+        R = s < mx || (planar_on && !plane_goes_left(idx));
+    } else {
+        float p = f4c(lo, axis);
+        if (plane_goes_left(idx)) {
+            L = p <= s;
+            R = s < p;
+        } else {
+            L = p < s;
+            R = s <= p;
+        }
+    }
+}
+
+// ---- split clipping (dev_ClipSegment / dev_ClipTriToBounds, kdtree_tri.cu:190-316) ----
+
+struct PList {
+    int n;
+    f3 p[12];
+};
+
+__device__ __forceinline__ float f3c(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+__device__ __forceinline__ void f3set(f3& v, int a, float x) {
+    if (a == 0) v.x = x;
+    else if (a == 1) v.y = x;
+    else v.z = x;
+}
+
+__device__ void clip_segments(const PList& in, int axis, float val, float sense, PList& out) {
+    out.n = 0;
+    for (int i = 0; i < in.n; ++i) {
+        f3 a = in.p[i];
+        f3 b = in.p[(i + 1 == in.n) ? 0 : i + 1];
+        bool ins = (f3c(a, axis) - val) * sense >= 0.0f;
+        bool ine = (f3c(b, axis) - val) * sense >= 0.0f;
+        if (ins != ine) {
+            float t = (val - f3c(a, axis)) / (f3c(b, axis) - f3c(a, axis));
+            f3 h = mk3(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z));
+            f3set(h, axis, val);
+            out.p[out.n++] = h;
+        }
+        if (ine) out.p[out.n++] = b;
+    }
+}
+
+// Clipped bounds of a straddling triangle inside a child box, padded outward by 2^-20 of
+// the largest magnitude involved, clamped to the child box and to the previous bounds.
+__device__ void clip_bounds(const Geometry& G, uint32_t idx, float4& lo, float4& hi, const float4& bmin,
+                            const float4& bmax) {
+    PList a, b;
+    a.n = 3;
+    a.p[0] = xyz(G.v0[idx]);
+    a.p[1] = xyz(G.v1[idx]);
+    a.p[2] = xyz(G.v2[idx]);
+    for (int ax = 0; ax < 3; ++ax) {
+        if (f4c(lo, ax) != f4c(hi, ax)) {
+            clip_segments(a, ax, f4c(bmin, ax), 1.0f, b);
+            clip_segments(b, ax, f4c(bmax, ax), -1.0f, a);
+        }
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        float l, h;
+        float bl = f4c(bmin, ax), bh = f4c(bmax, ax);
+        if (a.n == 0) {
+            l = f4c(lo, ax);
+            h = f4c(hi, ax);
+        } else {
+            l = kInf;
+            h = -kInf;
+            for (int i = 0; i < a.n; ++i) {
+                l = fminf(f3c(a.p[i], ax), l);
+                h = fmaxf(f3c(a.p[i], ax), h);
+            }
+            float m = fmaxf(fmaxf(fabsf(l), fabsf(h)), fmaxf(fabsf(bl), fabsf(bh)));
+            float pad = m * (1.0f / 1048576.0f);
+            l = l - pad;
+            h = h + pad;
+        }
+        l = fmaxf(l, bl);
+        h = fminf(h, bh);
+        l = fmaxf(l, f4c(lo, ax));
+        h = fminf(h, f4c(hi, ax));
+        f4set(lo, ax, canon(l));
+        f4set(hi, ax, canon(h));
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_init_elems(Geometry G, uint32_t n, ElemView E) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 a = G.v0[i];
+    float4 lo, hi;
+    if (G.kind == MNRT_KIND_TRI) {
+        float4 b = G.v1[i], c = G.v2[i];
+        lo = make_float4(canon(fminf(a.x, fminf(b.x, c.x))), canon(fminf(a.y, fminf(b.y, c.y))),
+                         canon(fminf(a.z, fminf(b.z, c.z))), 0.0f);
+        hi = make_float4(canon(fmaxf(a.x, fmaxf(b.x, c.x))), canon(fmaxf(a.y, fmaxf(b.y, c.y))),
+                         canon(fmaxf(a.z, fmaxf(b.z, c.z))), 0.0f);
+    } else {
+        lo = hi = make_float4(canon(a.x), canon(a.y), canon(a.z), 0.0f);
+    }
+    E.idx[i] = i;
+    E.lo[i] = lo;
+    E.hi[i] = hi;
+    E.node[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_counts(uint32_t n, const uint32_t* __restrict__ cnt,
+                                                      uint32_t* __restrict__ cc) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) cc[p] = (cnt[p] + kChunk - 1) / kChunk;
+}
+
+// Chunk -> node by binary search over the chunk offsets (kernel_GenerateChunks,
+// kdtree_chunklist.cu:110-134, runs one serial thread per node instead).
+__global__ __launch_bounds__(256) void k_chunk_fill(uint32_t n, uint32_t max_chunks, const uint32_t* __restrict__ coff,
+                                                    const uint32_t* __restrict__ counters,
+                                                    const uint32_t* __restrict__ first,
+                                                    const uint32_t* __restrict__ cnt, uint32_t* __restrict__ ch_node,
+                                                    uint32_t* __restrict__ ch_first, uint32_t* __restrict__ ch_cnt) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= max_chunks || c >= counters[C_NCHUNKS]) return;
+    uint32_t lo = 0, hi = n;  // last p with coff[p] <= c
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (coff[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    uint32_t p = lo;
+    uint32_t local = c - coff[p];
+    ch_node[c] = p;
+    ch_first[c] = first[p] + local * kChunk;
+    ch_cnt[c] = min(kChunk, cnt[p] - local * kChunk);
+}
+
+__global__ __launch_bounds__(256) void k_tight_init(uint32_t n, uint32_t* __restrict__ tlo, uint32_t* __restrict__ thi) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * n) {
+        tlo[i] = 0xFFFFFFFFu;
+        thi[i] = 0u;
+    }
+}
+
+// Tight node bounds: per-chunk wave/LDS min/max, one atomic per chunk and component
+// (kernel_GenChunkAABB + two segmented reductions, kdtree.cu:310-445, KDTreeGPU.cpp:526-553).
+// min/max are exact, so the order of the atomics does not matter.
+__global__ __launch_bounds__(kChunk) void k_tight(uint32_t max_chunks, const uint32_t* __restrict__ counters,
+                                                  const uint32_t* __restrict__ ch_node,
+                                                  const uint32_t* __restrict__ ch_first,
+                                                  const uint32_t* __restrict__ ch_cnt, ElemView E, int kind,
+                                                  uint32_t* __restrict__ tlo, uint32_t* __restrict__ thi) {
+    __shared__ uint32_t s_red[6][kChunk / 64];
+    uint32_t c = blockIdx.x;
+    if (c >= max_chunks || c >= counters[C_NCHUNKS]) return;
+    uint32_t node = ch_node[c], first = ch_first[c], count = ch_cnt[c];
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    if (threadIdx.x < count) {
+        float4 lo = E.lo[first + threadIdx.x];
+        float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[first + threadIdx.x] : lo;
+        v[0] = fenc(lo.x);
+        v[1] = fenc(lo.y);
+        v[2] = fenc(lo.z);
+        v[3] = fenc(hi.x);
+        v[4] = fenc(hi.y);
+        v[5] = fenc(hi.z);
+    }
+#pragma unroll
+    for (uint32_t o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = min(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
+#pragma unroll
+        for (int k = 3; k < 6; ++k) v[k] = max(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
+    }
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) s_red[k][wave] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < kChunk / 64; ++w) {
+            for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_red[k][w]);
+            for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_red[k][w]);
+        }
+        for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * node + k], v[k]);
+        for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * node + k], v[3 + k]);
+    }
+}
+
+__global__ void k_root_box_from_tight(const uint32_t* __restrict__ tlo, const uint32_t* __restrict__ thi,
+                                      float4* __restrict__ lo, float4* __restrict__ hi, float* __restrict__ host_box) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float4 l = make_float4(fdec(tlo[0]), fdec(tlo[1]), fdec(tlo[2]), 0.0f);
+    float4 h = make_float4(fdec(thi[0]), fdec(thi[1]), fdec(thi[2]), 0.0f);
+    lo[0] = l;
+    hi[0] = h;
+    host_box[0] = l.x;
+    host_box[1] = l.y;
+    host_box[2] = l.z;
+    host_box[3] = h.x;
+    host_box[4] = h.y;
+    host_box[5] = h.z;
+}
+
+struct PlanView {
+    uint32_t* newcnt;  // build nodes created per active node (count phase) / base (write phase)
+    uint32_t* axis;
+    float* split;
+    float4* lo;        // box after the empty-space cuts
+    float4* hi;
+    uint32_t* level;   // level of the node that is split (after cuts)
+    uint32_t* childL;  // build-node id of the left child (right = +1)
+    uint32_t* cur;     // build-node id of the node that is split or becomes a leaf
+    uint32_t* leaf;    // 1: level cap reached, the node is a leaf with its elements
+    uint32_t* noprog;  // 1: every element went to both children, the node is a leaf
+};
+
+// Empty-space cuts then the spatial median split (kernel_CanCutOffEmptySpace,
+// kernel_EmptySpaceCutting, kernel_SplitLargeNodes; kdtree.cu:467-682).  Sides are cut in
+// the reference's order (min x, y, z, then max x, y, z); each cut turns the node into an
+// inner node with an empty leaf and a tighter copy.  The median splits the longest axis
+// of the cut box, ties to the lower axis.
+template <bool kWrite>
+__global__ __launch_bounds__(256) void k_plan(uint32_t n, NodeView A, const uint32_t* __restrict__ tlo,
+                                              const uint32_t* __restrict__ thi, float ratio, PlanView P,
+                                              const uint32_t* __restrict__ base, uint32_t bn_base, BnView B,
+                                              uint32_t* __restrict__ counters) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float lo[3], hi[3], tl[3], th[3];
+    float4 l4 = A.lo[p], h4 = A.hi[p];
+    lo[0] = l4.x; lo[1] = l4.y; lo[2] = l4.z;
+    hi[0] = h4.x; hi[1] = h4.y; hi[2] = h4.z;
+    for (int k = 0; k < 3; ++k) {
+        tl[k] = fdec(tlo[3 * p + k]);
+        th[k] = fdec(thi[3 * p + k]);
+    }
+    uint32_t level = A.level[p];
+    uint32_t cur = A.bn[p];
+    uint32_t next_id = kWrite ? bn_base + base[p] : 0u;
+    uint32_t ncuts = 0;
+    for (int isMax = 0; isMax < 2; ++isMax) {
+        for (int ax = 0; ax < 3; ++ax) {
+            if (level >= MNRT_MAX_LEVEL) continue;
+            float total = hi[ax] - lo[ax];
+            float empty = isMax ? (hi[ax] - th[ax]) : (tl[ax] - lo[ax]);
+            if (!(empty > total * ratio)) continue;
+            float s = isMax ? th[ax] : tl[ax];
+            ++ncuts;
+            if (kWrite) {
+                uint32_t eleaf = next_id++, cont = next_id++;
+                B.type[cur] = (uint32_t)ax;
+                B.split[cur] = __float_as_uint(s);
+                B.left[cur] = isMax ? cont : eleaf;
+                B.right[cur] = isMax ? eleaf : cont;
+                B.type[eleaf] = BN_LEAF;
+                B.cnt[eleaf] = 0;
+                B.aux[eleaf] = 0;
+                B.level[eleaf] = level + 1;
+                B.type[cont] = BN_PENDING;
+                B.level[cont] = level + 1;
+                cur = cont;
+            }
+            if (isMax) hi[ax] = s;
+            else lo[ax] = s;
+            level += 1;
+        }
+    }
+    if (!kWrite) {
+        P.newcnt[p] = 2 * ncuts + (level >= MNRT_MAX_LEVEL ? 0u : 2u);
+        return;
+    }
+    atomicMax(&counters[C_MAXLEVEL], level);
+    P.cur[p] = cur;
+    P.level[p] = level;
+    P.lo[p] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    P.hi[p] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    P.noprog[p] = 0;
+    if (level >= MNRT_MAX_LEVEL) {
+        P.leaf[p] = 1;
+        P.axis[p] = 0;
+        P.split[p] = 0.0f;
+        P.childL[p] = kNone;
+        B.type[cur] = BN_LEAF;
+        return;
+    }
+    float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    uint32_t axis = 0;
+    float emax = ex;
+    if (ey > emax) { axis = 1; emax = ey; }
+    if (ez > emax) axis = 2;
+    float s = lo[axis] + 0.5f * (hi[axis] - lo[axis]);
+    uint32_t L = next_id++, R = next_id++;
+    P.leaf[p] = 0;
+    P.axis[p] = axis;
+    P.split[p] = s;
+    P.childL[p] = L;
+    B.type[cur] = axis;
+    B.split[cur] = __float_as_uint(s);
+    B.left[cur] = L;
+    B.right[cur] = R;
+    B.level[L] = level + 1;
+    B.level[R] = level + 1;
+    B.type[L] = BN_PENDING;
+    B.type[R] = BN_PENDING;
+}
+
+__global__ __launch_bounds__(256) void k_classify(uint32_t ne, ElemView E, int kind, PlanView P,
+                                                  uint32_t* __restrict__ fl) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    uint32_t p = E.node[e];
+    bool L = false, R = false;
+    if (!P.leaf[p]) {
+        float4 lo = E.lo[e];
+        float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[e] : lo;
+        classify(kind, E.idx[e], lo, hi, P.axis[p], P.split[p], L, R);
+    }
+    fl[e] = L ? 1u : 0u;
+    fl[ne + e] = R ? 1u : 0u;
+}
+
+struct ChildView {
+    uint32_t* type;  // BN_SMALL, BN_PENDING (large), BN_DEAD
+    uint32_t* cnt;
+    uint32_t* fpos;  // flag-scan position of the child's first element
+    uint32_t* sflag;
+    uint32_t* lflag;
+    uint32_t* scnt;
+    uint32_t* lcnt;
+    uint32_t* sidx;  // scans of the four columns above
+    uint32_t* lidx;
+    uint32_t* soff;
+    uint32_t* loff;
+};
+
+// Child counts from the flag scan, the no-progress test, and the small/large split
+// (kernel_MarkSmallNodes, kdtree.cu:849-875: small iff count <= smallNodeMax).
+__global__ __launch_bounds__(256) void k_child_plan(uint32_t n, uint32_t ne, NodeView A, PlanView P,
+                                                    const uint32_t* __restrict__ pos, uint32_t small_max,
+                                                    ChildView Cv) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= 2 * n) return;
+    uint32_t side = c >= n ? 1u : 0u;
+    uint32_t p = c - side * n;
+    uint32_t type = BN_DEAD, cnt = 0, fpos = 0;
+    if (!P.leaf[p]) {
+        uint32_t f = A.first[p], m = A.cnt[p];
+        uint32_t nl = pos[f + m] - pos[f];
+        uint32_t nr = pos[ne + f + m] - pos[ne + f];
+        bool noprog = (nl == m && nr == m);
+        if (noprog) {
+            if (side == 0) P.noprog[p] = 1;
+        } else {
+            fpos = pos[side * ne + f];
+            cnt = side ? nr : nl;
+            type = (cnt <= small_max) ? BN_SMALL : BN_PENDING;
+        }
+    }
+    Cv.type[c] = type;
+    Cv.cnt[c] = cnt;
+    Cv.fpos[c] = fpos;
+    Cv.sflag[c] = type == BN_SMALL ? 1u : 0u;
+    Cv.lflag[c] = type == BN_PENDING ? 1u : 0u;
+    Cv.scnt[c] = type == BN_SMALL ? cnt : 0u;
+    Cv.lcnt[c] = type == BN_PENDING ? cnt : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_child_write(uint32_t n, PlanView P, ChildView Cv, uint32_t small_base,
+                                                     uint32_t small_elem_base, SmallView S, NodeView Nx,
+                                                     BnView B, uint32_t* __restrict__ counters) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= 2 * n) return;
+    uint32_t side = c >= n ? 1u : 0u;
+    uint32_t p = c - side * n;
+    if (P.leaf[p]) return;
+    if (P.noprog[p]) {
+        if (side == 0) {
+            B.type[P.cur[p]] = BN_LEAF;
+            B.type[P.childL[p]] = BN_DEAD;
+            B.type[P.childL[p] + 1] = BN_DEAD;
+        }
+        return;
+    }
+    uint32_t bn = P.childL[p] + side;
+    uint32_t level = P.level[p] + 1;
+    atomicMax(&counters[C_MAXLEVEL], level);
+    float4 lo = P.lo[p], hi = P.hi[p];
+    if (side == 0) f4set(hi, P.axis[p], P.split[p]);
+    else f4set(lo, P.axis[p], P.split[p]);
+    uint32_t type = Cv.type[c];
+    if (type == BN_SMALL) {
+        uint32_t s = small_base + Cv.sidx[c];
+        S.first[s] = small_elem_base + Cv.soff[c];
+        S.cnt[s] = Cv.cnt[c];
+        S.level[s] = level;
+        S.bn[s] = bn;
+        S.lo[s] = lo;
+        S.hi[s] = hi;
+        B.type[bn] = BN_SMALL;
+        B.aux[bn] = s;
+    } else {
+        uint32_t l = Cv.lidx[c];
+        Nx.lo[l] = lo;
+        Nx.hi[l] = hi;
+        Nx.first[l] = Cv.loff[c];
+        Nx.cnt[l] = Cv.cnt[c];
+        Nx.level[l] = level;
+        Nx.bn[l] = bn;
+        B.type[bn] = BN_PENDING;
+    }
+}
+
+// Elements of large-stage leaves (level cap, no progress) are copied out before the
+// level's element arrays are released.
+__global__ __launch_bounds__(256) void k_leaf_copy(uint32_t n, NodeView A, PlanView P, ElemView E,
+                                                   uint32_t* __restrict__ ll_idx, BnView B,
+                                                   uint32_t* __restrict__ counters) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !(P.leaf[p] || P.noprog[p])) return;
+    uint32_t m = A.cnt[p], f = A.first[p];
+    uint32_t start = atomicAdd(&counters[C_LLCOUNT], m);
+    for (uint32_t i = 0; i < m; ++i) ll_idx[start + i] = E.idx[f + i];
+    uint32_t cur = P.cur[p];
+    B.aux[cur] = start;
+    B.cnt[cur] = m;
+}
+
+// Scatter each element into its children: small roots append to the small-element
+// buffer, large children to the next level's arrays.  Straddling triangles are clipped
+// to each child box (kernel_PerformSplitClipping, kdtree_tri.cu:389-489).
+__global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t ne, ElemView E, Geometry G, PlanView P,
+                                                 ChildView Cv, const uint32_t* __restrict__ pos,
+                                                 uint32_t small_elem_base, ElemView Sm, uint32_t sm_cap, ElemView Nx,
+                                                 uint32_t nx_cap, uint32_t* __restrict__ counters) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    uint32_t p = E.node[e];
+    if (P.leaf[p] || P.noprog[p]) return;
+    uint32_t idx = E.idx[e];
+    float4 lo = E.lo[e];
+    float4 hi = (G.kind == MNRT_KIND_TRI) ? E.hi[e] : lo;
+    uint32_t axis = P.axis[p];
+    float s = P.split[p];
+    bool L, R;
+    classify(G.kind, idx, lo, hi, axis, s, L, R);
+    bool straddle = (G.kind == MNRT_KIND_TRI) && (f4c(lo, axis) < s && s < f4c(hi, axis));
+    for (uint32_t side = 0; side < 2; ++side) {
+        if (!(side ? R : L)) continue;
+        uint32_t c = side * n + p;
+        uint32_t type = Cv.type[c];
+        if (type == BN_DEAD) continue;
+        uint32_t rank = pos[side * ne + e] - Cv.fpos[c];
+        float4 clo = lo, chi = hi;
+        if (straddle) {
+            float4 bmin = P.lo[p], bmax = P.hi[p];
+            if (side == 0) f4set(bmax, axis, s);
+            else f4set(bmin, axis, s);
+            clip_bounds(G, idx, clo, chi, bmin, bmax);
+        }
+        if (type == BN_SMALL) {
+            uint32_t d = small_elem_base + Cv.soff[c] + rank;
+            if (d >= sm_cap) {
+                atomicOr(&counters[C_ERR], 1u);
+                continue;
+            }
+            Sm.idx[d] = idx;
+            Sm.lo[d] = clo;
+            if (G.kind == MNRT_KIND_TRI) Sm.hi[d] = chi;
+        } else {
+            uint32_t d = Cv.loff[c] + rank;
+            if (d >= nx_cap) {
+                atomicOr(&counters[C_ERR], 2u);
+                continue;
+            }
+            Nx.idx[d] = idx;
+            Nx.lo[d] = clo;
+            if (G.kind == MNRT_KIND_TRI) Nx.hi[d] = chi;
+            Nx.node[d] = Cv.lidx[c];
+        }
+    }
+}
+
+// ---- small-node stage: one wave64 per small root ----------------------------------------
+
+constexpr int kSmallWaves = 4;
+constexpr int kSmallStack = MNRT_MAX_LEVEL + 4;
+
+struct SmallStackEnt {
+    uint64_t mask;
+    float lo[3], hi[3];
+    uint32_t level;
+    uint32_t patch;  // node whose right-child word gets this entry's address, or kNone
+    uint32_t axis;
+};
+
+struct SmallOut {
+    uint32_t* nsize;  // pass 1: nodes per small subtree
+    uint32_t* rsize;  // pass 1: references per small subtree
+    const uint32_t* bn_addr;  // pass 2: subtree start address / reference base
+    const uint32_t* bn_rbase;
+    uint32_t* nodes;
+    uint32_t* refs;
+    uint32_t* counters;
+    uint32_t max_nodes;
+    uint32_t max_refs;
+};
+
+template <bool kEmit>
+__global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, SmallView S, ElemView Sm, int kind,
+                                                            float max_qr, SmallOut O) {
+    __shared__ float s_lo[kSmallWaves][3][64];
+    __shared__ float s_hi[kSmallWaves][3][64];
+    __shared__ uint32_t s_idx[kSmallWaves][64];
+    __shared__ float s_cpos[kSmallWaves][384];
+    __shared__ uint64_t s_mL[kSmallWaves][384];
+    __shared__ uint64_t s_mR[kSmallWaves][384];
+    __shared__ SmallStackEnt s_stack[kSmallWaves][kSmallStack];
+
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t r = blockIdx.x * kSmallWaves + wave;
+    if (r >= nsmall) return;  // uniform per wave
+    float(*elo)[64] = s_lo[wave];
+    float(*ehi)[64] = s_hi[wave];
+    uint32_t* eidx = s_idx[wave];
+    float* cpos = s_cpos[wave];
+    uint64_t* mL = s_mL[wave];
+    uint64_t* mR = s_mR[wave];
+    SmallStackEnt* stack = s_stack[wave];
+
+    const uint32_t first = S.first[r], n = S.cnt[r];
+    if (lane < n) {
+        float4 lo = Sm.lo[first + lane];
+        float4 hi = (kind == MNRT_KIND_TRI) ? Sm.hi[first + lane] : lo;
+        elo[0][lane] = lo.x; elo[1][lane] = lo.y; elo[2][lane] = lo.z;
+        ehi[0][lane] = hi.x; ehi[1][lane] = hi.y; ehi[2][lane] = hi.z;
+        eidx[lane] = Sm.idx[first + lane];
+    }
+    wave_sync();
+    // Candidates axis-major, minimums then maximums (kernel_CreateSplitCandidates,
+    // kdtree.cu:1035-1082); masks as kernel_InitSplitMasks (kdtree.cu:1110-1251).
+    const uint32_t per = 3 * n;
+    const uint32_t ncand = (kind == MNRT_KIND_TRI) ? 2 * per : per;
+    for (uint32_t c = lane; c < ncand; c += 64) {
+        uint32_t isMax = c / per, rem = c - isMax * per, axis = rem / n, i = rem - axis * n;
+        float sp = isMax ? ehi[axis][i] : elo[axis][i];
+        uint64_t ml = 0, mr = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            bool L, R;
+            if (kind == MNRT_KIND_TRI) {
+                float mn = elo[axis][j], mx = ehi[axis][j];
+                L = mn < sp;
+                R = sp < mx;
+                if (j == i) {
+                    if (isMax) L = true;
+                    else R = true;
+                } else if (mn == mx && mn == sp) {
+                    if (plane_goes_left(eidx[j])) L = true;
+                    else R = true;
+                }
+            } else {
+                float pv = elo[axis][j];
+                if (plane_goes_left(eidx[j])) {
+                    L = pv <= sp;
+                    R = sp < pv;
+                } else {
+                    L = pv < sp;
+                    R = sp <= pv;
+                }
+            }
+            if (L) ml |= 1ull << j;
+            if (R) mr |= 1ull << j;
+        }
+        cpos[c] = sp;
+        mL[c] = ml;
+        mR[c] = mr;
+    }
+
+    uint32_t addr = 0, rbase = 0, leaves = 0, maxlvl = 0;
+    if (kEmit) {
+        uint32_t bn = S.bn[r];
+        addr = O.bn_addr[bn];
+        rbase = O.bn_rbase[bn];
+    }
+    int sp = 0;
+    if (lane == 0) {
+        SmallStackEnt e;
+        e.mask = (n == 64) ? ~0ull : ((1ull << n) - 1ull);
+        float4 lo = S.lo[r], hi = S.hi[r];
+        e.lo[0] = lo.x; e.lo[1] = lo.y; e.lo[2] = lo.z;
+        e.hi[0] = hi.x; e.hi[1] = hi.y; e.hi[2] = hi.z;
+        e.level = S.level[r];
+        e.patch = kNone;
+        e.axis = 0;
+        stack[0] = e;
+    }
+    sp = 1;
+    wave_sync();
+    const float qr2 = 2.0f * max_qr;
+    while (sp > 0) {
+        --sp;
+        SmallStackEnt e = stack[sp];
+        wave_sync();
+        if (kEmit && (addr >= O.max_nodes || rbase > O.max_refs)) {
+            if (lane == 0) atomicOr(&O.counters[C_ERR], 4u);
+            break;
+        }
+        if (kEmit && e.patch != kNone && lane == 0) O.nodes[2 * e.patch + 1] = (e.axis << 30) | addr;
+        maxlvl = max(maxlvl, e.level);
+        const uint32_t cnt = (uint32_t)__popcll(e.mask);
+        float best_cost = kInf;
+        uint32_t best = kNone;
+        if (cnt > 0 && e.level < MNRT_MAX_LEVEL) {
+            float ext[3] = {e.hi[0] - e.lo[0], e.hi[1] - e.lo[1], e.hi[2] - e.lo[2]};
+            float denom;
+            if (kind == MNRT_KIND_POINT) {
+                float vol = 1.0f;
+                for (int i = 0; i < 3; ++i) vol *= ext[i] + qr2;
+                denom = 1.0f / vol;
+            } else {
+                float area = 2.0f * (ext[0] * ext[1] + ext[0] * ext[2] + ext[1] * ext[2]);
+                denom = 1.0f / area;
+            }
+            for (uint32_t c = lane; c < ncand; c += 64) {
+                uint32_t a = (c % per) / n;
+                float s = cpos[c];
+                if (!(e.lo[a] < s && s < e.hi[a])) continue;
+                uint32_t nl = (uint32_t)__popcll(mL[c] & e.mask);
+                uint32_t nr = (uint32_t)__popcll(mR[c] & e.mask);
+                if (nl == 0 || nr == 0) continue;
+                uint32_t o1 = (a + 1) % 3, o2 = (a + 2) % 3;
+                float cost;
+                if (kind == MNRT_KIND_POINT) {
+                    float vl = (ext[o1] + qr2) * (s - e.lo[a] + qr2) * (ext[o2] + qr2);
+                    float vr = (ext[o1] + qr2) * (e.hi[a] - s + qr2) * (ext[o2] + qr2);
+                    cost = ((float)nl * vl + (float)nr * vr) * denom + kCostTraverse;
+                } else {
+                    float al = 2.0f * (ext[o1] * ext[o2] + (s - e.lo[a]) * (ext[o1] + ext[o2]));
+                    float ar = 2.0f * (ext[o1] * ext[o2] + (e.hi[a] - s) * (ext[o1] + ext[o2]));
+                    cost = ((float)nl * al + (float)nr * ar) * denom + kCostTraverse;
+                }
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best = c;
+                }
+            }
+#pragma unroll
+            for (uint32_t o = 32; o > 0; o >>= 1) {
+                float oc = __shfl_xor(best_cost, o, 64);
+                uint32_t ob = (uint32_t)__shfl_xor((int)best, o, 64);
+                if (oc < best_cost || (oc == best_cost && ob < best)) {
+                    best_cost = oc;
+                    best = ob;
+                }
+            }
+        }
+        if (best == kNone || !(best_cost < (float)cnt)) {
+            if (kEmit && rbase + cnt > O.max_refs) {
+                if (lane == 0) atomicOr(&O.counters[C_ERR], 8u);
+                break;
+            }
+            if (kEmit) {
+                if (lane == 0) {
+                    O.nodes[2 * addr] = cnt;
+                    O.nodes[2 * addr + 1] = (MNRT_LEAF_TAG << 30) | rbase;
+                }
+                if (lane < n && ((e.mask >> lane) & 1ull)) {
+                    uint64_t below = (lane == 0) ? 0ull : (e.mask & ((1ull << lane) - 1ull));
+                    O.refs[rbase + (uint32_t)__popcll(below)] = eidx[lane];
+                }
+            }
+            ++addr;
+            rbase += cnt;
+            ++leaves;
+            continue;
+        }
+        const uint32_t a = (best % per) / n;
+        const float s = cpos[best];
+        if (kEmit && lane == 0) {
+            O.nodes[2 * addr] = __float_as_uint(s);
+            O.nodes[2 * addr + 1] = a << 30;
+        }
+        if (lane == 0) {
+            SmallStackEnt R = e, L = e;
+            R.mask = mR[best] & e.mask;
+            R.lo[a] = s;
+            R.level = e.level + 1;
+            R.patch = addr;
+            R.axis = a;
+            L.mask = mL[best] & e.mask;
+            L.hi[a] = s;
+            L.level = e.level + 1;
+            L.patch = kNone;
+            L.axis = 0;
+            stack[sp] = R;
+            stack[sp + 1] = L;
+        }
+        sp += 2;
+        ++addr;
+        wave_sync();
+    }
+    if (lane == 0) {
+        if (kEmit) {
+            atomicAdd(&O.counters[C_LEAVES], leaves);
+            atomicMax(&O.counters[C_MAXLEVEL], maxlvl);
+        } else {
+            O.nsize[r] = addr;
+            O.rsize[r] = rbase;
+        }
+    }
+}
+
+// ---- layout: subtree sizes bottom-up, addresses top-down, one workgroup ------------------
+
+struct LayoutView {
+    uint32_t* size;
+    uint32_t* rsize;
+    uint32_t* addr;
+    uint32_t* rbase;
+};
+
+__global__ __launch_bounds__(1024) void k_layout(uint32_t nbn, uint32_t maxlvl, BnView B, const uint32_t* __restrict__ s_nsize,
+                                                 const uint32_t* __restrict__ s_rsize, LayoutView Lv,
+                                                 uint32_t* __restrict__ counters) {
+    for (int lvl = (int)maxlvl; lvl >= 0; --lvl) {
+        for (uint32_t i = threadIdx.x; i < nbn; i += blockDim.x) {
+            if (B.level[i] != (uint32_t)lvl) continue;
+            uint32_t t = B.type[i];
+            if (t <= 2) {
+                uint32_t l = B.left[i], r = B.right[i];
+                Lv.size[i] = 1 + Lv.size[l] + Lv.size[r];
+                Lv.rsize[i] = Lv.rsize[l] + Lv.rsize[r];
+            } else if (t == BN_LEAF) {
+                Lv.size[i] = 1;
+                Lv.rsize[i] = B.cnt[i];
+            } else if (t == BN_SMALL) {
+                Lv.size[i] = s_nsize[B.aux[i]];
+                Lv.rsize[i] = s_rsize[B.aux[i]];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        Lv.addr[0] = 0;
+        Lv.rbase[0] = 0;
+        counters[C_TOTAL_NODES] = Lv.size[0];
+        counters[C_TOTAL_REFS] = Lv.rsize[0];
+    }
+    __syncthreads();
+    for (uint32_t lvl = 0; lvl <= maxlvl; ++lvl) {
+        for (uint32_t i = threadIdx.x; i < nbn; i += blockDim.x) {
+            if (B.level[i] != lvl || B.type[i] > 2) continue;
+            uint32_t l = B.left[i], r = B.right[i];
+            uint32_t a = Lv.addr[i], rb = Lv.rbase[i];
+            Lv.addr[l] = a + 1;
+            Lv.rbase[l] = rb;
+            Lv.addr[r] = a + 1 + Lv.size[l];
+            Lv.rbase[r] = rb + Lv.rsize[l];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_emit_large(uint32_t nbn, BnView B, LayoutView Lv, const uint32_t* __restrict__ ll_idx,
+                                                    uint32_t* __restrict__ nodes, uint32_t* __restrict__ refs,
+                                                    uint32_t* __restrict__ counters, uint32_t max_nodes,
+                                                    uint32_t max_refs) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbn) return;
+    uint32_t t = B.type[i];
+    if (t <= BN_LEAF && (Lv.addr[i] >= max_nodes || (t == BN_LEAF && Lv.rbase[i] + B.cnt[i] > max_refs) ||
+                         (t <= 2 && Lv.addr[B.right[i]] >= max_nodes))) {
+        atomicOr(&counters[C_ERR], 16u);
+        return;
+    }
+    if (t <= 2) {
+        uint32_t a = Lv.addr[i];
+        nodes[2 * a] = B.split[i];
+        nodes[2 * a + 1] = (t << 30) | Lv.addr[B.right[i]];
+        atomicMax(&counters[C_MAXLEVEL], B.level[i]);
+    } else if (t == BN_LEAF) {
+        uint32_t a = Lv.addr[i], rb = Lv.rbase[i], m = B.cnt[i];
+        nodes[2 * a] = m;
+        nodes[2 * a + 1] = (MNRT_LEAF_TAG << 30) | rb;
+        for (uint32_t k = 0; k < m; ++k) refs[rb + k] = ll_idx[B.aux[i] + k];
+        atomicAdd(&counters[C_LEAVES], 1u);
+        atomicMax(&counters[C_MAXLEVEL], B.level[i]);
+    }
+}
+
+__global__ void k_set_root_small(SmallView S, BnView B, const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                 uint32_t n) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    S.first[0] = 0;
+    S.cnt[0] = n;
+    S.level[0] = 0;
+    S.bn[0] = 0;
+    S.lo[0] = lo[0];
+    S.hi[0] = hi[0];
+    B.type[0] = BN_SMALL;
+    B.aux[0] = 0;
+    B.level[0] = 0;
+}
+
+__global__ void k_set_root_active(NodeView A, BnView B, uint32_t n) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    A.first[0] = 0;
+    A.cnt[0] = n;
+    A.level[0] = 0;
+    A.bn[0] = 0;
+    B.type[0] = BN_PENDING;
+    B.level[0] = 0;
+}
+
+}  // namespace mnrt
+
+// =========================================================================================
+// host driver
+// =========================================================================================
+namespace {
+
+using namespace mnrt;
+
+// Stream-ordered device allocations released together.
+class Pool {
+   public:
+    explicit Pool(hipStream_t st) : st_(st) {}
+    ~Pool() {
+        for (void* p : ptrs_) (void)hipFreeAsync(p, st_);
+    }
+    template <typename T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+        if (hipMallocAsync(&p, bytes, st_) != hipSuccess) {
+            ok = false;
+            return nullptr;
+        }
+        ptrs_.push_back(p);
+        return (T*)p;
+    }
+    void release(void* p) {
+        if (!p) return;
+        auto it = std::find(ptrs_.begin(), ptrs_.end(), p);
+        if (it != ptrs_.end()) {
+            (void)hipFreeAsync(p, st_);
+            ptrs_.erase(it);
+        }
+    }
+    bool ok = true;
+
+   private:
+    hipStream_t st_;
+    std::vector<void*> ptrs_;
+};
+
+struct Grow {
+    uint32_t cap = 0;
+};
+
+inline dim3 blocks_for(uint64_t n, uint32_t bs = 256) { return dim3((uint32_t)std::max<uint64_t>(1, (n + bs - 1) / bs)); }
+
+int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2, uint32_t n,
+               const mnrt_build_params* prm, mnrt_tree* out, hipStream_t st) {
+    if (!out || !prm || (n > 0 && !d_v0)) return MNRT_ERR_ARG;
+    if (kind == MNRT_KIND_TRI && n > 0 && (!d_v1 || !d_v2)) return MNRT_ERR_ARG;
+    if (prm->small_node_max < 1 || prm->small_node_max > 64) return MNRT_ERR_ARG;
+    if (n > (MNRT_INDEX_MASK >> 2)) return MNRT_ERR_CAPACITY;
+    Geometry G{kind, (const float4*)d_v0, (const float4*)d_v1 ? (const float4*)d_v1 : (const float4*)d_v0,
+               (const float4*)d_v2 ? (const float4*)d_v2 : (const float4*)d_v0};
+    Pool pool(st);
+    uint32_t* counters = pool.get<uint32_t>(C_COUNT);
+    uint32_t* h_counters = nullptr;
+    MNRT_HIP_CHECK(hipHostMalloc((void**)&h_counters, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    struct HostFree {
+        uint32_t* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } host_free{h_counters};
+    float* h_box = (float*)(h_counters + 32);
+    float* d_box = (float*)pool.get<float>(8);
+    if (!pool.ok) return MNRT_ERR_NOMEM;
+    MNRT_HIP_CHECK(hipMemsetAsync(counters, 0, C_COUNT * sizeof(uint32_t), st));
+
+    auto sync_counters = [&]() -> int {
+        MNRT_HIP_CHECK(hipMemcpyAsync(h_counters, counters, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        MNRT_HIP_CHECK(hipStreamSynchronize(st));
+        return MNRT_OK;
+    };
+
+    // ---- root elements and box ----
+    ElemView E{};
+    uint32_t ne = n;
+    E.idx = pool.get<uint32_t>(ne);
+    E.lo = pool.get<float4>(ne);
+    E.hi = kind == MNRT_KIND_TRI ? pool.get<float4>(ne) : E.lo;
+    E.node = pool.get<uint32_t>(ne);
+    NodeView A{};
+    uint32_t na = 1;
+    A.lo = pool.get<float4>(1);
+    A.hi = pool.get<float4>(1);
+    A.first = pool.get<uint32_t>(1);
+    A.cnt = pool.get<uint32_t>(1);
+    A.level = pool.get<uint32_t>(1);
+    A.bn = pool.get<uint32_t>(1);
+    if (!pool.ok) return MNRT_ERR_NOMEM;
+    if (n > 0) hipLaunchKernelGGL(k_init_elems, blocks_for(n), dim3(256), 0, st, G, n, E);
+
+    uint32_t* tlo = nullptr;
+    uint32_t* thi = nullptr;
+    float root[6];
+    float4 root_lo4, root_hi4;
+    uint32_t zero = 0, n_copy = n;
+    if (prm->has_root_box || n == 0) {
+        for (int k = 0; k < 3; ++k) {
+            root[k] = prm->has_root_box ? prm->root_min[k] + 0.0f : kInf;
+            root[3 + k] = prm->has_root_box ? prm->root_max[k] + 0.0f : -kInf;
+        }
+        root_lo4 = make_float4(root[0], root[1], root[2], 0.0f);
+        root_hi4 = make_float4(root[3], root[4], root[5], 0.0f);
+        MNRT_HIP_CHECK(hipMemcpyAsync(A.lo, &root_lo4, sizeof(float4), hipMemcpyHostToDevice, st));
+        MNRT_HIP_CHECK(hipMemcpyAsync(A.hi, &root_hi4, sizeof(float4), hipMemcpyHostToDevice, st));
+        MNRT_HIP_CHECK(hipStreamSynchronize(st));
+    } else {
+        // tight bounds of all elements: one node covering everything
+        uint32_t nch = (n + kChunk - 1) / kChunk;
+        uint32_t* cc = pool.get<uint32_t>(1);
+        uint32_t* chn = pool.get<uint32_t>(nch);
+        uint32_t* chf = pool.get<uint32_t>(nch);
+        uint32_t* chc = pool.get<uint32_t>(nch);
+        tlo = pool.get<uint32_t>(3);
+        thi = pool.get<uint32_t>(3);
+        uint32_t* sc = pool.get<uint32_t>(scan_scratch_words(1));
+        if (!pool.ok) return MNRT_ERR_NOMEM;
+        MNRT_HIP_CHECK(hipMemcpyAsync(A.first, &zero, 4, hipMemcpyHostToDevice, st));
+        MNRT_HIP_CHECK(hipMemcpyAsync(A.cnt, &n_copy, 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_chunk_counts, dim3(1), dim3(256), 0, st, 1u, A.cnt, cc);
+        MNRT_HIP_CHECK(scan_exclusive(cc, cc, 1, counters + C_NCHUNKS, sc, st));
+        hipLaunchKernelGGL(k_chunk_fill, blocks_for(nch), dim3(256), 0, st, 1u, nch, cc, counters, A.first, A.cnt,
+                           chn, chf, chc);
+        hipLaunchKernelGGL(k_tight_init, dim3(1), dim3(256), 0, st, 1u, tlo, thi);
+        hipLaunchKernelGGL(k_tight, dim3(nch), dim3(kChunk), 0, st, nch, counters, chn, chf, chc, E, kind, tlo, thi);
+        hipLaunchKernelGGL(k_root_box_from_tight, dim3(1), dim3(1), 0, st, tlo, thi, A.lo, A.hi, d_box);
+        MNRT_HIP_CHECK(hipMemcpyAsync(h_box, d_box, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
+        MNRT_HIP_CHECK(hipStreamSynchronize(st));
+        for (int k = 0; k < 6; ++k) root[k] = h_box[k];
+        pool.release(cc);
+        pool.release(chn);
+        pool.release(chf);
+        pool.release(chc);
+        pool.release(sc);
+    }
+
+    // ---- growable tables ----
+    uint32_t bn_cap = std::max<uint32_t>(1024, 8 * (n / std::max<uint32_t>(1, prm->small_node_max)) + 64);
+    uint32_t* bn_buf = pool.get<uint32_t>((size_t)7 * bn_cap);
+    auto bn_view = [&](uint32_t* buf, uint32_t cap) {
+        return BnView{buf, buf + cap, buf + 2ull * cap, buf + 3ull * cap, buf + 4ull * cap, buf + 5ull * cap,
+                      buf + 6ull * cap};
+    };
+    BnView B = bn_view(bn_buf, bn_cap);
+    uint32_t nbn = 1;
+
+    uint32_t s_cap = std::max<uint32_t>(256, 4 * (n / std::max<uint32_t>(1, prm->small_node_max)) + 16);
+    auto alloc_small = [&](uint32_t cap) {
+        SmallView S{};
+        S.first = pool.get<uint32_t>(cap);
+        S.cnt = pool.get<uint32_t>(cap);
+        S.level = pool.get<uint32_t>(cap);
+        S.bn = pool.get<uint32_t>(cap);
+        S.lo = pool.get<float4>(cap);
+        S.hi = pool.get<float4>(cap);
+        return S;
+    };
+    SmallView S = alloc_small(s_cap);
+    uint32_t nsmall = 0;
+
+    uint32_t se_cap = std::max<uint32_t>(1024, n + n / 2);
+    auto alloc_elems = [&](uint32_t cap, bool with_node) {
+        ElemView V{};
+        V.idx = pool.get<uint32_t>(cap);
+        V.lo = pool.get<float4>(cap);
+        V.hi = kind == MNRT_KIND_TRI ? pool.get<float4>(cap) : V.lo;
+        V.node = with_node ? pool.get<uint32_t>(cap) : nullptr;
+        return V;
+    };
+    auto release_elems = [&](ElemView& V) {
+        pool.release(V.idx);
+        pool.release(V.lo);
+        if (kind == MNRT_KIND_TRI) pool.release(V.hi);
+        pool.release(V.node);
+        V = ElemView{};
+    };
+    ElemView Sm = alloc_elems(se_cap, false);
+    uint32_t nse = 0;
+
+    uint32_t ll_cap = 0;
+    uint32_t* ll_idx = nullptr;
+    uint32_t ll_count = 0;
+    if (!pool.ok) return MNRT_ERR_NOMEM;
+
+    uint32_t maxlvl = 0;
+
+    if (n <= prm->small_node_max) {
+        // the root itself is a small root
+        if (n > 0) {
+            MNRT_HIP_CHECK(hipMemcpyAsync(Sm.idx, E.idx, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(Sm.lo, E.lo, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
+            if (kind == MNRT_KIND_TRI)
+                MNRT_HIP_CHECK(hipMemcpyAsync(Sm.hi, E.hi, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
+        }
+        hipLaunchKernelGGL(k_set_root_small, dim3(1), dim3(1), 0, st, S, B, A.lo, A.hi, n);
+        nsmall = 1;
+        nse = n;
+        na = 0;
+    } else {
+        hipLaunchKernelGGL(k_set_root_active, dim3(1), dim3(1), 0, st, A, B, n);
+    }
+
+    // ---- large-node stage ----
+    while (na > 0) {
+        // capacity for this level's new build nodes / small roots / leaf copies
+        if (nbn + 14ull * na > bn_cap) {
+            uint32_t ncap = std::max<uint32_t>(2 * bn_cap, nbn + 14 * na + 1024);
+            uint32_t* nb = pool.get<uint32_t>((size_t)7 * ncap);
+            if (!pool.ok) return MNRT_ERR_NOMEM;
+            for (int col = 0; col < 7; ++col)
+                MNRT_HIP_CHECK(hipMemcpyAsync(nb + (size_t)col * ncap, bn_buf + (size_t)col * bn_cap, (size_t)nbn * 4,
+                                              hipMemcpyDeviceToDevice, st));
+            pool.release(bn_buf);
+            bn_buf = nb;
+            bn_cap = ncap;
+            B = bn_view(bn_buf, bn_cap);
+        }
+        if (nsmall + 2ull * na > s_cap) {
+            uint32_t ncap = std::max<uint32_t>(2 * s_cap, nsmall + 2 * na + 256);
+            SmallView NS = alloc_small(ncap);
+            if (!pool.ok) return MNRT_ERR_NOMEM;
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.first, S.first, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.cnt, S.cnt, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.level, S.level, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.bn, S.bn, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.lo, S.lo, (size_t)nsmall * 16, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.hi, S.hi, (size_t)nsmall * 16, hipMemcpyDeviceToDevice, st));
+            pool.release(S.first);
+            pool.release(S.cnt);
+            pool.release(S.level);
+            pool.release(S.bn);
+            pool.release(S.lo);
+            pool.release(S.hi);
+            S = NS;
+            s_cap = ncap;
+        }
+        if (ll_count + (uint64_t)ne > ll_cap) {
+            uint32_t ncap = std::max<uint32_t>(2 * ll_cap, ll_count + ne + 1024);
+            uint32_t* nl = pool.get<uint32_t>(ncap);
+            if (!pool.ok) return MNRT_ERR_NOMEM;
+            if (ll_count) MNRT_HIP_CHECK(hipMemcpyAsync(nl, ll_idx, (size_t)ll_count * 4, hipMemcpyDeviceToDevice, st));
+            pool.release(ll_idx);
+            ll_idx = nl;
+            ll_cap = ncap;
+        }
+
+        // per-level temporaries
+        uint32_t max_chunks = ne / kChunk + na;
+        Pool lv(st);
+        uint32_t* cc = lv.get<uint32_t>(na);
+        uint32_t* chn = lv.get<uint32_t>(max_chunks);
+        uint32_t* chf = lv.get<uint32_t>(max_chunks);
+        uint32_t* chc = lv.get<uint32_t>(max_chunks);
+        tlo = lv.get<uint32_t>(3ull * na);
+        thi = lv.get<uint32_t>(3ull * na);
+        PlanView P{};
+        P.newcnt = lv.get<uint32_t>(na);
+        P.axis = lv.get<uint32_t>(na);
+        P.split = lv.get<float>(na);
+        P.lo = lv.get<float4>(na);
+        P.hi = lv.get<float4>(na);
+        P.level = lv.get<uint32_t>(na);
+        P.childL = lv.get<uint32_t>(na);
+        P.cur = lv.get<uint32_t>(na);
+        P.leaf = lv.get<uint32_t>(na);
+        P.noprog = lv.get<uint32_t>(na);
+        uint32_t* fl = lv.get<uint32_t>(2ull * ne + 1);
+        uint32_t* pos = lv.get<uint32_t>(2ull * ne + 1);
+        ChildView Cv{};
+        uint32_t* cbuf = lv.get<uint32_t>(11ull * 2 * na);
+        uint32_t* scr = lv.get<uint32_t>(scan_scratch_words(std::max<uint64_t>(2ull * ne + 1, 2ull * na)));
+        if (!lv.ok) return MNRT_ERR_NOMEM;
+        {
+            uint32_t m = 2 * na;
+            Cv.type = cbuf;
+            Cv.cnt = cbuf + m;
+            Cv.fpos = cbuf + 2 * m;
+            Cv.sflag = cbuf + 3 * m;
+            Cv.lflag = cbuf + 4 * m;
+            Cv.scnt = cbuf + 5 * m;
+            Cv.lcnt = cbuf + 6 * m;
+            Cv.sidx = cbuf + 7 * m;
+            Cv.lidx = cbuf + 8 * m;
+            Cv.soff = cbuf + 9 * m;
+            Cv.loff = cbuf + 10 * m;
+        }
+
+        // chunks + tight bounds
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(na), dim3(256), 0, st, na, A.cnt, cc);
+        MNRT_HIP_CHECK(scan_exclusive(cc, cc, na, counters + C_NCHUNKS, scr, st));
+        hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, na, max_chunks, cc, counters,
+                           A.first, A.cnt, chn, chf, chc);
+        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * na), dim3(256), 0, st, na, tlo, thi);
+        hipLaunchKernelGGL(k_tight, dim3(max_chunks), dim3(kChunk), 0, st, max_chunks, counters, chn, chf, chc, E,
+                           kind, tlo, thi);
+        // cuts + median split
+        hipLaunchKernelGGL((k_plan<false>), blocks_for(na), dim3(256), 0, st, na, A, tlo, thi,
+                           prm->empty_space_ratio, P, (const uint32_t*)nullptr, 0u, B, counters);
+        MNRT_HIP_CHECK(scan_exclusive(P.newcnt, P.newcnt, na, counters + C_NEWNODES, scr, st));
+        hipLaunchKernelGGL((k_plan<true>), blocks_for(na), dim3(256), 0, st, na, A, tlo, thi,
+                           prm->empty_space_ratio, P, (const uint32_t*)P.newcnt, nbn, B, counters);
+        // classification + compaction positions
+        hipLaunchKernelGGL(k_classify, blocks_for(ne), dim3(256), 0, st, ne, E, kind, P, fl);
+        MNRT_HIP_CHECK(scan_exclusive(fl, pos, 2 * ne, pos + 2 * ne, scr, st));
+        // children
+        hipLaunchKernelGGL(k_child_plan, blocks_for(2ull * na), dim3(256), 0, st, na, ne, A, P, pos,
+                           prm->small_node_max, Cv);
+        MNRT_HIP_CHECK(scan_exclusive(Cv.sflag, Cv.sidx, 2 * na, counters + C_NSMALL_NEW, scr, st));
+        MNRT_HIP_CHECK(scan_exclusive(Cv.lflag, Cv.lidx, 2 * na, counters + C_NNEXT, scr, st));
+        MNRT_HIP_CHECK(scan_exclusive(Cv.scnt, Cv.soff, 2 * na, counters + C_ESMALL_NEW, scr, st));
+        MNRT_HIP_CHECK(scan_exclusive(Cv.lcnt, Cv.loff, 2 * na, counters + C_ENEXT, scr, st));
+        int rc = sync_counters();
+        if (rc != MNRT_OK) return rc;
+        const uint32_t n_next = h_counters[C_NNEXT], e_next = h_counters[C_ENEXT];
+        const uint32_t n_small_new = h_counters[C_NSMALL_NEW], e_small_new = h_counters[C_ESMALL_NEW];
+        if ((uint64_t)nse + e_small_new > se_cap) {
+            uint32_t ncap = (uint32_t)std::max<uint64_t>(2ull * se_cap, (uint64_t)nse + e_small_new + 1024);
+            ElemView NS = alloc_elems(ncap, false);
+            if (!pool.ok) return MNRT_ERR_NOMEM;
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.idx, Sm.idx, (size_t)nse * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.lo, Sm.lo, (size_t)nse * 16, hipMemcpyDeviceToDevice, st));
+            if (kind == MNRT_KIND_TRI)
+                MNRT_HIP_CHECK(hipMemcpyAsync(NS.hi, Sm.hi, (size_t)nse * 16, hipMemcpyDeviceToDevice, st));
+            release_elems(Sm);
+            Sm = NS;
+            se_cap = ncap;
+        }
+        ElemView Nx = alloc_elems(std::max<uint32_t>(e_next, 1), true);
+        NodeView An{};
+        An.lo = pool.get<float4>(std::max<uint32_t>(n_next, 1));
+        An.hi = pool.get<float4>(std::max<uint32_t>(n_next, 1));
+        An.first = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
+        An.cnt = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
+        An.level = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
+        An.bn = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
+        if (!pool.ok) return MNRT_ERR_NOMEM;
+        hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, na, P, Cv, nsmall, nse, S, An, B,
+                           counters);
+        hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, na, A, P, E, ll_idx, B, counters);
+        hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, na, ne, E, G, P, Cv, pos, nse, Sm, se_cap, Nx,
+                           e_next, counters);
+        MNRT_HIP_CHECK(hipGetLastError());
+        nbn += h_counters[C_NEWNODES];
+        nsmall += n_small_new;
+        nse += e_small_new;
+        // next level
+        release_elems(E);
+        pool.release(A.lo);
+        pool.release(A.hi);
+        pool.release(A.first);
+        pool.release(A.cnt);
+        pool.release(A.level);
+        pool.release(A.bn);
+        E = Nx;
+        A = An;
+        ne = e_next;
+        na = n_next;
+        // the level-cap / no-progress leaves of this level were counted by k_leaf_copy
+        rc = sync_counters();
+        if (rc != MNRT_OK) return rc;
+        ll_count = h_counters[C_LLCOUNT];
+        maxlvl = h_counters[C_MAXLEVEL];
+    }
+    release_elems(E);
+
+    // ---- small-node stage, pass 1: subtree sizes ----
+    uint32_t* s_nsize = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
+    uint32_t* s_rsize = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
+    if (!pool.ok) return MNRT_ERR_NOMEM;
+    SmallOut O{};
+    O.nsize = s_nsize;
+    O.rsize = s_rsize;
+    O.counters = counters;
+    if (nsmall > 0)
+        hipLaunchKernelGGL((k_small<false>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
+                           st, nsmall, S, Sm, kind, prm->max_query_radius, O);
+    // ---- layout ----
+    LayoutView Lv{};
+    Lv.size = pool.get<uint32_t>(nbn);
+    Lv.rsize = pool.get<uint32_t>(nbn);
+    Lv.addr = pool.get<uint32_t>(nbn);
+    Lv.rbase = pool.get<uint32_t>(nbn);
+    if (!pool.ok) return MNRT_ERR_NOMEM;
+    hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, st, nbn, maxlvl, B, s_nsize, s_rsize, Lv, counters);
+    int rc = sync_counters();
+    if (rc != MNRT_OK) return rc;
+    const uint32_t total_nodes = h_counters[C_TOTAL_NODES], total_refs = h_counters[C_TOTAL_REFS];
+    if (total_nodes == 0 || total_nodes > MNRT_INDEX_MASK || total_refs > MNRT_INDEX_MASK) return MNRT_ERR_CAPACITY;
+
+    // ---- emission into the caller-visible tree ----
+    uint32_t* d_nodes = nullptr;
+    uint32_t* d_refs = nullptr;
+    MNRT_HIP_CHECK(hipMalloc((void**)&d_nodes, (size_t)total_nodes * 8));
+    if (hipMalloc((void**)&d_refs, std::max<size_t>((size_t)total_refs * 4, 16)) != hipSuccess) {
+        (void)hipFree(d_nodes);
+        return MNRT_ERR_NOMEM;
+    }
+    MNRT_HIP_CHECK(hipMemsetAsync(counters + C_LEAVES, 0, 4, st));
+    MNRT_HIP_CHECK(hipMemsetAsync(counters + C_MAXLEVEL, 0, 4, st));
+    hipLaunchKernelGGL(k_emit_large, blocks_for(nbn), dim3(256), 0, st, nbn, B, Lv, ll_idx, d_nodes, d_refs, counters,
+                       total_nodes, total_refs);
+    O.bn_addr = Lv.addr;
+    O.bn_rbase = Lv.rbase;
+    O.nodes = d_nodes;
+    O.refs = d_refs;
+    O.max_nodes = total_nodes;
+    O.max_refs = total_refs;
+    if (nsmall > 0)
+        hipLaunchKernelGGL((k_small<true>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
+                           st, nsmall, S, Sm, kind, prm->max_query_radius, O);
+    rc = sync_counters();
+    if (rc != MNRT_OK || hipGetLastError() != hipSuccess || h_counters[C_ERR] != 0) {
+        (void)hipFree(d_nodes);
+        (void)hipFree(d_refs);
+        return rc != MNRT_OK ? rc : (h_counters[C_ERR] != 0 ? MNRT_ERR_STATE : MNRT_ERR_HIP);
+    }
+    out->kind = (uint32_t)kind;
+    out->num_nodes = total_nodes;
+    out->num_refs = total_refs;
+    out->num_elems = n;
+    out->num_leaves = h_counters[C_LEAVES];
+    out->max_level = h_counters[C_MAXLEVEL];
+    for (int k = 0; k < 3; ++k) {
+        out->root_min[k] = root[k];
+        out->root_max[k] = root[3 + k];
+    }
+    out->d_nodes = d_nodes;
+    out->d_refs = d_refs;
+    out->d_recs = nullptr;
+    out->owns_nodes = 1;
+    out->owns_recs = 0;
+    return MNRT_OK;
+}
+
+}  // namespace
+
+extern "C" int mnrt_build_tri(const float* d_v0, const float* d_v1, const float* d_v2, uint32_t n,
+                              const mnrt_build_params* params, mnrt_tree* out, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    int rc = build_impl(MNRT_KIND_TRI, d_v0, d_v1, d_v2, n, params, out, st);
+    if (rc != MNRT_OK) return rc;
+    rc = mnrt_tree_attach_tri_records(out, d_v0, d_v1, d_v2, stream);
+    if (rc != MNRT_OK) mnrt_tree_release(out);
+    return rc;
+}
+
+extern "C" int mnrt_build_points(const float* d_pos4, uint32_t n, const mnrt_build_params* params, mnrt_tree* out,
+                                 void* stream) {
+    return build_impl(MNRT_KIND_POINT, d_pos4, nullptr, nullptr, n, params, out, (hipStream_t)stream);
+}

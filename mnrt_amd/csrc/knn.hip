@@ -1,0 +1,363 @@
+// knn.hip — exact k-nearest-photon radiance estimate on gfx950.
+//
+// Replaces the reference's query-radius estimate + range search + gather:
+//   KDTreePoint::ComputeQueryRadii (KDTreePoint.cpp:229-248, kdtree_point.cu:229-546),
+//   dev_RangeSearch (photon_knn.cu:113-235) and kernel_PhotonGather
+//   (photon_knn.cu:487-552, wrapper KernelPMPhotonGather :833-844).
+//
+// Design (one wave64 per query):
+//   * the point tree is walked by the whole wave with wave-uniform control flow (node
+//     words are scalar loads); far children go onto a per-wave LDS stack with their
+//     squared plane distance, as dev_RangeSearch pushes them (photon_knn.cu:146-171);
+//   * a leaf's photon records are contiguous, so its photons are tested one per lane
+//     with one coalesced 16-byte load each;
+//   * candidates (key = d2 bits << 32 | photon index, so integer order is (d2, index)
+//     order) are appended to a per-wave LDS buffer with ballot + mbcnt compaction; when
+//     k are known or the buffer would overflow, a wave-wide bitonic sort keeps the k
+//     best and fixes the pruning threshold;
+//   * the estimate sums the front-facing powers of the final set in (d2, index) order.
+#include "mnrt_common.h"
+
+namespace mnrt {
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kKnnStack = 64;
+
+struct KnnArgs {
+    const uint2* nodes;
+    const float4* recs;  // 2 float4 per reference
+    uint32_t k;
+    float r2max;
+    float r_max;
+    uint32_t n;
+    const float4* pos;
+    const float4* nrm;
+    const float4* dir;
+    const float4* infl;
+    const float4* clr;
+    float4* rad_out;      // accumulate target (slot = query), or per-query temp when idx_sp
+    uint32_t accumulate;  // 1: rad_out[q] += rad, 0: rad_out[q] = rad
+    float* out_r2;
+    float4* out_irr;
+    uint32_t* out_nbr;
+    uint32_t* out_count;
+    float* out_radius;
+};
+
+// Sorts keys[0..size) ascending (positions follow their keys), size a power of two.
+template <int CAP>
+__device__ __forceinline__ void wave_bitonic(uint64_t* keys, uint32_t* pos, uint32_t size, uint32_t lane) {
+    for (uint32_t k2 = 2; k2 <= size; k2 <<= 1) {
+        for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = lane; t < (size >> 1); t += 64) {
+                uint32_t i = 2 * j * (t / j) + (t % j);
+                uint32_t l = i + j;
+                uint64_t a = keys[i], b = keys[l];
+                bool asc = (i & k2) == 0;
+                if ((a > b) == asc) {
+                    keys[i] = b;
+                    keys[l] = a;
+                    uint32_t pa = pos[i];
+                    pos[i] = pos[l];
+                    pos[l] = pa;
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+template <int CAP, bool kRadiance>
+__global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
+    __shared__ uint64_t s_key[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_snode[kWavesPerBlock][kKnnStack];
+    __shared__ float s_spd2[kWavesPerBlock][kKnnStack];
+
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t q = blockIdx.x * kWavesPerBlock + wave;
+    if (q >= A.n) return;  // uniform per wave
+    uint64_t* keys = s_key[wave];
+    uint32_t* kpos = s_pos[wave];
+    uint32_t* snode = s_snode[wave];
+    float* spd2 = s_spd2[wave];
+
+    const float4 qp4 = A.pos[q];
+    const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
+    const f3 qp = mk3(qx, qy, qz);
+    const uint32_t k = A.k;
+    const float r2max = A.r2max;
+
+    uint32_t cnt = 0;        // candidates in the buffer
+    bool full = false;       // the buffer's first k are the k best so far
+    uint64_t thr_key = ~0ull;
+    float thr_d2 = kInf;
+    int sp = 0;
+    uint32_t node = 0;
+
+    auto select = [&](uint32_t keep) {
+        uint32_t size = pow2_at_least(cnt);
+        for (uint32_t t = cnt + lane; t < size; t += 64) {
+            keys[t] = ~0ull;
+            kpos[t] = 0u;
+        }
+        wave_sync();
+        wave_bitonic<CAP>(keys, kpos, size, lane);
+        if (cnt > keep) cnt = keep;
+        if (cnt == k) {
+            full = true;
+            thr_key = keys[k - 1];
+            thr_d2 = __uint_as_float((uint32_t)(thr_key >> 32));
+        }
+    };
+
+    for (;;) {
+        uint2 nd = A.nodes[node];
+        nd.x = uniform(nd.x);
+        nd.y = uniform(nd.y);
+        while (!is_leaf(nd.y)) {
+            uint32_t a = nd.y >> 30;
+            float s = __uint_as_float(nd.x);
+            float qa = comp(qp, a);
+            float diff = qa - s;
+            float pd2 = diff * diff;
+            uint32_t left = node + 1, right = nd.y & MNRT_INDEX_MASK;
+            uint32_t nearc = left, farc = right;
+            if (qa > s) {
+                nearc = right;
+                farc = left;
+            }
+            if (pd2 < r2max && sp < kKnnStack) {
+                if (lane == 0) {
+                    snode[sp] = farc;
+                    spd2[sp] = pd2;
+                }
+                ++sp;
+            }
+            node = nearc;
+            nd = A.nodes[node];
+            nd.x = uniform(nd.x);
+            nd.y = uniform(nd.y);
+        }
+        wave_sync();
+        const uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
+        for (uint32_t base = 0; base < count; base += 64) {
+            uint32_t j = base + lane;
+            bool take = false;
+            uint64_t key = 0;
+            if (j < count) {
+                float4 p = A.recs[2ull * (begin + j)];
+                f3 df = sub3(xyz(p), qp);
+                float d2 = dot3(df, df);
+                key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
+                take = (d2 < r2max) && (key < thr_key);
+            }
+            uint64_t m = __ballot(take);
+            uint32_t nm = (uint32_t)__popcll(m);
+            if (nm == 0) continue;
+            if (cnt + nm > (uint32_t)CAP) {
+                select(k);
+                take = take && (key < thr_key);
+                m = __ballot(take);
+                nm = (uint32_t)__popcll(m);
+            }
+            if (take) {
+                uint32_t slot = cnt + mask_prefix(m);
+                keys[slot] = key;
+                kpos[slot] = begin + j;
+            }
+            cnt += nm;
+            wave_sync();
+        }
+        if (!full && cnt >= k) select(k);
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            float pd2 = spd2[sp];
+            if (full && pd2 > thr_d2) continue;
+            node = snode[sp];
+            found = true;
+            break;
+        }
+        wave_sync();
+        if (!found) break;
+    }
+    select(k);
+    const uint32_t found = cnt;  // min(#photons closer than r_max, k)
+
+    float r2 = r2max;
+    if (found >= k && k > 0) r2 = __uint_as_float((uint32_t)(keys[k - 1] >> 32));
+
+    if (A.out_count && lane == 0) A.out_count[q] = found;
+    if (A.out_nbr) {
+        for (uint32_t i = lane; i < k; i += 64)
+            A.out_nbr[(size_t)q * k + i] = (i < found) ? (uint32_t)(keys[i] & 0xFFFFFFFFull) : 0xFFFFFFFFu;
+    }
+    if (!kRadiance) {
+        if (A.out_radius && lane == 0) A.out_radius[q] = (found >= k && k > 0) ? sqrtf(r2) : A.r_max;
+        if (A.out_r2 && lane == 0) A.out_r2[q] = r2;
+        return;
+    }
+
+    // Normal flipped toward the source ray (photon_knn.cu:500-505).
+    f3 nq = xyz(A.nrm[q]);
+    f3 rd = xyz(A.dir[q]);
+    f3 nrd = mk3(-rd.x, -rd.y, -rd.z);
+    if (dot3(nrd, nq) < 0.0f) nq = mk3(nq.x * -1.0f, nq.y * -1.0f, nq.z * -1.0f);
+
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t i = 0; i < found; ++i) {
+        uint32_t p = kpos[i];
+        float4 pw = A.recs[2ull * p + 1];
+        f3 pd = oct_decode(__float_as_uint(pw.w));
+        if (dot3(nq, pd) < 0.0f) {
+            sx = sx + pw.x;
+            sy = sy + pw.y;
+            sz = sz + pw.z;
+        }
+    }
+    float ix = 0.0f, iy = 0.0f, iz = 0.0f;
+    if (r2 > 0.0f) {
+        float den = r2 * kPi;
+        ix = sx / den;
+        iy = sy / den;
+        iz = sz / den;
+    }
+    float4 c4 = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    if (A.clr) {
+        c4 = A.clr[q];
+        if (found < 8) ix = iy = iz = 0.0f;
+    }
+    float4 in4 = A.infl[q];
+    float rx = (((c4.w * c4.x) * kInvPi) * ix) * in4.x;
+    float ry = (((c4.w * c4.y) * kInvPi) * iy) * in4.y;
+    float rz = (((c4.w * c4.z) * kInvPi) * iz) * in4.z;
+    if (lane == 0) {
+        if (A.out_r2) A.out_r2[q] = r2;
+        if (A.out_irr) A.out_irr[q] = make_float4(ix, iy, iz, 0.0f);
+        if (A.accumulate) {
+            float4 o = A.rad_out[q];
+            A.rad_out[q] = make_float4(o.x + rx, o.y + ry, o.z + rz, o.w + 0.0f);
+        } else {
+            A.rad_out[q] = make_float4(rx, ry, rz, 0.0f);
+        }
+    }
+}
+
+// Adds per-query radiance into slots given by a slot index per query (final gathering,
+// PhotonMap::GatherFor, PhotonMap.cpp:234-270).  A run of equal consecutive slots is
+// summed in query order by the run's first query, so sorted slot indices (as final
+// gathering produces them) give a deterministic result.
+__global__ __launch_bounds__(256) void k_scatter_runs(const float4* __restrict__ per_query,
+                                                      const uint32_t* __restrict__ idx_sp, uint32_t n,
+                                                      float4* __restrict__ io_rad) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t slot = idx_sp[i];
+    if (i > 0 && idx_sp[i - 1] == slot) return;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    for (uint32_t j = i; j < n && idx_sp[j] == slot; ++j) {
+        float4 r = per_query[j];
+        x = x + r.x;
+        y = y + r.y;
+        z = z + r.z;
+    }
+    atomicAdd(&io_rad[slot].x, x);
+    atomicAdd(&io_rad[slot].y, y);
+    atomicAdd(&io_rad[slot].z, z);
+}
+
+template <bool kRadiance>
+int launch_knn(const KnnArgs& A, hipStream_t st) {
+    dim3 block(64 * kWavesPerBlock), grid((A.n + kWavesPerBlock - 1) / kWavesPerBlock);
+    uint32_t need = A.k + 64;
+    if (need <= 128)
+        hipLaunchKernelGGL((k_knn<128, kRadiance>), grid, block, 0, st, A);
+    else if (need <= 256)
+        hipLaunchKernelGGL((k_knn<256, kRadiance>), grid, block, 0, st, A);
+    else
+        hipLaunchKernelGGL((k_knn<512, kRadiance>), grid, block, 0, st, A);
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
+
+}  // namespace mnrt
+
+static int knn_check(const mnrt_tree* pmap, uint32_t k) {
+    if (!pmap) return MNRT_ERR_ARG;
+    if (k == 0 || k > MNRT_KNN_MAX_K) return MNRT_ERR_ARG;
+    if (pmap->kind != MNRT_KIND_POINT || !pmap->d_nodes || !pmap->d_recs || pmap->num_nodes == 0)
+        return MNRT_ERR_STATE;
+    if (pmap->max_level > MNRT_MAX_LEVEL) return MNRT_ERR_CAPACITY;
+    return MNRT_OK;
+}
+
+extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                               const float* d_pos4, const float* d_nrm4, const float* d_dir4,
+                               const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,
+                               float* d_io_rad4, float* d_out_r2, float* d_out_irr4,
+                               uint32_t* d_out_nbr, uint32_t* d_out_count, void* stream) {
+    int rc = knn_check(pmap, k);
+    if (rc != MNRT_OK) return rc;
+    if (n == 0) return MNRT_OK;
+    if (!d_pos4 || !d_nrm4 || !d_dir4 || !d_infl4 || !d_io_rad4 || !(r_max > 0.0f)) return MNRT_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    mnrt::KnnArgs A{};
+    A.nodes = (const uint2*)pmap->d_nodes;
+    A.recs = (const float4*)pmap->d_recs;
+    A.k = k;
+    A.r2max = r_max * r_max;
+    A.r_max = r_max;
+    A.n = n;
+    A.pos = (const float4*)d_pos4;
+    A.nrm = (const float4*)d_nrm4;
+    A.dir = (const float4*)d_dir4;
+    A.infl = (const float4*)d_infl4;
+    A.clr = (const float4*)d_clr4;
+    A.out_r2 = d_out_r2;
+    A.out_irr = (float4*)d_out_irr4;
+    A.out_nbr = d_out_nbr;
+    A.out_count = d_out_count;
+    float4* tmp = nullptr;
+    if (d_idx_sp) {
+        MNRT_HIP_CHECK(hipMallocAsync((void**)&tmp, (size_t)n * sizeof(float4), st));
+        A.rad_out = tmp;
+        A.accumulate = 0;
+    } else {
+        A.rad_out = (float4*)d_io_rad4;
+        A.accumulate = 1;
+    }
+    rc = mnrt::launch_knn<true>(A, st);
+    if (rc == MNRT_OK && d_idx_sp) {
+        hipLaunchKernelGGL(mnrt::k_scatter_runs, dim3((n + 255) / 256), dim3(256), 0, st, tmp, d_idx_sp,
+                           n, (float4*)d_io_rad4);
+        if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
+    }
+    if (tmp && hipFreeAsync(tmp, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
+    return rc;
+}
+
+extern "C" int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                              const float* d_pos4, float* d_out_radius, void* stream) {
+    int rc = knn_check(pmap, k);
+    if (rc != MNRT_OK) return rc;
+    if (n == 0) return MNRT_OK;
+    if (!d_pos4 || !d_out_radius || !(r_max > 0.0f)) return MNRT_ERR_ARG;
+    mnrt::KnnArgs A{};
+    A.nodes = (const uint2*)pmap->d_nodes;
+    A.recs = (const float4*)pmap->d_recs;
+    A.k = k;
+    A.r2max = r_max * r_max;
+    A.r_max = r_max;
+    A.n = n;
+    A.pos = (const float4*)d_pos4;
+    A.out_radius = d_out_radius;
+    return mnrt::launch_knn<false>(A, (hipStream_t)stream);
+}

@@ -1,0 +1,132 @@
+// mnrt_common.h — device helpers shared by the gfx950 kernels of the kd-tree hot path.
+//
+// All fp32 arithmetic here is written in a fixed operation order and compiled with
+// -ffp-contract=off and IEEE division / square root, so a result is bit-identical to any
+// other implementation that performs the same operations in the same order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mnrt_kd.h"
+
+namespace mnrt {
+
+constexpr float kInf = 3.402823466e+38f;              // FLT_MAX, MN_INFINITY (MNMath.h:60)
+constexpr float kPi = 3.14159265358979323846f;        // MN_PI (MNMath.h:53)
+constexpr float kInvPi = 0.31830988618379067154f;     // MN_INV_PI (MNMath.h:55)
+constexpr float kCostTraverse = 3.0f;                 // KD_COST_TRAVERSE (kdtree.cu:51)
+// Relative widening of traversal interval decisions (2^-16).
+constexpr float kTravTolLo = 1.0f - 1.0f / 65536.0f;
+constexpr float kTravTolHi = 1.0f + 1.0f / 65536.0f;
+
+struct f3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float comp(f3 v, uint32_t a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+
+__device__ __forceinline__ bool is_leaf(uint32_t w1) { return (w1 >> 30) == MNRT_LEAF_TAG; }
+
+// Moller-Trumbore with precomputed edges (dev_RayTriIntersect, intersect_dev.h:77-99):
+// det == 0 misses, barycentric edges inclusive, lambda may be negative.
+__device__ __forceinline__ bool ray_tri(f3 a, f3 e1, f3 e2, f3 o, f3 d, float& lam, float& u,
+                                        float& v) {
+    f3 pvec = cross3(d, e2);
+    float det = dot3(e1, pvec);
+    if (det == 0.0f) return false;
+    float inv_det = 1.0f / det;
+    f3 tvec = sub3(o, a);
+    u = dot3(tvec, pvec) * inv_det;
+    f3 qvec = cross3(tvec, e1);
+    v = dot3(d, qvec) * inv_det;
+    lam = dot3(e2, qvec) * inv_det;
+    return u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f;
+}
+
+// Slab test (dev_RayBoxIntersect, intersect_dev.h:124-164), no early out.
+__device__ __forceinline__ bool ray_box(const float* bmin, const float* bmax, f3 o, f3 inv,
+                                        float t0, float t1, float& tn, float& tf) {
+    bool hit = true;
+    float oo[3] = {o.x, o.y, o.z}, ii[3] = {inv.x, inv.y, inv.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float v1 = (bmin[i] - oo[i]) * ii[i];
+        float v2 = (bmax[i] - oo[i]) * ii[i];
+        float tnear = v1, tfar = v2;
+        if (v1 > v2) tnear = v2;
+        if (v1 > v2) tfar = v1;
+        t0 = (tnear > t0) ? tnear : t0;
+        t1 = (tfar < t1) ? tfar : t1;
+        if (t0 > t1) hit = false;
+    }
+    tn = t0;
+    tf = t1;
+    return hit;
+}
+
+__device__ __forceinline__ float sgn1(float x) { return x >= 0.0f ? 1.0f : -1.0f; }
+
+// Octahedral direction code: two 16-bit signed coordinates.  Decoding does not
+// normalise; callers only use the sign of a dot product.
+__device__ __forceinline__ uint32_t oct_encode(float x, float y, float z) {
+    float s = fabsf(x) + fabsf(y) + fabsf(z);
+    float u = x / s, v = y / s;
+    if (z < 0.0f) {
+        float ou = u, ov = v;
+        u = (1.0f - fabsf(ov)) * sgn1(ou);
+        v = (1.0f - fabsf(ou)) * sgn1(ov);
+    }
+    float qu = rintf(u * 32767.0f), qv = rintf(v * 32767.0f);
+    qu = fminf(32767.0f, fmaxf(-32767.0f, qu));
+    qv = fminf(32767.0f, fmaxf(-32767.0f, qv));
+    int32_t iu = (int32_t)qu, iv = (int32_t)qv;
+    return ((uint32_t)(uint16_t)(int16_t)iu) | (((uint32_t)(uint16_t)(int16_t)iv) << 16);
+}
+
+__device__ __forceinline__ f3 oct_decode(uint32_t c) {
+    int16_t iu = (int16_t)(uint16_t)(c & 0xFFFFu), iv = (int16_t)(uint16_t)(c >> 16);
+    float u = (float)iu / 32767.0f, v = (float)iv / 32767.0f;
+    float z = (1.0f - fabsf(u)) - fabsf(v);
+    if (z < 0.0f) {
+        float ou = u;
+        u = (1.0f - fabsf(v)) * sgn1(ou);
+        v = (1.0f - fabsf(ou)) * sgn1(v);
+    }
+    return f3{u, v, z};
+}
+
+// Wave-local ordering of LDS accesses (all lanes of one wave, no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniformf(float v) {
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+
+// Lanes below this one whose predicate is set (exclusive prefix of a 64-bit ballot).
+__device__ __forceinline__ uint32_t mask_prefix(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+}  // namespace mnrt
+
+// Status plumbing shared by the C-ABI translation units.
+#define MNRT_HIP_CHECK(expr)                       \
+    do {                                           \
+        hipError_t _e = (expr);                    \
+        if (_e != hipSuccess) return MNRT_ERR_HIP; \
+    } while (0)

@@ -1,0 +1,100 @@
+// scan.h — device-wide exclusive prefix sum of uint32 arrays (wave64 + LDS), used by
+// the kd-tree build for stream compaction.  Replaces the CUDPP scans the reference calls
+// through MNCudaPrimitives::Scan (MNCudaPrimitives.cpp:160).
+#pragma once
+
+#include "mnrt_common.h"
+
+namespace mnrt {
+
+constexpr uint32_t kScanBlock = 256;
+constexpr uint32_t kScanPerThread = 8;
+constexpr uint32_t kScanTile = kScanBlock * kScanPerThread;  // 2048 elements per block
+
+// Inclusive scan of one value per lane across a wave64.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one 2048-element tile per block; writes the tile total to sums.
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t* in, uint32_t n,
+                                                           uint32_t* out,
+                                                           uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_wave[kScanBlock / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)tid * kScanPerThread;
+    uint32_t v[kScanPerThread];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPerThread; ++j) {
+        size_t i = base + j;
+        v[j] = (i < n) ? in[i] : 0u;
+        local += v[j];
+    }
+    uint32_t incl = wave_incl_scan(local, lane);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    for (uint32_t w = 0; w < wave; ++w) wave_off += s_wave[w];
+    uint32_t run = wave_off + incl - local;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPerThread; ++j) {
+        size_t i = base + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+    if (tid == kScanBlock - 1) sums[blockIdx.x] = wave_off + incl;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(uint32_t* __restrict__ out, uint32_t n,
+                                                         const uint32_t* __restrict__ offs) {
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    const uint32_t add = offs[blockIdx.x];
+    for (uint32_t j = threadIdx.x; j < kScanTile; j += kScanBlock) {
+        size_t i = base + j;
+        if (i < n) out[i] += add;
+    }
+}
+
+__global__ void k_scan_total(const uint32_t* __restrict__ in, const uint32_t* __restrict__ out, uint32_t n,
+                             uint32_t* __restrict__ total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = (n == 0) ? 0u : out[n - 1] + in[n - 1];
+}
+
+// Scratch words needed by scan_exclusive for n elements.
+inline size_t scan_scratch_words(uint32_t n) {
+    size_t words = 0;
+    uint64_t m = n;
+    while (m > 1) {
+        m = (m + kScanTile - 1) / kScanTile;
+        words += 2 * m + 2;
+    }
+    return words + 4;
+}
+
+// out[i] = sum(in[0..i)); *total (device word, may be null) = sum(in[0..n)).
+// in and out may alias.  scratch: scan_scratch_words(n) device words.
+inline hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
+                                 uint32_t* scratch, hipStream_t st) {
+    if (n == 0) return total ? hipMemsetAsync(total, 0, 4, st) : hipSuccess;
+    uint32_t blocks = (n + kScanTile - 1) / kScanTile;
+    uint32_t* sums = scratch;
+    uint32_t* offs = scratch + blocks + 1;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(blocks), dim3(kScanBlock), 0, st, in, n, out, sums);
+    if (blocks == 1) {
+        if (total) return hipMemcpyAsync(total, sums, 4, hipMemcpyDeviceToDevice, st);
+        return hipGetLastError();
+    }
+    hipError_t e = scan_exclusive(sums, offs, blocks, nullptr, scratch + 2 * blocks + 2, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(kScanBlock), 0, st, out, n, offs);
+    if (total) hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, sums, offs, blocks, total);
+    return hipGetLastError();
+}
+
+}  // namespace mnrt

@@ -1,0 +1,126 @@
+"""ctypes binding of libmnrt_kd.so (the C ABI declared in include/mnrt_kd.h).
+
+Device memory and streams come from PyTorch (torch.cuda tensors and streams); the
+library sees only raw device pointers.  There is no fallback: if the library is missing
+or the device is not a gfx950, every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmnrt_kd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mnrt_kd.h")
+
+KIND_TRI = 0
+KIND_POINT = 1
+MAX_LEVEL = 48
+KNN_MAX_K = 256
+
+u32 = ctypes.c_uint32
+f32 = ctypes.c_float
+vp = ctypes.c_void_p
+
+
+class BuildParams(ctypes.Structure):
+    """mnrt_build_params (KDTreeGPU::Set* parameters, KDTreeGPU.h:248-280)."""
+
+    _fields_ = [
+        ("empty_space_ratio", f32),
+        ("small_node_max", u32),
+        ("max_query_radius", f32),
+        ("has_root_box", u32),
+        ("root_min", f32 * 3),
+        ("root_max", f32 * 3),
+    ]
+
+
+class Tree(ctypes.Structure):
+    """mnrt_tree: device-resident packed preorder kd-tree."""
+
+    _fields_ = [
+        ("kind", u32),
+        ("num_nodes", u32),
+        ("num_refs", u32),
+        ("num_elems", u32),
+        ("num_leaves", u32),
+        ("max_level", u32),
+        ("root_min", f32 * 3),
+        ("root_max", f32 * 3),
+        ("d_nodes", vp),
+        ("d_refs", vp),
+        ("d_recs", vp),
+        ("owns_nodes", u32),
+        ("owns_recs", u32),
+    ]
+
+
+class MnrtError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads libmnrt_kd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MnrtError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (make -C mnrt_amd/csrc)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.mnrt_status_string.restype = ctypes.c_char_p
+    L.mnrt_status_string.argtypes = [ctypes.c_int]
+    L.mnrt_abi_version.restype = ctypes.c_int
+    L.mnrt_device_init.argtypes = [ctypes.c_int]
+    TP = ctypes.POINTER(Tree)
+    L.mnrt_build_tri.argtypes = [vp, vp, vp, u32, ctypes.POINTER(BuildParams), TP, vp]
+    L.mnrt_build_points.argtypes = [vp, u32, ctypes.POINTER(BuildParams), TP, vp]
+    L.mnrt_tree_attach_tri_records.argtypes = [TP, vp, vp, vp, vp]
+    L.mnrt_tree_attach_photon_records.argtypes = [TP, vp, vp, vp, vp]
+    L.mnrt_tree_release.argtypes = [TP]
+    L.mnrt_tree_cost.argtypes = [TP, f32, ctypes.POINTER(ctypes.c_double), vp]
+    L.mnrt_trace_closest.argtypes = [TP, u32, vp, vp, f32, vp, vp, vp, vp, vp, vp]
+    L.mnrt_knn_gather.argtypes = [TP, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mnrt_knn_radii.argtypes = [TP, u32, f32, u32, vp, vp, vp]
+    for name in ("mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
+                 "mnrt_tree_attach_photon_records", "mnrt_tree_release", "mnrt_tree_cost",
+                 "mnrt_trace_closest", "mnrt_knn_gather", "mnrt_knn_radii", "mnrt_device_init"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = load().mnrt_status_string(rc).decode()
+        raise MnrtError(f"{what}: {msg} ({rc})")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise MnrtError("expected a device tensor")
+    if not t.is_contiguous():
+        raise MnrtError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def stream_handle(stream=None) -> int | None:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def exported_symbols_in_header() -> list[str]:
+    """Names of the functions include/mnrt_kd.h declares."""
+    import re
+
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(mnrt_\w+)\s*\(", text, re.M)))

@@ -1,0 +1,1265 @@
+// oracle.cpp — CPU restatement of the MNRT kd-tree hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load this library, and only as the checker / CPU baseline.  The
+// product (mnrt_amd/) never links, loads or calls it.
+//
+// It restates, with its own code, the algorithms of the reference (jonike/mnrt):
+//   build      KDTreeGPU.cpp:175-1209, kdtree.cu:467-1631, kdtree_tri.cu:190-489,
+//              KDTreeTri.cpp:46-91, KDTreePoint.cpp:120-134
+//   traversal  raytracing.cu:503-645, intersect_dev.h:77-164
+//   kNN        photon_knn.cu:113-235 (range search, replaced by exact kNN),
+//              photon_knn.cu:487-552 (gather / radiance)
+// with the deterministic rules that replace the reference's random and racy choices:
+//   * elements lying in a split plane go left for an even element index, right for an
+//     odd one (the reference draws Mersenne-Twister uniforms, kdtree.cu:788-815,1226-1232);
+//   * the best small-node split is the lowest candidate index among equal costs
+//     (the reference races on s_idxMin, kdtree.cu:1458-1463);
+//   * closest hits tie-break equal lambda to the lower triangle index (the reference
+//     keeps the first found, raytracing.cu:627);
+//   * kNN sets are ordered by (d2, photon index).
+// Every deviation is listed in the repository's design notes.
+//
+// Floating point: compiled with -ffp-contract=off and without fast-math so that every
+// fp32 operation rounds exactly as the HIP kernels' operations (same order, IEEE
+// division and square root).  An fp64 path evaluates the radiance estimate for the
+// tolerance checks.
+//
+// Parity pins (tests/test_oracle_*.py): closed-form intersections, brute-force closest
+// hits over all triangles, brute-force sorted kNN, closed-form radiance cases, build
+// invariants (every triangle referenced by every leaf its geometry overlaps, in fp64),
+// SAH/VVH cost on hand-computed trees, oct encoding round trips.
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr uint32_t kLeafTag = 3u;
+constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
+constexpr uint32_t kMaxLevel = 48u;          // mirrors MNRT_MAX_LEVEL
+constexpr float kCostTraverse = 3.0f;        // KD_COST_TRAVERSE, kdtree.cu:51
+constexpr float kPi = 3.14159265358979323846f;     // MN_PI, MNMath.h:53
+constexpr float kInvPi = 0.31830988618379067154f;  // MN_INV_PI, MNMath.h:55
+constexpr float kInf = FLT_MAX;              // MN_INFINITY, MNMath.h:60
+// Relative tolerance of the traversal's interval decisions (see trace_one).
+constexpr float kTravTolLo = 1.0f - 1.0f / 65536.0f;
+constexpr float kTravTolHi = 1.0f + 1.0f / 65536.0f;
+
+inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+struct V3 { float v[3]; };
+inline V3 mk(float x, float y, float z) { V3 r; r.v[0] = x; r.v[1] = y; r.v[2] = z; return r; }
+inline V3 sub(const V3& a, const V3& b) { return mk(a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]); }
+inline float dot(const V3& a, const V3& b) { return a.v[0] * b.v[0] + a.v[1] * b.v[1] + a.v[2] * b.v[2]; }
+inline V3 cross(const V3& a, const V3& b) {
+    return mk(a.v[1] * b.v[2] - a.v[2] * b.v[1], a.v[2] * b.v[0] - a.v[0] * b.v[2],
+              a.v[0] * b.v[1] - a.v[1] * b.v[0]);
+}
+inline V3 ld4(const float* p, size_t i) { return mk(p[4 * i], p[4 * i + 1], p[4 * i + 2]); }
+
+void parallel_for(uint32_t n, int nthreads, const std::function<void(uint32_t, uint32_t)>& fn) {
+    if (nthreads <= 1 || n < 2) { fn(0, n); return; }
+    std::vector<std::thread> th;
+    uint32_t chunk = (n + nthreads - 1) / nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        uint32_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back(fn, b, e);
+    }
+    for (auto& t : th) t.join();
+}
+
+// ---------------------------------------------------------------------------------
+// Intersection tests (intersect_dev.h:77-99 and 124-164)
+// ---------------------------------------------------------------------------------
+
+// Moller-Trumbore as in dev_RayTriIntersect: det == 0 misses, inclusive barycentric
+// edges, lambda may be negative.  e1 = b - a and e2 = c - a.
+inline bool ray_tri(const V3& a, const V3& e1, const V3& e2, const V3& o, const V3& d,
+                    float& lam, float& u, float& v) {
+    V3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    if (det == 0.0f) return false;
+    float inv_det = 1.0f / det;
+    V3 tvec = sub(o, a);
+    u = dot(tvec, pvec) * inv_det;
+    V3 qvec = cross(tvec, e1);
+    v = dot(d, qvec) * inv_det;
+    lam = dot(e2, qvec) * inv_det;
+    return u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f;
+}
+
+// Slab test as in dev_RayBoxIntersect: no early out, interval narrowed per axis.
+inline bool ray_box(const float* bmin, const float* bmax, const V3& o, const V3& inv, float t0,
+                    float t1, float& tn, float& tf) {
+    bool hit = true;
+    for (int i = 0; i < 3; ++i) {
+        float v1 = (bmin[i] - o.v[i]) * inv.v[i];
+        float v2 = (bmax[i] - o.v[i]) * inv.v[i];
+        float tnear = v1, tfar = v2;
+        if (v1 > v2) tnear = v2;
+        if (v1 > v2) tfar = v1;
+        t0 = (tnear > t0) ? tnear : t0;
+        t1 = (tfar < t1) ? tfar : t1;
+        if (t0 > t1) hit = false;
+    }
+    tn = t0;
+    tf = t1;
+    return hit;
+}
+
+// ---------------------------------------------------------------------------------
+// Octahedral direction code (photon records).  The reference stores the photon
+// direction as spherical angles and rebuilds it with sinf/cosf (photon_dev.h:71-116,
+// mncudautil_dev.h:345-375); records here keep two 16-bit octahedral coordinates so the
+// kNN kernel needs no trigonometry.  Only the sign of dot(normal, direction) is used,
+// so decoding does not normalise.
+// ---------------------------------------------------------------------------------
+inline float sgn1(float x) { return x >= 0.0f ? 1.0f : -1.0f; }
+
+uint32_t oct_encode(float x, float y, float z) {
+    float s = std::fabs(x) + std::fabs(y) + std::fabs(z);
+    float u = x / s, v = y / s;
+    if (z < 0.0f) {
+        float ou = u, ov = v;
+        u = (1.0f - std::fabs(ov)) * sgn1(ou);
+        v = (1.0f - std::fabs(ou)) * sgn1(ov);
+    }
+    float qu = std::rint(u * 32767.0f), qv = std::rint(v * 32767.0f);
+    qu = std::min(32767.0f, std::max(-32767.0f, qu));
+    qv = std::min(32767.0f, std::max(-32767.0f, qv));
+    int32_t iu = (int32_t)qu, iv = (int32_t)qv;
+    return ((uint32_t)(uint16_t)(int16_t)iu) | (((uint32_t)(uint16_t)(int16_t)iv) << 16);
+}
+
+V3 oct_decode(uint32_t c) {
+    int16_t iu = (int16_t)(uint16_t)(c & 0xFFFFu), iv = (int16_t)(uint16_t)(c >> 16);
+    float u = (float)iu / 32767.0f, v = (float)iv / 32767.0f;
+    float z = (1.0f - std::fabs(u)) - std::fabs(v);
+    if (z < 0.0f) {
+        float ou = u;
+        u = (1.0f - std::fabs(v)) * sgn1(ou);
+        v = (1.0f - std::fabs(ou)) * sgn1(v);
+    }
+    return mk(u, v, z);
+}
+
+// ---------------------------------------------------------------------------------
+// kd-tree build
+// ---------------------------------------------------------------------------------
+
+struct Elem {
+    uint32_t idx;
+    float lo[3], hi[3];
+};
+
+struct BNode {
+    bool leaf = true;
+    int axis = 0;
+    float split = 0.0f;
+    int left = -1, right = -1;
+    std::vector<uint32_t> elems;  // leaf only: original element indices, ascending
+};
+
+struct BuildCtx {
+    int kind;  // 0 tri, 1 point
+    const float* v0;
+    const float* v1;
+    const float* v2;  // tri vertices (float4) or points (v0)
+    float ratio;
+    uint32_t small_max;
+    float max_qr;
+    std::vector<BNode> T;
+    uint32_t max_level = 0;
+};
+
+// Deterministic side of an element lying in the split plane (replaces the MT uniforms
+// of kdtree.cu:788-815 and 1226-1232).
+inline bool plane_goes_left(uint32_t idx) { return (idx & 1u) == 0u; }
+
+// -0 -> +0, so min/max results and split words never depend on the sign of zero.
+inline float canon(float x) { return x + 0.0f; }
+
+// Classification of a large-node element (kernel_MarkLeftRightElements, kdtree.cu:752-824).
+inline void classify(int kind, const Elem& e, int axis, float s, bool& L, bool& R) {
+    if (kind == 0) {
+        float mn = e.lo[axis], mx = e.hi[axis];
+        bool planar_on = (mn == s && mn == mx);
+        L = mn < s || (planar_on && plane_goes_left(e.idx));
+        R = s < mx || (planar_on && !plane_goes_left(e.idx));
+    } else {
+        float p = e.lo[axis];
+        if (plane_goes_left(e.idx)) { L = p <= s; R = s < p; }
+        else { L = p < s; R = s <= p; }
+    }
+}
+
+// Sutherland-Hodgman clipping (dev_ClipSegment / dev_ClipSegments, kdtree_tri.cu:190-257).
+struct PList { int n = 0; V3 p[12]; };
+
+void clip_segments(const PList& in, int axis, float val, float sense, PList& out) {
+    out.n = 0;
+    for (int i = 0; i < in.n; ++i) {
+        const V3& a = in.p[i];
+        const V3& b = in.p[(i + 1 == in.n) ? 0 : i + 1];
+        bool ins = (a.v[axis] - val) * sense >= 0.0f;
+        bool ine = (b.v[axis] - val) * sense >= 0.0f;
+        if (ins != ine) {
+            float t = (val - a.v[axis]) / (b.v[axis] - a.v[axis]);
+            V3 h = mk(a.v[0] + t * (b.v[0] - a.v[0]), a.v[1] + t * (b.v[1] - a.v[1]),
+                      a.v[2] + t * (b.v[2] - a.v[2]));
+            h.v[axis] = val;
+            out.p[out.n++] = h;
+        }
+        if (ine) out.p[out.n++] = b;
+    }
+}
+
+// Clipped bounds of a straddling triangle in a child box (dev_ClipTriToBounds and
+// kernel_PerformSplitClipping, kdtree_tri.cu:283-489), rounded outward: the clipped
+// polygon's bounds are padded by 2^-20 of the largest magnitude on each axis, then
+// clamped to the child box and to the element's previous bounds.  The reference clamps
+// only the split axis and can lose a triangle by rounding; this keeps every triangle in
+// every child its geometry reaches.
+void clip_bounds(const BuildCtx& C, Elem& e, const float* bmin, const float* bmax) {
+    PList a, b;
+    a.n = 3;
+    a.p[0] = ld4(C.v0, e.idx);
+    a.p[1] = ld4(C.v1, e.idx);
+    a.p[2] = ld4(C.v2, e.idx);
+    for (int ax = 0; ax < 3; ++ax) {
+        if (e.lo[ax] != e.hi[ax]) {
+            clip_segments(a, ax, bmin[ax], 1.0f, b);
+            clip_segments(b, ax, bmax[ax], -1.0f, a);
+        }
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        float lo, hi;
+        if (a.n == 0) {
+            lo = e.lo[ax];
+            hi = e.hi[ax];
+        } else {
+            lo = kInf;
+            hi = -kInf;
+            for (int i = 0; i < a.n; ++i) {
+                lo = std::fmin(a.p[i].v[ax], lo);
+                hi = std::fmax(a.p[i].v[ax], hi);
+            }
+            float m = std::fmax(std::fmax(std::fabs(lo), std::fabs(hi)),
+                                std::fmax(std::fabs(bmin[ax]), std::fabs(bmax[ax])));
+            float pad = m * (1.0f / 1048576.0f);
+            lo = lo - pad;
+            hi = hi + pad;
+        }
+        lo = std::fmax(lo, bmin[ax]);
+        hi = std::fmin(hi, bmax[ax]);
+        lo = std::fmax(lo, e.lo[ax]);
+        hi = std::fmin(hi, e.hi[ax]);
+        e.lo[ax] = canon(lo);
+        e.hi[ax] = canon(hi);
+    }
+}
+
+int new_node(BuildCtx& C) {
+    C.T.emplace_back();
+    return (int)C.T.size() - 1;
+}
+
+void make_leaf(BuildCtx& C, int id, const std::vector<Elem>& el) {
+    BNode& n = C.T[id];
+    n.leaf = true;
+    n.elems.clear();
+    for (const Elem& e : el) n.elems.push_back(e.idx);
+}
+
+// Small-node stage for one small root (PreProcessSmallNodes/ProcessSmallNodes,
+// KDTreeGPU.cpp:945-1154; kernel_CreateSplitCandidates, kernel_InitSplitMasks,
+// kernel_FindBestSplits, kernel_SplitSmallNodes, kernel_GenerateENAFromMasks,
+// kdtree.cu:1035-1631).  Candidates are the element points, axis-major, minimums before
+// maximums; 64-bit left/right masks are fixed per small root; each node takes the split
+// with the lowest SAH (triangles) or VVH (points) cost, lowest candidate index on ties,
+// among candidates strictly inside its box, and splits iff that cost < its element count.
+struct SmallRoot {
+    std::vector<Elem> el;
+    float bmin[3], bmax[3];
+    uint32_t level;
+    int id;
+};
+
+struct Cand {
+    float pos;
+    int axis;
+    uint64_t mL, mR;
+};
+
+void small_recurse(BuildCtx& C, const SmallRoot& R, const std::vector<Cand>& cand, int id,
+                   uint64_t mask, const float* bmin, const float* bmax, uint32_t level) {
+    C.max_level = std::max(C.max_level, level);
+    uint32_t cnt = (uint32_t)__builtin_popcountll(mask);
+    int best = -1;
+    float best_cost = kInf;
+    if (cnt > 0 && level < kMaxLevel) {
+        float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
+        float denom;
+        float qr2 = 2.0f * C.max_qr;
+        if (C.kind == 1) {
+            float vol = 1.0f;
+            for (int i = 0; i < 3; ++i) vol *= ext[i] + qr2;
+            denom = 1.0f / vol;
+        } else {
+            float area = 2.0f * (ext[0] * ext[1] + ext[0] * ext[2] + ext[1] * ext[2]);
+            denom = 1.0f / area;
+        }
+        for (size_t c = 0; c < cand.size(); ++c) {
+            const Cand& k = cand[c];
+            int a = k.axis;
+            float s = k.pos;
+            if (!(bmin[a] < s && s < bmax[a])) continue;
+            uint32_t nl = (uint32_t)__builtin_popcountll(k.mL & mask);
+            uint32_t nr = (uint32_t)__builtin_popcountll(k.mR & mask);
+            if (nl == 0 || nr == 0) continue;
+            int o1 = (a + 1) % 3, o2 = (a + 2) % 3;
+            float cost;
+            if (C.kind == 1) {
+                float vl = (ext[o1] + qr2) * (s - bmin[a] + qr2) * (ext[o2] + qr2);
+                float vr = (ext[o1] + qr2) * (bmax[a] - s + qr2) * (ext[o2] + qr2);
+                cost = ((float)nl * vl + (float)nr * vr) * denom + kCostTraverse;
+            } else {
+                float al = 2.0f * (ext[o1] * ext[o2] + (s - bmin[a]) * (ext[o1] + ext[o2]));
+                float ar = 2.0f * (ext[o1] * ext[o2] + (bmax[a] - s) * (ext[o1] + ext[o2]));
+                cost = ((float)nl * al + (float)nr * ar) * denom + kCostTraverse;
+            }
+            if (cost < best_cost) {  // strict: lowest index wins ties; NaN never wins
+                best_cost = cost;
+                best = (int)c;
+            }
+        }
+    }
+    if (best < 0 || !(best_cost < (float)cnt)) {
+        BNode& n = C.T[id];
+        n.leaf = true;
+        n.elems.clear();
+        for (size_t i = 0; i < R.el.size(); ++i)
+            if (mask >> i & 1ull) n.elems.push_back(R.el[i].idx);
+        return;
+    }
+    const Cand& k = cand[best];
+    int l = new_node(C), r = new_node(C);
+    BNode& n = C.T[id];
+    n.leaf = false;
+    n.axis = k.axis;
+    n.split = k.pos;
+    n.left = l;
+    n.right = r;
+    float lmax[3] = {bmax[0], bmax[1], bmax[2]};
+    lmax[k.axis] = k.pos;
+    float rmin[3] = {bmin[0], bmin[1], bmin[2]};
+    rmin[k.axis] = k.pos;
+    uint64_t ml = k.mL & mask, mr = k.mR & mask;
+    small_recurse(C, R, cand, l, ml, bmin, lmax, level + 1);
+    small_recurse(C, R, cand, r, mr, rmin, bmax, level + 1);
+}
+
+void small_stage(BuildCtx& C, const SmallRoot& R) {
+    uint32_t n = (uint32_t)R.el.size();
+    std::vector<Cand> cand;
+    int passes = (C.kind == 0) ? 2 : 1;
+    for (int isMax = 0; isMax < passes; ++isMax) {
+        for (int axis = 0; axis < 3; ++axis) {
+            for (uint32_t i = 0; i < n; ++i) {
+                Cand c;
+                c.axis = axis;
+                c.pos = isMax ? R.el[i].hi[axis] : R.el[i].lo[axis];
+                c.mL = c.mR = 0;
+                for (uint32_t j = 0; j < n; ++j) {
+                    const Elem& e = R.el[j];
+                    bool L, Rr;
+                    if (C.kind == 0) {
+                        float mn = e.lo[axis], mx = e.hi[axis];
+                        L = mn < c.pos;
+                        Rr = c.pos < mx;
+                        if (j == i) {
+                            if (isMax) L = true;
+                            else Rr = true;
+                        } else if (mn == mx && mn == c.pos) {
+                            if (plane_goes_left(e.idx)) L = true;
+                            else Rr = true;
+                        }
+                    } else {
+                        float p = e.lo[axis];
+                        if (plane_goes_left(e.idx)) { L = p <= c.pos; Rr = c.pos < p; }
+                        else { L = p < c.pos; Rr = c.pos <= p; }
+                    }
+                    if (L) c.mL |= 1ull << j;
+                    if (Rr) c.mR |= 1ull << j;
+                }
+                cand.push_back(c);
+            }
+        }
+    }
+    uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1ull);
+    small_recurse(C, R, cand, R.id, all, R.bmin, R.bmax, R.level);
+}
+
+struct Active {
+    float inh_lo[3], inh_hi[3];
+    std::vector<Elem> el;
+    uint32_t level;
+    int id;
+};
+
+// Large-node stage (LargeNodeStage/ProcessLargeNodes, KDTreeGPU.cpp:247-782; empty space
+// cutting kdtree.cu:467-612 and 1986-2033; median split kdtree.cu:629-682).  A node whose
+// median split would put every element into both children becomes a leaf (the
+// reference loops forever there, F5 in the survey of the reference).
+void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>& smalls) {
+    while (!active.empty()) {
+        std::vector<Active> next;
+        for (Active& A : active) {
+            C.max_level = std::max(C.max_level, A.level);
+            float tlo[3] = {kInf, kInf, kInf}, thi[3] = {-kInf, -kInf, -kInf};
+            for (const Elem& e : A.el)
+                for (int a = 0; a < 3; ++a) {
+                    tlo[a] = std::fmin(tlo[a], e.lo[a]);
+                    thi[a] = std::fmax(thi[a], e.hi[a]);
+                }
+            int cur = A.id;
+            for (int isMax = 0; isMax < 2; ++isMax) {
+                for (int ax = 0; ax < 3; ++ax) {
+                    if (A.level >= kMaxLevel) continue;
+                    float total = A.inh_hi[ax] - A.inh_lo[ax];
+                    float empty = isMax ? (A.inh_hi[ax] - thi[ax]) : (tlo[ax] - A.inh_lo[ax]);
+                    if (!(empty > total * C.ratio)) continue;
+                    float s = isMax ? thi[ax] : tlo[ax];
+                    int eleaf = new_node(C), cont = new_node(C);
+                    BNode& n = C.T[cur];
+                    n.leaf = false;
+                    n.axis = ax;
+                    n.split = s;
+                    n.left = isMax ? cont : eleaf;
+                    n.right = isMax ? eleaf : cont;
+                    if (isMax) A.inh_hi[ax] = s;
+                    else A.inh_lo[ax] = s;
+                    A.level += 1;
+                    C.max_level = std::max(C.max_level, A.level);
+                    cur = cont;
+                }
+            }
+            if (A.level >= kMaxLevel) {
+                make_leaf(C, cur, A.el);
+                continue;
+            }
+            float ex = A.inh_hi[0] - A.inh_lo[0], ey = A.inh_hi[1] - A.inh_lo[1],
+                  ez = A.inh_hi[2] - A.inh_lo[2];
+            // Longest axis, ties to the lower axis.  The reference's test
+            // (kdtree.cu:641-647) falls back to x whenever y and z tie for longest even
+            // if x is shorter, which slices cube-like scenes into ever thinner x slabs.
+            int axis = 0;
+            float emax = ex;
+            if (ey > emax) { axis = 1; emax = ey; }
+            if (ez > emax) axis = 2;
+            float s = A.inh_lo[axis] + 0.5f * (A.inh_hi[axis] - A.inh_lo[axis]);
+            Active Lc, Rc;
+            for (int a = 0; a < 3; ++a) {
+                Lc.inh_lo[a] = Rc.inh_lo[a] = A.inh_lo[a];
+                Lc.inh_hi[a] = Rc.inh_hi[a] = A.inh_hi[a];
+            }
+            Lc.inh_hi[axis] = s;
+            Rc.inh_lo[axis] = s;
+            for (const Elem& e : A.el) {
+                bool L, R;
+                classify(C.kind, e, axis, s, L, R);
+                if (C.kind == 0 && (e.lo[axis] < s && s < e.hi[axis])) {
+                    Elem el = e, er = e;
+                    clip_bounds(C, el, Lc.inh_lo, Lc.inh_hi);
+                    clip_bounds(C, er, Rc.inh_lo, Rc.inh_hi);
+                    Lc.el.push_back(el);
+                    Rc.el.push_back(er);
+                } else {
+                    if (L) Lc.el.push_back(e);
+                    if (R) Rc.el.push_back(e);
+                }
+            }
+            if (Lc.el.size() == A.el.size() && Rc.el.size() == A.el.size()) {
+                make_leaf(C, cur, A.el);
+                continue;
+            }
+            int l = new_node(C), r = new_node(C);
+            BNode& n = C.T[cur];
+            n.leaf = false;
+            n.axis = axis;
+            n.split = s;
+            n.left = l;
+            n.right = r;
+            Lc.level = Rc.level = A.level + 1;
+            C.max_level = std::max(C.max_level, Lc.level);
+            Lc.id = l;
+            Rc.id = r;
+            for (Active* ch : {&Lc, &Rc}) {
+                if (ch->el.size() <= C.small_max) {
+                    SmallRoot sr;
+                    sr.el = std::move(ch->el);
+                    for (int a = 0; a < 3; ++a) {
+                        sr.bmin[a] = ch->inh_lo[a];
+                        sr.bmax[a] = ch->inh_hi[a];
+                    }
+                    sr.level = ch->level;
+                    sr.id = ch->id;
+                    smalls.push_back(std::move(sr));
+                } else {
+                    next.push_back(std::move(*ch));
+                }
+            }
+        }
+        active.swap(next);
+    }
+}
+
+struct OTree {
+    int kind;
+    std::vector<uint32_t> nodes;
+    std::vector<uint32_t> refs;
+    float root[6];
+    uint32_t max_level;
+    uint32_t num_leaves;
+    uint32_t n_elems;
+};
+
+uint32_t emit(const BuildCtx& C, int id, OTree& T) {
+    uint32_t at = (uint32_t)(T.nodes.size() / 2);
+    T.nodes.push_back(0);
+    T.nodes.push_back(0);
+    const BNode& n = C.T[id];
+    if (n.leaf) {
+        T.nodes[2 * at] = (uint32_t)n.elems.size();
+        T.nodes[2 * at + 1] = (kLeafTag << 30) | (uint32_t)T.refs.size();
+        for (uint32_t e : n.elems) T.refs.push_back(e);
+        T.num_leaves++;
+    } else {
+        emit(C, n.left, T);
+        uint32_t r = emit(C, n.right, T);
+        T.nodes[2 * at] = fbits(n.split);
+        T.nodes[2 * at + 1] = ((uint32_t)n.axis << 30) | r;
+    }
+    return at;
+}
+
+OTree* build(int kind, const float* v0, const float* v1, const float* v2, uint32_t n, float ratio,
+             uint32_t small_max, float max_qr, int has_root, const float* root6) {
+    BuildCtx C;
+    C.kind = kind;
+    C.v0 = v0;
+    C.v1 = v1;
+    C.v2 = v2;
+    C.ratio = ratio;
+    C.small_max = small_max;
+    C.max_qr = max_qr;
+    Active root;
+    root.level = 0;
+    root.el.resize(n);
+    float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+    for (uint32_t i = 0; i < n; ++i) {
+        Elem& e = root.el[i];
+        e.idx = i;
+        V3 a = ld4(v0, i);
+        if (kind == 0) {
+            V3 b = ld4(v1, i), c = ld4(v2, i);
+            for (int k = 0; k < 3; ++k) {
+                e.lo[k] = canon(std::fmin(a.v[k], std::fmin(b.v[k], c.v[k])));
+                e.hi[k] = canon(std::fmax(a.v[k], std::fmax(b.v[k], c.v[k])));
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) e.lo[k] = e.hi[k] = canon(a.v[k]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::fmin(lo[k], e.lo[k]);
+            hi[k] = std::fmax(hi[k], e.hi[k]);
+        }
+    }
+    if (has_root) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = canon(root6[k]);
+            hi[k] = canon(root6[3 + k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        root.inh_lo[k] = lo[k];
+        root.inh_hi[k] = hi[k];
+    }
+    root.id = new_node(C);
+    std::vector<SmallRoot> smalls;
+    if (n <= small_max) {
+        SmallRoot sr;
+        sr.el = root.el;
+        for (int k = 0; k < 3; ++k) {
+            sr.bmin[k] = lo[k];
+            sr.bmax[k] = hi[k];
+        }
+        sr.level = 0;
+        sr.id = root.id;
+        smalls.push_back(std::move(sr));
+    } else {
+        large_stage(C, {root}, smalls);
+    }
+    for (const SmallRoot& sr : smalls) small_stage(C, sr);
+    OTree* T = new OTree();
+    T->kind = kind;
+    T->num_leaves = 0;
+    T->n_elems = n;
+    for (int k = 0; k < 3; ++k) {
+        T->root[k] = lo[k];
+        T->root[3 + k] = hi[k];
+    }
+    emit(C, root.id, *T);
+    T->max_level = C.max_level;
+    return T;
+}
+
+// ---------------------------------------------------------------------------------
+// Traversal (dev_FindNextIntersectionKDWhileWhile<true>, raytracing.cu:503-645).
+// Interval decisions are widened by a relative 2^-16 so that rounding of the split
+// parameter never hides a hit that the brute force over all triangles finds:
+//   NaN split parameter (origin in the plane, direction parallel): both children;
+//   tsplit < 0 or tsplit > tmax*(1+tol): near child only;
+//   tsplit < tmin*(1-tol): far child only;
+//   otherwise both, near on [tmin, min(tmax, tsplit*(1+tol))], far on
+//   [max(tmin, tsplit*(1-tol)), tmax];
+// the near side is where the origin lies, an origin in the plane taking the side the
+// direction leaves (the reference uses |o - split| < 1e-8 and the direction,
+// raytracing.cu:584-586).  Stacked nodes starting after the best hit are skipped.
+// ---------------------------------------------------------------------------------
+
+struct TraceCounts {
+    uint64_t inner = 0, leaves = 0, refs = 0, tests = 0;
+};
+
+struct TriSoup {
+    const float* v0;
+    const float* v1;
+    const float* v2;
+};
+
+inline void tri_test(const TriSoup& S, uint32_t t, const V3& o, const V3& d, float eps,
+                     float& best_t, int32_t& best_tri, float& bu, float& bv) {
+    V3 a = ld4(S.v0, t), b = ld4(S.v1, t), c = ld4(S.v2, t);
+    V3 e1 = sub(b, a), e2 = sub(c, a);
+    float lam, u, v;
+    if (ray_tri(a, e1, e2, o, d, lam, u, v) && lam > eps) {
+        if (lam < best_t || (lam == best_t && best_tri >= 0 && (int32_t)t < best_tri)) {
+            best_t = lam;
+            best_tri = (int32_t)t;
+            bu = u;
+            bv = v;
+        }
+    }
+}
+
+void trace_one(const uint32_t* nodes, const uint32_t* refs, const float* root, const TriSoup& S,
+               const V3& o, const V3& d, float eps, int32_t& out_tri, float& out_t, float& ou,
+               float& ov, TraceCounts& cnt) {
+    V3 inv = mk(1.0f / d.v[0], 1.0f / d.v[1], 1.0f / d.v[2]);
+    float t0, t1;
+    out_tri = -1;
+    out_t = 0.0f;
+    ou = ov = 0.0f;
+    if (!ray_box(root, root + 3, o, inv, eps, kInf, t0, t1)) return;
+    t1 = t1 + eps;
+    float best_t = t1, bu = 0.0f, bv = 0.0f;
+    int32_t best_tri = -1;
+    struct Ent { uint32_t node; float tmin, tmax; };
+    std::vector<Ent> stack;
+    stack.reserve(64);
+    uint32_t node = 0;
+    float tmin = t0, tmax = t1;
+    for (;;) {
+        for (;;) {
+            uint32_t w1 = nodes[2 * node + 1];
+            if ((w1 >> 30) == kLeafTag) break;
+            cnt.inner++;
+            int a = (int)(w1 >> 30);
+            float s = bitsf(nodes[2 * node]);
+            uint32_t left = node + 1, right = w1 & kIndexMask;
+            float oa = o.v[a], da = d.v[a];
+            float tsplit = (s - oa) * inv.v[a];
+            bool below = (oa < s) || (oa == s && da > 0.0f);
+            uint32_t first = below ? left : right, second = below ? right : left;
+            if (tsplit != tsplit) {
+                stack.push_back({second, tmin, tmax});
+                node = first;
+            } else if (tsplit < 0.0f || tsplit > tmax * kTravTolHi) {
+                node = first;
+            } else if (tsplit < tmin * kTravTolLo) {
+                node = second;
+            } else {
+                float fmin = std::fmax(tmin, tsplit * kTravTolLo);
+                stack.push_back({second, fmin, tmax});
+                tmax = std::fmin(tmax, tsplit * kTravTolHi);
+                node = first;
+            }
+        }
+        cnt.leaves++;
+        uint32_t count = nodes[2 * node], begin = nodes[2 * node + 1] & kIndexMask;
+        for (uint32_t i = 0; i < count; ++i) {
+            cnt.refs++;
+            cnt.tests++;
+            tri_test(S, refs[begin + i], o, d, eps, best_t, best_tri, bu, bv);
+        }
+        // Stacked nodes that start after the best hit are skipped.  (The reference's
+        // `outLambda < tMax` early exit, raytracing.cu:637-641, ends the whole search;
+        // a sibling pushed for a ray lying in the split plane covers the same interval
+        // as the subtree searched first, so it may still hold an equal or closer hit.)
+        bool found = false;
+        while (!stack.empty()) {
+            Ent e = stack.back();
+            stack.pop_back();
+            if (best_tri >= 0 && best_t < e.tmin) continue;
+            node = e.node;
+            tmin = e.tmin;
+            tmax = e.tmax;
+            found = true;
+            break;
+        }
+        if (!found) break;
+    }
+    out_tri = best_tri;
+    if (best_tri >= 0) {
+        out_t = best_t;
+        ou = bu;
+        ov = bv;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Exact kNN (replaces the Zhou radius estimate + range search, kdtree_point.cu:229-267,
+// photon_knn.cu:113-235).  Depth-first as dev_RangeSearch: near child where the query
+// lies (p > split goes right), far child pushed when its plane distance^2 < r_max^2;
+// a bounded max-heap of (d2, index) keeps the k nearest with d2 < r_max^2, and a popped
+// far child is skipped once the heap is full and its plane distance^2 exceeds the k-th
+// d2.
+// ---------------------------------------------------------------------------------
+
+struct KnnCounts {
+    uint64_t inner = 0, leaves = 0, refs = 0, tests = 0, accepted = 0;
+};
+
+inline uint64_t kkey(float d2, uint32_t idx) { return ((uint64_t)fbits(d2) << 32) | idx; }
+
+inline float dist2(const V3& p, const V3& q) {
+    V3 df = sub(p, q);
+    return dot(df, df);
+}
+
+uint32_t knn_one(const uint32_t* nodes, const uint32_t* refs, const float* ppos, const V3& q,
+                 uint32_t k, float r2max, uint64_t* heap, KnnCounts& cnt) {
+    uint32_t hn = 0;
+    struct Ent { uint32_t node; float pd2; };
+    Ent stack[128];
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        for (;;) {
+            uint32_t w1 = nodes[2 * node + 1];
+            if ((w1 >> 30) == kLeafTag) break;
+            cnt.inner++;
+            int a = (int)(w1 >> 30);
+            float s = bitsf(nodes[2 * node]);
+            float diff = q.v[a] - s;
+            float pd2 = diff * diff;
+            uint32_t left = node + 1, right = w1 & kIndexMask;
+            uint32_t nearc = left, farc = right;
+            if (q.v[a] > s) { nearc = right; farc = left; }
+            if (pd2 < r2max) stack[sp++] = {farc, pd2};
+            node = nearc;
+        }
+        cnt.leaves++;
+        uint32_t count = nodes[2 * node], begin = nodes[2 * node + 1] & kIndexMask;
+        for (uint32_t i = 0; i < count; ++i) {
+            cnt.refs++;
+            cnt.tests++;
+            uint32_t pi = refs[begin + i];
+            float d2 = dist2(ld4(ppos, pi), q);
+            if (!(d2 < r2max)) continue;
+            uint64_t key = kkey(d2, pi);
+            if (hn < k) {
+                heap[hn++] = key;
+                std::push_heap(heap, heap + hn);
+                cnt.accepted++;
+            } else if (key < heap[0]) {
+                std::pop_heap(heap, heap + hn);
+                heap[hn - 1] = key;
+                std::push_heap(heap, heap + hn);
+                cnt.accepted++;
+            }
+        }
+        bool found = false;
+        while (sp > 0) {
+            Ent e = stack[--sp];
+            if (hn == k && e.pd2 > bitsf((uint32_t)(heap[0] >> 32))) continue;
+            node = e.node;
+            found = true;
+            break;
+        }
+        if (!found) break;
+    }
+    std::sort_heap(heap, heap + hn);
+    return hn;
+}
+
+// Radiance estimate of kernel_PhotonGather (photon_knn.cu:487-552) over an exact,
+// sorted neighbour set.  fp32 path: the kernel's operation order (sequential sum in
+// (d2, index) order, component-wise division by r2*pi).  fp64 path: same set and same
+// front-facing decisions, continuous arithmetic in double.
+struct RadIn {
+    const float* ppos;
+    const float* ppow;
+    const uint32_t* pdir;  // oct codes
+    const float* qpos;
+    const float* qnrm;
+    const float* qdir;
+    const float* qinfl;
+    const float* qclr;
+};
+
+void radiance_one(const RadIn& I, uint32_t q, const uint32_t* nbr, uint32_t cnt, uint32_t k,
+                  float rmax, float* out32, double* out64, float* irr32, double* irr64,
+                  float* r2out) {
+    V3 n = ld4(I.qnrm, q), d = ld4(I.qdir, q), qp = ld4(I.qpos, q);
+    V3 nd = mk(-d.v[0], -d.v[1], -d.v[2]);
+    if (dot(nd, n) < 0.0f) n = mk(n.v[0] * -1.0f, n.v[1] * -1.0f, n.v[2] * -1.0f);
+    float r2max = rmax * rmax;
+    float s[3] = {0.0f, 0.0f, 0.0f};
+    double s64[3] = {0.0, 0.0, 0.0};
+    for (uint32_t i = 0; i < cnt && i < k; ++i) {
+        uint32_t p = nbr[i];
+        V3 pd = oct_decode(I.pdir[p]);
+        if (dot(n, pd) < 0.0f) {
+            for (int c = 0; c < 3; ++c) {
+                s[c] = s[c] + I.ppow[4 * p + c];
+                s64[c] += (double)I.ppow[4 * p + c];
+            }
+        }
+    }
+    float r2 = r2max;
+    double r2d = (double)rmax * (double)rmax;
+    if (cnt >= k && k > 0) {
+        uint32_t p = nbr[k - 1];
+        r2 = dist2(ld4(I.ppos, p), qp);
+        double dd = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            double df = (double)I.ppos[4 * p + c] - (double)qp.v[c];
+            dd += df * df;
+        }
+        r2d = dd;
+    }
+    if (r2out) r2out[q] = r2;
+    float irr[3] = {0.0f, 0.0f, 0.0f};
+    double irrd[3] = {0.0, 0.0, 0.0};
+    if (r2 > 0.0f) {
+        float den = r2 * kPi;
+        for (int c = 0; c < 3; ++c) irr[c] = s[c] / den;
+    }
+    if (r2d > 0.0) {
+        for (int c = 0; c < 3; ++c) irrd[c] = s64[c] / (r2d * M_PI);
+    }
+    if (I.qclr && cnt < 8) {
+        for (int c = 0; c < 3; ++c) {
+            irr[c] = 0.0f;
+            irrd[c] = 0.0;
+        }
+    }
+    float c4[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    if (I.qclr)
+        for (int c = 0; c < 4; ++c) c4[c] = I.qclr[4 * q + c];
+    for (int c = 0; c < 3; ++c) {
+        float infl = I.qinfl[4 * q + c];
+        float rad = (((c4[3] * c4[c]) * kInvPi) * irr[c]) * infl;
+        if (out32) out32[4 * q + c] = rad;
+        if (out64)
+            out64[4 * q + c] = (double)c4[3] * (double)c4[c] / M_PI * irrd[c] * (double)infl;
+        if (irr32) irr32[4 * q + c] = irr[c];
+        if (irr64) irr64[4 * q + c] = irrd[c];
+    }
+    if (out32) out32[4 * q + 3] = 0.0f;
+    if (out64) out64[4 * q + 3] = 0.0;
+    if (irr32) irr32[4 * q + 3] = 0.0f;
+    if (irr64) irr64[4 * q + 3] = 0.0;
+}
+
+// Tree cost over the spatial node boxes (SAH for triangles, VVH for points).
+double tree_cost(const uint32_t* nodes, uint32_t num_nodes, const float* root, int kind,
+                 float max_qr) {
+    if (num_nodes == 0) return 0.0;
+    auto measure = [&](const double* lo, const double* hi) {
+        double e[3];
+        for (int i = 0; i < 3; ++i) e[i] = hi[i] - lo[i];
+        if (kind == 1) {
+            double q = 2.0 * max_qr;
+            return (e[0] + q) * (e[1] + q) * (e[2] + q);
+        }
+        return 2.0 * (e[0] * e[1] + e[0] * e[2] + e[1] * e[2]);
+    };
+    struct Ent { uint32_t node; double lo[3], hi[3]; };
+    std::vector<Ent> st;
+    Ent r;
+    r.node = 0;
+    for (int i = 0; i < 3; ++i) {
+        r.lo[i] = root[i];
+        r.hi[i] = root[3 + i];
+    }
+    double mroot = measure(r.lo, r.hi);
+    st.push_back(r);
+    double cost = 0.0;
+    while (!st.empty()) {
+        Ent e = st.back();
+        st.pop_back();
+        double m = measure(e.lo, e.hi) / mroot;
+        uint32_t w1 = nodes[2 * e.node + 1];
+        if ((w1 >> 30) == kLeafTag) {
+            cost += (double)nodes[2 * e.node] * m;
+            continue;
+        }
+        cost += (double)kCostTraverse * m;
+        int a = (int)(w1 >> 30);
+        double s = (double)bitsf(nodes[2 * e.node]);
+        Ent L = e, R = e;
+        L.node = e.node + 1;
+        L.hi[a] = s;
+        R.node = w1 & kIndexMask;
+        R.lo[a] = s;
+        st.push_back(R);
+        st.push_back(L);
+    }
+    return cost;
+}
+
+// fp64 triangle / box overlap by separating axes (13 axes), used to check that every
+// leaf whose box the triangle's geometry touches references it.
+bool tri_box_overlap(const double* lo, const double* hi, const double tv[3][3]) {
+    double c[3], h[3], v[3][3];
+    for (int i = 0; i < 3; ++i) {
+        c[i] = 0.5 * (lo[i] + hi[i]);
+        h[i] = 0.5 * (hi[i] - lo[i]);
+    }
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) v[j][i] = tv[j][i] - c[i];
+    for (int i = 0; i < 3; ++i) {
+        double mn = std::min({v[0][i], v[1][i], v[2][i]}), mx = std::max({v[0][i], v[1][i], v[2][i]});
+        if (mn > h[i] || mx < -h[i]) return false;
+    }
+    double e[3][3];
+    for (int i = 0; i < 3; ++i) {
+        e[0][i] = v[1][i] - v[0][i];
+        e[1][i] = v[2][i] - v[1][i];
+        e[2][i] = v[0][i] - v[2][i];
+    }
+    double nrm[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                     e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+    {
+        double d = nrm[0] * v[0][0] + nrm[1] * v[0][1] + nrm[2] * v[0][2];
+        double r = h[0] * std::fabs(nrm[0]) + h[1] * std::fabs(nrm[1]) + h[2] * std::fabs(nrm[2]);
+        if (std::fabs(d) > r) return false;
+    }
+    for (int ei = 0; ei < 3; ++ei) {
+        for (int ax = 0; ax < 3; ++ax) {
+            double a[3] = {0, 0, 0};
+            double u[3] = {0, 0, 0};
+            u[ax] = 1.0;
+            a[0] = u[1] * e[ei][2] - u[2] * e[ei][1];
+            a[1] = u[2] * e[ei][0] - u[0] * e[ei][2];
+            a[2] = u[0] * e[ei][1] - u[1] * e[ei][0];
+            double p0 = a[0] * v[0][0] + a[1] * v[0][1] + a[2] * v[0][2];
+            double p1 = a[0] * v[1][0] + a[1] * v[1][1] + a[2] * v[1][2];
+            double p2 = a[0] * v[2][0] + a[1] * v[2][1] + a[2] * v[2][2];
+            double r = h[0] * std::fabs(a[0]) + h[1] * std::fabs(a[1]) + h[2] * std::fabs(a[2]);
+            double mn = std::min({p0, p1, p2}), mx = std::max({p0, p1, p2});
+            if (mn > r * (1 + 1e-12) || mx < -r * (1 + 1e-12)) return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+// =================================================================================
+// C API (ctypes)
+// =================================================================================
+extern "C" {
+
+void* orc_build(int kind, const float* v0, const float* v1, const float* v2, uint32_t n,
+                float ratio, uint32_t small_max, float max_qr, int has_root, const float* root6) {
+    if (small_max < 1 || small_max > 64) return nullptr;
+    return build(kind, v0, v1, v2, n, ratio, small_max, max_qr, has_root, root6);
+}
+uint32_t orc_tree_num_nodes(void* h) { return (uint32_t)(((OTree*)h)->nodes.size() / 2); }
+uint32_t orc_tree_num_refs(void* h) { return (uint32_t)((OTree*)h)->refs.size(); }
+uint32_t orc_tree_num_leaves(void* h) { return ((OTree*)h)->num_leaves; }
+uint32_t orc_tree_max_level(void* h) { return ((OTree*)h)->max_level; }
+void orc_tree_root(void* h, float* out6) { std::memcpy(out6, ((OTree*)h)->root, 24); }
+void orc_tree_copy(void* h, uint32_t* nodes, uint32_t* refs) {
+    OTree* T = (OTree*)h;
+    std::memcpy(nodes, T->nodes.data(), T->nodes.size() * 4);
+    if (!T->refs.empty()) std::memcpy(refs, T->refs.data(), T->refs.size() * 4);
+}
+void orc_tree_free(void* h) { delete (OTree*)h; }
+
+double orc_tree_cost(const uint32_t* nodes, uint32_t num_nodes, const float* root6, int kind,
+                     float max_qr) {
+    return tree_cost(nodes, num_nodes, root6, kind, max_qr);
+}
+
+// Structural check of a packed preorder tree: every index in range, every node reached
+// exactly once, leaf reference ranges tile [0, num_refs) in preorder, each leaf's
+// references ascending.  Returns 0 when valid, else a code naming the first failure.
+int orc_tree_check_structure(const uint32_t* nodes, uint32_t num_nodes, const uint32_t* refs,
+                             uint32_t num_refs, uint32_t num_elems) {
+    if (num_nodes == 0) return 1;
+    uint32_t next_ref = 0;
+    std::vector<uint32_t> st{0};
+    uint32_t expect = 0;
+    while (!st.empty()) {
+        uint32_t nd = st.back();
+        st.pop_back();
+        if (nd != expect) return 2;  // preorder: nodes appear in order
+        expect++;
+        uint32_t w1 = nodes[2 * nd + 1];
+        if ((w1 >> 30) == kLeafTag) {
+            uint32_t cnt = nodes[2 * nd], b = w1 & kIndexMask;
+            if (b != next_ref) return 3;
+            if ((uint64_t)b + cnt > num_refs) return 4;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                if (refs[b + i] >= num_elems) return 5;
+                if (i > 0 && refs[b + i] <= refs[b + i - 1]) return 6;
+            }
+            next_ref = b + cnt;
+        } else {
+            uint32_t r = w1 & kIndexMask;
+            if (r <= nd + 1 || r >= num_nodes) return 7;
+            st.push_back(r);
+            st.push_back(nd + 1);
+        }
+    }
+    if (expect != num_nodes) return 8;
+    if (next_ref != num_refs) return 9;
+    return 0;
+}
+
+// Coverage check: every leaf box the triangle's geometry touches (fp64 separating-axis
+// test, boxes shrunk by rel_shrink of the root extent) must reference the triangle.
+// Returns the number of violations.
+uint64_t orc_tree_check_coverage(const uint32_t* nodes, const uint32_t* refs, const float* root6,
+                                 const float* v0, const float* v1, const float* v2, uint32_t ntri,
+                                 double rel_shrink) {
+    struct Ent { uint32_t node; double lo[3], hi[3]; };
+    std::vector<Ent> st;
+    Ent r;
+    r.node = 0;
+    double ext = 0;
+    for (int i = 0; i < 3; ++i) {
+        r.lo[i] = root6[i];
+        r.hi[i] = root6[3 + i];
+        ext = std::max(ext, r.hi[i] - r.lo[i]);
+    }
+    st.push_back(r);
+    uint64_t bad = 0;
+    double sh = rel_shrink * ext;
+    while (!st.empty()) {
+        Ent e = st.back();
+        st.pop_back();
+        uint32_t w1 = nodes[2 * e.node + 1];
+        if ((w1 >> 30) != kLeafTag) {
+            int a = (int)(w1 >> 30);
+            double s = bitsf(nodes[2 * e.node]);
+            Ent L = e, R = e;
+            L.node = e.node + 1;
+            L.hi[a] = s;
+            R.node = w1 & kIndexMask;
+            R.lo[a] = s;
+            st.push_back(L);
+            st.push_back(R);
+            continue;
+        }
+        uint32_t cnt = nodes[2 * e.node], b = w1 & kIndexMask;
+        double lo[3], hi[3];
+        bool degenerate = false;
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = e.lo[i] + sh;
+            hi[i] = e.hi[i] - sh;
+            if (lo[i] > hi[i]) degenerate = true;
+        }
+        if (degenerate) continue;
+        const uint32_t* lr = refs + b;
+        for (uint32_t t = 0; t < ntri; ++t) {
+            double tv[3][3];
+            for (int c = 0; c < 3; ++c) {
+                tv[0][c] = v0[4 * t + c];
+                tv[1][c] = v1[4 * t + c];
+                tv[2][c] = v2[4 * t + c];
+            }
+            if (!tri_box_overlap(lo, hi, tv)) continue;
+            if (!std::binary_search(lr, lr + cnt, t)) bad++;
+        }
+    }
+    return bad;
+}
+
+// Closest hits by tree traversal.  counters[4] += inner visits, leaf visits, leaf
+// references read, triangle tests.
+void orc_trace(const uint32_t* nodes, const uint32_t* refs, const float* root6, const float* v0,
+               const float* v1, const float* v2, const float* orig4, const float* dir4, uint32_t n,
+               float eps, int32_t* out_tri, float* out_t, float* out_uv, uint64_t* counters,
+               int nthreads) {
+    TriSoup S{v0, v1, v2};
+    std::vector<TraceCounts> per(std::max(1, nthreads));
+    std::atomic<int> slot{0};
+    parallel_for(n, nthreads, [&](uint32_t b, uint32_t e) {
+        TraceCounts& c = per[slot++];
+        for (uint32_t i = b; i < e; ++i) {
+            float u, v;
+            trace_one(nodes, refs, root6, S, ld4(orig4, i), ld4(dir4, i), eps, out_tri[i], out_t[i],
+                      u, v, c);
+            if (out_uv) {
+                out_uv[2 * i] = u;
+                out_uv[2 * i + 1] = v;
+            }
+        }
+    });
+    if (counters) {
+        for (auto& c : per) {
+            counters[0] += c.inner;
+            counters[1] += c.leaves;
+            counters[2] += c.refs;
+            counters[3] += c.tests;
+        }
+    }
+}
+
+// Closest hits by testing every triangle (same root clip, epsilon and tie rule).
+void orc_trace_brute(const float* root6, const float* v0, const float* v1, const float* v2,
+                     uint32_t ntri, const float* orig4, const float* dir4, uint32_t n, float eps,
+                     int32_t* out_tri, float* out_t, float* out_uv, int nthreads) {
+    TriSoup S{v0, v1, v2};
+    parallel_for(n, nthreads, [&](uint32_t b, uint32_t e) {
+        for (uint32_t i = b; i < e; ++i) {
+            V3 o = ld4(orig4, i), d = ld4(dir4, i);
+            V3 inv = mk(1.0f / d.v[0], 1.0f / d.v[1], 1.0f / d.v[2]);
+            float t0, t1;
+            out_tri[i] = -1;
+            out_t[i] = 0.0f;
+            if (out_uv) out_uv[2 * i] = out_uv[2 * i + 1] = 0.0f;
+            if (!ray_box(root6, root6 + 3, o, inv, eps, kInf, t0, t1)) continue;
+            float best_t = t1 + eps, bu = 0, bv = 0;
+            int32_t best = -1;
+            for (uint32_t t = 0; t < ntri; ++t) tri_test(S, t, o, d, eps, best_t, best, bu, bv);
+            out_tri[i] = best;
+            if (best >= 0) {
+                out_t[i] = best_t;
+                if (out_uv) {
+                    out_uv[2 * i] = bu;
+                    out_uv[2 * i + 1] = bv;
+                }
+            }
+        }
+    });
+}
+
+// Exact kNN through the point tree.  out_nbr: nq*k indices sorted by (d2, index),
+// 0xFFFFFFFF past the count.  counters[5] += inner, leaves, refs, distance tests,
+// heap insertions.
+void orc_knn(const uint32_t* nodes, const uint32_t* refs, const float* ppos4, const float* qpos4,
+             uint32_t nq, uint32_t k, float rmax, uint32_t* out_nbr, uint32_t* out_count,
+             uint64_t* counters, int nthreads) {
+    float r2 = rmax * rmax;
+    std::vector<KnnCounts> per(std::max(1, nthreads));
+    std::atomic<int> slot{0};
+    parallel_for(nq, nthreads, [&](uint32_t b, uint32_t e) {
+        KnnCounts& c = per[slot++];
+        std::vector<uint64_t> heap(k + 1);
+        for (uint32_t q = b; q < e; ++q) {
+            uint32_t hn = knn_one(nodes, refs, ppos4, ld4(qpos4, q), k, r2, heap.data(), c);
+            out_count[q] = hn;
+            if (out_nbr)
+                for (uint32_t i = 0; i < k; ++i)
+                    out_nbr[(size_t)q * k + i] = (i < hn) ? (uint32_t)(heap[i] & 0xFFFFFFFFu) : 0xFFFFFFFFu;
+        }
+    });
+    if (counters) {
+        for (auto& c : per) {
+            counters[0] += c.inner;
+            counters[1] += c.leaves;
+            counters[2] += c.refs;
+            counters[3] += c.tests;
+            counters[4] += c.accepted;
+        }
+    }
+}
+
+// Exact kNN by testing every photon.
+void orc_knn_brute(const float* ppos4, uint32_t np, const float* qpos4, uint32_t nq, uint32_t k,
+                   float rmax, uint32_t* out_nbr, uint32_t* out_count, int nthreads) {
+    float r2 = rmax * rmax;
+    parallel_for(nq, nthreads, [&](uint32_t b, uint32_t e) {
+        std::vector<uint64_t> keys;
+        for (uint32_t q = b; q < e; ++q) {
+            keys.clear();
+            V3 qp = ld4(qpos4, q);
+            for (uint32_t p = 0; p < np; ++p) {
+                float d2 = dist2(ld4(ppos4, p), qp);
+                if (d2 < r2) keys.push_back(kkey(d2, p));
+            }
+            uint32_t m = (uint32_t)std::min<size_t>(k, keys.size());
+            std::partial_sort(keys.begin(), keys.begin() + m, keys.end());
+            out_count[q] = m;
+            for (uint32_t i = 0; i < k; ++i)
+                out_nbr[(size_t)q * k + i] = (i < m) ? (uint32_t)(keys[i] & 0xFFFFFFFFu) : 0xFFFFFFFFu;
+        }
+    });
+}
+
+// Radiance of each query from its sorted neighbour set (fp32 kernel order and fp64).
+// The accumulation target is zero-initialised here (out = estimate, not +=).
+void orc_radiance(const uint32_t* nbr, const uint32_t* count, uint32_t k, float rmax,
+                  const float* ppos4, const float* ppow4, const uint32_t* pdir_oct,
+                  const float* qpos4, const float* qnrm4, const float* qdir4, const float* qinfl4,
+                  const float* qclr4, uint32_t nq, float* out32, double* out64, float* irr32,
+                  double* irr64, float* r2out, int nthreads) {
+    RadIn I{ppos4, ppow4, pdir_oct, qpos4, qnrm4, qdir4, qinfl4, qclr4};
+    parallel_for(nq, nthreads, [&](uint32_t b, uint32_t e) {
+        for (uint32_t q = b; q < e; ++q)
+            radiance_one(I, q, nbr + (size_t)q * k, count[q], k, rmax, out32, out64, irr32, irr64,
+                         r2out);
+    });
+}
+
+uint32_t orc_oct_encode(float x, float y, float z) { return oct_encode(x, y, z); }
+void orc_oct_encode_n(const float* dir4, uint32_t n, uint32_t* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = oct_encode(dir4[4 * i], dir4[4 * i + 1], dir4[4 * i + 2]);
+}
+void orc_oct_decode(uint32_t c, float* out3) {
+    V3 v = oct_decode(c);
+    out3[0] = v.v[0];
+    out3[1] = v.v[1];
+    out3[2] = v.v[2];
+}
+
+// Single-ray / single-triangle test for closed-form checks.
+int orc_ray_tri(const float* a, const float* b, const float* c, const float* o, const float* d,
+                float* out3) {
+    V3 A = mk(a[0], a[1], a[2]), B = mk(b[0], b[1], b[2]), Cc = mk(c[0], c[1], c[2]);
+    float lam = 0.0f, u = 0.0f, v = 0.0f;
+    bool hit = ray_tri(A, sub(B, A), sub(Cc, A), mk(o[0], o[1], o[2]), mk(d[0], d[1], d[2]), lam, u, v);
+    out3[0] = lam;
+    out3[1] = u;
+    out3[2] = v;
+    return hit ? 1 : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py, never by the product package.  See oracle.cpp for what each function
+restates.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+vp = ctypes.c_void_p
+
+
+def build_lib(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(os.path.join(_HERE, "oracle.cpp")):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"], check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        build_lib()
+    L = ctypes.CDLL(_LIB)
+    L.orc_build.restype = vp
+    L.orc_build.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32,
+                            ctypes.c_float, ctypes.c_int, vp]
+    for name in ("orc_tree_num_nodes", "orc_tree_num_refs", "orc_tree_num_leaves", "orc_tree_max_level"):
+        getattr(L, name).restype = ctypes.c_uint32
+        getattr(L, name).argtypes = [vp]
+    L.orc_tree_root.argtypes = [vp, f32p]
+    L.orc_tree_copy.argtypes = [vp, u32p, u32p]
+    L.orc_tree_free.argtypes = [vp]
+    L.orc_tree_cost.restype = ctypes.c_double
+    L.orc_tree_cost.argtypes = [u32p, ctypes.c_uint32, f32p, ctypes.c_int, ctypes.c_float]
+    L.orc_tree_check_structure.restype = ctypes.c_int
+    L.orc_tree_check_structure.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32]
+    L.orc_tree_check_coverage.restype = ctypes.c_uint64
+    L.orc_tree_check_coverage.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_double]
+    L.orc_trace.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_float,
+                            i32p, f32p, f32p, u64p, ctypes.c_int]
+    L.orc_trace_brute.argtypes = [f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, f32p, ctypes.c_uint32,
+                                  ctypes.c_float, i32p, f32p, f32p, ctypes.c_int]
+    L.orc_knn.argtypes = [u32p, u32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, u32p, u32p,
+                          u64p, ctypes.c_int]
+    L.orc_knn_brute.argtypes = [f32p, ctypes.c_uint32, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                u32p, u32p, ctypes.c_int]
+    L.orc_radiance.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_float, f32p, f32p, u32p, f32p, f32p, f32p,
+                               f32p, vp, ctypes.c_uint32, f32p, f64p, f32p, f64p, f32p, ctypes.c_int]
+    L.orc_oct_encode.restype = ctypes.c_uint32
+    L.orc_oct_encode.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.orc_oct_encode_n.argtypes = [f32p, ctypes.c_uint32, u32p]
+    L.orc_oct_decode.argtypes = [ctypes.c_uint32, f32p]
+    L.orc_ray_tri.restype = ctypes.c_int
+    L.orc_ray_tri.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
+    _lib = L
+    return L
+
+
+def _c(a, dt=np.float32):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _threads(n):
+    return n if n else max(1, os.cpu_count() or 1)
+
+
+class Tree:
+    """A packed preorder tree (the library's node format) held in host memory."""
+
+    def __init__(self, kind, nodes, refs, root, max_level=0, num_leaves=0, num_elems=0):
+        self.kind = kind
+        self.nodes = nodes
+        self.refs = refs
+        self.root = root
+        self.max_level = max_level
+        self.num_leaves = num_leaves
+        self.num_elems = num_elems
+
+    @property
+    def num_nodes(self):
+        return self.nodes.size // 2
+
+
+def build(kind, v0, v1=None, v2=None, ratio=None, small_max=None, max_qr=1.0, root=None) -> Tree:
+    L = lib()
+    if ratio is None:
+        ratio = 0.25 if kind == 0 else 0.1
+    if small_max is None:
+        small_max = 64 if kind == 0 else 32
+    v0 = _c(v0)
+    v1 = _c(v1) if v1 is not None else v0
+    v2 = _c(v2) if v2 is not None else v0
+    r6 = _c(root).ravel() if root is not None else np.zeros(6, np.float32)
+    h = L.orc_build(kind, v0.ctypes.data, v1.ctypes.data, v2.ctypes.data, v0.shape[0], ratio, small_max,
+                    max_qr, 1 if root is not None else 0, r6.ctypes.data)
+    if not h:
+        raise ValueError("oracle build rejected its parameters")
+    try:
+        nn, nr = L.orc_tree_num_nodes(h), L.orc_tree_num_refs(h)
+        nodes = np.zeros(2 * nn, np.uint32)
+        refs = np.zeros(max(nr, 1), np.uint32)
+        L.orc_tree_copy(h, nodes, refs)
+        rt = np.zeros(6, np.float32)
+        L.orc_tree_root(h, rt)
+        return Tree(kind, nodes, refs[:nr], rt, L.orc_tree_max_level(h), L.orc_tree_num_leaves(h), v0.shape[0])
+    finally:
+        L.orc_tree_free(h)
+
+
+def tree_cost(nodes, root, kind, max_qr=1.0) -> float:
+    nodes = _c(nodes, np.uint32)
+    return lib().orc_tree_cost(nodes, nodes.size // 2, _c(root).ravel(), kind, max_qr)
+
+
+def check_structure(nodes, refs, num_elems) -> int:
+    nodes = _c(nodes, np.uint32)
+    refs = _c(refs, np.uint32) if len(refs) else np.zeros(1, np.uint32)
+    return lib().orc_tree_check_structure(nodes, nodes.size // 2, refs, len(refs), num_elems)
+
+
+def check_coverage(nodes, refs, root, v0, v1, v2, rel_shrink=1e-6) -> int:
+    refs = _c(refs, np.uint32) if len(refs) else np.zeros(1, np.uint32)
+    return int(lib().orc_tree_check_coverage(_c(nodes, np.uint32), refs, _c(root).ravel(), _c(v0), _c(v1), _c(v2),
+                                             v0.shape[0], rel_shrink))
+
+
+def trace(tree: Tree, v0, v1, v2, orig, dirs, eps, nthreads=0):
+    n = orig.shape[0]
+    tri = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    uv = np.zeros(2 * n, np.float32)
+    cnt = np.zeros(4, np.uint64)
+    refs = tree.refs if tree.refs.size else np.zeros(1, np.uint32)
+    lib().orc_trace(tree.nodes, refs, tree.root, _c(v0), _c(v1), _c(v2), _c(orig), _c(dirs), n, eps, tri, t, uv,
+                    cnt, _threads(nthreads))
+    return tri, t, uv.reshape(n, 2), cnt
+
+
+def trace_brute(root, v0, v1, v2, orig, dirs, eps, nthreads=0):
+    n = orig.shape[0]
+    tri = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    uv = np.zeros(2 * n, np.float32)
+    lib().orc_trace_brute(_c(root).ravel(), _c(v0), _c(v1), _c(v2), v0.shape[0], _c(orig), _c(dirs), n, eps, tri,
+                          t, uv, _threads(nthreads))
+    return tri, t, uv.reshape(n, 2)
+
+
+def knn(tree: Tree, ppos, qpos, k, rmax, nthreads=0):
+    nq = qpos.shape[0]
+    nbr = np.zeros(nq * k, np.uint32)
+    cnt = np.zeros(nq, np.uint32)
+    ctr = np.zeros(5, np.uint64)
+    refs = tree.refs if tree.refs.size else np.zeros(1, np.uint32)
+    lib().orc_knn(tree.nodes, refs, _c(ppos), _c(qpos), nq, k, rmax, nbr, cnt, ctr, _threads(nthreads))
+    return nbr.reshape(nq, k), cnt, ctr
+
+
+def knn_brute(ppos, qpos, k, rmax, nthreads=0):
+    nq = qpos.shape[0]
+    nbr = np.zeros(nq * k, np.uint32)
+    cnt = np.zeros(nq, np.uint32)
+    lib().orc_knn_brute(_c(ppos), ppos.shape[0], _c(qpos), nq, k, rmax, nbr, cnt, _threads(nthreads))
+    return nbr.reshape(nq, k), cnt
+
+
+def oct_encode(dirs4) -> np.ndarray:
+    d = _c(dirs4)
+    out = np.zeros(d.shape[0], np.uint32)
+    lib().orc_oct_encode_n(d, d.shape[0], out)
+    return out
+
+
+def oct_decode(code: int) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_oct_decode(int(code), out)
+    return out
+
+
+def radiance(nbr, count, k, rmax, ppos, ppow, pdir4, qpos, qnrm, qdir, qinfl, qclr=None, nthreads=0):
+    nq = qpos.shape[0]
+    codes = oct_encode(pdir4)
+    out32 = np.zeros(4 * nq, np.float32)
+    out64 = np.zeros(4 * nq, np.float64)
+    irr32 = np.zeros(4 * nq, np.float32)
+    irr64 = np.zeros(4 * nq, np.float64)
+    r2 = np.zeros(nq, np.float32)
+    clr = _c(qclr) if qclr is not None else None
+    lib().orc_radiance(_c(nbr, np.uint32).ravel(), _c(count, np.uint32), k, rmax, _c(ppos), _c(ppow), codes,
+                       _c(qpos), _c(qnrm), _c(qdir), _c(qinfl), clr.ctypes.data if clr is not None else None, nq,
+                       out32, out64, irr32, irr64, r2, _threads(nthreads))
+    return (out32.reshape(nq, 4), out64.reshape(nq, 4), irr32.reshape(nq, 4), irr64.reshape(nq, 4), r2)
+
+
+def ray_tri(a, b, c, o, d):
+    out = np.zeros(3, np.float32)
+    hit = lib().orc_ray_tri(_c(a), _c(b), _c(c), _c(o), _c(d), out)
+    return bool(hit), out

@@ -1,0 +1,80 @@
+"""GPU exact-kNN radiance estimate vs the oracle on the same point tree: neighbour
+sets bit-exact (sorted by (d2, photon index)) and equal to the brute force; radiance
+bit-exact to the oracle's fp32 path and within 1e-5 relative of its fp64 path."""
+
+import numpy as np
+import pytest
+
+from mnrt_amd import api, scene as S
+from helpers import hit_queries
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-5
+
+
+def _run(orc, cuda, sc, ppos, ppow, pdir, q, k, rmax, brute=True, clr=True):
+    P = orc.build(1, ppos, max_qr=rmax, root=np.concatenate(sc.bounds()))
+    nbr, cnt, _ = orc.knn(P, ppos, q["pos"], k, rmax)
+    if brute:
+        bn, bc = orc.knn_brute(ppos, q["pos"], k, rmax)
+        assert np.array_equal(nbr, bn) and np.array_equal(cnt, bc)
+    o32, o64, i32, i64, r2 = orc.radiance(nbr, cnt, k, rmax, ppos, ppow, pdir, q["pos"], q["nrm"], q["dir"],
+                                          q["infl"], q["clr"] if clr else None)
+    DP = api.tree_from_host(1, P.nodes, P.refs, P.root, ppos.shape[0], P.max_level)
+    api.attach_photon_records(DP, ppos, ppow, pdir)
+    out = api.knn_gather(DP, k, rmax, q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"] if clr else None,
+                         want_nbr=True, want_r2=True, want_irr=True, want_count=True)
+    cuda.cuda.synchronize()
+    gn = out["nbr"].cpu().numpy().view(np.uint32)
+    gc = out["count"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(gc, cnt), f"{(gc != cnt).sum()} counts differ"
+    assert np.array_equal(gn, nbr), f"{(gn != nbr).any(axis=1).sum()} neighbour sets differ"
+    grad = out["rad"].cpu().numpy()
+    assert np.array_equal(grad.view(np.uint32), o32.view(np.uint32)), "radiance not bit-exact to fp32 oracle"
+    assert np.array_equal(out["r2"].cpu().numpy().view(np.uint32), r2.view(np.uint32))
+    scale = np.abs(o64).max(axis=1, keepdims=True) + 1e-30
+    err = np.abs(grad.astype(np.float64) - o64) / scale
+    assert err.max() <= RTOL64, f"max rel err vs fp64 {err.max():.3e}"
+    return gn, gc
+
+
+def test_knn_small_scene_k50(orc, cuda):
+    sc = S.scene_small(subdiv=5)
+    ppos, ppow, pdir = S.random_surface_photons(sc, 65536, seed=0)
+    o, d, _ = S.camera_rays(128, 128)
+    q = hit_queries(orc, sc, o, d)
+    print(f"knn: {len(q['pos'])} queries k=50", flush=True)
+    _run(orc, cuda, sc, ppos, ppow, pdir, q, 50, sc.r_max())
+
+
+@pytest.mark.parametrize("k", [1, 8, 64, 100, 200, 256])
+def test_knn_k_sweep(orc, cuda, k):
+    sc = S.scene_small(subdiv=3)
+    ppos, ppow, pdir = S.random_surface_photons(sc, 20000, seed=k)
+    o, d, _ = S.camera_rays(48, 48)
+    q = hit_queries(orc, sc, o, d)
+    print(f"knn: k={k}", flush=True)
+    _run(orc, cuda, sc, ppos, ppow, pdir, q, k, sc.r_max())
+
+
+def test_knn_small_radius_and_few_photons(orc, cuda):
+    # many queries find fewer than k (and fewer than 8) photons: r2 = r_max^2, E = 0
+    sc = S.scene_small(subdiv=2)
+    ppos, ppow, pdir = S.random_surface_photons(sc, 500, seed=3)
+    o, d, _ = S.camera_rays(40, 40)
+    q = hit_queries(orc, sc, o, d)
+    _, gc = _run(orc, cuda, sc, ppos, ppow, pdir, q, 50, 0.05)
+    assert (gc < 8).any() and (gc > 0).any()
+    _run(orc, cuda, sc, ppos, ppow, pdir, q, 50, 0.05, clr=False)
+
+
+def test_knn_coincident_photons_tie_by_index(orc, cuda):
+    # clusters of identical photon positions: equal d2, order and cut by photon index
+    sc = S.scene_small(subdiv=2)
+    ppos, ppow, pdir = S.random_surface_photons(sc, 4000, seed=4)
+    ppos[1000:3000] = ppos[np.repeat(np.arange(100), 20)]
+    o, d, _ = S.camera_rays(32, 32)
+    q = hit_queries(orc, sc, o, d)
+    q["pos"][:50] = ppos[:50]  # queries exactly on photons (d2 = 0)
+    _run(orc, cuda, sc, ppos, ppow, pdir, q, 30, sc.r_max())
