@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""Benchmark of the MNRT kd-tree hot path on one MI355X (gfx950).
+
+One step = one pass of the hot path over the plain large workload (about 1.03M
+triangles, 1M traced photons, 1920x1080 primary rays plus one cosine bounce ray per hit,
+k = 100 kNN radiance estimate at every hit):
+
+  build the triangle kd-tree            mnrt_build_tri        (KDTreeTri + BuildTree)
+  trace primary rays                    mnrt_trace_closest    (KernelRTTraceRays)
+  shading points + bounce rays + trace  mnrt_shading_points, mnrt_bounce_rays, mnrt_trace_closest
+  build the photon kd-tree              mnrt_build_points     (KDTreePoint + BuildTree)
+  kNN radiance at all hits              mnrt_knn_gather       (KernelPMPhotonGather)
+
+Inputs are resident in HBM before the timed region (the photon map is traced once during
+set-up with mnrt_photons_generate).  `value` counts rays traced plus kNN queries answered
+per second of step time.  The roofline entry is for the kNN gather kernel; its
+algorithmic bytes come from the oracle's instrumented replay of a deterministic sample
+of the same queries on the same tree.  The CPU baseline is the oracle (C++ restatement of
+the reference algorithms) on the host cores, on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s + M kNN queries/s (k=50) at %HBM peak; batched vs plain; bwd/fwd ratio"
+
+
+def trace_bytes(counts, n_rays):
+    """Traversal algorithmic bytes: 24 B ray in + 16 B hit out per ray, 8 B per inner and
+    per leaf visit, 4 B per leaf reference, 36 B per triangle test."""
+    inner, leaves, refs, tests = [float(c) for c in counts]
+    return 40.0 * n_rays + 8.0 * inner + 8.0 * leaves + 4.0 * refs + 36.0 * tests
+
+
+def knn_bytes(counts, n_q, n_set):
+    """kNN algorithmic bytes: 24 B query in + 12 B radiance out, 8 B per node visit,
+    4 B per leaf reference, 12 B per photon distance test, 16 B per neighbour kept."""
+    inner, leaves, refs, tests, _ = [float(c) for c in counts]
+    return 36.0 * n_q + 8.0 * (inner + leaves) + 4.0 * refs + 12.0 * tests + 16.0 * float(n_set)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", choices=["large", "small"], default="large")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--sample", type=int, default=64, help="1-in-N subsample for the oracle replay")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the hot path shards rays/queries across GPUs by replication, see the design notes)",
+              file=sys.stderr)
+        sys.exit(3)
+
+    import torch
+
+    from mnrt_amd import api, native, scene as S
+
+    native.check(native.load().mnrt_device_init(0), "mnrt_device_init")
+    torch.cuda.set_device(0)
+
+    t_setup = time.time()
+    if args.config == "large":
+        sc = S.scene_large()
+        W, H, K, NPH = 1920, 1080, 100, 1_000_000
+        bounce = True
+        workload = "plain-large: box + 48 icospheres (subdiv 5) + 50k random tris, 1M photons, 1920x1080 + 1 bounce, k=100"
+    else:
+        sc = S.scene_small(subdiv=5)
+        W, H, K, NPH = 512, 512, 50, 65536
+        bounce = False
+        workload = "plain-small: box + icosphere subdiv 5, 65536 photons, 512x512 primary, k=50"
+    eps, rmax = sc.ray_epsilon(), sc.r_max()
+    lo, hi = sc.bounds()
+    root = np.concatenate([lo, hi]).astype(np.float32)
+    v0, v1, v2, alb = (api.dev(a) for a in (sc.v0, sc.v1, sc.v2, sc.albedo))
+    o_np, d_np, _ = S.camera_rays(W, H)
+    go, gd = api.dev(o_np), api.dev(d_np)
+    # photon map (set-up, not timed)
+    T0 = api.build_tri(v0, v1, v2)
+    ph = api.photons_generate(T0, v0, v1, v2, alb, sc.lights, NPH, 4, 0, eps)
+    T0.release()
+    torch.cuda.synchronize()
+    ppos, ppow, pdir = ph["pos"].contiguous(), ph["power"].contiguous(), ph["dir"].contiguous()
+    pparams = api.default_params(1, root, rmax)
+    print(f"# setup {time.time() - t_setup:.1f}s: {sc.num_tris} tris, {ph['count']} photons from {ph['paths']} paths",
+          file=sys.stderr, flush=True)
+
+    stream = torch.cuda.current_stream()
+    phases = ["build_tri", "trace_primary", "shade_bounce", "trace_bounce", "build_points", "knn"]
+
+    def step(record):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
+        kev = []
+        ev[0].record(stream)
+        T = api.build_tri(v0, v1, v2)
+        ev[1].record(stream)
+        tri, t, _, _ = api.trace_closest(T, go, gd, eps)
+        ev[2].record(stream)
+        q1 = api.shading_points(go, gd, tri, t, v0, v1, v2, alb)
+        queries = [q1]
+        n_b = 0
+        if bounce:
+            bo, bd, binfl = api.bounce_rays(q1, 3)
+            ev[3].record(stream)
+            tri2, t2, _, _ = api.trace_closest(T, bo, bd, eps)
+            ev[4].record(stream)
+            q2 = api.shading_points(bo, bd, tri2, t2, v0, v1, v2, alb, infl=binfl)
+            queries.append(q2)
+            n_b = bo.shape[0]
+        else:
+            ev[3].record(stream)
+            ev[4].record(stream)
+        P = api.build_points(ppos, pparams)
+        api.attach_photon_records(P, ppos, ppow, pdir)
+        ev[5].record(stream)
+        outs = []
+        for q in queries:
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            outs.append(api.knn_gather(P, K, rmax, q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"]))
+            b.record(stream)
+            kev.append((a, b, q["count"]))
+        ev[6].record(stream)
+        torch.cuda.synchronize()
+        res = dict(rays=go.shape[0] + n_b, queries=sum(q["count"] for q in queries),
+                   ms={p: ev[i].elapsed_time(ev[i + 1]) for i, p in enumerate(phases)},
+                   knn_launch_ms=[(a.elapsed_time(b), n) for a, b, n in kev],
+                   trace_ms=ev[1].elapsed_time(ev[2]) + (ev[3].elapsed_time(ev[4]) if bounce else 0.0))
+        if record is not None:
+            record.update(T=T, P=P, queries=queries, tri=tri, tri2=tri2 if bounce else None,
+                          bounce_rays=(bo, bd) if bounce else None)
+        else:
+            T.release()
+            P.release()
+        return res
+
+    for _ in range(args.warmup):
+        step(None)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    results = [step(None) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    ms_step = wall * 1e3 / args.steps
+    rays, queries = results[0]["rays"], results[0]["queries"]
+    phase_ms = {p: float(np.mean([r["ms"][p] for r in results])) for p in phases}
+    knn_launch = [x for r in results for x in r["knn_launch_ms"]]
+    knn_ms = float(np.sum([m for m, _ in knn_launch])) / args.steps
+    trace_ms = float(np.mean([r["trace_ms"] for r in results]))
+    value = (rays + queries) / (ms_step * 1e-3) / 1e6
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "M(rays+kNN queries)/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, traced photons)",
+        "config": {"workload": workload, "triangles": sc.num_tris, "photons": ph["count"], "rays": rays,
+                   "knn_queries": queries, "k": K, "parallelism": "single GPU"},
+        "phase_ms": {p: round(v, 3) for p, v in phase_ms.items()},
+        "trace_mrays_per_s": round(rays / (trace_ms * 1e-3) / 1e6, 2),
+        "knn_mqueries_per_s": round(queries / (knn_ms * 1e-3) / 1e6, 3),
+    }
+
+    # ---- oracle replay on a sample: algorithmic bytes, then the CPU baseline ----
+    rec = {}
+    step(rec)
+    from oracle import oracle as O
+
+    T, P = rec["T"], rec["P"]
+    pn, pr = P.nodes_host(), P.refs_host()
+    ptree = O.Tree(1, pn, pr, P.root, P.max_level, 0, ppos.shape[0])
+    ppos_h = ppos.cpu().numpy()
+    qpos = np.concatenate([q["pos"].cpu().numpy() for q in rec["queries"]])
+    qs = qpos[:: args.sample]
+    ncpu = os.cpu_count() or 1
+    tk = time.time()
+    nbr, cnt, kctr = O.knn(ptree, ppos_h, qs, K, rmax, nthreads=ncpu)
+    cpu_knn_s = time.time() - tk
+    per_q = knn_bytes(kctr, len(qs), int(np.minimum(cnt, K).sum())) / len(qs)
+    tn, tr = T.nodes_host(), T.refs_host()
+    ttree = O.Tree(0, tn, tr, T.root, T.max_level, 0, sc.num_tris)
+    ray_o = [o_np]
+    ray_d = [d_np]
+    if rec["bounce_rays"] is not None:
+        ray_o.append(rec["bounce_rays"][0].cpu().numpy())
+        ray_d.append(rec["bounce_rays"][1].cpu().numpy())
+    ro = np.concatenate(ray_o)[:: args.sample]
+    rd = np.concatenate(ray_d)[:: args.sample]
+    tt = time.time()
+    _, _, _, tctr = O.trace(ttree, sc.v0, sc.v1, sc.v2, ro, rd, eps, nthreads=ncpu)
+    cpu_trace_s = time.time() - tt
+    per_ray = trace_bytes(tctr, len(ro)) / len(ro)
+    T.release()
+    P.release()
+
+    launches = [(m, n) for m, n in knn_launch]
+    avg_ms = float(np.mean([m for m, _ in launches]))
+    avg_q = float(np.mean([n for _, n in launches]))
+    achieved = per_q * avg_q / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"k_knn_{args.config}_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                       "kernel": "k_knn (mnrt_knn_gather)", "alg_bytes_per_query": round(per_q, 1),
+                       "avg_launch_ms": round(avg_ms, 4), "queries_per_launch": avg_q}
+    trace_gbs = per_ray * rays / (trace_ms * 1e-3) / 1e9
+    out["trace_roofline"] = {"kernel": "k_trace_closest", "achieved": round(trace_gbs, 2), "unit": "GB/s",
+                             "frac": round(trace_gbs / HBM_PEAK_GBS, 5), "alg_bytes_per_ray": round(per_ray, 1)}
+    out["knn_pct_hbm"] = round(100.0 * achieved / HBM_PEAK_GBS, 3)
+    out["trace_pct_hbm"] = round(100.0 * trace_gbs / HBM_PEAK_GBS, 3)
+
+    if not args.no_cpu:
+        tb = time.time()
+        O.build(0, sc.v0, sc.v1, sc.v2)
+        cpu_build_tri = time.time() - tb
+        tb = time.time()
+        O.build(1, ppos_h, max_qr=rmax, root=root)
+        cpu_build_pts = time.time() - tb
+        cpu_step = cpu_build_tri + cpu_build_pts + rays * (cpu_trace_s / len(ro)) + queries * (cpu_knn_s / len(qs))
+        out["cpu_baseline"] = {
+            "value": round((rays + queries) / cpu_step / 1e6, 4), "unit": "M(rays+kNN queries)/s", "cores": ncpu,
+            "kind": "port",
+            "sample": (f"oracle on the host: both tree builds in full (1 thread: {cpu_build_tri:.2f}s + "
+                       f"{cpu_build_pts:.2f}s), traversal of 1 in {args.sample} rays ({len(ro)} rays, {cpu_trace_s:.2f}s) "
+                       f"and kNN of 1 in {args.sample} queries ({len(qs)}, {cpu_knn_s:.2f}s) on {ncpu} threads, "
+                       f"scaled to the full step ({cpu_step:.1f}s)"),
+            "gpu_speedup": round(cpu_step / (ms_step * 1e-3), 1)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

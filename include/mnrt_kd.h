@@ -173,6 +173,48 @@ int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
 int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                    const float* d_pos4, float* d_out_radius, void* stream);
 
+/* --- photon tracing ------------------------------------------------------------------ */
+
+/* RTCore::RebuildPhotonMaps for the global map (RTCore.cpp:454-638; emission
+ * photon_build.cu:79, tracing raytracing.cu:1146, Russian roulette photon_build.cu:220,
+ * scattering photon_build.cu:278): traces photon paths from quad lights through the
+ * triangle tree until `target` photons are stored at diffuse hits (at most
+ * max_bounces + 1 per path, Russian roulette with p = 0.5 past bounce index 3), then
+ * scales their power by 1 / paths shot.
+ *   d_albedo4: per-triangle diffuse RGB (0 = absorbing, e.g. the light itself).
+ *   d_lights: num_lights records of 4 float4: corner, edge u, edge v, emitted power RGB;
+ *             path p starts on light p mod num_lights with power * num_lights.
+ * Outputs (target entries): position, power, incoming direction (float4 each); optional
+ * meta (light | bounce << 8), source path index, and the 4 triangles the path scattered
+ * from before the photon (0xFFFFFFFF padded) for reweighting by emission and albedo.
+ * *out_count = photons stored, *out_paths = paths shot. */
+int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, const float* d_v1,
+                          const float* d_v2, const float* d_albedo4, const float* d_lights,
+                          uint32_t num_lights, uint32_t target, uint32_t max_bounces, uint32_t seed,
+                          float eps, float* d_out_pos4, float* d_out_pow4, float* d_out_dir4,
+                          uint32_t* d_out_meta, uint32_t* d_out_path, uint32_t* d_out_path_tris,
+                          uint32_t* out_count, uint32_t* out_paths, void* stream);
+
+/* --- hits to queries -------------------------------------------------------------------- */
+
+/* Compaction of valid hits (RTCore::TraceRays, RTCore.cpp:416-429) into shading points:
+ * position o + d*t, unit geometric normal (KernelBSDFGetNormalsAtHit, bsdf.cu:317-367),
+ * source ray direction, per-triangle diffuse colour (alpha in w) and ray influence
+ * (d_infl4 per ray, or NULL for 1).  Outputs hold up to n entries; *out_count = hits.
+ * Optional: source ray index and hit triangle per shading point. */
+int mnrt_shading_points(uint32_t n, const float* d_orig4, const float* d_dir4, const int32_t* d_hit_tri,
+                        const float* d_hit_t, const float* d_v0, const float* d_v1, const float* d_v2,
+                        const float* d_albedo4, const float* d_infl4, float* d_out_pos4, float* d_out_nrm4,
+                        float* d_out_dir4, float* d_out_clr4, float* d_out_infl4, uint32_t* d_out_src,
+                        int32_t* d_out_tri, uint32_t* out_count, void* stream);
+
+/* One cosine-distributed diffuse bounce ray per shading point (kernel_genFinalGatherRays,
+ * raygen.cu:406-487) about the normal turned toward the arriving ray; optional influence
+ * output = the point's diffuse colour. */
+int mnrt_bounce_rays(uint32_t n, const float* d_pos4, const float* d_nrm4, const float* d_dir4,
+                     const float* d_clr4, uint32_t seed, float* d_out_orig4, float* d_out_dir4,
+                     float* d_out_infl4, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

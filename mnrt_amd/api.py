@@ -216,3 +216,65 @@ def knn_radii(P: DeviceTree, k: int, r_max: float, pos):
     N.check(N.load().mnrt_knn_radii(P.ref(), k, r_max, pos.shape[0], N.ptr(pos), N.ptr(out), N.stream_handle()),
             "mnrt_knn_radii")
     return out
+
+
+def light_records(lights) -> np.ndarray:
+    """(m, 4, 3) corner / edge u / edge v / power -> (m, 4, 4) float4 records."""
+    lights = np.asarray(lights, np.float32).reshape(-1, 4, 3)
+    rec = np.zeros((lights.shape[0], 4, 4), np.float32)
+    rec[:, :, :3] = lights
+    return rec
+
+
+def photons_generate(T: DeviceTree, v0, v1, v2, albedo, lights, target: int, max_bounces=4, seed=0, eps=1e-4,
+                     want_meta=False):
+    """Traced photon map of `target` photons (RTCore::RebuildPhotonMaps)."""
+    torch = _torch()
+    v0, v1, v2, albedo = dev(v0), dev(v1), dev(v2), dev(albedo)
+    lr = dev(light_records(lights))
+    pos = torch.empty((target, 4), dtype=torch.float32, device="cuda")
+    pw = torch.empty((target, 4), dtype=torch.float32, device="cuda")
+    dr = torch.empty((target, 4), dtype=torch.float32, device="cuda")
+    meta = torch.empty(target, dtype=torch.int32, device="cuda") if want_meta else None
+    path = torch.empty(target, dtype=torch.int32, device="cuda") if want_meta else None
+    tris = torch.empty((target, 4), dtype=torch.int32, device="cuda") if want_meta else None
+    cnt, shot = N.u32(0), N.u32(0)
+    N.check(N.load().mnrt_photons_generate(T.ref(), N.ptr(v0), N.ptr(v1), N.ptr(v2), N.ptr(albedo), N.ptr(lr),
+                                           lr.shape[0], target, max_bounces, seed, eps, N.ptr(pos), N.ptr(pw),
+                                           N.ptr(dr), N.ptr(meta), N.ptr(path), N.ptr(tris), ctypes.byref(cnt),
+                                           ctypes.byref(shot), N.stream_handle()), "mnrt_photons_generate")
+    n = cnt.value
+    out = dict(pos=pos[:n], power=pw[:n], dir=dr[:n], count=n, paths=shot.value)
+    if want_meta:
+        out.update(meta=meta[:n], path=path[:n], tris=tris[:n])
+    return out
+
+
+def shading_points(orig, dirs, tri, t, v0, v1, v2, albedo, infl=None, want_src=False):
+    """Compacted shading points at the valid hits."""
+    torch = _torch()
+    n = orig.shape[0]
+    outs = [torch.empty((max(n, 1), 4), dtype=torch.float32, device="cuda") for _ in range(5)]
+    src = torch.empty(max(n, 1), dtype=torch.int32, device="cuda") if want_src else None
+    otri = torch.empty(max(n, 1), dtype=torch.int32, device="cuda") if want_src else None
+    cnt = N.u32(0)
+    N.check(N.load().mnrt_shading_points(n, N.ptr(orig), N.ptr(dirs), N.ptr(tri), N.ptr(t), N.ptr(v0), N.ptr(v1),
+                                         N.ptr(v2), N.ptr(albedo), N.ptr(infl), *[N.ptr(o) for o in outs],
+                                         N.ptr(src), N.ptr(otri), ctypes.byref(cnt), N.stream_handle()),
+            "mnrt_shading_points")
+    m = cnt.value
+    q = dict(pos=outs[0][:m], nrm=outs[1][:m], dir=outs[2][:m], clr=outs[3][:m], infl=outs[4][:m], count=m)
+    if want_src:
+        q.update(src=src[:m], tri=otri[:m])
+    return q
+
+
+def bounce_rays(q, seed: int):
+    torch = _torch()
+    n = q["pos"].shape[0]
+    o = torch.empty((max(n, 1), 4), dtype=torch.float32, device="cuda")
+    d = torch.empty((max(n, 1), 4), dtype=torch.float32, device="cuda")
+    infl = torch.empty((max(n, 1), 4), dtype=torch.float32, device="cuda")
+    N.check(N.load().mnrt_bounce_rays(n, N.ptr(q["pos"]), N.ptr(q["nrm"]), N.ptr(q["dir"]), N.ptr(q["clr"]), seed,
+                                      N.ptr(o), N.ptr(d), N.ptr(infl), N.stream_handle()), "mnrt_bounce_rays")
+    return o[:n], d[:n], infl[:n]

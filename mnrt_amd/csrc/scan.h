@@ -6,6 +6,7 @@
 #include "mnrt_common.h"
 
 namespace mnrt {
+namespace {  // one private copy per translation unit
 
 constexpr uint32_t kScanBlock = 256;
 constexpr uint32_t kScanPerThread = 8;
@@ -97,4 +98,5 @@ inline hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, 
     return hipGetLastError();
 }
 
+}  // namespace
 }  // namespace mnrt

@@ -86,7 +86,12 @@ def load():
     L.mnrt_trace_closest.argtypes = [TP, u32, vp, vp, f32, vp, vp, vp, vp, vp, vp]
     L.mnrt_knn_gather.argtypes = [TP, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.mnrt_knn_radii.argtypes = [TP, u32, f32, u32, vp, vp, vp]
-    for name in ("mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
+    L.mnrt_photons_generate.argtypes = [TP, vp, vp, vp, vp, vp, u32, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp,
+                                        ctypes.POINTER(u32), ctypes.POINTER(u32), vp]
+    L.mnrt_shading_points.argtypes = [u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                      ctypes.POINTER(u32), vp]
+    L.mnrt_bounce_rays.argtypes = [u32, vp, vp, vp, vp, u32, vp, vp, vp, vp]
+    for name in ("mnrt_photons_generate", "mnrt_shading_points", "mnrt_bounce_rays", "mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
                  "mnrt_tree_attach_photon_records", "mnrt_tree_release", "mnrt_tree_cost",
                  "mnrt_trace_closest", "mnrt_knn_gather", "mnrt_knn_radii", "mnrt_device_init"):
         getattr(L, name).restype = ctypes.c_int

@@ -987,6 +987,116 @@ bool tri_box_overlap(const double* lo, const double* hi, const double tv[3][3]) 
     return true;
 }
 
+// ---------------------------------------------------------------------------------
+// Photon tracing (RTCore::RebuildPhotonMaps, RTCore.cpp:454-638; emission
+// photon_build.cu:79, Russian roulette photon_build.cu:220-241, scattering
+// photon_build.cu:278): one path at a time, photons stored at every diffuse hit, Russian
+// roulette (p = 0.5, power x2) past bounce index 3, cosine scattering about the normal
+// facing the incoming ray.  Random numbers: an integer hash of (seed, path, bounce, draw)
+// (lowbias32 finaliser); cosine directions: Malley's method with rejection-sampled disk
+// points and the Duff et al. 2017 orthonormal basis; only +, *, / and sqrt, so any IEEE
+// fp32 implementation with the same operation order draws the same paths.
+// ---------------------------------------------------------------------------------
+inline uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+inline float rnd01(uint32_t seed, uint32_t path, uint32_t bounce, uint32_t draw) {
+    uint32_t h = hash32(seed ^ hash32(path ^ hash32(bounce * 0x9E3779B9u ^ hash32(draw))));
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+V3 cosine_dir(const V3& n, uint32_t seed, uint32_t path, uint32_t bounce) {
+    float x = 0.0f, y = 0.0f, r2 = 0.0f;
+    bool ok = false;
+    for (uint32_t t = 0; t < 32 && !ok; ++t) {
+        x = 2.0f * rnd01(seed, path, bounce, 2 * t + 1) - 1.0f;
+        y = 2.0f * rnd01(seed, path, bounce, 2 * t + 2) - 1.0f;
+        r2 = x * x + y * y;
+        ok = r2 < 1.0f;
+    }
+    if (!ok) {
+        x = 0.0f;
+        y = 0.0f;
+        r2 = 0.0f;
+    }
+    float z = std::sqrt(1.0f - r2);
+    const float nx = n.v[0], ny = n.v[1], nz = n.v[2];
+    float sign = nz >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sign + nz);
+    float b = nx * ny * a;
+    float Tx = 1.0f + sign * nx * nx * a, Ty = sign * b, Tz = -sign * nx;
+    float Bx = b, By = sign + ny * ny * a, Bz = -ny;
+    return mk(x * Tx + y * Bx + z * nx, x * Ty + y * By + z * ny, x * Tz + y * Bz + z * nz);
+}
+
+V3 unit_normal(const V3& e1, const V3& e2) {
+    V3 n = cross(e1, e2);
+    float len = std::sqrt(dot(n, n));
+    return mk(n.v[0] / len, n.v[1] / len, n.v[2] / len);
+}
+
+struct PhotonSlot {
+    bool valid = false;
+    float pos[3], pw[3], dir[3];
+    uint32_t meta;
+    uint32_t tris[4];
+};
+
+void photon_path(const uint32_t* nodes, const uint32_t* refs, const float* root, const TriSoup& S,
+                 const float* albedo, const float* lights, uint32_t num_lights, uint32_t path,
+                 uint32_t max_bounces, uint32_t seed, float eps, PhotonSlot* out) {
+    const uint32_t slots = max_bounces + 1;
+    for (uint32_t b = 0; b < slots; ++b) out[b].valid = false;
+    const uint32_t light = path % num_lights;
+    const float* L = lights + 16 * light;
+    V3 c = mk(L[0], L[1], L[2]), eu = mk(L[4], L[5], L[6]), ev = mk(L[8], L[9], L[10]);
+    float u = rnd01(seed, path, 0xFFFFu, 0), v = rnd01(seed, path, 0xFFFFu, 1);
+    V3 o = mk(c.v[0] + u * eu.v[0] + v * ev.v[0], c.v[1] + u * eu.v[1] + v * ev.v[1],
+              c.v[2] + u * eu.v[2] + v * ev.v[2]);
+    V3 ln = unit_normal(eu, ev);
+    V3 d = cosine_dir(ln, seed, path, 0xFFFEu);
+    float nl = (float)num_lights;
+    float pw[3] = {L[12] * nl, L[13] * nl, L[14] * nl};
+    uint32_t tris[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    TraceCounts cnt;
+    for (uint32_t b = 0; b < slots; ++b) {
+        int32_t tri;
+        float t, hu, hv;
+        trace_one(nodes, refs, root, S, o, d, eps, tri, t, hu, hv, cnt);
+        if (tri < 0) break;
+        const float* alb = albedo + 4 * (size_t)tri;
+        if (!(alb[0] > 0.0f || alb[1] > 0.0f || alb[2] > 0.0f)) break;
+        V3 x = mk(o.v[0] + d.v[0] * t, o.v[1] + d.v[1] * t, o.v[2] + d.v[2] * t);
+        if (b > 3) {
+            if (!(rnd01(seed, path, b, 100) <= 0.5f)) break;
+            for (int k = 0; k < 3; ++k) pw[k] = pw[k] * 2.0f;
+        }
+        PhotonSlot& ps = out[b];
+        ps.valid = true;
+        for (int k = 0; k < 3; ++k) {
+            ps.pos[k] = x.v[k];
+            ps.pw[k] = pw[k];
+            ps.dir[k] = d.v[k];
+        }
+        ps.meta = light | (b << 8);
+        for (int k = 0; k < 4; ++k) ps.tris[k] = tris[k];
+        if (b + 1 >= slots) break;
+        V3 a0 = ld4(S.v0, tri), a1 = ld4(S.v1, tri), a2 = ld4(S.v2, tri);
+        V3 n = unit_normal(sub(a1, a0), sub(a2, a0));
+        if (dot(n, d) > 0.0f) n = mk(n.v[0] * -1.0f, n.v[1] * -1.0f, n.v[2] * -1.0f);
+        d = cosine_dir(n, seed, path, b);
+        o = x;
+        for (int k = 0; k < 3; ++k) pw[k] = pw[k] * alb[k];
+        if (b < 4) tris[b] = (uint32_t)tri;
+    }
+}
+
 }  // namespace
 
 // =================================================================================
@@ -1237,6 +1347,60 @@ void orc_radiance(const uint32_t* nbr, const uint32_t* count, uint32_t k, float 
             radiance_one(I, q, nbr + (size_t)q * k, count[q], k, rmax, out32, out64, irr32, irr64,
                          r2out);
     });
+}
+
+// Photon map of `target` photons (first `target` stored photons in (path, bounce) order,
+// power scaled by 1 / paths shot up to the last kept photon).  Output arrays: float4
+// position / power / direction, meta (light | bounce << 8), path index, 4 path triangles.
+void orc_photons(const uint32_t* nodes, const uint32_t* refs, const float* root6, const float* v0,
+                 const float* v1, const float* v2, const float* albedo4, const float* lights,
+                 uint32_t num_lights, uint32_t target, uint32_t max_bounces, uint32_t seed, float eps,
+                 float* out_pos4, float* out_pow4, float* out_dir4, uint32_t* out_meta, uint32_t* out_path,
+                 uint32_t* out_tris, uint32_t* out_count, uint32_t* out_paths, int nthreads) {
+    TriSoup S{v0, v1, v2};
+    const uint32_t slots = max_bounces + 1;
+    const uint64_t max_paths = 100ull * target;
+    uint32_t stored = 0, shot = 0;
+    const uint32_t chunk = 65536;
+    std::vector<PhotonSlot> buf((size_t)chunk * slots);
+    while (stored < target && shot < max_paths) {
+        uint32_t np = (uint32_t)std::min<uint64_t>(chunk, max_paths - shot);
+        parallel_for(np, nthreads, [&](uint32_t b, uint32_t e) {
+            for (uint32_t i = b; i < e; ++i)
+                photon_path(nodes, refs, root6, S, albedo4, lights, num_lights, shot + i, max_bounces, seed, eps,
+                            &buf[(size_t)i * slots]);
+        });
+        bool done = false;
+        for (uint32_t i = 0; i < np && !done; ++i) {
+            for (uint32_t b = 0; b < slots; ++b) {
+                const PhotonSlot& ps = buf[(size_t)i * slots + b];
+                if (!ps.valid) continue;
+                for (int k = 0; k < 3; ++k) {
+                    out_pos4[4 * stored + k] = ps.pos[k];
+                    out_pow4[4 * stored + k] = ps.pw[k];
+                    out_dir4[4 * stored + k] = ps.dir[k];
+                }
+                out_pos4[4 * stored + 3] = out_pow4[4 * stored + 3] = out_dir4[4 * stored + 3] = 0.0f;
+                if (out_meta) out_meta[stored] = ps.meta;
+                if (out_path) out_path[stored] = shot + i;
+                if (out_tris)
+                    for (int k = 0; k < 4; ++k) out_tris[4 * (size_t)stored + k] = ps.tris[k];
+                if (++stored == target) {
+                    shot = shot + i + 1;
+                    done = true;
+                    break;
+                }
+            }
+        }
+        if (!done) shot += np;
+    }
+    if (stored > 0 && shot > 0) {
+        float scale = 1.0f / (float)shot;
+        for (uint32_t i = 0; i < stored; ++i)
+            for (int k = 0; k < 3; ++k) out_pow4[4 * i + k] = out_pow4[4 * i + k] * scale;
+    }
+    *out_count = stored;
+    *out_paths = shot;
 }
 
 uint32_t orc_oct_encode(float x, float y, float z) { return oct_encode(x, y, z); }

@@ -67,6 +67,9 @@ def lib():
     L.orc_oct_encode.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
     L.orc_oct_encode_n.argtypes = [f32p, ctypes.c_uint32, u32p]
     L.orc_oct_decode.argtypes = [ctypes.c_uint32, f32p]
+    L.orc_photons.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32,
+                              ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, f32p, f32p, f32p, u32p, u32p, u32p,
+                              ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
     L.orc_ray_tri.restype = ctypes.c_int
     L.orc_ray_tri.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
     _lib = L
@@ -213,3 +216,26 @@ def ray_tri(a, b, c, o, d):
     out = np.zeros(3, np.float32)
     hit = lib().orc_ray_tri(_c(a), _c(b), _c(c), _c(o), _c(d), out)
     return bool(hit), out
+
+
+def photons(tree: Tree, v0, v1, v2, albedo, lights, target, max_bounces=4, seed=0, eps=1e-4, nthreads=0):
+    """Traced photon map: dict of pos, power, dir (float4), meta, path, tris, count, paths."""
+    lights = _c(np.asarray(lights, np.float32).reshape(-1, 4, 3))
+    nl = lights.shape[0]
+    lrec = np.zeros((nl, 4, 4), np.float32)
+    lrec[:, :, :3] = lights
+    pos = np.zeros((target, 4), np.float32)
+    pw = np.zeros((target, 4), np.float32)
+    dr = np.zeros((target, 4), np.float32)
+    meta = np.zeros(target, np.uint32)
+    path = np.zeros(target, np.uint32)
+    tris = np.zeros(4 * target, np.uint32)
+    cnt = ctypes.c_uint32(0)
+    shot = ctypes.c_uint32(0)
+    refs = tree.refs if tree.refs.size else np.zeros(1, np.uint32)
+    lib().orc_photons(tree.nodes, refs, tree.root, _c(v0), _c(v1), _c(v2), _c(albedo), _c(lrec).ravel(), nl,
+                      target, max_bounces, seed, eps, pos, pw, dr, meta, path, tris, ctypes.byref(cnt),
+                      ctypes.byref(shot), _threads(nthreads))
+    n = cnt.value
+    return dict(pos=pos[:n], power=pw[:n], dir=dr[:n], meta=meta[:n], path=path[:n], tris=tris.reshape(-1, 4)[:n],
+                count=n, paths=shot.value)
