@@ -1,0 +1,46 @@
+"""The C ABI library loads without a GPU, exports every function include/mnrt_kd.h
+declares, and rejects bad arguments before touching a device."""
+
+import ctypes
+
+import pytest
+
+from mnrt_amd import native as N
+
+
+def test_exports_every_declared_symbol():
+    L = N.load()
+    names = N.exported_symbols_in_header()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_status_strings():
+    L = N.load()
+    assert L.mnrt_status_string(0) == b"ok"
+    assert L.mnrt_status_string(-1) == b"invalid argument"
+    assert L.mnrt_abi_version() == 1
+
+
+def test_argument_errors_without_device():
+    L = N.load()
+    t = N.Tree()
+    assert L.mnrt_build_tri(None, None, None, 10, None, ctypes.byref(t), None) == -1
+    p = N.BuildParams()
+    p.small_node_max = 65
+    assert L.mnrt_build_points(None, 0, ctypes.byref(p), ctypes.byref(t), None) == -1
+    assert L.mnrt_trace_closest(None, 1, None, None, 0.0, None, None, None, None, None, None) == -1
+    assert L.mnrt_tree_release(None) == -1
+    t.kind = N.KIND_POINT
+    assert L.mnrt_knn_gather(ctypes.byref(t), 0, 1.0, 1, *([None] * 12)) == -1
+    assert L.mnrt_knn_gather(ctypes.byref(t), 5, 1.0, 1, *([None] * 12)) == -6  # no nodes yet
+    t.kind = N.KIND_TRI
+    assert L.mnrt_trace_closest(ctypes.byref(t), 1, None, None, 0.0, None, None, None, None, None, None) == -1
+
+
+def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(N, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(N, "_lib", None)
+    with pytest.raises(N.MnrtError):
+        N.load()
