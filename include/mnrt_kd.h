@@ -99,6 +99,9 @@ typedef struct mnrt_tree {
     float* d_recs;       /* num_refs * (12 | 8) floats, see header comment */
     uint32_t owns_nodes; /* library allocated d_nodes/d_refs */
     uint32_t owns_recs;  /* library allocated d_recs */
+    uint32_t* d_qnodes;  /* query-time node words (NULL: d_nodes), see mnrt_tree_collapse */
+    uint32_t qleaf_max;  /* subtrees with at most this many references are leaves in d_qnodes */
+    uint32_t owns_qnodes;
 } mnrt_tree;
 
 const char* mnrt_status_string(int status);
@@ -127,6 +130,13 @@ int mnrt_tree_attach_tri_records(mnrt_tree* tree, const float* d_v0, const float
  * spherical angles in the w components (photon_dev.h:71-116). */
 int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_pos4, const float* d_pow4,
                                     const float* d_dir4, void* stream);
+
+/* Query-time node words: every inner node whose subtree holds at most max_refs leaf
+ * references becomes a leaf over that contiguous reference range (a subtree's leaves are
+ * consecutive in preorder).  A wave then tests such a subtree's elements in one step.
+ * The build's node words (d_nodes) are unchanged; queries read d_qnodes.  Query results
+ * do not depend on it.  mnrt_tree_attach_photon_records applies max_refs = 64. */
+int mnrt_tree_collapse(mnrt_tree* tree, uint32_t max_refs, void* stream);
 
 /* KDTreeGPU::Destroy() (KDTreeGPU.cpp:237-245): frees what the library allocated. */
 int mnrt_tree_release(mnrt_tree* tree);
@@ -167,6 +177,20 @@ int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                     const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,
                     float* d_io_rad4, float* d_out_r2, float* d_out_irr4, uint32_t* d_out_nbr,
                     uint32_t* d_out_count, void* stream);
+
+/* Adjoint of mnrt_knn_gather (no reference counterpart): given dLoss/dradiance per query
+ * (float4, w ignored) and the forward's r2 and neighbour counts (d_out_r2, d_out_count),
+ * recomputes every query's neighbour set (identical to the forward's) and writes
+ *   d_grad_pow4[p] = dLoss/dpower of photon p (original photon index, num_elems entries),
+ *   d_grad_clr4[q] = dLoss/d(diffuse colour rgb) of query q (optional).
+ * Per-photon sums use 64-bit fixed-point atomic adds (scale a power of two fixed from the
+ * largest weight), so results do not depend on scheduling.  d_idx_sp slots are not
+ * supported here (radiance per query). */
+int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                        const float* d_pos4, const float* d_nrm4, const float* d_dir4,
+                        const float* d_infl4, const float* d_clr4, const float* d_grad_rad4,
+                        const float* d_r2, const uint32_t* d_count, float* d_grad_pow4,
+                        float* d_grad_clr4, void* stream);
 
 /* KDTreePoint::ComputeQueryRadii (KDTreePoint.cpp:229-248): radius to the k-th nearest
  * photon, capped at r_max (r_max when fewer than k are closer). */

@@ -75,15 +75,18 @@ class DeviceTree:
                 "mnrt_tree_cost")
         return c.value
 
+    def _owns(self):
+        return self.t.owns_nodes or self.t.owns_recs or self.t.owns_qnodes
+
     def release(self):
-        if self.t.owns_nodes or self.t.owns_recs:
+        if self._owns():
             _torch().cuda.synchronize()
             N.check(N.load().mnrt_tree_release(self.ref()), "mnrt_tree_release")
         self._keep = []
 
     def __del__(self):
         try:
-            if self.t.owns_nodes or self.t.owns_recs:
+            if self._owns():
                 self.release()
         except Exception:
             pass
@@ -207,6 +210,20 @@ def knn_gather(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, clr=No
                                      N.ptr(clr), N.ptr(idx_sp), N.ptr(io_rad), N.ptr(r2), N.ptr(irr), N.ptr(nbr),
                                      N.ptr(cnt), N.stream_handle()), "mnrt_knn_gather")
     return dict(rad=io_rad, nbr=nbr.view(n, k) if nbr is not None else None, r2=r2, irr=irr, count=cnt)
+
+
+def knn_gather_bwd(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, clr, grad_rad, r2, count,
+                   want_grad_clr=True):
+    """Adjoint of knn_gather (neighbour sets recomputed): (dLoss/dpower per photon, dLoss/dcolour per query)."""
+    torch = _torch()
+    n = pos.shape[0]
+    gp = torch.empty((P.t.num_elems, 4), dtype=torch.float32, device="cuda")
+    gc = torch.empty((n, 4), dtype=torch.float32, device="cuda") if want_grad_clr else None
+    N.check(N.load().mnrt_knn_gather_bwd(P.ref(), k, r_max, n, N.ptr(dev(pos)), N.ptr(dev(nrm)), N.ptr(dev(dirs)),
+                                         N.ptr(dev(infl)), N.ptr(dev(clr) if clr is not None else None),
+                                         N.ptr(grad_rad.contiguous()), N.ptr(r2), N.ptr(count), N.ptr(gp), N.ptr(gc),
+                                         N.stream_handle()), "mnrt_knn_gather_bwd")
+    return gp, gc
 
 
 def knn_radii(P: DeviceTree, k: int, r_max: float, pos):

@@ -846,12 +846,31 @@ struct LayoutView {
     uint32_t* rbase;
 };
 
-__global__ __launch_bounds__(1024) void k_layout(uint32_t nbn, uint32_t maxlvl, BnView B, const uint32_t* __restrict__ s_nsize,
+// Build nodes bucketed by level (the order inside a level does not matter).
+__global__ __launch_bounds__(256) void k_level_hist(uint32_t nbn, BnView B, uint32_t* __restrict__ hist) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbn || B.type[i] >= BN_PENDING) return;
+    atomicAdd(&hist[B.level[i]], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_level_scatter(uint32_t nbn, BnView B, uint32_t* __restrict__ cursor,
+                                                       uint32_t* __restrict__ list) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbn || B.type[i] >= BN_PENDING) return;
+    list[atomicAdd(&cursor[B.level[i]], 1u)] = i;
+}
+
+// Subtree sizes bottom-up then preorder addresses top-down, one workgroup walking the
+// level buckets (KDTreeGPU::PreorderTraversal launches two kernels per level over all
+// nodes, KDTreeGPU.cpp:1156-1209).
+__global__ __launch_bounds__(1024) void k_layout(uint32_t maxlvl, const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ list, BnView B,
+                                                 const uint32_t* __restrict__ s_nsize,
                                                  const uint32_t* __restrict__ s_rsize, LayoutView Lv,
                                                  uint32_t* __restrict__ counters) {
     for (int lvl = (int)maxlvl; lvl >= 0; --lvl) {
-        for (uint32_t i = threadIdx.x; i < nbn; i += blockDim.x) {
-            if (B.level[i] != (uint32_t)lvl) continue;
+        for (uint32_t j = off[lvl] + threadIdx.x; j < off[lvl + 1]; j += blockDim.x) {
+            uint32_t i = list[j];
             uint32_t t = B.type[i];
             if (t <= 2) {
                 uint32_t l = B.left[i], r = B.right[i];
@@ -875,8 +894,9 @@ __global__ __launch_bounds__(1024) void k_layout(uint32_t nbn, uint32_t maxlvl, 
     }
     __syncthreads();
     for (uint32_t lvl = 0; lvl <= maxlvl; ++lvl) {
-        for (uint32_t i = threadIdx.x; i < nbn; i += blockDim.x) {
-            if (B.level[i] != lvl || B.type[i] > 2) continue;
+        for (uint32_t j = off[lvl] + threadIdx.x; j < off[lvl + 1]; j += blockDim.x) {
+            uint32_t i = list[j];
+            if (B.type[i] > 2) continue;
             uint32_t l = B.left[i], r = B.right[i];
             uint32_t a = Lv.addr[i], rb = Lv.rbase[i];
             Lv.addr[l] = a + 1;
@@ -1144,6 +1164,38 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
     }
 
     // ---- large-node stage ----
+    // Workspaces grow geometrically and are reused across levels; next-level arrays are
+    // sized by the upper bound (every element / node goes to at most two children), so
+    // the host waits on the device once per level, to read the level's counters.
+    uint32_t lv_na_cap = 0, lv_ne_cap = 0;
+    void* lv_buf = nullptr;
+    uint32_t *cc = nullptr, *chn = nullptr, *chf = nullptr, *chc = nullptr, *fl = nullptr, *pos = nullptr,
+             *cbuf = nullptr, *scr = nullptr;
+    PlanView P{};
+    ElemView Nx{};
+    uint32_t nx_cap = 0;
+    NodeView An{};
+    uint32_t an_cap = 0;
+    auto release_nodes = [&](NodeView& V) {
+        pool.release(V.lo);
+        pool.release(V.hi);
+        pool.release(V.first);
+        pool.release(V.cnt);
+        pool.release(V.level);
+        pool.release(V.bn);
+        V = NodeView{};
+    };
+    auto alloc_nodes = [&](uint32_t cap) {
+        NodeView V{};
+        V.lo = pool.get<float4>(cap);
+        V.hi = pool.get<float4>(cap);
+        V.first = pool.get<uint32_t>(cap);
+        V.cnt = pool.get<uint32_t>(cap);
+        V.level = pool.get<uint32_t>(cap);
+        V.bn = pool.get<uint32_t>(cap);
+        return V;
+    };
+    uint32_t e_cap = n, a_cap = 1;
     while (na > 0) {
         // capacity for this level's new build nodes / small roots / leaf copies
         if (nbn + 14ull * na > bn_cap) {
@@ -1186,33 +1238,71 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
             ll_idx = nl;
             ll_cap = ncap;
         }
-
-        // per-level temporaries
-        uint32_t max_chunks = ne / kChunk + na;
-        Pool lv(st);
-        uint32_t* cc = lv.get<uint32_t>(na);
-        uint32_t* chn = lv.get<uint32_t>(max_chunks);
-        uint32_t* chf = lv.get<uint32_t>(max_chunks);
-        uint32_t* chc = lv.get<uint32_t>(max_chunks);
-        tlo = lv.get<uint32_t>(3ull * na);
-        thi = lv.get<uint32_t>(3ull * na);
-        PlanView P{};
-        P.newcnt = lv.get<uint32_t>(na);
-        P.axis = lv.get<uint32_t>(na);
-        P.split = lv.get<float>(na);
-        P.lo = lv.get<float4>(na);
-        P.hi = lv.get<float4>(na);
-        P.level = lv.get<uint32_t>(na);
-        P.childL = lv.get<uint32_t>(na);
-        P.cur = lv.get<uint32_t>(na);
-        P.leaf = lv.get<uint32_t>(na);
-        P.noprog = lv.get<uint32_t>(na);
-        uint32_t* fl = lv.get<uint32_t>(2ull * ne + 1);
-        uint32_t* pos = lv.get<uint32_t>(2ull * ne + 1);
+        if ((uint64_t)nse + 2ull * ne > se_cap) {
+            uint32_t ncap = (uint32_t)std::max<uint64_t>(2ull * se_cap, (uint64_t)nse + 2ull * ne + 1024);
+            ElemView NS = alloc_elems(ncap, false);
+            if (!pool.ok) return MNRT_ERR_NOMEM;
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.idx, Sm.idx, (size_t)nse * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.lo, Sm.lo, (size_t)nse * 16, hipMemcpyDeviceToDevice, st));
+            if (kind == MNRT_KIND_TRI)
+                MNRT_HIP_CHECK(hipMemcpyAsync(NS.hi, Sm.hi, (size_t)nse * 16, hipMemcpyDeviceToDevice, st));
+            release_elems(Sm);
+            Sm = NS;
+            se_cap = ncap;
+        }
+        if (2ull * ne > nx_cap) {
+            release_elems(Nx);
+            nx_cap = (uint32_t)std::max<uint64_t>(2ull * ne, 1024);
+            Nx = alloc_elems(nx_cap, true);
+        }
+        if (2ull * na > an_cap) {
+            release_nodes(An);
+            an_cap = std::max<uint32_t>(2 * na, 256);
+            An = alloc_nodes(an_cap);
+        }
+        // per-level temporaries, one buffer
+        const uint32_t max_chunks = ne / kChunk + na;
+        if (na > lv_na_cap || ne > lv_ne_cap) {
+            pool.release(lv_buf);
+            lv_na_cap = std::max<uint32_t>(na, lv_na_cap + lv_na_cap / 2);
+            lv_ne_cap = std::max<uint32_t>(ne, lv_ne_cap + lv_ne_cap / 2);
+            const uint64_t A_ = lv_na_cap, E_ = lv_ne_cap, C_ = E_ / kChunk + A_;
+            const uint64_t words = A_ /*cc*/ + 3 * C_ /*chunks*/ + 6 * A_ /*tight*/ + 14 * A_ /*plan*/ +
+                                   2 * (2 * E_ + 1) /*fl, pos*/ + 22 * A_ /*child*/ +
+                                   scan_scratch_words((uint32_t)std::max<uint64_t>(2 * E_ + 1, 2 * A_)) + 64;
+            lv_buf = pool.get<uint32_t>(words);
+            if (!pool.ok) return MNRT_ERR_NOMEM;
+        }
+        {
+            uint32_t* w = (uint32_t*)lv_buf;
+            const uint64_t A_ = lv_na_cap, E_ = lv_ne_cap, C_ = E_ / kChunk + A_;
+            auto take = [&](uint64_t k) {
+                uint32_t* r = w;
+                w += (k + 3) & ~3ull;  // keep 16-byte alignment for float4 columns
+                return r;
+            };
+            cc = take(A_);
+            chn = take(C_);
+            chf = take(C_);
+            chc = take(C_);
+            tlo = take(3 * A_);
+            thi = take(3 * A_);
+            P.newcnt = take(A_);
+            P.axis = take(A_);
+            P.split = (float*)take(A_);
+            P.lo = (float4*)take(4 * A_);
+            P.hi = (float4*)take(4 * A_);
+            P.level = take(A_);
+            P.childL = take(A_);
+            P.cur = take(A_);
+            P.leaf = take(A_);
+            P.noprog = take(A_);
+            fl = take(2 * E_ + 1);
+            pos = take(2 * E_ + 1);
+            cbuf = take(22 * A_);
+            scr = w;
+        }
         ChildView Cv{};
-        uint32_t* cbuf = lv.get<uint32_t>(11ull * 2 * na);
-        uint32_t* scr = lv.get<uint32_t>(scan_scratch_words(std::max<uint64_t>(2ull * ne + 1, 2ull * na)));
-        if (!lv.ok) return MNRT_ERR_NOMEM;
         {
             uint32_t m = 2 * na;
             Cv.type = cbuf;
@@ -1252,59 +1342,32 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
         MNRT_HIP_CHECK(scan_exclusive(Cv.lflag, Cv.lidx, 2 * na, counters + C_NNEXT, scr, st));
         MNRT_HIP_CHECK(scan_exclusive(Cv.scnt, Cv.soff, 2 * na, counters + C_ESMALL_NEW, scr, st));
         MNRT_HIP_CHECK(scan_exclusive(Cv.lcnt, Cv.loff, 2 * na, counters + C_ENEXT, scr, st));
-        int rc = sync_counters();
-        if (rc != MNRT_OK) return rc;
-        const uint32_t n_next = h_counters[C_NNEXT], e_next = h_counters[C_ENEXT];
-        const uint32_t n_small_new = h_counters[C_NSMALL_NEW], e_small_new = h_counters[C_ESMALL_NEW];
-        if ((uint64_t)nse + e_small_new > se_cap) {
-            uint32_t ncap = (uint32_t)std::max<uint64_t>(2ull * se_cap, (uint64_t)nse + e_small_new + 1024);
-            ElemView NS = alloc_elems(ncap, false);
-            if (!pool.ok) return MNRT_ERR_NOMEM;
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.idx, Sm.idx, (size_t)nse * 4, hipMemcpyDeviceToDevice, st));
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.lo, Sm.lo, (size_t)nse * 16, hipMemcpyDeviceToDevice, st));
-            if (kind == MNRT_KIND_TRI)
-                MNRT_HIP_CHECK(hipMemcpyAsync(NS.hi, Sm.hi, (size_t)nse * 16, hipMemcpyDeviceToDevice, st));
-            release_elems(Sm);
-            Sm = NS;
-            se_cap = ncap;
-        }
-        ElemView Nx = alloc_elems(std::max<uint32_t>(e_next, 1), true);
-        NodeView An{};
-        An.lo = pool.get<float4>(std::max<uint32_t>(n_next, 1));
-        An.hi = pool.get<float4>(std::max<uint32_t>(n_next, 1));
-        An.first = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
-        An.cnt = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
-        An.level = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
-        An.bn = pool.get<uint32_t>(std::max<uint32_t>(n_next, 1));
-        if (!pool.ok) return MNRT_ERR_NOMEM;
         hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, na, P, Cv, nsmall, nse, S, An, B,
                            counters);
         hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, na, A, P, E, ll_idx, B, counters);
         hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, na, ne, E, G, P, Cv, pos, nse, Sm, se_cap, Nx,
-                           e_next, counters);
+                           nx_cap, counters);
         MNRT_HIP_CHECK(hipGetLastError());
-        nbn += h_counters[C_NEWNODES];
-        nsmall += n_small_new;
-        nse += e_small_new;
-        // next level
-        release_elems(E);
-        pool.release(A.lo);
-        pool.release(A.hi);
-        pool.release(A.first);
-        pool.release(A.cnt);
-        pool.release(A.level);
-        pool.release(A.bn);
-        E = Nx;
-        A = An;
-        ne = e_next;
-        na = n_next;
-        // the level-cap / no-progress leaves of this level were counted by k_leaf_copy
-        rc = sync_counters();
+        int rc = sync_counters();
         if (rc != MNRT_OK) return rc;
+        if (h_counters[C_ERR] != 0) return MNRT_ERR_STATE;
+        const uint32_t n_next = h_counters[C_NNEXT], e_next = h_counters[C_ENEXT];
+        nbn += h_counters[C_NEWNODES];
+        nsmall += h_counters[C_NSMALL_NEW];
+        nse += h_counters[C_ESMALL_NEW];
         ll_count = h_counters[C_LLCOUNT];
         maxlvl = h_counters[C_MAXLEVEL];
+        // next level: swap the ping-pong arrays
+        std::swap(E, Nx);
+        std::swap(e_cap, nx_cap);
+        std::swap(A, An);
+        std::swap(a_cap, an_cap);
+        ne = e_next;
+        na = n_next;
     }
     release_elems(E);
+    release_elems(Nx);
+    pool.release(lv_buf);
 
     // ---- small-node stage, pass 1: subtree sizes ----
     uint32_t* s_nsize = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
@@ -1324,7 +1387,23 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
     Lv.addr = pool.get<uint32_t>(nbn);
     Lv.rbase = pool.get<uint32_t>(nbn);
     if (!pool.ok) return MNRT_ERR_NOMEM;
-    hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, st, nbn, maxlvl, B, s_nsize, s_rsize, Lv, counters);
+    {
+        // level buckets: 64 counts, their exclusive scan (+ total at [64]), node lists
+        uint32_t* lvb = pool.get<uint32_t>(64 + 65 + 64 + nbn + scan_scratch_words(64));
+        if (!pool.ok) return MNRT_ERR_NOMEM;
+        uint32_t* hist = lvb;
+        uint32_t* off = lvb + 64;
+        uint32_t* cursor = off + 65;
+        uint32_t* list = cursor + 64;
+        uint32_t* sc = list + nbn;
+        MNRT_HIP_CHECK(hipMemsetAsync(hist, 0, 64 * 4, st));
+        hipLaunchKernelGGL(k_level_hist, blocks_for(nbn), dim3(256), 0, st, nbn, B, hist);
+        MNRT_HIP_CHECK(scan_exclusive(hist, off, 64, off + 64, sc, st));
+        MNRT_HIP_CHECK(hipMemcpyAsync(cursor, off, 64 * 4, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_level_scatter, blocks_for(nbn), dim3(256), 0, st, nbn, B, cursor, list);
+        hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, st, std::min<uint32_t>(maxlvl, 63), off, list, B,
+                           s_nsize, s_rsize, Lv, counters);
+    }
     int rc = sync_counters();
     if (rc != MNRT_OK) return rc;
     const uint32_t total_nodes = h_counters[C_TOTAL_NODES], total_refs = h_counters[C_TOTAL_REFS];

@@ -7,8 +7,11 @@
 //
 // Design (one wave64 per query):
 //   * the point tree is walked by the whole wave with wave-uniform control flow (node
-//     words are scalar loads); far children go onto a per-wave LDS stack with their
-//     squared plane distance, as dev_RangeSearch pushes them (photon_knn.cu:146-171);
+//     words are scalar loads) over the query-time node words, in which every subtree of
+//     at most 64 photons is one leaf; far children go onto a per-wave LDS stack with
+//     their box distance (dev_RangeSearch pushes them by plane distance,
+//     photon_knn.cu:146-171).  Rounding is monotone, so the fp32 box distance never
+//     exceeds the fp32 distance of a photon inside the box and pruning by it is exact;
 //   * a leaf's photon records are contiguous, so its photons are tested one per lane
 //     with one coalesced 16-byte load each;
 //   * candidates (key = d2 bits << 32 | photon index, so integer order is (d2, index)
@@ -16,6 +19,9 @@
 //     k are known or the buffer would overflow, a wave-wide bitonic sort keeps the k
 //     best and fixes the pruning threshold;
 //   * the estimate sums the front-facing powers of the final set in (d2, index) order.
+#include <cmath>
+#include <cstring>
+
 #include "mnrt_common.h"
 
 namespace mnrt {
@@ -42,7 +48,14 @@ struct KnnArgs {
     uint32_t* out_nbr;
     uint32_t* out_count;
     float* out_radius;
+    // backward
+    const float4* grad;            // dLoss/dradiance per query
+    unsigned long long* acc;       // 3 fixed-point accumulators per photon (original index)
+    double fix_scale;              // accumulator units per unit gradient (a power of two)
+    float4* out_grad_clr;          // dLoss/d(diffuse colour) per query
 };
+
+enum : int { kModeRadii = 0, kModeRadiance = 1, kModeBackward = 2 };
 
 // Sorts keys[0..size) ascending (positions follow their keys), size a power of two.
 template <int CAP>
@@ -73,12 +86,12 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t v) {
     return p;
 }
 
-template <int CAP, bool kRadiance>
+template <int CAP, int kMode>
 __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
     __shared__ uint64_t s_key[kWavesPerBlock][CAP];
     __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
     __shared__ uint32_t s_snode[kWavesPerBlock][kKnnStack];
-    __shared__ float s_spd2[kWavesPerBlock][kKnnStack];
+    __shared__ float4 s_soff[kWavesPerBlock][kKnnStack];
 
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63u;
@@ -87,7 +100,7 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
     uint64_t* keys = s_key[wave];
     uint32_t* kpos = s_pos[wave];
     uint32_t* snode = s_snode[wave];
-    float* spd2 = s_spd2[wave];
+    float4* soff = s_soff[wave];
 
     const float4 qp4 = A.pos[q];
     const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
@@ -101,6 +114,8 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
     float thr_d2 = kInf;
     int sp = 0;
     uint32_t node = 0;
+    // per-axis distance from the query to the current node's box (0 inside)
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
 
     auto select = [&](uint32_t keep) {
         uint32_t size = pow2_at_least(cnt);
@@ -127,17 +142,19 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
             float s = __uint_as_float(nd.x);
             float qa = comp(qp, a);
             float diff = qa - s;
-            float pd2 = diff * diff;
             uint32_t left = node + 1, right = nd.y & MNRT_INDEX_MASK;
             uint32_t nearc = left, farc = right;
             if (qa > s) {
                 nearc = right;
                 farc = left;
             }
-            if (pd2 < r2max && sp < kKnnStack) {
+            // far child's box distance: the split plane replaces this axis' offset
+            float fx = a == 0 ? diff : ox, fy = a == 1 ? diff : oy, fz = a == 2 ? diff : oz;
+            float bd2 = fx * fx + fy * fy + fz * fz;
+            if (bd2 < r2max && !(full && bd2 > thr_d2) && sp < kKnnStack) {
                 if (lane == 0) {
                     snode[sp] = farc;
-                    spd2[sp] = pd2;
+                    soff[sp] = make_float4(fx, fy, fz, bd2);
                 }
                 ++sp;
             }
@@ -180,9 +197,12 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
         bool found = false;
         while (sp > 0) {
             --sp;
-            float pd2 = spd2[sp];
-            if (full && pd2 > thr_d2) continue;
+            float4 f = soff[sp];
+            if (full && f.w > thr_d2) continue;
             node = snode[sp];
+            ox = f.x;
+            oy = f.y;
+            oz = f.z;
             found = true;
             break;
         }
@@ -200,7 +220,7 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
         for (uint32_t i = lane; i < k; i += 64)
             A.out_nbr[(size_t)q * k + i] = (i < found) ? (uint32_t)(keys[i] & 0xFFFFFFFFull) : 0xFFFFFFFFu;
     }
-    if (!kRadiance) {
+    if (kMode == kModeRadii) {
         if (A.out_radius && lane == 0) A.out_radius[q] = (found >= k && k > 0) ? sqrtf(r2) : A.r_max;
         if (A.out_r2 && lane == 0) A.out_r2[q] = r2;
         return;
@@ -236,6 +256,39 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
         if (found < 8) ix = iy = iz = 0.0f;
     }
     float4 in4 = A.infl[q];
+    if (kMode == kModeBackward) {
+        // Adjoint with the neighbour set fixed: L_c = ((alpha rho_c / pi) S_c / (pi r2)) infl_c,
+        // so dLoss/dS_c = G_c (alpha rho_c / pi) infl_c / (pi r2) for every front-facing
+        // neighbour's power, and dLoss/drho_c = G_c (alpha / pi) E_c infl_c.
+        float4 g = A.grad[q];
+        bool gated = (A.clr && found < 8) || !(r2 > 0.0f);
+        float den = r2 * kPi;
+        float wx = gated ? 0.0f : (g.x * (((c4.w * c4.x) * kInvPi) * in4.x)) / den;
+        float wy = gated ? 0.0f : (g.y * (((c4.w * c4.y) * kInvPi) * in4.y)) / den;
+        float wz = gated ? 0.0f : (g.z * (((c4.w * c4.z) * kInvPi) * in4.z)) / den;
+        if (A.out_grad_clr && lane == 0)
+            A.out_grad_clr[q] = make_float4(g.x * (((c4.w * kInvPi) * ix) * in4.x),
+                                            g.y * (((c4.w * kInvPi) * iy) * in4.y),
+                                            g.z * (((c4.w * kInvPi) * iz) * in4.z), 0.0f);
+        // fixed-point adds are associative, so the per-photon sums do not depend on the
+        // order in which queries arrive
+        long long fx = __double2ll_rn((double)wx * A.fix_scale);
+        long long fy = __double2ll_rn((double)wy * A.fix_scale);
+        long long fz = __double2ll_rn((double)wz * A.fix_scale);
+        if (fx != 0 || fy != 0 || fz != 0) {
+            for (uint32_t i = lane; i < found; i += 64) {
+                float4 pw = A.recs[2ull * kpos[i] + 1];
+                f3 pd = oct_decode(__float_as_uint(pw.w));
+                if (dot3(nq, pd) < 0.0f) {
+                    size_t p = (size_t)(keys[i] & 0xFFFFFFFFull);
+                    atomicAdd(&A.acc[3 * p + 0], (unsigned long long)fx);
+                    atomicAdd(&A.acc[3 * p + 1], (unsigned long long)fy);
+                    atomicAdd(&A.acc[3 * p + 2], (unsigned long long)fz);
+                }
+            }
+        }
+        return;
+    }
     float rx = (((c4.w * c4.x) * kInvPi) * ix) * in4.x;
     float ry = (((c4.w * c4.y) * kInvPi) * iy) * in4.y;
     float rz = (((c4.w * c4.z) * kInvPi) * iz) * in4.z;
@@ -274,16 +327,51 @@ __global__ __launch_bounds__(256) void k_scatter_runs(const float4* __restrict__
     atomicAdd(&io_rad[slot].z, z);
 }
 
-template <bool kRadiance>
+// Bound on |weight| per query for the fixed-point scale of the backward pass, from the
+// forward's r2 and neighbour counts (same expressions as the backward kernel).
+__global__ __launch_bounds__(256) void k_bwd_wmax(uint32_t n, const float4* __restrict__ grad,
+                                                  const float4* __restrict__ clr, const float4* __restrict__ infl,
+                                                  const float* __restrict__ r2, const uint32_t* __restrict__ count,
+                                                  uint32_t* __restrict__ wmax_bits) {
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    float m = 0.0f;
+    if (q < n) {
+        float4 g = grad[q], in4 = infl[q];
+        float4 c4 = clr ? clr[q] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        float rr = r2[q];
+        bool gated = (clr && count[q] < 8) || !(rr > 0.0f);
+        if (!gated) {
+            float den = rr * kPi;
+            float wx = (g.x * (((c4.w * c4.x) * kInvPi) * in4.x)) / den;
+            float wy = (g.y * (((c4.w * c4.y) * kInvPi) * in4.y)) / den;
+            float wz = (g.z * (((c4.w * c4.z) * kInvPi) * in4.z)) / den;
+            m = fmaxf(fabsf(wx), fmaxf(fabsf(wy), fabsf(wz)));
+            if (!(m <= kInf)) m = kInf;  // NaN / inf gradients saturate the bound
+        }
+    }
+    for (uint32_t o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63u) == 0 && m > 0.0f) atomicMax(wmax_bits, __float_as_uint(m));
+}
+
+__global__ __launch_bounds__(256) void k_fix_to_float(uint32_t np, const unsigned long long* __restrict__ acc,
+                                                      double inv_scale, float4* __restrict__ out) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    out[p] = make_float4((float)((double)(long long)acc[3ull * p] * inv_scale),
+                         (float)((double)(long long)acc[3ull * p + 1] * inv_scale),
+                         (float)((double)(long long)acc[3ull * p + 2] * inv_scale), 0.0f);
+}
+
+template <int kMode>
 int launch_knn(const KnnArgs& A, hipStream_t st) {
     dim3 block(64 * kWavesPerBlock), grid((A.n + kWavesPerBlock - 1) / kWavesPerBlock);
     uint32_t need = A.k + 64;
     if (need <= 128)
-        hipLaunchKernelGGL((k_knn<128, kRadiance>), grid, block, 0, st, A);
+        hipLaunchKernelGGL((k_knn<128, kMode>), grid, block, 0, st, A);
     else if (need <= 256)
-        hipLaunchKernelGGL((k_knn<256, kRadiance>), grid, block, 0, st, A);
+        hipLaunchKernelGGL((k_knn<256, kMode>), grid, block, 0, st, A);
     else
-        hipLaunchKernelGGL((k_knn<512, kRadiance>), grid, block, 0, st, A);
+        hipLaunchKernelGGL((k_knn<512, kMode>), grid, block, 0, st, A);
     MNRT_HIP_CHECK(hipGetLastError());
     return MNRT_OK;
 }
@@ -310,7 +398,7 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
     if (!d_pos4 || !d_nrm4 || !d_dir4 || !d_infl4 || !d_io_rad4 || !(r_max > 0.0f)) return MNRT_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     mnrt::KnnArgs A{};
-    A.nodes = (const uint2*)pmap->d_nodes;
+    A.nodes = (const uint2*)(pmap->d_qnodes ? pmap->d_qnodes : pmap->d_nodes);
     A.recs = (const float4*)pmap->d_recs;
     A.k = k;
     A.r2max = r_max * r_max;
@@ -334,7 +422,7 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
         A.rad_out = (float4*)d_io_rad4;
         A.accumulate = 1;
     }
-    rc = mnrt::launch_knn<true>(A, st);
+    rc = mnrt::launch_knn<mnrt::kModeRadiance>(A, st);
     if (rc == MNRT_OK && d_idx_sp) {
         hipLaunchKernelGGL(mnrt::k_scatter_runs, dim3((n + 255) / 256), dim3(256), 0, st, tmp, d_idx_sp,
                            n, (float4*)d_io_rad4);
@@ -351,7 +439,7 @@ extern "C" int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, ui
     if (n == 0) return MNRT_OK;
     if (!d_pos4 || !d_out_radius || !(r_max > 0.0f)) return MNRT_ERR_ARG;
     mnrt::KnnArgs A{};
-    A.nodes = (const uint2*)pmap->d_nodes;
+    A.nodes = (const uint2*)(pmap->d_qnodes ? pmap->d_qnodes : pmap->d_nodes);
     A.recs = (const float4*)pmap->d_recs;
     A.k = k;
     A.r2max = r_max * r_max;
@@ -359,5 +447,93 @@ extern "C" int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, ui
     A.n = n;
     A.pos = (const float4*)d_pos4;
     A.out_radius = d_out_radius;
-    return mnrt::launch_knn<false>(A, (hipStream_t)stream);
+    return mnrt::launch_knn<mnrt::kModeRadii>(A, (hipStream_t)stream);
+}
+
+extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                                   const float* d_pos4, const float* d_nrm4, const float* d_dir4,
+                                   const float* d_infl4, const float* d_clr4, const float* d_grad_rad4,
+                                   const float* d_r2, const uint32_t* d_count, float* d_grad_pow4,
+                                   float* d_grad_clr4, void* stream) {
+    int rc = knn_check(pmap, k);
+    if (rc != MNRT_OK) return rc;
+    if (!d_grad_pow4) return MNRT_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t np = pmap->num_elems;
+    if (n == 0 || np == 0) {
+        if (np) MNRT_HIP_CHECK(hipMemsetAsync(d_grad_pow4, 0, (size_t)np * 16, st));
+        return MNRT_OK;
+    }
+    if (!d_pos4 || !d_nrm4 || !d_dir4 || !d_infl4 || !d_grad_rad4 || !d_r2 || !d_count || !(r_max > 0.0f))
+        return MNRT_ERR_ARG;
+    void* buf = nullptr;
+    MNRT_HIP_CHECK(hipMallocAsync(&buf, (size_t)np * 24 + 256, st));
+    unsigned long long* acc = (unsigned long long*)buf;
+    uint32_t* wmax = (uint32_t*)((char*)buf + (size_t)np * 24);
+    rc = MNRT_OK;
+    uint32_t h_wmax = 0;
+    if (hipMemsetAsync(buf, 0, (size_t)np * 24 + 4, st) != hipSuccess) rc = MNRT_ERR_HIP;
+    if (rc == MNRT_OK) {
+        hipLaunchKernelGGL(mnrt::k_bwd_wmax, dim3((n + 255) / 256), dim3(256), 0, st, n, (const float4*)d_grad_rad4,
+                           (const float4*)d_clr4, (const float4*)d_infl4, d_r2, d_count, wmax);
+        if (hipMemcpyAsync(&h_wmax, wmax, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = MNRT_ERR_HIP;
+    }
+    float wm = 0.0f;
+    std::memcpy(&wm, &h_wmax, 4);
+    double scale = 0.0;
+    if (rc == MNRT_OK && wm > 0.0f && wm <= mnrt::kInf) {
+        // |sum| <= n * wmax * scale < 2^62
+        int e = 0;
+        std::frexp(4.611686018427387904e18 / ((double)n * (double)wm), &e);
+        scale = std::ldexp(1.0, e - 1);
+    } else if (rc == MNRT_OK && wm > mnrt::kInf) {
+        rc = MNRT_ERR_ARG;  // non-finite upstream gradient
+    }
+    if (rc == MNRT_OK && scale > 0.0) {
+        mnrt::KnnArgs A{};
+        A.nodes = (const uint2*)(pmap->d_qnodes ? pmap->d_qnodes : pmap->d_nodes);
+        A.recs = (const float4*)pmap->d_recs;
+        A.k = k;
+        A.r2max = r_max * r_max;
+        A.r_max = r_max;
+        A.n = n;
+        A.pos = (const float4*)d_pos4;
+        A.nrm = (const float4*)d_nrm4;
+        A.dir = (const float4*)d_dir4;
+        A.infl = (const float4*)d_infl4;
+        A.clr = (const float4*)d_clr4;
+        A.grad = (const float4*)d_grad_rad4;
+        A.acc = acc;
+        A.fix_scale = scale;
+        A.out_grad_clr = (float4*)d_grad_clr4;
+        rc = mnrt::launch_knn<mnrt::kModeBackward>(A, st);
+    } else if (rc == MNRT_OK && d_grad_clr4) {
+        // all weights zero: dLoss/drho still needs the recomputed irradiance
+        mnrt::KnnArgs A{};
+        A.nodes = (const uint2*)(pmap->d_qnodes ? pmap->d_qnodes : pmap->d_nodes);
+        A.recs = (const float4*)pmap->d_recs;
+        A.k = k;
+        A.r2max = r_max * r_max;
+        A.r_max = r_max;
+        A.n = n;
+        A.pos = (const float4*)d_pos4;
+        A.nrm = (const float4*)d_nrm4;
+        A.dir = (const float4*)d_dir4;
+        A.infl = (const float4*)d_infl4;
+        A.clr = (const float4*)d_clr4;
+        A.grad = (const float4*)d_grad_rad4;
+        A.acc = acc;
+        A.fix_scale = 0.0;
+        A.out_grad_clr = (float4*)d_grad_clr4;
+        rc = mnrt::launch_knn<mnrt::kModeBackward>(A, st);
+    }
+    if (rc == MNRT_OK) {
+        hipLaunchKernelGGL(mnrt::k_fix_to_float, dim3((np + 255) / 256), dim3(256), 0, st, np, acc,
+                           scale > 0.0 ? 1.0 / scale : 0.0, (float4*)d_grad_pow4);
+        if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
+    }
+    (void)hipFreeAsync(buf, st);
+    return rc;
 }

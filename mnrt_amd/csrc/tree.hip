@@ -38,7 +38,54 @@ __global__ __launch_bounds__(256) void k_photon_records(const uint32_t* __restri
     recs[2ull * r + 1] = make_float4(w.x, w.y, w.z, __uint_as_float(oct_encode(d.x, d.y, d.z)));
 }
 
+// One lane per node: the subtree's reference range runs from its leftmost leaf's first
+// reference to the end of its rightmost leaf.
+__global__ __launch_bounds__(256) void k_collapse(const uint2* __restrict__ nodes, uint32_t num_nodes,
+                                                  uint32_t max_refs, uint2* __restrict__ qnodes) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_nodes) return;
+    uint2 nd = nodes[i];
+    if (is_leaf(nd.y)) {
+        qnodes[i] = nd;
+        return;
+    }
+    uint32_t l = i;
+    uint2 a = nd;
+    for (uint32_t s = 0; s <= MNRT_MAX_LEVEL + 1 && !is_leaf(a.y); ++s) {
+        l = l + 1;
+        a = nodes[l];
+    }
+    uint32_t r = i;
+    uint2 b = nd;
+    for (uint32_t s = 0; s <= MNRT_MAX_LEVEL + 1 && !is_leaf(b.y); ++s) {
+        r = b.y & MNRT_INDEX_MASK;
+        b = nodes[r];
+    }
+    uint32_t first = a.y & MNRT_INDEX_MASK;
+    uint32_t end = (b.y & MNRT_INDEX_MASK) + b.x;
+    if (is_leaf(a.y) && is_leaf(b.y) && end >= first && end - first <= max_refs)
+        qnodes[i] = make_uint2(end - first, (MNRT_LEAF_TAG << 30) | first);
+    else
+        qnodes[i] = nd;
+}
+
 }  // namespace mnrt
+
+extern "C" int mnrt_tree_collapse(mnrt_tree* tree, uint32_t max_refs, void* stream) {
+    if (!tree || !tree->d_nodes || tree->num_nodes == 0) return MNRT_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    if (!tree->d_qnodes) {
+        void* p = nullptr;
+        if (hipMalloc(&p, (size_t)tree->num_nodes * 8) != hipSuccess) return MNRT_ERR_NOMEM;
+        tree->d_qnodes = (uint32_t*)p;
+        tree->owns_qnodes = 1;
+    }
+    hipLaunchKernelGGL(mnrt::k_collapse, dim3((tree->num_nodes + 255) / 256), dim3(256), 0, st,
+                       (const uint2*)tree->d_nodes, tree->num_nodes, max_refs, (uint2*)tree->d_qnodes);
+    MNRT_HIP_CHECK(hipGetLastError());
+    tree->qleaf_max = max_refs;
+    return MNRT_OK;
+}
 
 static int ensure_recs(mnrt_tree* t, size_t floats_per_ref, hipStream_t st) {
     if (t->d_recs) return MNRT_OK;
@@ -81,7 +128,7 @@ extern "C" int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_p
                        tree->d_refs, tree->num_refs, (const float4*)d_pos4, (const float4*)d_pow4,
                        (const float4*)d_dir4, (float4*)tree->d_recs);
     MNRT_HIP_CHECK(hipGetLastError());
-    return MNRT_OK;
+    return mnrt_tree_collapse(tree, 64, stream);
 }
 
 extern "C" int mnrt_tree_release(mnrt_tree* tree) {
@@ -92,6 +139,9 @@ extern "C" int mnrt_tree_release(mnrt_tree* tree) {
         if (tree->d_nodes && hipFree(tree->d_nodes) != hipSuccess) rc = MNRT_ERR_HIP;
         if (tree->d_refs && hipFree(tree->d_refs) != hipSuccess) rc = MNRT_ERR_HIP;
     }
+    if (tree->owns_qnodes && tree->d_qnodes && hipFree(tree->d_qnodes) != hipSuccess) rc = MNRT_ERR_HIP;
+    if (tree->owns_qnodes) tree->d_qnodes = nullptr;
+    tree->owns_qnodes = 0;
     if (tree->owns_recs) tree->d_recs = nullptr;
     if (tree->owns_nodes) {
         tree->d_nodes = nullptr;

@@ -54,6 +54,9 @@ class Tree(ctypes.Structure):
         ("d_recs", vp),
         ("owns_nodes", u32),
         ("owns_recs", u32),
+        ("d_qnodes", vp),
+        ("qleaf_max", u32),
+        ("owns_qnodes", u32),
     ]
 
 
@@ -82,16 +85,18 @@ def load():
     L.mnrt_tree_attach_tri_records.argtypes = [TP, vp, vp, vp, vp]
     L.mnrt_tree_attach_photon_records.argtypes = [TP, vp, vp, vp, vp]
     L.mnrt_tree_release.argtypes = [TP]
+    L.mnrt_tree_collapse.argtypes = [TP, u32, vp]
     L.mnrt_tree_cost.argtypes = [TP, f32, ctypes.POINTER(ctypes.c_double), vp]
     L.mnrt_trace_closest.argtypes = [TP, u32, vp, vp, f32, vp, vp, vp, vp, vp, vp]
     L.mnrt_knn_gather.argtypes = [TP, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.mnrt_knn_radii.argtypes = [TP, u32, f32, u32, vp, vp, vp]
+    L.mnrt_knn_gather_bwd.argtypes = [TP, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.mnrt_photons_generate.argtypes = [TP, vp, vp, vp, vp, vp, u32, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp,
                                         ctypes.POINTER(u32), ctypes.POINTER(u32), vp]
     L.mnrt_shading_points.argtypes = [u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(u32), vp]
     L.mnrt_bounce_rays.argtypes = [u32, vp, vp, vp, vp, u32, vp, vp, vp, vp]
-    for name in ("mnrt_photons_generate", "mnrt_shading_points", "mnrt_bounce_rays", "mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
+    for name in ("mnrt_knn_gather_bwd", "mnrt_tree_collapse", "mnrt_photons_generate", "mnrt_shading_points", "mnrt_bounce_rays", "mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
                  "mnrt_tree_attach_photon_records", "mnrt_tree_release", "mnrt_tree_cost",
                  "mnrt_trace_closest", "mnrt_knn_gather", "mnrt_knn_radii", "mnrt_device_init"):
         getattr(L, name).restype = ctypes.c_int

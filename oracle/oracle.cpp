@@ -1403,6 +1403,56 @@ void orc_photons(const uint32_t* nodes, const uint32_t* refs, const float* root6
     *out_paths = shot;
 }
 
+// Adjoint of the radiance estimate with neighbour sets fixed (fp64): for upstream
+// gradient G (nq float4) writes dLoss/dpower per photon (np x 4 double, accumulated) and
+// dLoss/d(colour) per query (nq x 4 double).  Front-facing decisions and the 8-photon
+// cut are the fp32 ones of radiance_one.
+void orc_radiance_grad(const uint32_t* nbr, const uint32_t* count, uint32_t k, float rmax, const float* ppos4,
+                       const float* ppow4, const uint32_t* pdir_oct, uint32_t np, const float* qpos4,
+                       const float* qnrm4, const float* qdir4, const float* qinfl4, const float* qclr4, uint32_t nq,
+                       const float* grad4, double* out_gpow, double* out_gclr) {
+    std::memset(out_gpow, 0, sizeof(double) * 4 * (size_t)np);
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint32_t* nb = nbr + (size_t)q * k;
+        uint32_t cnt = count[q];
+        V3 n = ld4(qnrm4, q), d = ld4(qdir4, q), qp = ld4(qpos4, q);
+        V3 nd = mk(-d.v[0], -d.v[1], -d.v[2]);
+        if (dot(nd, n) < 0.0f) n = mk(n.v[0] * -1.0f, n.v[1] * -1.0f, n.v[2] * -1.0f);
+        double r2d = (double)rmax * (double)rmax;
+        if (cnt >= k && k > 0) {
+            uint32_t p = nb[k - 1];
+            double dd = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                double df = (double)ppos4[4 * p + c] - (double)qp.v[c];
+                dd += df * df;
+            }
+            r2d = dd;
+        }
+        double s64[3] = {0, 0, 0};
+        for (uint32_t i = 0; i < cnt && i < k; ++i) {
+            uint32_t p = nb[i];
+            if (dot(n, oct_decode(pdir_oct[p])) < 0.0f)
+                for (int c = 0; c < 3; ++c) s64[c] += (double)ppow4[4 * p + c];
+        }
+        double c4[4] = {1, 1, 1, 1};
+        if (qclr4)
+            for (int c = 0; c < 4; ++c) c4[c] = qclr4[4 * q + c];
+        bool gated = (qclr4 && cnt < 8) || !(r2d > 0.0);
+        for (int c = 0; c < 3; ++c) {
+            double g = grad4[4 * q + c], infl = qinfl4[4 * q + c];
+            double w = gated ? 0.0 : g * c4[3] * c4[c] / M_PI * infl / (M_PI * r2d);
+            double e = gated ? 0.0 : s64[c] / (M_PI * r2d);
+            if (out_gclr) out_gclr[4 * q + c] = g * c4[3] / M_PI * e * infl;
+            if (w != 0.0)
+                for (uint32_t i = 0; i < cnt && i < k; ++i) {
+                    uint32_t p = nb[i];
+                    if (dot(n, oct_decode(pdir_oct[p])) < 0.0f) out_gpow[4 * p + c] += w;
+                }
+        }
+        if (out_gclr) out_gclr[4 * q + 3] = 0.0;
+    }
+}
+
 uint32_t orc_oct_encode(float x, float y, float z) { return oct_encode(x, y, z); }
 void orc_oct_encode_n(const float* dir4, uint32_t n, uint32_t* out) {
     for (uint32_t i = 0; i < n; ++i) out[i] = oct_encode(dir4[4 * i], dir4[4 * i + 1], dir4[4 * i + 2]);

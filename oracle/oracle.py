@@ -70,6 +70,8 @@ def lib():
     L.orc_photons.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32,
                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, f32p, f32p, f32p, u32p, u32p, u32p,
                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+    L.orc_radiance_grad.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_float, f32p, f32p, u32p, ctypes.c_uint32,
+                                    f32p, f32p, f32p, f32p, vp, ctypes.c_uint32, f32p, f64p, f64p]
     L.orc_ray_tri.restype = ctypes.c_int
     L.orc_ray_tri.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
     _lib = L
@@ -210,6 +212,19 @@ def radiance(nbr, count, k, rmax, ppos, ppow, pdir4, qpos, qnrm, qdir, qinfl, qc
                        _c(qpos), _c(qnrm), _c(qdir), _c(qinfl), clr.ctypes.data if clr is not None else None, nq,
                        out32, out64, irr32, irr64, r2, _threads(nthreads))
     return (out32.reshape(nq, 4), out64.reshape(nq, 4), irr32.reshape(nq, 4), irr64.reshape(nq, 4), r2)
+
+
+def radiance_grad(nbr, count, k, rmax, ppos, ppow, pdir4, qpos, qnrm, qdir, qinfl, qclr, grad):
+    """fp64 adjoint of the estimate with the neighbour sets fixed: (dLoss/dpower, dLoss/dcolour)."""
+    nq, np_ = qpos.shape[0], ppos.shape[0]
+    codes = oct_encode(pdir4)
+    gp = np.zeros(4 * np_, np.float64)
+    gc = np.zeros(4 * nq, np.float64)
+    clr = _c(qclr) if qclr is not None else None
+    lib().orc_radiance_grad(_c(nbr, np.uint32).ravel(), _c(count, np.uint32), k, rmax, _c(ppos), _c(ppow), codes,
+                            np_, _c(qpos), _c(qnrm), _c(qdir), _c(qinfl), clr.ctypes.data if clr is not None else None,
+                            nq, _c(grad), gp, gc)
+    return gp.reshape(np_, 4), gc.reshape(nq, 4)
 
 
 def ray_tri(a, b, c, o, d):
