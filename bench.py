@@ -50,6 +50,190 @@ def knn_bytes(counts, n_q, n_set):
     return 36.0 * n_q + 8.0 * (inner + leaves) + 4.0 * refs + 12.0 * tests + 16.0 * float(n_set)
 
 
+def _offsets(counts):
+    off = np.zeros(len(counts) + 1, np.uint32)
+    off[1:] = np.cumsum(np.asarray(counts, np.int64))
+    return off
+
+
+def batched_leg(args, torch, api, S):
+    """Batched workload: 256 independent scenes like the small one (box, icosphere,
+    random triangles, one light), triangle / photon / query counts log-uniform in
+    [1k, 50k] / [4k, 256k] / [16k, 1M] (seed 4).  One step builds every triangle tree and
+    every photon tree (one batched build each), then traces all primary rays, compacts
+    the hits and answers kNN (k = 50) at all hits, each phase one persistent launch
+    replayed from a hipGraph.  The same work through the single-scene entry points, scene
+    after scene, is timed beside it."""
+    t0 = time.time()
+    NS, K = args.batched_scenes, 50
+    rng = np.random.default_rng(4)
+
+    def lu(lo, hi):
+        return np.exp(rng.uniform(np.log(lo), np.log(hi), NS))
+
+    ntri, nph, nq = lu(1000, 50000).astype(int), lu(4096, 262144).astype(int), lu(16384, 1 << 20).astype(int)
+    scs = [S.scene_sized(int(n), seed=4 + i) for i, n in enumerate(ntri)]
+    v0, v1, v2, alb = (api.dev(np.concatenate([getattr(sc, a) for sc in scs]).astype(np.float32))
+                       for a in ("v0", "v1", "v2", "albedo"))
+    toff = _offsets([sc.num_tris for sc in scs])
+    dims = []
+    for q in nq:
+        w = 8 * max(1, int(round(np.sqrt(q) / 8)))
+        dims.append((w, max(1, int(round(q / w)))))
+    roff = _offsets([w * h for w, h in dims])
+    R = int(roff[-1])
+    go = torch.empty((R, 4), dtype=torch.float32, device="cuda")
+    gd = torch.empty((R, 4), dtype=torch.float32, device="cuda")
+    for s, (w, h) in enumerate(dims):
+        api.primary_rays(w, h, out_orig=go[roff[s]:roff[s + 1]], out_dir=gd[roff[s]:roff[s + 1]])
+    eps = np.array([sc.ray_epsilon() for sc in scs], np.float32)
+    rmax = np.array([sc.r_max() for sc in scs], np.float32)
+    boxes = np.stack([np.concatenate(sc.bounds()) for sc in scs]).astype(np.float32)
+    tsl = [slice(int(toff[s]), int(toff[s + 1])) for s in range(NS)]
+    rsl = [slice(int(roff[s]), int(roff[s + 1])) for s in range(NS)]
+    TB = api.build_tri_batch(v0, v1, v2, toff)
+    ph = [api.photons_generate(TB[s], v0[tsl[s]], v1[tsl[s]], v2[tsl[s]], alb[tsl[s]], scs[s].lights, int(nph[s]),
+                               4, s, float(eps[s])) for s in range(NS)]
+    poff = _offsets([g["count"] for g in ph])
+    ppos, ppow, pdir = (torch.cat([g[k] for g in ph]).contiguous() for k in ("pos", "power", "dir"))
+    psl = [slice(int(poff[s]), int(poff[s + 1])) for s in range(NS)]
+    del ph
+
+    def build_all():
+        tb = api.build_tri_batch(v0, v1, v2, toff)
+        pb = api.build_points_batch(ppos, poff, root_boxes=boxes, max_query_radius=rmax)
+        api.attach_photon_records_batch(pb, ppos, ppow, pdir)
+        return tb, pb
+
+    TB.release()
+    TB, PB = build_all()
+    Q = api.QueryBatch(TB, toff, roff, eps, PB, rmax, K)
+    f4 = lambda: torch.empty((R, 4), dtype=torch.float32, device="cuda")
+    bufs = dict(orig4=go, dir4=gd, v0=v0, v1=v1, v2=v2, albedo4=alb,
+                hit_tri=torch.empty(R, dtype=torch.int32, device="cuda"),
+                hit_t=torch.empty(R, dtype=torch.float32, device="cuda"),
+                sp_pos4=f4(), sp_nrm4=f4(), sp_dir4=f4(), sp_clr4=f4(), sp_infl4=f4(),
+                sp_count=torch.zeros(NS, dtype=torch.int32, device="cuda"), io_rad4=f4())
+    Q.make_io(**bufs)
+    tt = time.time()
+    tuning = Q.autotune(max_slots=min(R, 8 << 20))
+    tune_s = time.time() - tt
+    for ph_ in (1, 2, 4):
+        Q.capture(ph_)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    stream = torch.cuda.current_stream()
+    names = ["build_tri", "build_points", "trace", "shade", "knn"]
+
+    def batched_step():
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        torch.cuda.synchronize()
+        w0 = time.time()
+        ev[0].record(stream)
+        tb = api.build_tri_batch(v0, v1, v2, toff)
+        ev[1].record(stream)
+        pb = api.build_points_batch(ppos, poff, root_boxes=boxes, max_query_radius=rmax)
+        api.attach_photon_records_batch(pb, ppos, ppow, pdir)
+        Q.update_trees(tb, pb)
+        bufs["io_rad4"].zero_()
+        ev[2].record(stream)
+        for i, ph_ in enumerate((1, 2, 4)):
+            Q.replay(ph_)
+            ev[3 + i].record(stream)
+        torch.cuda.synchronize()
+        wall = time.time() - w0
+        return (tb, pb), {n: ev[i].elapsed_time(ev[i + 1]) for i, n in enumerate(names)}, wall
+
+    def plain_step():
+        keep = []
+        torch.cuda.synchronize()
+        w0 = time.time()
+        for s in range(NS):
+            T = api.build_tri(v0[tsl[s]], v1[tsl[s]], v2[tsl[s]])
+            P = api.build_points(ppos[psl[s]], api.default_params(1, boxes[s], float(rmax[s])))
+            api.attach_photon_records(P, ppos[psl[s]], ppow[psl[s]], pdir[psl[s]])
+            tri, t, _, _ = api.trace_closest(T, go[rsl[s]], gd[rsl[s]], float(eps[s]))
+            q = api.shading_points(go[rsl[s]], gd[rsl[s]], tri, t, v0[tsl[s]], v1[tsl[s]], v2[tsl[s]], alb[tsl[s]])
+            api.knn_gather(P, K, float(rmax[s]), q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"])
+            keep.append((T, P))
+        torch.cuda.synchronize()
+        return keep, time.time() - w0
+
+    steps = max(1, args.batched_steps)
+    phase, walls = [], []
+    for it in range(steps + 1):  # the first is a warm-up
+        trees, ms, wall = batched_step()
+        if it > 0:
+            phase.append(ms)
+            walls.append(wall)
+        Q.update_trees(TB, PB)
+        for t_ in trees:
+            t_.release()
+    batched_s = float(np.mean(walls))
+    hits = int(bufs["sp_count"].sum().item())
+    pwalls = []
+    for it in range(steps + 1):
+        keep, wall = plain_step()
+        if it > 0:
+            pwalls.append(wall)
+        for T, P in keep:
+            T.release()
+            P.release()
+    plain_s = float(np.mean(pwalls))
+    Q.close()
+    TB.release()
+    PB.release()
+    return {
+        "workload": (f"{NS} scenes: triangles log-U[1k,50k], photons log-U[4k,256k], queries log-U[16k,1M] "
+                     "(seed 4), k=50; one step = all tree builds + all traversals + compaction + all kNN"),
+        "scenes": NS, "triangles": int(toff[-1]), "photons": int(poff[-1]), "rays": R, "knn_queries": hits,
+        "value": round(hits / batched_s / 1e6, 3), "unit": "M kNN queries/s (step incl. builds and traversal)",
+        "ms_per_step": round(batched_s * 1e3, 3), "plain_ms_per_step": round(plain_s * 1e3, 3),
+        "plain_value": round(hits / plain_s / 1e6, 3), "speedup_vs_plain": round(plain_s / batched_s, 2),
+        "phase_ms": {n: round(float(np.mean([p[n] for p in phase])), 3) for n in names},
+        "tuning": tuning, "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 1), "steps": steps}
+
+
+def adjoint_leg(args, torch, api, S):
+    """Differentiable workload: the small scene with 3 lights, 1M traced photons,
+    1920x1080 primary rays, k = 50 at all hits; forward = radiance with r2 and counts
+    kept (mnrt_knn_gather), backward = the adjoint (mnrt_knn_gather_bwd: neighbour sets
+    recomputed, dLoss/dpower per photon and dLoss/dcolour per query)."""
+    K = 50
+    sc = S.scene_small(subdiv=5, num_lights=3)
+    eps, rmax = sc.ray_epsilon(), sc.r_max()
+    v = [api.dev(a) for a in (sc.v0, sc.v1, sc.v2, sc.albedo)]
+    T = api.build_tri(*v[:3])
+    ph = api.photons_generate(T, *v, sc.lights, 1_000_000, 4, 0, eps)
+    o, d, _ = api.primary_rays(1920, 1080)
+    tri, t, _, _ = api.trace_closest(T, o, d, eps)
+    q = api.shading_points(o, d, tri, t, *v)
+    P = api.build_points(ph["pos"], api.default_params(1, np.concatenate(sc.bounds()), rmax))
+    api.attach_photon_records(P, ph["pos"], ph["power"], ph["dir"])
+    grad = torch.rand((q["count"], 4), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    fwd, bwd = [], []
+    for it in range(4):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        out = api.knn_gather(P, K, rmax, q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"], want_r2=True,
+                             want_count=True)
+        e[1].record(stream)
+        api.knn_gather_bwd(P, K, rmax, q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"], grad, out["r2"],
+                           out["count"])
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        if it > 0:
+            fwd.append(e[0].elapsed_time(e[1]))
+            bwd.append(e[1].elapsed_time(e[2]))
+    T.release()
+    P.release()
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    return {"workload": "small scene + 3 lights, 1M photons, 1920x1080 primary hits, k=50",
+            "photons": ph["count"], "queries": q["count"], "fwd_ms": round(f, 3), "bwd_ms": round(b, 3),
+            "bwd_fwd_ratio": round(b / f, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +242,10 @@ def main():
     ap.add_argument("--config", choices=["large", "small"], default="large")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sample", type=int, default=64, help="1-in-N subsample for the oracle replay")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -250,6 +438,12 @@ def main():
                        f"and kNN of 1 in {args.sample} queries ({len(qs)}, {cpu_knn_s:.2f}s) on {ncpu} threads, "
                        f"scaled to the full step ({cpu_step:.1f}s)"),
             "gpu_speedup": round(cpu_step / (ms_step * 1e-3), 1)}
+    if not args.no_batched:
+        out["batched"] = batched_leg(args, torch, api, S)
+        out["batched_vs_plain"] = out["batched"]["speedup_vs_plain"]
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(args, torch, api, S)
+        out["bwd_fwd_ratio"] = out["adjoint"]["bwd_fwd_ratio"]
     print(json.dumps(out))
 
 

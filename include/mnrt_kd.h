@@ -123,6 +123,34 @@ int mnrt_build_tri(const float* d_v0, const float* d_v1, const float* d_v2, uint
 int mnrt_build_points(const float* d_pos4, uint32_t n, const mnrt_build_params* params,
                       mnrt_tree* out, void* stream);
 
+/* Batched builds (no reference counterpart; the reference builds one KDTreeGPU per
+ * scene, KDTreeGPU.cpp:175-209): num_scenes scenes whose elements are concatenated in
+ * one input, scene s holding entries h_offsets[s] .. h_offsets[s+1]-1 (host array of
+ * num_scenes + 1).  All scenes go through one level loop, one small-node launch and one
+ * layout pass; out[s] (num_scenes trees) equals what mnrt_build_tri / mnrt_build_points
+ * gives for scene s alone, word for word, with scene-local element indices.  The trees
+ * are views into shared arrays owned by out[0]: mnrt_tree_release(&out[0]) frees the
+ * batch, release of out[s > 0] frees nothing.  h_root_boxes: NULL (tight bounds) or
+ * 6 floats (min xyz, max xyz) per scene; h_max_query_radius: NULL (params value) or one
+ * VVH radius per scene.  params->has_root_box is ignored. */
+int mnrt_build_tri_batch(const float* d_v0, const float* d_v1, const float* d_v2,
+                         const uint32_t* h_offsets, uint32_t num_scenes,
+                         const mnrt_build_params* params, const float* h_root_boxes,
+                         mnrt_tree* out, void* stream);
+int mnrt_build_points_batch(const float* d_pos4, const uint32_t* h_offsets, uint32_t num_scenes,
+                            const mnrt_build_params* params, const float* h_root_boxes,
+                            const float* h_max_query_radius, mnrt_tree* out, void* stream);
+/* Records of a batch (trees from one batched build, inputs concatenated as for the
+ * build): one shared record array owned by trees[0]; photon batches are collapsed with
+ * max_refs = 64 like mnrt_tree_attach_photon_records. */
+int mnrt_tree_attach_tri_records_batch(mnrt_tree* trees, uint32_t num_scenes, const float* d_v0,
+                                       const float* d_v1, const float* d_v2,
+                                       const uint32_t* h_offsets, void* stream);
+int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t num_scenes,
+                                          const float* d_pos4, const float* d_pow4,
+                                          const float* d_dir4, const uint32_t* h_offsets,
+                                          void* stream);
+
 /* Leaf-ordered records for a tree whose nodes/refs exist. */
 int mnrt_tree_attach_tri_records(mnrt_tree* tree, const float* d_v0, const float* d_v1,
                                  const float* d_v2, void* stream);
@@ -232,12 +260,107 @@ int mnrt_shading_points(uint32_t n, const float* d_orig4, const float* d_dir4, c
                         float* d_out_dir4, float* d_out_clr4, float* d_out_infl4, uint32_t* d_out_src,
                         int32_t* d_out_tri, uint32_t* out_count, void* stream);
 
+/* Pinhole primary rays through pixel centres (kernel_genPrimaryRays, raygen.cu:171-240)
+ * in tile order (tile x tile pixel tiles row by row, pixels row by row inside a tile,
+ * edge tiles clipped): width * height rays.  h_cam (host, 14 doubles): eye xyz, unit
+ * forward xyz, unit right xyz, up xyz, tan(vfov / 2), aspect (width / height).  The
+ * direction is computed in double and rounded once to float.  Optional pixel index
+ * py * width + px per ray. */
+int mnrt_primary_rays(uint32_t width, uint32_t height, uint32_t tile, const double* h_cam,
+                      float* d_out_orig4, float* d_out_dir4, uint32_t* d_out_pixel, void* stream);
+
 /* One cosine-distributed diffuse bounce ray per shading point (kernel_genFinalGatherRays,
  * raygen.cu:406-487) about the normal turned toward the arriving ray; optional influence
  * output = the point's diffuse colour. */
 int mnrt_bounce_rays(uint32_t n, const float* d_pos4, const float* d_nrm4, const float* d_dir4,
                      const float* d_clr4, uint32_t seed, float* d_out_orig4, float* d_out_dir4,
                      float* d_out_infl4, void* stream);
+
+/* --- batches of scenes (no reference counterpart) ----------------------------------- */
+
+/* A batch is many independent scenes traced and queried together: the reference runs
+ * KernelRTTraceRays / KernelPMPhotonGather once per scene with the scene bound into
+ * global device state (RTCore.cpp:404-439, PhotonMap.cpp:195-232).  Scene s owns ray and
+ * query slots ray_offsets[s] .. ray_offsets[s+1]-1 of every per-ray / per-query array
+ * and triangles tri_offsets[s] .. of the concatenated vertex and albedo arrays.  Each
+ * phase is one persistent launch (workgroups take tiles from device work counters) and
+ * enqueues kernels only, with no allocation and no host synchronisation, so a phase can
+ * be captured in a hipGraph (mnrt_batch_capture) and replayed.  Per scene, results are
+ * identical to mnrt_trace_closest / mnrt_shading_points / mnrt_knn_gather on that scene. */
+typedef struct mnrt_batch mnrt_batch;
+
+typedef struct mnrt_batch_desc {
+    uint32_t num_scenes;
+    const mnrt_tree* tri_trees;    /* num_scenes triangle trees with records */
+    const mnrt_tree* photon_maps;  /* num_scenes point trees with photon records, or NULL */
+    const uint32_t* tri_offsets;   /* host, num_scenes + 1 */
+    const uint32_t* ray_offsets;   /* host, num_scenes + 1 */
+    const float* eps;              /* host, ray epsilon per scene */
+    const float* r_max;            /* host, kNN radius cap per scene (photon_maps only) */
+    uint32_t k;                    /* neighbours per query, 1..MNRT_KNN_MAX_K */
+} mnrt_batch_desc;
+
+/* Launch shape of the phases; mnrt_batch_autotune picks it by timing candidates. */
+typedef struct mnrt_batch_tuning {
+    uint32_t trace_block; /* lanes per workgroup: 64, 128, 256 */
+    uint32_t trace_tile;  /* rays per work item, a multiple of trace_block */
+    uint32_t trace_grid;  /* persistent workgroups */
+    uint32_t trace_stack; /* traversal stack entries: 16, 24, 32, 50; >= every tree's max_level */
+    uint32_t knn_waves;   /* waves (one query each) per workgroup: 1, 2, 4 */
+    uint32_t knn_chunk;   /* query slots per work item */
+    uint32_t knn_grid;    /* persistent workgroups */
+    uint32_t knn_cap;     /* LDS candidate entries per wave: 128, 256, 512; >= k + 64 */
+    uint32_t parts;       /* work-counter shares: 1, or 8 (one per XCD) */
+} mnrt_batch_tuning;
+
+/* Buffers of the phases (slot-indexed unless stated). */
+typedef struct mnrt_batch_io {
+    const float* d_orig4;
+    const float* d_dir4;
+    const float* d_v0;
+    const float* d_v1;
+    const float* d_v2;
+    const float* d_albedo4; /* per triangle */
+    const float* d_infl4;   /* per ray, or NULL (1) */
+    int32_t* d_hit_tri;     /* trace out: scene-local triangle or -1 */
+    float* d_hit_t;
+    float* d_hit_bary2;     /* optional */
+    float* d_sp_pos4;       /* shade out: scene s's shading points from slot ray_offsets[s] */
+    float* d_sp_nrm4;
+    float* d_sp_dir4;
+    float* d_sp_clr4;
+    float* d_sp_infl4;
+    uint32_t* d_sp_src;     /* optional: scene-local source ray */
+    uint32_t* d_sp_count;   /* num_scenes: shading points per scene */
+    float* d_io_rad4;       /* kNN: radiance accumulated per shading point */
+    float* d_out_r2;        /* optional */
+    uint32_t* d_out_nbr;    /* optional, k per slot */
+    uint32_t* d_out_count;  /* optional */
+} mnrt_batch_io;
+
+#define MNRT_PHASE_TRACE 1u
+#define MNRT_PHASE_SHADE 2u
+#define MNRT_PHASE_KNN 4u
+
+/* Copies the per-scene tables to the device (the trees must outlive the batch). */
+int mnrt_batch_create(const mnrt_batch_desc* desc, mnrt_batch** out);
+int mnrt_batch_destroy(mnrt_batch* batch);
+int mnrt_batch_get_tuning(const mnrt_batch* batch, mnrt_batch_tuning* out);
+/* Synchronous (re-uploads the work tables); drops captured graphs. */
+int mnrt_batch_set_tuning(mnrt_batch* batch, const mnrt_batch_tuning* tuning);
+/* Times candidate launch shapes of the trace and kNN phases on io (a prefix of at most
+ * max_slots slots; 0 = all) and keeps the fastest.  Writes io's outputs. */
+int mnrt_batch_autotune(mnrt_batch* batch, const mnrt_batch_io* io, uint32_t max_slots, void* stream);
+/* Points the batch at new trees of the same scenes (rebuilt geometry or photons; same
+ * ray offsets, epsilons and radii).  Synchronous.  Captured graphs stay valid unless a
+ * tree is deeper than the tuned traversal stack: then the stack grows and the graphs are
+ * dropped (mnrt_batch_replay returns MNRT_ERR_STATE until recaptured). */
+int mnrt_batch_update_trees(mnrt_batch* batch, const mnrt_tree* tri_trees, const mnrt_tree* photon_maps);
+/* Enqueue one phase (kernels only). */
+int mnrt_batch_run(mnrt_batch* batch, const mnrt_batch_io* io, uint32_t phases, void* stream);
+/* Captures the given phases with io into a graph kept under that phase mask. */
+int mnrt_batch_capture(mnrt_batch* batch, const mnrt_batch_io* io, uint32_t phases, void* stream);
+int mnrt_batch_replay(mnrt_batch* batch, uint32_t phases, void* stream);
 
 #ifdef __cplusplus
 }

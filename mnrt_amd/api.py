@@ -25,10 +25,11 @@ def _torch():
 class DeviceTree:
     """A device-resident tree plus the tensors that keep caller-provided arrays alive."""
 
-    def __init__(self, kind: int):
-        self.t = N.Tree()
+    def __init__(self, kind: int, t=None, batch=None):
+        self.t = t if t is not None else N.Tree()
         self.t.kind = kind
         self._keep = []
+        self._batch = batch  # views of a TreeBatch never release
 
     @property
     def kind(self):
@@ -76,7 +77,7 @@ class DeviceTree:
         return c.value
 
     def _owns(self):
-        return self.t.owns_nodes or self.t.owns_recs or self.t.owns_qnodes
+        return self._batch is None and (self.t.owns_nodes or self.t.owns_recs or self.t.owns_qnodes)
 
     def release(self):
         if self._owns():
@@ -90,6 +91,89 @@ class DeviceTree:
                 self.release()
         except Exception:
             pass
+
+
+class TreeBatch:
+    """Trees of one batched build: views into arrays owned by the first tree."""
+
+    def __init__(self, kind: int, offsets):
+        self.kind = kind
+        self.offsets = np.ascontiguousarray(offsets, np.uint32)
+        self.num_scenes = len(self.offsets) - 1
+        self.arr = (N.Tree * self.num_scenes)()
+        self._keep = []
+
+    def __len__(self):
+        return self.num_scenes
+
+    def __getitem__(self, s: int) -> DeviceTree:
+        if not 0 <= s < self.num_scenes:
+            raise IndexError(s)
+        return DeviceTree(self.kind, self.arr[s], batch=self)
+
+    def ref(self):
+        return ctypes.cast(self.arr, ctypes.POINTER(N.Tree))
+
+    def offsets_ref(self):
+        return self.offsets.ctypes.data_as(ctypes.POINTER(N.u32))
+
+    def release(self):
+        if self.num_scenes and (self.arr[0].owns_nodes or self.arr[0].owns_recs or self.arr[0].owns_qnodes):
+            _torch().cuda.synchronize()
+            N.check(N.load().mnrt_tree_release(ctypes.byref(self.arr[0])), "mnrt_tree_release")
+        self._keep = []
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def _f32_ptr(a, per: int, n: int):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    if a.size != per * n:
+        raise N.MnrtError(f"expected {per * n} floats, got {a.size}")
+    return a, a.ctypes.data_as(ctypes.POINTER(N.f32))
+
+
+def build_tri_batch(v0, v1, v2, offsets, params: N.BuildParams | None = None, root_boxes=None) -> TreeBatch:
+    """Trees of len(offsets)-1 scenes whose triangles are concatenated in v0/v1/v2."""
+    v0, v1, v2 = dev(v0), dev(v1), dev(v2)
+    B = TreeBatch(N.KIND_TRI, offsets)
+    if int(B.offsets[-1]) != v0.shape[0]:
+        raise N.MnrtError("offsets do not cover the input")
+    params = params or default_params(N.KIND_TRI)
+    rb, rbp = _f32_ptr(root_boxes, 6, B.num_scenes)
+    B._keep += [v0, v1, v2]
+    N.check(N.load().mnrt_build_tri_batch(N.ptr(v0), N.ptr(v1), N.ptr(v2), B.offsets_ref(), B.num_scenes,
+                                          ctypes.byref(params), rbp, B.ref(), N.stream_handle()),
+            "mnrt_build_tri_batch")
+    return B
+
+
+def build_points_batch(pos, offsets, params: N.BuildParams | None = None, root_boxes=None,
+                       max_query_radius=None) -> TreeBatch:
+    pos = dev(pos)
+    B = TreeBatch(N.KIND_POINT, offsets)
+    if int(B.offsets[-1]) != pos.shape[0]:
+        raise N.MnrtError("offsets do not cover the input")
+    params = params or default_params(N.KIND_POINT)
+    rb, rbp = _f32_ptr(root_boxes, 6, B.num_scenes)
+    qr, qrp = _f32_ptr(max_query_radius, 1, B.num_scenes)
+    N.check(N.load().mnrt_build_points_batch(N.ptr(pos), B.offsets_ref(), B.num_scenes, ctypes.byref(params), rbp,
+                                             qrp, B.ref(), N.stream_handle()), "mnrt_build_points_batch")
+    return B
+
+
+def attach_photon_records_batch(B: TreeBatch, pos, power, dirs):
+    pos, power, dirs = dev(pos), dev(power), dev(dirs)
+    B._keep += [pos, power, dirs]
+    N.check(N.load().mnrt_tree_attach_photon_records_batch(B.ref(), B.num_scenes, N.ptr(pos), N.ptr(power),
+                                                           N.ptr(dirs), B.offsets_ref(), N.stream_handle()),
+            "mnrt_tree_attach_photon_records_batch")
 
 
 def _copy_from_ptr(dst, src_ptr: int, nbytes: int):
@@ -286,6 +370,22 @@ def shading_points(orig, dirs, tri, t, v0, v1, v2, albedo, infl=None, want_src=F
     return q
 
 
+def primary_rays(width: int, height: int, eye=(0.0, 0.0, 0.9), target=(0.0, 0.0, 0.0), vfov_deg=90.0, tile=8,
+                 out_orig=None, out_dir=None, want_pixel=False):
+    """Camera rays on the device, equal to scene.camera_rays (mnrt_primary_rays)."""
+    from . import scene as S
+
+    torch = _torch()
+    n = width * height
+    cam = np.ascontiguousarray(S.camera_basis(width, height, eye, target, vfov_deg))
+    o = out_orig if out_orig is not None else torch.empty((max(n, 1), 4), dtype=torch.float32, device="cuda")
+    d = out_dir if out_dir is not None else torch.empty((max(n, 1), 4), dtype=torch.float32, device="cuda")
+    pix = torch.empty(max(n, 1), dtype=torch.int32, device="cuda") if want_pixel else None
+    N.check(N.load().mnrt_primary_rays(width, height, tile, cam.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       N.ptr(o), N.ptr(d), N.ptr(pix), N.stream_handle()), "mnrt_primary_rays")
+    return o[:n], d[:n], (pix[:n] if pix is not None else None)
+
+
 def bounce_rays(q, seed: int):
     torch = _torch()
     n = q["pos"].shape[0]
@@ -295,3 +395,111 @@ def bounce_rays(q, seed: int):
     N.check(N.load().mnrt_bounce_rays(n, N.ptr(q["pos"]), N.ptr(q["nrm"]), N.ptr(q["dir"]), N.ptr(q["clr"]), seed,
                                       N.ptr(o), N.ptr(d), N.ptr(infl), N.stream_handle()), "mnrt_bounce_rays")
     return o[:n], d[:n], infl[:n]
+
+
+class QueryBatch:
+    """Many scenes traced, shaded and queried together (mnrt_batch_*).
+
+    tri_trees / photon_maps: TreeBatch or list of DeviceTree (photon_maps None: trace
+    only).  tri_offsets: (S+1) triangle offsets into the concatenated vertex / albedo
+    arrays; ray_offsets: (S+1) ray slot offsets; eps, r_max: per scene."""
+
+    def __init__(self, tri_trees, tri_offsets, ray_offsets, eps, photon_maps=None, r_max=None, k: int = 50):
+        self.S = len(ray_offsets) - 1
+        self._tri_arr = self._tree_array(tri_trees)
+        self._pm_arr = self._tree_array(photon_maps) if photon_maps is not None else None
+        self.tri_offsets = np.ascontiguousarray(tri_offsets, np.uint32)
+        self.ray_offsets = np.ascontiguousarray(ray_offsets, np.uint32)
+        self.eps = np.ascontiguousarray(eps, np.float32)
+        self.r_max = np.ascontiguousarray(r_max if r_max is not None else np.ones(self.S), np.float32)
+        self.k = k
+        self._refs = (tri_trees, photon_maps)
+        d = N.BatchDesc()
+        d.num_scenes = self.S
+        d.tri_trees = ctypes.cast(self._tri_arr, ctypes.POINTER(N.Tree))
+        d.photon_maps = ctypes.cast(self._pm_arr, ctypes.POINTER(N.Tree)) if self._pm_arr is not None else None
+        d.tri_offsets = self.tri_offsets.ctypes.data_as(ctypes.POINTER(N.u32))
+        d.ray_offsets = self.ray_offsets.ctypes.data_as(ctypes.POINTER(N.u32))
+        d.eps = self.eps.ctypes.data_as(ctypes.POINTER(N.f32))
+        d.r_max = self.r_max.ctypes.data_as(ctypes.POINTER(N.f32))
+        d.k = k
+        h = ctypes.c_void_p()
+        N.check(N.load().mnrt_batch_create(ctypes.byref(d), ctypes.byref(h)), "mnrt_batch_create")
+        self.h = h
+        self._io = None
+        self._io_keep = None
+
+    @staticmethod
+    def _tree_array(trees):
+        if isinstance(trees, TreeBatch):
+            return trees.arr
+        arr = (N.Tree * len(trees))()
+        for i, t in enumerate(trees):
+            arr[i] = t.t
+        return arr
+
+    def update_trees(self, tri_trees, photon_maps=None):
+        """Points the batch at rebuilt trees of the same scenes (captured graphs stay valid)."""
+        self._tri_arr = self._tree_array(tri_trees)
+        self._pm_arr = self._tree_array(photon_maps) if photon_maps is not None else None
+        self._refs = (tri_trees, photon_maps)
+        N.check(N.load().mnrt_batch_update_trees(
+            self.h, ctypes.cast(self._tri_arr, ctypes.POINTER(N.Tree)),
+            ctypes.cast(self._pm_arr, ctypes.POINTER(N.Tree)) if self._pm_arr is not None else None),
+            "mnrt_batch_update_trees")
+
+    def make_io(self, **bufs):
+        """BatchIO from named device tensors (names as mnrt_batch_io without d_)."""
+        io = N.BatchIO()
+        keep = {}
+        for name in N.BATCH_IO_FIELDS:
+            t = bufs.pop(name[2:], None)
+            if t is not None:
+                keep[name] = t
+                setattr(io, name, N.ptr(t))
+        if bufs:
+            raise N.MnrtError(f"unknown batch buffers: {sorted(bufs)}")
+        self._io, self._io_keep = io, keep
+        return io
+
+    def run(self, phases: int, stream=None):
+        N.check(N.load().mnrt_batch_run(self.h, ctypes.byref(self._io), phases, N.stream_handle(stream)),
+                "mnrt_batch_run")
+
+    def capture(self, phases: int, stream=None):
+        N.check(N.load().mnrt_batch_capture(self.h, ctypes.byref(self._io), phases, N.stream_handle(stream)),
+                "mnrt_batch_capture")
+
+    def replay(self, phases: int, stream=None):
+        N.check(N.load().mnrt_batch_replay(self.h, phases, N.stream_handle(stream)), "mnrt_batch_replay")
+
+    def autotune(self, max_slots: int = 0, stream=None):
+        N.check(N.load().mnrt_batch_autotune(self.h, ctypes.byref(self._io), max_slots, N.stream_handle(stream)),
+                "mnrt_batch_autotune")
+        return self.tuning
+
+    @property
+    def tuning(self) -> dict:
+        t = N.BatchTuning()
+        N.check(N.load().mnrt_batch_get_tuning(self.h, ctypes.byref(t)), "mnrt_batch_get_tuning")
+        return t.as_dict()
+
+    @tuning.setter
+    def tuning(self, d: dict):
+        t = N.BatchTuning()
+        N.check(N.load().mnrt_batch_get_tuning(self.h, ctypes.byref(t)), "mnrt_batch_get_tuning")
+        for k_, v in d.items():
+            setattr(t, k_, int(v))
+        N.check(N.load().mnrt_batch_set_tuning(self.h, ctypes.byref(t)), "mnrt_batch_set_tuning")
+
+    def close(self):
+        if self.h:
+            _torch().cuda.synchronize()
+            N.check(N.load().mnrt_batch_destroy(self.h), "mnrt_batch_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <vector>
 
+#include <cstring>
+
 #include "scan.h"
 
 namespace mnrt {
@@ -55,7 +57,9 @@ enum : uint32_t {
 };
 
 // Build-node table: one entry per node of the large-stage tree (inner nodes, empty
-// and capped leaves, small roots), 7 uint32 columns of `cap` entries.
+// and capped leaves, small roots), 8 uint32 columns of `cap` entries.  Build nodes
+// 0..S-1 are the roots of the S scenes built together.
+constexpr int kBnCols = 8;
 struct BnView {
     uint32_t* type;   // 0..2 inner split axis, BN_LEAF, BN_SMALL, BN_PENDING, BN_DEAD
     uint32_t* split;  // split position bits (inner)
@@ -64,6 +68,7 @@ struct BnView {
     uint32_t* level;
     uint32_t* aux;  // small-root index (BN_SMALL) or first copied element (BN_LEAF)
     uint32_t* cnt;  // element count (BN_LEAF)
+    uint32_t* scene;
 };
 
 struct SmallView {  // small roots
@@ -73,6 +78,7 @@ struct SmallView {  // small roots
     uint32_t* bn;
     float4* lo;
     float4* hi;
+    uint32_t* scene;
 };
 
 struct ElemView {  // element arrays: original index, bounds, owning active node
@@ -89,14 +95,29 @@ struct NodeView {  // active large nodes
     uint32_t* cnt;
     uint32_t* level;
     uint32_t* bn;
+    uint32_t* scene;
 };
 
+// Concatenated input of S scenes: element e of scene s is entry base[s] + e; element
+// indices inside a build are scene-local, as a single-scene build would number them.
 struct Geometry {
     int kind;
     const float4* v0;
     const float4* v1;
     const float4* v2;
+    const uint32_t* base;  // S + 1 offsets
+    uint32_t num_scenes;
 };
+
+__device__ __forceinline__ uint32_t scene_of(const uint32_t* base, uint32_t nscenes, uint32_t g) {
+    uint32_t lo = 0, hi = nscenes;  // last s with base[s] <= g
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 __device__ __forceinline__ uint32_t fenc(float f) {
     uint32_t u = __float_as_uint(f);
@@ -217,6 +238,7 @@ __device__ void clip_bounds(const Geometry& G, uint32_t idx, float4& lo, float4&
 __global__ __launch_bounds__(256) void k_init_elems(Geometry G, uint32_t n, ElemView E) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t s = scene_of(G.base, G.num_scenes, i);
     float4 a = G.v0[i];
     float4 lo, hi;
     if (G.kind == MNRT_KIND_TRI) {
@@ -228,10 +250,10 @@ __global__ __launch_bounds__(256) void k_init_elems(Geometry G, uint32_t n, Elem
     } else {
         lo = hi = make_float4(canon(a.x), canon(a.y), canon(a.z), 0.0f);
     }
-    E.idx[i] = i;
+    E.idx[i] = i - G.base[s];
     E.lo[i] = lo;
     E.hi[i] = hi;
-    E.node[i] = 0;
+    E.node[i] = s;
 }
 
 __global__ __launch_bounds__(256) void k_chunk_counts(uint32_t n, const uint32_t* __restrict__ cnt,
@@ -314,19 +336,85 @@ __global__ __launch_bounds__(kChunk) void k_tight(uint32_t max_chunks, const uin
     }
 }
 
-__global__ void k_root_box_from_tight(const uint32_t* __restrict__ tlo, const uint32_t* __restrict__ thi,
-                                      float4* __restrict__ lo, float4* __restrict__ hi, float* __restrict__ host_box) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    float4 l = make_float4(fdec(tlo[0]), fdec(tlo[1]), fdec(tlo[2]), 0.0f);
-    float4 h = make_float4(fdec(thi[0]), fdec(thi[1]), fdec(thi[2]), 0.0f);
-    lo[0] = l;
-    hi[0] = h;
-    host_box[0] = l.x;
-    host_box[1] = l.y;
-    host_box[2] = l.z;
-    host_box[3] = h.x;
-    host_box[4] = h.y;
-    host_box[5] = h.z;
+// Per-scene placement decided on the host: an active (large) root or a small root, its
+// index in that list and its first element there.
+struct RootPlan {
+    const uint32_t* kind;   // 0 active, 1 small
+    const uint32_t* index;
+    const uint32_t* off;
+};
+
+// Root boxes (given, or the tight bounds of the scene's elements; (+max, -max) for an
+// empty scene) and the root entries of the active / small lists.
+__global__ __launch_bounds__(256) void k_roots_setup(uint32_t nscenes, const uint32_t* __restrict__ base,
+                                                     const uint32_t* __restrict__ tlo,
+                                                     const uint32_t* __restrict__ thi,
+                                                     const float* __restrict__ given, RootPlan R, NodeView A,
+                                                     SmallView S, BnView B, float* __restrict__ out_box) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscenes) return;
+    const uint32_t cnt = base[s + 1] - base[s];
+    float4 lo, hi;
+    if (given) {
+        lo = make_float4(canon(given[6 * s + 0]), canon(given[6 * s + 1]), canon(given[6 * s + 2]), 0.0f);
+        hi = make_float4(canon(given[6 * s + 3]), canon(given[6 * s + 4]), canon(given[6 * s + 5]), 0.0f);
+    } else if (cnt == 0) {
+        lo = make_float4(kInf, kInf, kInf, 0.0f);
+        hi = make_float4(-kInf, -kInf, -kInf, 0.0f);
+    } else {
+        lo = make_float4(fdec(tlo[3 * s]), fdec(tlo[3 * s + 1]), fdec(tlo[3 * s + 2]), 0.0f);
+        hi = make_float4(fdec(thi[3 * s]), fdec(thi[3 * s + 1]), fdec(thi[3 * s + 2]), 0.0f);
+    }
+    B.level[s] = 0;
+    B.scene[s] = s;
+    const uint32_t i = R.index[s];
+    if (R.kind[s] == 0) {
+        A.lo[i] = lo;
+        A.hi[i] = hi;
+        A.first[i] = R.off[s];
+        A.cnt[i] = cnt;
+        A.level[i] = 0;
+        A.bn[i] = s;
+        A.scene[i] = s;
+        B.type[s] = BN_PENDING;
+    } else {
+        S.first[i] = R.off[s];
+        S.cnt[i] = cnt;
+        S.level[i] = 0;
+        S.bn[i] = s;
+        S.lo[i] = lo;
+        S.hi[i] = hi;
+        S.scene[i] = s;
+        B.type[s] = BN_SMALL;
+        B.aux[s] = i;
+    }
+    out_box[6 * s + 0] = lo.x;
+    out_box[6 * s + 1] = lo.y;
+    out_box[6 * s + 2] = lo.z;
+    out_box[6 * s + 3] = hi.x;
+    out_box[6 * s + 4] = hi.y;
+    out_box[6 * s + 5] = hi.z;
+}
+
+// Elements of large scenes go to the first level's arrays, those of small scenes to the
+// small-root buffer; each keeps its scene order.
+__global__ __launch_bounds__(256) void k_partition_elems(uint32_t n, ElemView In, int kind, RootPlan R, ElemView E,
+                                                         ElemView Sm) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t s = In.node[g], local = In.idx[g];
+    const uint32_t d = R.off[s] + local;
+    float4 lo = In.lo[g];
+    if (R.kind[s] == 0) {
+        E.idx[d] = local;
+        E.lo[d] = lo;
+        if (kind == MNRT_KIND_TRI) E.hi[d] = In.hi[g];
+        E.node[d] = R.index[s];
+    } else {
+        Sm.idx[d] = local;
+        Sm.lo[d] = lo;
+        if (kind == MNRT_KIND_TRI) Sm.hi[d] = In.hi[g];
+    }
 }
 
 struct PlanView {
@@ -384,8 +472,10 @@ __global__ __launch_bounds__(256) void k_plan(uint32_t n, NodeView A, const uint
                 B.cnt[eleaf] = 0;
                 B.aux[eleaf] = 0;
                 B.level[eleaf] = level + 1;
+                B.scene[eleaf] = A.scene[p];
                 B.type[cont] = BN_PENDING;
                 B.level[cont] = level + 1;
+                B.scene[cont] = A.scene[p];
                 cur = cont;
             }
             if (isMax) hi[ax] = s;
@@ -430,6 +520,8 @@ __global__ __launch_bounds__(256) void k_plan(uint32_t n, NodeView A, const uint
     B.level[R] = level + 1;
     B.type[L] = BN_PENDING;
     B.type[R] = BN_PENDING;
+    B.scene[L] = A.scene[p];
+    B.scene[R] = A.scene[p];
 }
 
 __global__ __launch_bounds__(256) void k_classify(uint32_t ne, ElemView E, int kind, PlanView P,
@@ -493,9 +585,9 @@ __global__ __launch_bounds__(256) void k_child_plan(uint32_t n, uint32_t ne, Nod
     Cv.lcnt[c] = type == BN_PENDING ? cnt : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_child_write(uint32_t n, PlanView P, ChildView Cv, uint32_t small_base,
-                                                     uint32_t small_elem_base, SmallView S, NodeView Nx,
-                                                     BnView B, uint32_t* __restrict__ counters) {
+__global__ __launch_bounds__(256) void k_child_write(uint32_t n, NodeView A, PlanView P, ChildView Cv,
+                                                     uint32_t small_base, uint32_t small_elem_base, SmallView S,
+                                                     NodeView Nx, BnView B, uint32_t* __restrict__ counters) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= 2 * n) return;
     uint32_t side = c >= n ? 1u : 0u;
@@ -524,6 +616,7 @@ __global__ __launch_bounds__(256) void k_child_write(uint32_t n, PlanView P, Chi
         S.bn[s] = bn;
         S.lo[s] = lo;
         S.hi[s] = hi;
+        S.scene[s] = A.scene[p];
         B.type[bn] = BN_SMALL;
         B.aux[bn] = s;
     } else {
@@ -534,6 +627,7 @@ __global__ __launch_bounds__(256) void k_child_write(uint32_t n, PlanView P, Chi
         Nx.cnt[l] = Cv.cnt[c];
         Nx.level[l] = level;
         Nx.bn[l] = bn;
+        Nx.scene[l] = A.scene[p];
         B.type[bn] = BN_PENDING;
     }
 }
@@ -556,7 +650,8 @@ __global__ __launch_bounds__(256) void k_leaf_copy(uint32_t n, NodeView A, PlanV
 // Scatter each element into its children: small roots append to the small-element
 // buffer, large children to the next level's arrays.  Straddling triangles are clipped
 // to each child box (kernel_PerformSplitClipping, kdtree_tri.cu:389-489).
-__global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t ne, ElemView E, Geometry G, PlanView P,
+__global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t ne, ElemView E, Geometry G, NodeView A,
+                                                 PlanView P,
                                                  ChildView Cv, const uint32_t* __restrict__ pos,
                                                  uint32_t small_elem_base, ElemView Sm, uint32_t sm_cap, ElemView Nx,
                                                  uint32_t nx_cap, uint32_t* __restrict__ counters) {
@@ -583,7 +678,7 @@ __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t ne, ElemVi
             float4 bmin = P.lo[p], bmax = P.hi[p];
             if (side == 0) f4set(bmax, axis, s);
             else f4set(bmin, axis, s);
-            clip_bounds(G, idx, clo, chi, bmin, bmax);
+            clip_bounds(G, G.base[A.scene[p]] + idx, clo, chi, bmin, bmax);
         }
         if (type == BN_SMALL) {
             uint32_t d = small_elem_base + Cv.soff[c] + rank;
@@ -631,11 +726,15 @@ struct SmallOut {
     uint32_t* counters;
     uint32_t max_nodes;
     uint32_t max_refs;
+    const uint32_t* node_base;  // per scene: first node / reference of its tree
+    const uint32_t* ref_base;
+    uint32_t* scene_leaves;     // per scene statistics
+    uint32_t* scene_maxlvl;
 };
 
 template <bool kEmit>
 __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, SmallView S, ElemView Sm, int kind,
-                                                            float max_qr, SmallOut O) {
+                                                            const float* __restrict__ scene_qr, SmallOut O) {
     __shared__ float s_lo[kSmallWaves][3][64];
     __shared__ float s_hi[kSmallWaves][3][64];
     __shared__ uint32_t s_idx[kSmallWaves][64];
@@ -704,6 +803,12 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     }
 
     uint32_t addr = 0, rbase = 0, leaves = 0, maxlvl = 0;
+    const uint32_t scene = S.scene[r];
+    // local addresses go into the node words; nodes / refs point at the scene's tree
+    uint32_t* nodes = kEmit ? O.nodes + 2ull * O.node_base[scene] : nullptr;
+    uint32_t* refs = kEmit ? O.refs + O.ref_base[scene] : nullptr;
+    uint32_t lim_nodes = kEmit ? O.max_nodes - O.node_base[scene] : 0u;
+    uint32_t lim_refs = kEmit ? O.max_refs - O.ref_base[scene] : 0u;
     if (kEmit) {
         uint32_t bn = S.bn[r];
         addr = O.bn_addr[bn];
@@ -723,16 +828,16 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     }
     sp = 1;
     wave_sync();
-    const float qr2 = 2.0f * max_qr;
+    const float qr2 = 2.0f * scene_qr[scene];
     while (sp > 0) {
         --sp;
         SmallStackEnt e = stack[sp];
         wave_sync();
-        if (kEmit && (addr >= O.max_nodes || rbase > O.max_refs)) {
+        if (kEmit && (addr >= lim_nodes || rbase > lim_refs)) {
             if (lane == 0) atomicOr(&O.counters[C_ERR], 4u);
             break;
         }
-        if (kEmit && e.patch != kNone && lane == 0) O.nodes[2 * e.patch + 1] = (e.axis << 30) | addr;
+        if (kEmit && e.patch != kNone && lane == 0) nodes[2 * e.patch + 1] = (e.axis << 30) | addr;
         maxlvl = max(maxlvl, e.level);
         const uint32_t cnt = (uint32_t)__popcll(e.mask);
         float best_cost = kInf;
@@ -782,18 +887,18 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
             }
         }
         if (best == kNone || !(best_cost < (float)cnt)) {
-            if (kEmit && rbase + cnt > O.max_refs) {
+            if (kEmit && rbase + cnt > lim_refs) {
                 if (lane == 0) atomicOr(&O.counters[C_ERR], 8u);
                 break;
             }
             if (kEmit) {
                 if (lane == 0) {
-                    O.nodes[2 * addr] = cnt;
-                    O.nodes[2 * addr + 1] = (MNRT_LEAF_TAG << 30) | rbase;
+                    nodes[2 * addr] = cnt;
+                    nodes[2 * addr + 1] = (MNRT_LEAF_TAG << 30) | rbase;
                 }
                 if (lane < n && ((e.mask >> lane) & 1ull)) {
                     uint64_t below = (lane == 0) ? 0ull : (e.mask & ((1ull << lane) - 1ull));
-                    O.refs[rbase + (uint32_t)__popcll(below)] = eidx[lane];
+                    refs[rbase + (uint32_t)__popcll(below)] = eidx[lane];
                 }
             }
             ++addr;
@@ -804,8 +909,8 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         const uint32_t a = (best % per) / n;
         const float s = cpos[best];
         if (kEmit && lane == 0) {
-            O.nodes[2 * addr] = __float_as_uint(s);
-            O.nodes[2 * addr + 1] = a << 30;
+            nodes[2 * addr] = __float_as_uint(s);
+            nodes[2 * addr + 1] = a << 30;
         }
         if (lane == 0) {
             SmallStackEnt R = e, L = e;
@@ -828,8 +933,8 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     }
     if (lane == 0) {
         if (kEmit) {
-            atomicAdd(&O.counters[C_LEAVES], leaves);
-            atomicMax(&O.counters[C_MAXLEVEL], maxlvl);
+            atomicAdd(&O.scene_leaves[scene], leaves);
+            atomicMax(&O.scene_maxlvl[scene], maxlvl);
         } else {
             O.nsize[r] = addr;
             O.rsize[r] = rbase;
@@ -886,11 +991,11 @@ __global__ __launch_bounds__(1024) void k_layout(uint32_t maxlvl, const uint32_t
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        Lv.addr[0] = 0;
-        Lv.rbase[0] = 0;
-        counters[C_TOTAL_NODES] = Lv.size[0];
-        counters[C_TOTAL_REFS] = Lv.rsize[0];
+    // every scene's root (level 0) starts its own tree at address 0
+    for (uint32_t j = off[0] + threadIdx.x; j < off[1]; j += blockDim.x) {
+        uint32_t i = list[j];
+        Lv.addr[i] = 0;
+        Lv.rbase[i] = 0;
     }
     __syncthreads();
     for (uint32_t lvl = 0; lvl <= maxlvl; ++lvl) {
@@ -909,12 +1014,20 @@ __global__ __launch_bounds__(1024) void k_layout(uint32_t maxlvl, const uint32_t
 }
 
 __global__ __launch_bounds__(256) void k_emit_large(uint32_t nbn, BnView B, LayoutView Lv, const uint32_t* __restrict__ ll_idx,
-                                                    uint32_t* __restrict__ nodes, uint32_t* __restrict__ refs,
-                                                    uint32_t* __restrict__ counters, uint32_t max_nodes,
-                                                    uint32_t max_refs) {
+                                                    uint32_t* __restrict__ nodes_all, uint32_t* __restrict__ refs_all,
+                                                    uint32_t* __restrict__ counters, uint32_t max_nodes_all,
+                                                    uint32_t max_refs_all, const uint32_t* __restrict__ node_base,
+                                                    const uint32_t* __restrict__ ref_base,
+                                                    uint32_t* __restrict__ scene_leaves,
+                                                    uint32_t* __restrict__ scene_maxlvl) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nbn) return;
     uint32_t t = B.type[i];
+    if (t > BN_LEAF) return;
+    const uint32_t sc = B.scene[i];
+    uint32_t* nodes = nodes_all + 2ull * node_base[sc];
+    uint32_t* refs = refs_all + ref_base[sc];
+    const uint32_t max_nodes = max_nodes_all - node_base[sc], max_refs = max_refs_all - ref_base[sc];
     if (t <= BN_LEAF && (Lv.addr[i] >= max_nodes || (t == BN_LEAF && Lv.rbase[i] + B.cnt[i] > max_refs) ||
                          (t <= 2 && Lv.addr[B.right[i]] >= max_nodes))) {
         atomicOr(&counters[C_ERR], 16u);
@@ -924,39 +1037,29 @@ __global__ __launch_bounds__(256) void k_emit_large(uint32_t nbn, BnView B, Layo
         uint32_t a = Lv.addr[i];
         nodes[2 * a] = B.split[i];
         nodes[2 * a + 1] = (t << 30) | Lv.addr[B.right[i]];
-        atomicMax(&counters[C_MAXLEVEL], B.level[i]);
+        atomicMax(&scene_maxlvl[sc], B.level[i]);
     } else if (t == BN_LEAF) {
         uint32_t a = Lv.addr[i], rb = Lv.rbase[i], m = B.cnt[i];
         nodes[2 * a] = m;
         nodes[2 * a + 1] = (MNRT_LEAF_TAG << 30) | rb;
         for (uint32_t k = 0; k < m; ++k) refs[rb + k] = ll_idx[B.aux[i] + k];
-        atomicAdd(&counters[C_LEAVES], 1u);
-        atomicMax(&counters[C_MAXLEVEL], B.level[i]);
+        atomicAdd(&scene_leaves[sc], 1u);
+        atomicMax(&scene_maxlvl[sc], B.level[i]);
     }
 }
 
-__global__ void k_set_root_small(SmallView S, BnView B, const float4* __restrict__ lo, const float4* __restrict__ hi,
-                                 uint32_t n) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    S.first[0] = 0;
-    S.cnt[0] = n;
-    S.level[0] = 0;
-    S.bn[0] = 0;
-    S.lo[0] = lo[0];
-    S.hi[0] = hi[0];
-    B.type[0] = BN_SMALL;
-    B.aux[0] = 0;
-    B.level[0] = 0;
+__global__ __launch_bounds__(256) void k_scene_counts(uint32_t nscenes, const uint32_t* __restrict__ base,
+                                                      uint32_t* __restrict__ cnt) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nscenes) cnt[s] = base[s + 1] - base[s];
 }
 
-__global__ void k_set_root_active(NodeView A, BnView B, uint32_t n) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    A.first[0] = 0;
-    A.cnt[0] = n;
-    A.level[0] = 0;
-    A.bn[0] = 0;
-    B.type[0] = BN_PENDING;
-    B.level[0] = 0;
+__global__ __launch_bounds__(256) void k_scene_sizes(uint32_t nscenes, LayoutView Lv, uint32_t* __restrict__ nodes,
+                                                     uint32_t* __restrict__ refs) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscenes) return;
+    nodes[s] = Lv.size[s];
+    refs[s] = Lv.rsize[s];
 }
 
 }  // namespace mnrt
@@ -973,13 +1076,13 @@ class Pool {
    public:
     explicit Pool(hipStream_t st) : st_(st) {}
     ~Pool() {
-        for (void* p : ptrs_) (void)hipFreeAsync(p, st_);
+        for (void* p : ptrs_) (void)pool_free(p, st_);
     }
     template <typename T>
     T* get(size_t count) {
         void* p = nullptr;
         size_t bytes = std::max<size_t>(count * sizeof(T), 16);
-        if (hipMallocAsync(&p, bytes, st_) != hipSuccess) {
+        if (pool_alloc(&p, bytes, st_) != hipSuccess) {
             ok = false;
             return nullptr;
         }
@@ -990,7 +1093,7 @@ class Pool {
         if (!p) return;
         auto it = std::find(ptrs_.begin(), ptrs_.end(), p);
         if (it != ptrs_.end()) {
-            (void)hipFreeAsync(p, st_);
+            (void)pool_free(p, st_);
             ptrs_.erase(it);
         }
     }
@@ -1007,121 +1110,92 @@ struct Grow {
 
 inline dim3 blocks_for(uint64_t n, uint32_t bs = 256) { return dim3((uint32_t)std::max<uint64_t>(1, (n + bs - 1) / bs)); }
 
-int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2, uint32_t n,
-               const mnrt_build_params* prm, mnrt_tree* out, hipStream_t st) {
-    if (!out || !prm || (n > 0 && !d_v0)) return MNRT_ERR_ARG;
-    if (kind == MNRT_KIND_TRI && n > 0 && (!d_v1 || !d_v2)) return MNRT_ERR_ARG;
+// Builds the trees of S scenes together: one level loop over the active nodes of all
+// scenes, one small-node launch for all small roots, one layout / emission pass.  Each
+// scene's tree equals the tree a single-scene build of it gives (indices are
+// scene-local, all decisions are per node).
+//   h_base: S + 1 element offsets into the concatenated input.
+//   h_roots: 6 floats per scene (or null: tight bounds).  h_qr: VVH radius per scene
+//   (or null: params->max_query_radius).
+//   out[s] views into shared arrays; out[0] owns them (mnrt_tree_release(&out[0])
+//   frees every tree of the batch).
+int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2, const uint32_t* h_base,
+                     uint32_t nscenes, const mnrt_build_params* prm, const float* h_roots, const float* h_qr,
+                     mnrt_tree* out, hipStream_t st) {
+    if (!out || !prm || !h_base || nscenes == 0) return MNRT_ERR_ARG;
+    const uint32_t ntot = h_base[nscenes];
+    for (uint32_t s = 0; s < nscenes; ++s)
+        if (h_base[s + 1] < h_base[s]) return MNRT_ERR_ARG;
+    if (ntot > 0 && !d_v0) return MNRT_ERR_ARG;
+    if (kind == MNRT_KIND_TRI && ntot > 0 && (!d_v1 || !d_v2)) return MNRT_ERR_ARG;
     if (prm->small_node_max < 1 || prm->small_node_max > 64) return MNRT_ERR_ARG;
-    if (n > (MNRT_INDEX_MASK >> 2)) return MNRT_ERR_CAPACITY;
-    Geometry G{kind, (const float4*)d_v0, (const float4*)d_v1 ? (const float4*)d_v1 : (const float4*)d_v0,
-               (const float4*)d_v2 ? (const float4*)d_v2 : (const float4*)d_v0};
+    if (ntot > (MNRT_INDEX_MASK >> 2)) return MNRT_ERR_CAPACITY;
+    const uint32_t S_ = nscenes;
     Pool pool(st);
     uint32_t* counters = pool.get<uint32_t>(C_COUNT);
-    uint32_t* h_counters = nullptr;
-    MNRT_HIP_CHECK(hipHostMalloc((void**)&h_counters, 64 * sizeof(uint32_t), hipHostMallocDefault));
-    struct HostFree {
-        uint32_t* p;
-        ~HostFree() { (void)hipHostFree(p); }
-    } host_free{h_counters};
-    float* h_box = (float*)(h_counters + 32);
-    float* d_box = (float*)pool.get<float>(8);
+    // per-scene device tables: base (S+1), plan kind/index/off (3S), qr (S), roots given (6S),
+    // root boxes out (6S), node/ref counts + bases (4S + 2), leaves / max level (2S)
+    uint32_t* d_tab = pool.get<uint32_t>(32ull * S_ + 64);
     if (!pool.ok) return MNRT_ERR_NOMEM;
+    uint32_t* d_base = d_tab;
+    uint32_t* d_pkind = d_base + S_ + 1;
+    uint32_t* d_pindex = d_pkind + S_;
+    uint32_t* d_poff = d_pindex + S_;
+    float* d_qr = (float*)(d_poff + S_);
+    float* d_given = d_qr + S_;
+    float* d_rootbox = d_given + 6 * S_;
+    uint32_t* d_snodes = (uint32_t*)(d_rootbox + 6 * S_);
+    uint32_t* d_srefs = d_snodes + S_;
+    uint32_t* d_nbase = d_srefs + S_;      // S + 1 (total at [S])
+    uint32_t* d_rbase = d_nbase + S_ + 1;  // S + 1
+    uint32_t* d_sleaves = d_rbase + S_ + 1;
+    uint32_t* d_smaxlvl = d_sleaves + S_;
+
+    // host staging in pinned memory: counters, the per-scene tables
+    const size_t hwords = 64 + 32ull * S_;
+    uint32_t* h_pin = (uint32_t*)host_staging(0, hwords * 4);
+    if (!h_pin) return MNRT_ERR_NOMEM;
+    uint32_t* h_counters = h_pin;
+    uint32_t* h_tab = h_pin + 64;
+    // RootPlan on the host: large scenes become active roots, small ones small roots
+    uint32_t na = 0, nsmall = 0, ne = 0, nse = 0;
+    {
+        uint32_t* hb = h_tab;
+        uint32_t* hk = hb + S_ + 1;
+        uint32_t* hi = hk + S_;
+        uint32_t* ho = hi + S_;
+        float* hq = (float*)(ho + S_);
+        float* hg = hq + S_;
+        for (uint32_t s = 0; s <= S_; ++s) hb[s] = h_base[s];
+        for (uint32_t s = 0; s < S_; ++s) {
+            const uint32_t cnt = h_base[s + 1] - h_base[s];
+            if (cnt > prm->small_node_max) {
+                hk[s] = 0;
+                hi[s] = na++;
+                ho[s] = ne;
+                ne += cnt;
+            } else {
+                hk[s] = 1;
+                hi[s] = nsmall++;
+                ho[s] = nse;
+                nse += cnt;
+            }
+            hq[s] = h_qr ? h_qr[s] : prm->max_query_radius;
+            for (int k = 0; k < 6; ++k) hg[6 * s + k] = h_roots ? h_roots[6 * s + k] : 0.0f;
+        }
+        MNRT_HIP_CHECK(hipMemcpyAsync(d_tab, h_tab, (size_t)(S_ + 1 + 4 * S_ + 6 * S_) * 4, hipMemcpyHostToDevice, st));
+    }
     MNRT_HIP_CHECK(hipMemsetAsync(counters, 0, C_COUNT * sizeof(uint32_t), st));
+    MNRT_HIP_CHECK(hipMemsetAsync(d_sleaves, 0, 2ull * S_ * 4, st));
+    Geometry G{kind, (const float4*)d_v0, d_v1 ? (const float4*)d_v1 : (const float4*)d_v0,
+               d_v2 ? (const float4*)d_v2 : (const float4*)d_v0, d_base, S_};
+    RootPlan RP{d_pkind, d_pindex, d_poff};
 
     auto sync_counters = [&]() -> int {
         MNRT_HIP_CHECK(hipMemcpyAsync(h_counters, counters, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         MNRT_HIP_CHECK(hipStreamSynchronize(st));
         return MNRT_OK;
     };
-
-    // ---- root elements and box ----
-    ElemView E{};
-    uint32_t ne = n;
-    E.idx = pool.get<uint32_t>(ne);
-    E.lo = pool.get<float4>(ne);
-    E.hi = kind == MNRT_KIND_TRI ? pool.get<float4>(ne) : E.lo;
-    E.node = pool.get<uint32_t>(ne);
-    NodeView A{};
-    uint32_t na = 1;
-    A.lo = pool.get<float4>(1);
-    A.hi = pool.get<float4>(1);
-    A.first = pool.get<uint32_t>(1);
-    A.cnt = pool.get<uint32_t>(1);
-    A.level = pool.get<uint32_t>(1);
-    A.bn = pool.get<uint32_t>(1);
-    if (!pool.ok) return MNRT_ERR_NOMEM;
-    if (n > 0) hipLaunchKernelGGL(k_init_elems, blocks_for(n), dim3(256), 0, st, G, n, E);
-
-    uint32_t* tlo = nullptr;
-    uint32_t* thi = nullptr;
-    float root[6];
-    float4 root_lo4, root_hi4;
-    uint32_t zero = 0, n_copy = n;
-    if (prm->has_root_box || n == 0) {
-        for (int k = 0; k < 3; ++k) {
-            root[k] = prm->has_root_box ? prm->root_min[k] + 0.0f : kInf;
-            root[3 + k] = prm->has_root_box ? prm->root_max[k] + 0.0f : -kInf;
-        }
-        root_lo4 = make_float4(root[0], root[1], root[2], 0.0f);
-        root_hi4 = make_float4(root[3], root[4], root[5], 0.0f);
-        MNRT_HIP_CHECK(hipMemcpyAsync(A.lo, &root_lo4, sizeof(float4), hipMemcpyHostToDevice, st));
-        MNRT_HIP_CHECK(hipMemcpyAsync(A.hi, &root_hi4, sizeof(float4), hipMemcpyHostToDevice, st));
-        MNRT_HIP_CHECK(hipStreamSynchronize(st));
-    } else {
-        // tight bounds of all elements: one node covering everything
-        uint32_t nch = (n + kChunk - 1) / kChunk;
-        uint32_t* cc = pool.get<uint32_t>(1);
-        uint32_t* chn = pool.get<uint32_t>(nch);
-        uint32_t* chf = pool.get<uint32_t>(nch);
-        uint32_t* chc = pool.get<uint32_t>(nch);
-        tlo = pool.get<uint32_t>(3);
-        thi = pool.get<uint32_t>(3);
-        uint32_t* sc = pool.get<uint32_t>(scan_scratch_words(1));
-        if (!pool.ok) return MNRT_ERR_NOMEM;
-        MNRT_HIP_CHECK(hipMemcpyAsync(A.first, &zero, 4, hipMemcpyHostToDevice, st));
-        MNRT_HIP_CHECK(hipMemcpyAsync(A.cnt, &n_copy, 4, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_chunk_counts, dim3(1), dim3(256), 0, st, 1u, A.cnt, cc);
-        MNRT_HIP_CHECK(scan_exclusive(cc, cc, 1, counters + C_NCHUNKS, sc, st));
-        hipLaunchKernelGGL(k_chunk_fill, blocks_for(nch), dim3(256), 0, st, 1u, nch, cc, counters, A.first, A.cnt,
-                           chn, chf, chc);
-        hipLaunchKernelGGL(k_tight_init, dim3(1), dim3(256), 0, st, 1u, tlo, thi);
-        hipLaunchKernelGGL(k_tight, dim3(nch), dim3(kChunk), 0, st, nch, counters, chn, chf, chc, E, kind, tlo, thi);
-        hipLaunchKernelGGL(k_root_box_from_tight, dim3(1), dim3(1), 0, st, tlo, thi, A.lo, A.hi, d_box);
-        MNRT_HIP_CHECK(hipMemcpyAsync(h_box, d_box, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
-        MNRT_HIP_CHECK(hipStreamSynchronize(st));
-        for (int k = 0; k < 6; ++k) root[k] = h_box[k];
-        pool.release(cc);
-        pool.release(chn);
-        pool.release(chf);
-        pool.release(chc);
-        pool.release(sc);
-    }
-
-    // ---- growable tables ----
-    uint32_t bn_cap = std::max<uint32_t>(1024, 8 * (n / std::max<uint32_t>(1, prm->small_node_max)) + 64);
-    uint32_t* bn_buf = pool.get<uint32_t>((size_t)7 * bn_cap);
-    auto bn_view = [&](uint32_t* buf, uint32_t cap) {
-        return BnView{buf, buf + cap, buf + 2ull * cap, buf + 3ull * cap, buf + 4ull * cap, buf + 5ull * cap,
-                      buf + 6ull * cap};
-    };
-    BnView B = bn_view(bn_buf, bn_cap);
-    uint32_t nbn = 1;
-
-    uint32_t s_cap = std::max<uint32_t>(256, 4 * (n / std::max<uint32_t>(1, prm->small_node_max)) + 16);
-    auto alloc_small = [&](uint32_t cap) {
-        SmallView S{};
-        S.first = pool.get<uint32_t>(cap);
-        S.cnt = pool.get<uint32_t>(cap);
-        S.level = pool.get<uint32_t>(cap);
-        S.bn = pool.get<uint32_t>(cap);
-        S.lo = pool.get<float4>(cap);
-        S.hi = pool.get<float4>(cap);
-        return S;
-    };
-    SmallView S = alloc_small(s_cap);
-    uint32_t nsmall = 0;
-
-    uint32_t se_cap = std::max<uint32_t>(1024, n + n / 2);
     auto alloc_elems = [&](uint32_t cap, bool with_node) {
         ElemView V{};
         V.idx = pool.get<uint32_t>(cap);
@@ -1137,31 +1211,96 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
         pool.release(V.node);
         V = ElemView{};
     };
+    auto alloc_nodes = [&](uint32_t cap) {
+        NodeView V{};
+        V.lo = pool.get<float4>(cap);
+        V.hi = pool.get<float4>(cap);
+        V.first = pool.get<uint32_t>(cap);
+        V.cnt = pool.get<uint32_t>(cap);
+        V.level = pool.get<uint32_t>(cap);
+        V.bn = pool.get<uint32_t>(cap);
+        V.scene = pool.get<uint32_t>(cap);
+        return V;
+    };
+    auto release_nodes = [&](NodeView& V) {
+        pool.release(V.lo);
+        pool.release(V.hi);
+        pool.release(V.first);
+        pool.release(V.cnt);
+        pool.release(V.level);
+        pool.release(V.bn);
+        pool.release(V.scene);
+        V = NodeView{};
+    };
+    auto alloc_small = [&](uint32_t cap) {
+        SmallView V{};
+        V.first = pool.get<uint32_t>(cap);
+        V.cnt = pool.get<uint32_t>(cap);
+        V.level = pool.get<uint32_t>(cap);
+        V.bn = pool.get<uint32_t>(cap);
+        V.lo = pool.get<float4>(cap);
+        V.hi = pool.get<float4>(cap);
+        V.scene = pool.get<uint32_t>(cap);
+        return V;
+    };
+
+    // ---- input elements, per-scene tight bounds, root placement ----
+    ElemView Ein = alloc_elems(std::max<uint32_t>(ntot, 1), true);
+    if (!pool.ok) return MNRT_ERR_NOMEM;
+    if (ntot > 0) hipLaunchKernelGGL(k_init_elems, blocks_for(ntot), dim3(256), 0, st, G, ntot, Ein);
+    uint32_t* tlo = nullptr;
+    uint32_t* thi = nullptr;
+    if (!h_roots && ntot > 0) {
+        // one node per scene over the concatenated elements
+        uint32_t max_chunks = ntot / kChunk + S_;
+        uint32_t* tb = pool.get<uint32_t>(3ull * max_chunks + 8ull * S_ + scan_scratch_words(S_) + 16);
+        if (!pool.ok) return MNRT_ERR_NOMEM;
+        uint32_t *chn = tb, *chf = chn + max_chunks, *chc = chf + max_chunks, *cc = chc + max_chunks;
+        uint32_t* cnts = cc + S_;
+        tlo = cnts + S_;
+        thi = tlo + 3 * S_;
+        uint32_t* sc = thi + 3 * S_;
+        hipLaunchKernelGGL(k_scene_counts, blocks_for(S_), dim3(256), 0, st, S_, d_base, cnts);
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, S_, cnts, cc);
+        MNRT_HIP_CHECK(scan_exclusive(cc, cc, S_, counters + C_NCHUNKS, sc, st));
+        hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, S_, max_chunks, cc, counters,
+                           d_base, cnts, chn, chf, chc);
+        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * S_), dim3(256), 0, st, S_, tlo, thi);
+        hipLaunchKernelGGL(k_tight, dim3(max_chunks), dim3(kChunk), 0, st, max_chunks, counters, chn, chf, chc, Ein,
+                           kind, tlo, thi);
+    }
+    // growable tables
+    uint32_t bn_cap = std::max<uint32_t>(1024, S_ + 8 * (ntot / std::max<uint32_t>(1, prm->small_node_max)) + 64);
+    uint32_t* bn_buf = pool.get<uint32_t>((size_t)kBnCols * bn_cap);
+    auto bn_view = [&](uint32_t* buf, uint32_t cap) {
+        return BnView{buf,           buf + cap,           buf + 2ull * cap, buf + 3ull * cap, buf + 4ull * cap,
+                      buf + 5ull * cap, buf + 6ull * cap, buf + 7ull * cap};
+    };
+    BnView B = bn_view(bn_buf, bn_cap);
+    uint32_t nbn = S_;
+    uint32_t s_cap = std::max<uint32_t>(256, nsmall + 4 * (ntot / std::max<uint32_t>(1, prm->small_node_max)) + 16);
+    SmallView SV = alloc_small(s_cap);
+    uint32_t se_cap = std::max<uint32_t>(1024, nse + ne + ne / 2);
     ElemView Sm = alloc_elems(se_cap, false);
-    uint32_t nse = 0;
+    uint32_t a_cap = std::max<uint32_t>(na, 1);
+    NodeView A = alloc_nodes(a_cap);
+    uint32_t e_cap = std::max<uint32_t>(ne, 1);
+    ElemView E = alloc_elems(e_cap, true);
+    if (!pool.ok) return MNRT_ERR_NOMEM;
+    hipLaunchKernelGGL(k_roots_setup, blocks_for(S_), dim3(256), 0, st, S_, d_base, tlo, thi,
+                       h_roots ? (const float*)d_given : (const float*)nullptr, RP, A, SV, B, d_rootbox);
+    if (ntot > 0) hipLaunchKernelGGL(k_partition_elems, blocks_for(ntot), dim3(256), 0, st, ntot, Ein, kind, RP, E, Sm);
+    MNRT_HIP_CHECK(hipGetLastError());
+    release_elems(Ein);
+    if (ne == 0) {
+        release_elems(E);
+        e_cap = 0;
+    }
 
     uint32_t ll_cap = 0;
     uint32_t* ll_idx = nullptr;
     uint32_t ll_count = 0;
-    if (!pool.ok) return MNRT_ERR_NOMEM;
-
     uint32_t maxlvl = 0;
-
-    if (n <= prm->small_node_max) {
-        // the root itself is a small root
-        if (n > 0) {
-            MNRT_HIP_CHECK(hipMemcpyAsync(Sm.idx, E.idx, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-            MNRT_HIP_CHECK(hipMemcpyAsync(Sm.lo, E.lo, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
-            if (kind == MNRT_KIND_TRI)
-                MNRT_HIP_CHECK(hipMemcpyAsync(Sm.hi, E.hi, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
-        }
-        hipLaunchKernelGGL(k_set_root_small, dim3(1), dim3(1), 0, st, S, B, A.lo, A.hi, n);
-        nsmall = 1;
-        nse = n;
-        na = 0;
-    } else {
-        hipLaunchKernelGGL(k_set_root_active, dim3(1), dim3(1), 0, st, A, B, n);
-    }
 
     // ---- large-node stage ----
     // Workspaces grow geometrically and are reused across levels; next-level arrays are
@@ -1176,33 +1315,12 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
     uint32_t nx_cap = 0;
     NodeView An{};
     uint32_t an_cap = 0;
-    auto release_nodes = [&](NodeView& V) {
-        pool.release(V.lo);
-        pool.release(V.hi);
-        pool.release(V.first);
-        pool.release(V.cnt);
-        pool.release(V.level);
-        pool.release(V.bn);
-        V = NodeView{};
-    };
-    auto alloc_nodes = [&](uint32_t cap) {
-        NodeView V{};
-        V.lo = pool.get<float4>(cap);
-        V.hi = pool.get<float4>(cap);
-        V.first = pool.get<uint32_t>(cap);
-        V.cnt = pool.get<uint32_t>(cap);
-        V.level = pool.get<uint32_t>(cap);
-        V.bn = pool.get<uint32_t>(cap);
-        return V;
-    };
-    uint32_t e_cap = n, a_cap = 1;
     while (na > 0) {
-        // capacity for this level's new build nodes / small roots / leaf copies
         if (nbn + 14ull * na > bn_cap) {
             uint32_t ncap = std::max<uint32_t>(2 * bn_cap, nbn + 14 * na + 1024);
-            uint32_t* nb = pool.get<uint32_t>((size_t)7 * ncap);
+            uint32_t* nb = pool.get<uint32_t>((size_t)kBnCols * ncap);
             if (!pool.ok) return MNRT_ERR_NOMEM;
-            for (int col = 0; col < 7; ++col)
+            for (int col = 0; col < kBnCols; ++col)
                 MNRT_HIP_CHECK(hipMemcpyAsync(nb + (size_t)col * ncap, bn_buf + (size_t)col * bn_cap, (size_t)nbn * 4,
                                               hipMemcpyDeviceToDevice, st));
             pool.release(bn_buf);
@@ -1214,19 +1332,21 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
             uint32_t ncap = std::max<uint32_t>(2 * s_cap, nsmall + 2 * na + 256);
             SmallView NS = alloc_small(ncap);
             if (!pool.ok) return MNRT_ERR_NOMEM;
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.first, S.first, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.cnt, S.cnt, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.level, S.level, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.bn, S.bn, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.lo, S.lo, (size_t)nsmall * 16, hipMemcpyDeviceToDevice, st));
-            MNRT_HIP_CHECK(hipMemcpyAsync(NS.hi, S.hi, (size_t)nsmall * 16, hipMemcpyDeviceToDevice, st));
-            pool.release(S.first);
-            pool.release(S.cnt);
-            pool.release(S.level);
-            pool.release(S.bn);
-            pool.release(S.lo);
-            pool.release(S.hi);
-            S = NS;
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.first, SV.first, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.cnt, SV.cnt, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.level, SV.level, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.bn, SV.bn, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.lo, SV.lo, (size_t)nsmall * 16, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.hi, SV.hi, (size_t)nsmall * 16, hipMemcpyDeviceToDevice, st));
+            MNRT_HIP_CHECK(hipMemcpyAsync(NS.scene, SV.scene, (size_t)nsmall * 4, hipMemcpyDeviceToDevice, st));
+            pool.release(SV.first);
+            pool.release(SV.cnt);
+            pool.release(SV.level);
+            pool.release(SV.bn);
+            pool.release(SV.lo);
+            pool.release(SV.hi);
+            pool.release(SV.scene);
+            SV = NS;
             s_cap = ncap;
         }
         if (ll_count + (uint64_t)ne > ll_cap) {
@@ -1260,16 +1380,17 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
             an_cap = std::max<uint32_t>(2 * na, 256);
             An = alloc_nodes(an_cap);
         }
-        // per-level temporaries, one buffer
         const uint32_t max_chunks = ne / kChunk + na;
         if (na > lv_na_cap || ne > lv_ne_cap) {
             pool.release(lv_buf);
             lv_na_cap = std::max<uint32_t>(na, lv_na_cap + lv_na_cap / 2);
             lv_ne_cap = std::max<uint32_t>(ne, lv_ne_cap + lv_ne_cap / 2);
             const uint64_t A_ = lv_na_cap, E_ = lv_ne_cap, C_ = E_ / kChunk + A_;
-            const uint64_t words = A_ /*cc*/ + 3 * C_ /*chunks*/ + 6 * A_ /*tight*/ + 14 * A_ /*plan*/ +
-                                   2 * (2 * E_ + 1) /*fl, pos*/ + 22 * A_ /*child*/ +
-                                   scan_scratch_words((uint32_t)std::max<uint64_t>(2 * E_ + 1, 2 * A_)) + 64;
+            auto r4 = [](uint64_t k) { return (k + 3) & ~3ull; };
+            // the columns handed out below, each rounded to 16 bytes
+            const uint64_t words = r4(A_) + 3 * r4(C_) + 2 * r4(3 * A_) + 9 * r4(A_) + 2 * r4(4 * A_) +
+                                   2 * r4(2 * E_ + 1) + r4(22 * A_) +
+                                   scan_scratch_words((uint32_t)std::max<uint64_t>(2 * E_ + 1, 2 * A_)) + 16;
             lv_buf = pool.get<uint32_t>(words);
             if (!pool.ok) return MNRT_ERR_NOMEM;
         }
@@ -1278,7 +1399,7 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
             const uint64_t A_ = lv_na_cap, E_ = lv_ne_cap, C_ = E_ / kChunk + A_;
             auto take = [&](uint64_t k) {
                 uint32_t* r = w;
-                w += (k + 3) & ~3ull;  // keep 16-byte alignment for float4 columns
+                w += (k + 3) & ~3ull;  // 16-byte alignment for float4 columns
                 return r;
             };
             cc = take(A_);
@@ -1317,8 +1438,6 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
             Cv.soff = cbuf + 9 * m;
             Cv.loff = cbuf + 10 * m;
         }
-
-        // chunks + tight bounds
         hipLaunchKernelGGL(k_chunk_counts, blocks_for(na), dim3(256), 0, st, na, A.cnt, cc);
         MNRT_HIP_CHECK(scan_exclusive(cc, cc, na, counters + C_NCHUNKS, scr, st));
         hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, na, max_chunks, cc, counters,
@@ -1326,27 +1445,24 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
         hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * na), dim3(256), 0, st, na, tlo, thi);
         hipLaunchKernelGGL(k_tight, dim3(max_chunks), dim3(kChunk), 0, st, max_chunks, counters, chn, chf, chc, E,
                            kind, tlo, thi);
-        // cuts + median split
         hipLaunchKernelGGL((k_plan<false>), blocks_for(na), dim3(256), 0, st, na, A, tlo, thi,
                            prm->empty_space_ratio, P, (const uint32_t*)nullptr, 0u, B, counters);
         MNRT_HIP_CHECK(scan_exclusive(P.newcnt, P.newcnt, na, counters + C_NEWNODES, scr, st));
         hipLaunchKernelGGL((k_plan<true>), blocks_for(na), dim3(256), 0, st, na, A, tlo, thi,
                            prm->empty_space_ratio, P, (const uint32_t*)P.newcnt, nbn, B, counters);
-        // classification + compaction positions
         hipLaunchKernelGGL(k_classify, blocks_for(ne), dim3(256), 0, st, ne, E, kind, P, fl);
         MNRT_HIP_CHECK(scan_exclusive(fl, pos, 2 * ne, pos + 2 * ne, scr, st));
-        // children
         hipLaunchKernelGGL(k_child_plan, blocks_for(2ull * na), dim3(256), 0, st, na, ne, A, P, pos,
                            prm->small_node_max, Cv);
         MNRT_HIP_CHECK(scan_exclusive(Cv.sflag, Cv.sidx, 2 * na, counters + C_NSMALL_NEW, scr, st));
         MNRT_HIP_CHECK(scan_exclusive(Cv.lflag, Cv.lidx, 2 * na, counters + C_NNEXT, scr, st));
         MNRT_HIP_CHECK(scan_exclusive(Cv.scnt, Cv.soff, 2 * na, counters + C_ESMALL_NEW, scr, st));
         MNRT_HIP_CHECK(scan_exclusive(Cv.lcnt, Cv.loff, 2 * na, counters + C_ENEXT, scr, st));
-        hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, na, P, Cv, nsmall, nse, S, An, B,
-                           counters);
+        hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, na, A, P, Cv, nsmall, nse, SV, An,
+                           B, counters);
         hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, na, A, P, E, ll_idx, B, counters);
-        hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, na, ne, E, G, P, Cv, pos, nse, Sm, se_cap, Nx,
-                           nx_cap, counters);
+        hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, na, ne, E, G, A, P, Cv, pos, nse, Sm, se_cap,
+                           Nx, nx_cap, counters);
         MNRT_HIP_CHECK(hipGetLastError());
         int rc = sync_counters();
         if (rc != MNRT_OK) return rc;
@@ -1357,7 +1473,6 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
         nse += h_counters[C_ESMALL_NEW];
         ll_count = h_counters[C_LLCOUNT];
         maxlvl = h_counters[C_MAXLEVEL];
-        // next level: swap the ping-pong arrays
         std::swap(E, Nx);
         std::swap(e_cap, nx_cap);
         std::swap(A, An);
@@ -1379,18 +1494,16 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
     O.counters = counters;
     if (nsmall > 0)
         hipLaunchKernelGGL((k_small<false>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
-                           st, nsmall, S, Sm, kind, prm->max_query_radius, O);
+                           st, nsmall, SV, Sm, kind, (const float*)d_qr, O);
     // ---- layout ----
     LayoutView Lv{};
     Lv.size = pool.get<uint32_t>(nbn);
     Lv.rsize = pool.get<uint32_t>(nbn);
     Lv.addr = pool.get<uint32_t>(nbn);
     Lv.rbase = pool.get<uint32_t>(nbn);
+    uint32_t* lvb = pool.get<uint32_t>(64 + 65 + 64 + nbn + scan_scratch_words(std::max<uint32_t>(64, S_ + 1)));
     if (!pool.ok) return MNRT_ERR_NOMEM;
     {
-        // level buckets: 64 counts, their exclusive scan (+ total at [64]), node lists
-        uint32_t* lvb = pool.get<uint32_t>(64 + 65 + 64 + nbn + scan_scratch_words(64));
-        if (!pool.ok) return MNRT_ERR_NOMEM;
         uint32_t* hist = lvb;
         uint32_t* off = lvb + 64;
         uint32_t* cursor = off + 65;
@@ -1403,54 +1516,77 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
         hipLaunchKernelGGL(k_level_scatter, blocks_for(nbn), dim3(256), 0, st, nbn, B, cursor, list);
         hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, st, std::min<uint32_t>(maxlvl, 63), off, list, B,
                            s_nsize, s_rsize, Lv, counters);
+        // per-scene sizes (root = build node s) and their offsets in the shared arrays
+        hipLaunchKernelGGL(k_scene_sizes, blocks_for(S_), dim3(256), 0, st, S_, Lv, d_snodes, d_srefs);
+        MNRT_HIP_CHECK(scan_exclusive(d_snodes, d_nbase, S_, d_nbase + S_, sc, st));
+        MNRT_HIP_CHECK(scan_exclusive(d_srefs, d_rbase, S_, d_rbase + S_, sc, st));
+        MNRT_HIP_CHECK(hipMemcpyAsync(h_counters + 40, d_nbase + S_, 4, hipMemcpyDeviceToHost, st));
+        MNRT_HIP_CHECK(hipMemcpyAsync(h_counters + 41, d_rbase + S_, 4, hipMemcpyDeviceToHost, st));
+        MNRT_HIP_CHECK(hipStreamSynchronize(st));
     }
-    int rc = sync_counters();
-    if (rc != MNRT_OK) return rc;
-    const uint32_t total_nodes = h_counters[C_TOTAL_NODES], total_refs = h_counters[C_TOTAL_REFS];
+    const uint32_t total_nodes = h_counters[40], total_refs = h_counters[41];
     if (total_nodes == 0 || total_nodes > MNRT_INDEX_MASK || total_refs > MNRT_INDEX_MASK) return MNRT_ERR_CAPACITY;
 
-    // ---- emission into the caller-visible tree ----
+    // ---- emission into the caller-visible trees ----
     uint32_t* d_nodes = nullptr;
     uint32_t* d_refs = nullptr;
-    MNRT_HIP_CHECK(hipMalloc((void**)&d_nodes, (size_t)total_nodes * 8));
-    if (hipMalloc((void**)&d_refs, std::max<size_t>((size_t)total_refs * 4, 16)) != hipSuccess) {
-        (void)hipFree(d_nodes);
+    if (pool_alloc((void**)&d_nodes, (size_t)total_nodes * 8, st) != hipSuccess) return MNRT_ERR_NOMEM;
+    if (pool_alloc((void**)&d_refs, (size_t)total_refs * 4, st) != hipSuccess) {
+        (void)pool_free(d_nodes, st);
         return MNRT_ERR_NOMEM;
     }
-    MNRT_HIP_CHECK(hipMemsetAsync(counters + C_LEAVES, 0, 4, st));
-    MNRT_HIP_CHECK(hipMemsetAsync(counters + C_MAXLEVEL, 0, 4, st));
     hipLaunchKernelGGL(k_emit_large, blocks_for(nbn), dim3(256), 0, st, nbn, B, Lv, ll_idx, d_nodes, d_refs, counters,
-                       total_nodes, total_refs);
+                       total_nodes, total_refs, d_nbase, d_rbase, d_sleaves, d_smaxlvl);
     O.bn_addr = Lv.addr;
     O.bn_rbase = Lv.rbase;
     O.nodes = d_nodes;
     O.refs = d_refs;
     O.max_nodes = total_nodes;
     O.max_refs = total_refs;
+    O.node_base = d_nbase;
+    O.ref_base = d_rbase;
+    O.scene_leaves = d_sleaves;
+    O.scene_maxlvl = d_smaxlvl;
     if (nsmall > 0)
         hipLaunchKernelGGL((k_small<true>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
-                           st, nsmall, S, Sm, kind, prm->max_query_radius, O);
-    rc = sync_counters();
+                           st, nsmall, SV, Sm, kind, (const float*)d_qr, O);
+    // per-scene results to the host: counts, bases, leaves, max level, root boxes
+    uint32_t* h_res = h_tab;
+    int rc = MNRT_OK;
+    if (hipMemcpyAsync(h_res, d_snodes, (size_t)(2 * S_ + 2 * (S_ + 1) + 2 * S_) * 4, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(h_res + 8 * S_ + 8, d_rootbox, (size_t)6 * S_ * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = MNRT_ERR_HIP;
+    if (rc == MNRT_OK) rc = sync_counters();
     if (rc != MNRT_OK || hipGetLastError() != hipSuccess || h_counters[C_ERR] != 0) {
-        (void)hipFree(d_nodes);
-        (void)hipFree(d_refs);
+        (void)pool_free(d_nodes, st);
+        (void)pool_free(d_refs, st);
         return rc != MNRT_OK ? rc : (h_counters[C_ERR] != 0 ? MNRT_ERR_STATE : MNRT_ERR_HIP);
     }
-    out->kind = (uint32_t)kind;
-    out->num_nodes = total_nodes;
-    out->num_refs = total_refs;
-    out->num_elems = n;
-    out->num_leaves = h_counters[C_LEAVES];
-    out->max_level = h_counters[C_MAXLEVEL];
-    for (int k = 0; k < 3; ++k) {
-        out->root_min[k] = root[k];
-        out->root_max[k] = root[3 + k];
+    const uint32_t* hn = h_res;
+    const uint32_t* hr = hn + S_;
+    const uint32_t* hnb = hr + S_;
+    const uint32_t* hrb = hnb + S_ + 1;
+    const uint32_t* hl = hrb + S_ + 1;
+    const uint32_t* hm = hl + S_;
+    const float* hbox = (const float*)(h_res + 8 * S_ + 8);
+    for (uint32_t s = 0; s < S_; ++s) {
+        mnrt_tree& t = out[s];
+        std::memset(&t, 0, sizeof(t));
+        t.kind = (uint32_t)kind;
+        t.num_nodes = hn[s];
+        t.num_refs = hr[s];
+        t.num_elems = h_base[s + 1] - h_base[s];
+        t.num_leaves = hl[s];
+        t.max_level = hm[s];
+        for (int k = 0; k < 3; ++k) {
+            t.root_min[k] = hbox[6 * s + k];
+            t.root_max[k] = hbox[6 * s + 3 + k];
+        }
+        t.d_nodes = d_nodes + 2ull * hnb[s];
+        t.d_refs = d_refs + hrb[s];
+        t.owns_nodes = (s == 0) ? 1u : 0u;
     }
-    out->d_nodes = d_nodes;
-    out->d_refs = d_refs;
-    out->d_recs = nullptr;
-    out->owns_nodes = 1;
-    out->owns_recs = 0;
     return MNRT_OK;
 }
 
@@ -1458,8 +1594,18 @@ int build_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2
 
 extern "C" int mnrt_build_tri(const float* d_v0, const float* d_v1, const float* d_v2, uint32_t n,
                               const mnrt_build_params* params, mnrt_tree* out, void* stream) {
-    hipStream_t st = (hipStream_t)stream;
-    int rc = build_impl(MNRT_KIND_TRI, d_v0, d_v1, d_v2, n, params, out, st);
+    if (!params || !out) return MNRT_ERR_ARG;
+    uint32_t base[2] = {0, n};
+    const float* root = params->has_root_box ? params->root_min : nullptr;
+    float r6[6];
+    if (root) {
+        for (int k = 0; k < 3; ++k) {
+            r6[k] = params->root_min[k];
+            r6[3 + k] = params->root_max[k];
+        }
+    }
+    int rc = build_batch_impl(MNRT_KIND_TRI, d_v0, d_v1, d_v2, base, 1, params, root ? r6 : nullptr, nullptr, out,
+                              (hipStream_t)stream);
     if (rc != MNRT_OK) return rc;
     rc = mnrt_tree_attach_tri_records(out, d_v0, d_v1, d_v2, stream);
     if (rc != MNRT_OK) mnrt_tree_release(out);
@@ -1468,5 +1614,31 @@ extern "C" int mnrt_build_tri(const float* d_v0, const float* d_v1, const float*
 
 extern "C" int mnrt_build_points(const float* d_pos4, uint32_t n, const mnrt_build_params* params, mnrt_tree* out,
                                  void* stream) {
-    return build_impl(MNRT_KIND_POINT, d_pos4, nullptr, nullptr, n, params, out, (hipStream_t)stream);
+    if (!params || !out) return MNRT_ERR_ARG;
+    uint32_t base[2] = {0, n};
+    float r6[6];
+    for (int k = 0; k < 3; ++k) {
+        r6[k] = params->root_min[k];
+        r6[3 + k] = params->root_max[k];
+    }
+    return build_batch_impl(MNRT_KIND_POINT, d_pos4, nullptr, nullptr, base, 1, params,
+                            params->has_root_box ? r6 : nullptr, nullptr, out, (hipStream_t)stream);
+}
+
+extern "C" int mnrt_build_tri_batch(const float* d_v0, const float* d_v1, const float* d_v2,
+                                    const uint32_t* h_offsets, uint32_t num_scenes, const mnrt_build_params* params,
+                                    const float* h_root_boxes, mnrt_tree* out, void* stream) {
+    int rc = build_batch_impl(MNRT_KIND_TRI, d_v0, d_v1, d_v2, h_offsets, num_scenes, params, h_root_boxes, nullptr,
+                              out, (hipStream_t)stream);
+    if (rc != MNRT_OK) return rc;
+    rc = mnrt_tree_attach_tri_records_batch(out, num_scenes, d_v0, d_v1, d_v2, h_offsets, stream);
+    if (rc != MNRT_OK) mnrt_tree_release(out);
+    return rc;
+}
+
+extern "C" int mnrt_build_points_batch(const float* d_pos4, const uint32_t* h_offsets, uint32_t num_scenes,
+                                       const mnrt_build_params* params, const float* h_root_boxes,
+                                       const float* h_max_query_radius, mnrt_tree* out, void* stream) {
+    return build_batch_impl(MNRT_KIND_POINT, d_pos4, nullptr, nullptr, h_offsets, num_scenes, params, h_root_boxes,
+                            h_max_query_radius, out, (hipStream_t)stream);
 }

@@ -22,38 +22,12 @@
 #include <cmath>
 #include <cstring>
 
-#include "mnrt_common.h"
+#include "batch_dev.h"
 
 namespace mnrt {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kKnnStack = 64;
-
-struct KnnArgs {
-    const uint2* nodes;
-    const float4* recs;  // 2 float4 per reference
-    uint32_t k;
-    float r2max;
-    float r_max;
-    uint32_t n;
-    const float4* pos;
-    const float4* nrm;
-    const float4* dir;
-    const float4* infl;
-    const float4* clr;
-    float4* rad_out;      // accumulate target (slot = query), or per-query temp when idx_sp
-    uint32_t accumulate;  // 1: rad_out[q] += rad, 0: rad_out[q] = rad
-    float* out_r2;
-    float4* out_irr;
-    uint32_t* out_nbr;
-    uint32_t* out_count;
-    float* out_radius;
-    // backward
-    const float4* grad;            // dLoss/dradiance per query
-    unsigned long long* acc;       // 3 fixed-point accumulators per photon (original index)
-    double fix_scale;              // accumulator units per unit gradient (a power of two)
-    float4* out_grad_clr;          // dLoss/d(diffuse colour) per query
-};
 
 enum : int { kModeRadii = 0, kModeRadiance = 1, kModeBackward = 2 };
 
@@ -86,22 +60,10 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t v) {
     return p;
 }
 
+// One query by one wave; keys / kpos: CAP LDS entries, snode / soff: kKnnStack.
 template <int CAP, int kMode>
-__global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
-    __shared__ uint64_t s_key[kWavesPerBlock][CAP];
-    __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
-    __shared__ uint32_t s_snode[kWavesPerBlock][kKnnStack];
-    __shared__ float4 s_soff[kWavesPerBlock][kKnnStack];
-
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t q = blockIdx.x * kWavesPerBlock + wave;
-    if (q >= A.n) return;  // uniform per wave
-    uint64_t* keys = s_key[wave];
-    uint32_t* kpos = s_pos[wave];
-    uint32_t* snode = s_snode[wave];
-    float4* soff = s_soff[wave];
-
+__device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t lane, uint64_t* keys, uint32_t* kpos,
+                                          uint32_t* snode, float4* soff) {
     const float4 qp4 = A.pos[q];
     const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
     const f3 qp = mk3(qx, qy, qz);
@@ -304,6 +266,65 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
     }
 }
 
+template <int CAP, int kMode>
+__global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
+    __shared__ uint64_t s_key[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_snode[kWavesPerBlock][kKnnStack];
+    __shared__ float4 s_soff[kWavesPerBlock][kKnnStack];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t q = blockIdx.x * kWavesPerBlock + wave;
+    if (q >= A.n) return;  // uniform per wave
+    knn_query<CAP, kMode>(A, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
+}
+
+// Batched radiance: waves take chunks of `chunk` consecutive query slots from a work
+// counter (one counter per XCD share, as k_trace_batch), slot g belonging to the scene
+// whose slot range holds it.  Slots at or past a scene's count (count[s], device side:
+// the number of shading points the batched compaction wrote) are left untouched.
+template <int CAP, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_knn_batch(KnnArgs G, const KnnScene* __restrict__ scenes,
+                                                           const uint32_t* __restrict__ slot_begin, uint32_t ns,
+                                                           const uint32_t* __restrict__ count, uint32_t chunk,
+                                                           uint32_t num_parts, const uint32_t* __restrict__ part_begin,
+                                                           const uint32_t* __restrict__ part_end,
+                                                           uint32_t* __restrict__ work) {
+    __shared__ uint64_t s_key[kWaves][CAP];
+    __shared__ uint32_t s_pos[kWaves][CAP];
+    __shared__ uint32_t s_snode[kWaves][kKnnStack];
+    __shared__ float4 s_soff[kWaves][kKnnStack];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t xcd = blockIdx.x % num_parts;
+    for (uint32_t part = 0; part < num_parts; ++part) {
+        const uint32_t p = (xcd + part) % num_parts;
+        const uint32_t pb = part_begin[p], pe = part_end[p];
+        for (;;) {
+            uint32_t c = 0;
+            if (lane == 0) c = atomicAdd(&work[p], 1u);
+            c = uniform(__shfl(c, 0, 64));
+            const uint64_t g0 = (uint64_t)pb + (uint64_t)c * chunk;
+            if (g0 >= pe) break;
+            const uint32_t g1 = (uint32_t)min<uint64_t>(g0 + chunk, pe);
+            uint32_t s = uniform(seg_of(slot_begin, ns, (uint32_t)g0));
+            for (uint32_t g = (uint32_t)g0; g < g1; ++g) {
+                while (g >= slot_begin[s + 1]) ++s;
+                const uint32_t b = slot_begin[s];
+                const uint32_t m = count ? count[s] : slot_begin[s + 1] - b;
+                if (g - b >= m) continue;
+                KnnArgs A = G;
+                A.nodes = scenes[s].nodes;
+                A.recs = scenes[s].recs;
+                A.r2max = scenes[s].r2max;
+                A.r_max = scenes[s].r_max;
+                knn_query<CAP, kModeRadiance>(A, g, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
+                wave_sync();
+            }
+        }
+    }
+}
+
 // Adds per-query radiance into slots given by a slot index per query (final gathering,
 // PhotonMap::GatherFor, PhotonMap.cpp:234-270).  A run of equal consecutive slots is
 // summed in query order by the run's first query, so sorted slot indices (as final
@@ -376,6 +397,29 @@ int launch_knn(const KnnArgs& A, hipStream_t st) {
     return MNRT_OK;
 }
 
+
+int launch_knn_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* slot_begin, uint32_t ns,
+                     const uint32_t* count, uint32_t chunk, uint32_t num_parts, const uint32_t* part_begin,
+                     const uint32_t* part_end, uint32_t* work, uint32_t cap, uint32_t waves, uint32_t grid,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, work, num_parts, 0u);
+#define MNRT_KB(C, W)                                                                                        \
+    hipLaunchKernelGGL((k_knn_batch<C, W>), dim3(grid), dim3(64 * W), 0, st, G, scenes, slot_begin, ns, count, \
+                       chunk, num_parts, part_begin, part_end, work)
+    if (cap == 128 && waves == 1) MNRT_KB(128, 1);
+    else if (cap == 128 && waves == 2) MNRT_KB(128, 2);
+    else if (cap == 128 && waves == 4) MNRT_KB(128, 4);
+    else if (cap == 256 && waves == 1) MNRT_KB(256, 1);
+    else if (cap == 256 && waves == 2) MNRT_KB(256, 2);
+    else if (cap == 256 && waves == 4) MNRT_KB(256, 4);
+    else if (cap == 512 && waves == 1) MNRT_KB(512, 1);
+    else if (cap == 512 && waves == 2) MNRT_KB(512, 2);
+    else if (cap == 512 && waves == 4) MNRT_KB(512, 4);
+    else return MNRT_ERR_ARG;
+#undef MNRT_KB
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
 }  // namespace mnrt
 
 static int knn_check(const mnrt_tree* pmap, uint32_t k) {
@@ -415,7 +459,7 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
     A.out_count = d_out_count;
     float4* tmp = nullptr;
     if (d_idx_sp) {
-        MNRT_HIP_CHECK(hipMallocAsync((void**)&tmp, (size_t)n * sizeof(float4), st));
+        MNRT_HIP_CHECK(mnrt::pool_alloc((void**)&tmp, (size_t)n * sizeof(float4), st));
         A.rad_out = tmp;
         A.accumulate = 0;
     } else {
@@ -428,7 +472,7 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
                            n, (float4*)d_io_rad4);
         if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
     }
-    if (tmp && hipFreeAsync(tmp, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
+    if (tmp && mnrt::pool_free(tmp, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
     return rc;
 }
 
@@ -467,7 +511,7 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
     if (!d_pos4 || !d_nrm4 || !d_dir4 || !d_infl4 || !d_grad_rad4 || !d_r2 || !d_count || !(r_max > 0.0f))
         return MNRT_ERR_ARG;
     void* buf = nullptr;
-    MNRT_HIP_CHECK(hipMallocAsync(&buf, (size_t)np * 24 + 256, st));
+    MNRT_HIP_CHECK(mnrt::pool_alloc(&buf, (size_t)np * 24 + 256, st));
     unsigned long long* acc = (unsigned long long*)buf;
     uint32_t* wmax = (uint32_t*)((char*)buf + (size_t)np * 24);
     rc = MNRT_OK;
@@ -534,6 +578,6 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
                            scale > 0.0 ? 1.0 / scale : 0.0, (float4*)d_grad_pow4);
         if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
     }
-    (void)hipFreeAsync(buf, st);
+    (void)mnrt::pool_free(buf, st);
     return rc;
 }

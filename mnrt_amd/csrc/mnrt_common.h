@@ -122,6 +122,16 @@ __device__ __forceinline__ uint32_t mask_prefix(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Device memory of the library: a stream-ordered pool per device that keeps freed blocks
+// for reuse (release threshold: never), so repeated builds and queries do not map and
+// unmap pages.  Tree arrays come from it too; mnrt_tree_release returns them on the
+// null stream.
+hipError_t pool_alloc(void** p, size_t bytes, hipStream_t st);
+hipError_t pool_free(void* p, hipStream_t st);
+// Pinned host staging buffer of at least `bytes`, one per host thread and slot, kept for
+// the thread's lifetime.
+void* host_staging(int slot, size_t bytes);
+
 }  // namespace mnrt
 
 // Status plumbing shared by the C-ABI translation units.

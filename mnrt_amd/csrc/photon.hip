@@ -171,8 +171,8 @@ extern "C" int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, c
     A.seed = seed;
     A.eps = eps;
     uint32_t stored = 0, shot = 0, last_path = 0;
-    uint32_t* h_tot = nullptr;
-    MNRT_HIP_CHECK(hipHostMalloc((void**)&h_tot, 16, hipHostMallocDefault));
+    uint32_t* h_tot = (uint32_t*)mnrt::host_staging(1, 16);
+    if (!h_tot) return MNRT_ERR_NOMEM;
     int rc = MNRT_OK;
     void* buf = nullptr;
     size_t cap_slots = 0;
@@ -184,9 +184,9 @@ extern "C" int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, c
             break;
         }
         if (nslots > cap_slots) {
-            if (buf) (void)hipFreeAsync(buf, st);
+            if (buf) (void)mnrt::pool_free(buf, st);
             size_t words = nslots * (4 * 3 + 1 + 1 + 4 + 1) + scan_scratch_words((uint32_t)nslots) + 8;
-            if (hipMallocAsync(&buf, words * 4, st) != hipSuccess) {
+            if (mnrt::pool_alloc(&buf, words * 4, st) != hipSuccess) {
                 buf = nullptr;
                 rc = MNRT_ERR_NOMEM;
                 break;
@@ -244,8 +244,7 @@ extern "C" int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, c
         shot += np;
         batch = std::max<uint32_t>(4096, (uint32_t)std::min<uint64_t>(2ull * batch, 1ull << 22));
     }
-    if (buf) (void)hipFreeAsync(buf, st);
-    (void)hipHostFree(h_tot);
+    if (buf) (void)mnrt::pool_free(buf, st);
     if (rc != MNRT_OK) return rc;
     if (stored > 0 && shot > 0) {
         float scale = 1.0f / (float)shot;

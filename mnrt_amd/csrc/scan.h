@@ -67,6 +67,10 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, const uint32_t* __
     if (threadIdx.x == 0 && blockIdx.x == 0) *total = (n == 0) ? 0u : out[n - 1] + in[n - 1];
 }
 
+__global__ void k_scan_word(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src ? *src : 0u;
+}
+
 // Scratch words needed by scan_exclusive for n elements.
 inline size_t scan_scratch_words(uint32_t n) {
     size_t words = 0;
@@ -82,13 +86,17 @@ inline size_t scan_scratch_words(uint32_t n) {
 // in and out may alias.  scratch: scan_scratch_words(n) device words.
 inline hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
                                  uint32_t* scratch, hipStream_t st) {
-    if (n == 0) return total ? hipMemsetAsync(total, 0, 4, st) : hipSuccess;
+    // kernels only (no memset / memcpy nodes), so a captured graph replays it exactly
+    if (n == 0) {
+        if (total) hipLaunchKernelGGL(k_scan_word, dim3(1), dim3(1), 0, st, total, (const uint32_t*)nullptr);
+        return hipGetLastError();
+    }
     uint32_t blocks = (n + kScanTile - 1) / kScanTile;
     uint32_t* sums = scratch;
     uint32_t* offs = scratch + blocks + 1;
     hipLaunchKernelGGL(k_scan_tiles, dim3(blocks), dim3(kScanBlock), 0, st, in, n, out, sums);
     if (blocks == 1) {
-        if (total) return hipMemcpyAsync(total, sums, 4, hipMemcpyDeviceToDevice, st);
+        if (total) hipLaunchKernelGGL(k_scan_word, dim3(1), dim3(1), 0, st, total, (const uint32_t*)sums);
         return hipGetLastError();
     }
     hipError_t e = scan_exclusive(sums, offs, blocks, nullptr, scratch + 2 * blocks + 2, st);

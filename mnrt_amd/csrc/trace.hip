@@ -13,6 +13,7 @@
 //     triangles whatever the tree;
 //   * the traversal stack is bounded by the build's level cap instead of KD_MAX_HEIGHT
 //     local-memory arrays with no overflow check.
+#include "batch_dev.h"
 #include "trace_dev.h"
 
 namespace mnrt {
@@ -58,3 +59,61 @@ extern "C" int mnrt_trace_closest(const mnrt_tree* tree, uint32_t n, const float
     MNRT_HIP_CHECK(hipGetLastError());
     return MNRT_OK;
 }
+
+// ---- batches: many trees, one persistent launch ----------------------------------------
+
+namespace mnrt {
+
+// Workgroups take tiles of consecutive rays from a work counter until every tile is
+// done; tile t belongs to the scene whose tile range holds it.  Each XCD (workgroup id
+// mod 8 on MI355X) has its own counter over a contiguous share of the tiles, so an
+// XCD's L2 holds the trees of the scenes it traces; a workgroup whose share is done
+// takes tiles from the next XCD's share.
+template <int kStack>
+__global__ __launch_bounds__(256) void k_trace_batch(TraceBatchArgs B) {
+    __shared__ uint32_t s_tile;
+    const uint32_t xcd = blockIdx.x % B.num_parts;
+    for (uint32_t part = 0; part < B.num_parts; ++part) {
+        const uint32_t p = (xcd + part) % B.num_parts;
+        const uint32_t end = B.part_end[p];
+        for (;;) {
+            if (threadIdx.x == 0) s_tile = B.part_begin[p] + atomicAdd(&B.work[p], 1u);
+            __syncthreads();
+            const uint32_t t = s_tile;
+            __syncthreads();
+            if (t >= end) break;
+            const uint32_t s = seg_of(B.tile_begin, B.num_scenes, t);
+            const TraceScene& d = B.scenes[s];
+            const uint32_t first = d.ray_begin + (t - B.tile_begin[s]) * B.tile_rays;
+            const uint32_t last = min(first + B.tile_rays, d.ray_end);
+            RootBox root;
+            for (int a = 0; a < 3; ++a) {
+                root.lo[a] = d.lo[a];
+                root.hi[a] = d.hi[a];
+            }
+            for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
+                f3 o = xyz(B.orig[i]);
+                f3 dd = xyz(B.dir[i]);
+                Hit h = trace_ray<kStack>(d.nodes, d.recs, root, o, dd, d.eps);
+                B.out_tri[i] = h.tri;
+                if (B.out_t) B.out_t[i] = h.t;
+                if (B.out_bary) B.out_bary[i] = make_float2(h.u, h.v);
+            }
+        }
+    }
+}
+
+int launch_trace_batch(const TraceBatchArgs& B, int stack, uint32_t grid, uint32_t block, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, B.work, B.num_parts, 0u);
+    switch (stack) {
+        case 16: hipLaunchKernelGGL(k_trace_batch<16>, dim3(grid), dim3(block), 0, st, B); break;
+        case 24: hipLaunchKernelGGL(k_trace_batch<24>, dim3(grid), dim3(block), 0, st, B); break;
+        case 32: hipLaunchKernelGGL(k_trace_batch<32>, dim3(grid), dim3(block), 0, st, B); break;
+        case kTraceStack: hipLaunchKernelGGL(k_trace_batch<kTraceStack>, dim3(grid), dim3(block), 0, st, B); break;
+        default: return MNRT_ERR_ARG;
+    }
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
+
+}  // namespace mnrt

@@ -28,6 +28,11 @@ struct Hit {
 
 constexpr int kTraceStack = MNRT_MAX_LEVEL + 2;
 
+// kStack: traversal stack entries.  Entries on the stack are far children of distinct
+// levels of the current path, so a tree of max_level L never needs more than L; a
+// smaller stack (fewer registers / scratch bytes) is exact for trees with
+// max_level <= kStack.
+template <int kStack = kTraceStack>
 __device__ __forceinline__ Hit trace_ray(const uint2* __restrict__ nodes, const float4* __restrict__ recs,
                                          const RootBox& root, f3 o, f3 d, float eps) {
     f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -37,8 +42,8 @@ __device__ __forceinline__ Hit trace_ray(const uint2* __restrict__ nodes, const 
     if (ray_box(root.lo, root.hi, o, inv, eps, kInf, t0, t1)) {
         t1 = t1 + eps;
         best_t = t1;
-        uint32_t snode[kTraceStack];
-        float stmin[kTraceStack], stmax[kTraceStack];
+        uint32_t snode[kStack];
+        float stmin[kStack], stmax[kStack];
         int sp = 0;
         uint32_t node = 0;
         float tmin = t0, tmax = t1;
@@ -53,7 +58,7 @@ __device__ __forceinline__ Hit trace_ray(const uint2* __restrict__ nodes, const 
                 bool below = (oa < s) || (oa == s && da > 0.0f);
                 uint32_t first = below ? left : right, second = below ? right : left;
                 if (tsplit != tsplit) {
-                    if (sp < kTraceStack) {
+                    if (sp < kStack) {
                         snode[sp] = second;
                         stmin[sp] = tmin;
                         stmax[sp] = tmax;
@@ -65,7 +70,7 @@ __device__ __forceinline__ Hit trace_ray(const uint2* __restrict__ nodes, const 
                 } else if (tsplit < tmin * kTravTolLo) {
                     node = second;
                 } else {
-                    if (sp < kTraceStack) {
+                    if (sp < kStack) {
                         snode[sp] = second;
                         stmin[sp] = fmaxf(tmin, tsplit * kTravTolLo);
                         stmax[sp] = tmax;

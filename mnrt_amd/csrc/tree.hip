@@ -4,10 +4,14 @@
 // (RTUpdateKernelData raytracing.cu:1516-1562, PMSetPhotonMap photon_knn.cu:746-771):
 // each leaf reference owns one record, so a leaf's elements are contiguous in HBM and a
 // wave reads them with coalesced loads.
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
-#include "mnrt_common.h"
+#include "batch_dev.h"
 
 namespace mnrt {
 
@@ -38,12 +42,53 @@ __global__ __launch_bounds__(256) void k_photon_records(const uint32_t* __restri
     recs[2ull * r + 1] = make_float4(w.x, w.y, w.z, __uint_as_float(oct_encode(d.x, d.y, d.z)));
 }
 
+// Batched records: reference r of the shared array belongs to tree s (rbase), its element
+// to input entry ebase[s] + refs[r].
+__global__ __launch_bounds__(256) void k_tri_records_batch(const uint32_t* __restrict__ refs, uint32_t num_refs,
+                                                           const uint32_t* __restrict__ rbase,
+                                                           const uint32_t* __restrict__ ebase, uint32_t ns,
+                                                           const float4* __restrict__ v0,
+                                                           const float4* __restrict__ v1,
+                                                           const float4* __restrict__ v2,
+                                                           float4* __restrict__ recs) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= num_refs) return;
+    uint32_t t = refs[r];
+    uint32_t g = ebase[seg_of(rbase, ns, r)] + t;
+    float4 a = v0[g], b = v1[g], c = v2[g];
+    recs[3ull * r + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(t));
+    recs[3ull * r + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+    recs[3ull * r + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_photon_records_batch(const uint32_t* __restrict__ refs, uint32_t num_refs,
+                                                              const uint32_t* __restrict__ rbase,
+                                                              const uint32_t* __restrict__ ebase, uint32_t ns,
+                                                              const float4* __restrict__ pos,
+                                                              const float4* __restrict__ pw,
+                                                              const float4* __restrict__ dir,
+                                                              float4* __restrict__ recs) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= num_refs) return;
+    uint32_t p = refs[r];
+    uint32_t g = ebase[seg_of(rbase, ns, r)] + p;
+    float4 x = pos[g], w = pw[g], d = dir[g];
+    recs[2ull * r + 0] = make_float4(x.x, x.y, x.z, __uint_as_float(p));
+    recs[2ull * r + 1] = make_float4(w.x, w.y, w.z, __uint_as_float(oct_encode(d.x, d.y, d.z)));
+}
+
 // One lane per node: the subtree's reference range runs from its leftmost leaf's first
-// reference to the end of its rightmost leaf.
-__global__ __launch_bounds__(256) void k_collapse(const uint2* __restrict__ nodes, uint32_t num_nodes,
-                                                  uint32_t max_refs, uint2* __restrict__ qnodes) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= num_nodes) return;
+// reference to the end of its rightmost leaf.  nbase (ns + 1 entries, or null for one
+// tree) places the trees of a batch in the shared node array.
+__global__ __launch_bounds__(256) void k_collapse(const uint2* __restrict__ nodes_all, uint32_t num_nodes,
+                                                  const uint32_t* __restrict__ nbase, uint32_t ns,
+                                                  uint32_t max_refs, uint2* __restrict__ qnodes_all) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= num_nodes) return;
+    const uint32_t b = nbase ? nbase[seg_of(nbase, ns, g)] : 0u;
+    const uint2* nodes = nodes_all + b;
+    uint2* qnodes = qnodes_all + b;
+    const uint32_t i = g - b;
     uint2 nd = nodes[i];
     if (is_leaf(nd.y)) {
         qnodes[i] = nd;
@@ -56,17 +101,70 @@ __global__ __launch_bounds__(256) void k_collapse(const uint2* __restrict__ node
         a = nodes[l];
     }
     uint32_t r = i;
-    uint2 b = nd;
-    for (uint32_t s = 0; s <= MNRT_MAX_LEVEL + 1 && !is_leaf(b.y); ++s) {
-        r = b.y & MNRT_INDEX_MASK;
-        b = nodes[r];
+    uint2 c = nd;
+    for (uint32_t s = 0; s <= MNRT_MAX_LEVEL + 1 && !is_leaf(c.y); ++s) {
+        r = c.y & MNRT_INDEX_MASK;
+        c = nodes[r];
     }
     uint32_t first = a.y & MNRT_INDEX_MASK;
-    uint32_t end = (b.y & MNRT_INDEX_MASK) + b.x;
-    if (is_leaf(a.y) && is_leaf(b.y) && end >= first && end - first <= max_refs)
+    uint32_t end = (c.y & MNRT_INDEX_MASK) + c.x;
+    if (is_leaf(a.y) && is_leaf(c.y) && end >= first && end - first <= max_refs)
         qnodes[i] = make_uint2(end - first, (MNRT_LEAF_TAG << 30) | first);
     else
         qnodes[i] = nd;
+}
+
+}  // namespace mnrt
+
+namespace mnrt {
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_pool_mutex;
+hipMemPool_t g_pools[kMaxDevices] = {};
+
+hipMemPool_t device_pool() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    if (!g_pools[dev]) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t mp = nullptr;
+        if (hipMemPoolCreate(&mp, &props) != hipSuccess) return nullptr;
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
+        g_pools[dev] = mp;
+    }
+    return g_pools[dev];
+}
+}  // namespace
+
+hipError_t pool_alloc(void** p, size_t bytes, hipStream_t st) {
+    hipMemPool_t mp = device_pool();
+    if (!mp) return hipErrorOutOfMemory;
+    return hipMallocFromPoolAsync(p, std::max<size_t>(bytes, 16), mp, st);
+}
+
+hipError_t pool_free(void* p, hipStream_t st) { return p ? hipFreeAsync(p, st) : hipSuccess; }
+
+void* host_staging(int slot, size_t bytes) {
+    // never freed: the buffers live until the process ends (a thread-exit free could run
+    // after the HIP runtime has shut down)
+    thread_local void* buf[2] = {nullptr, nullptr};
+    thread_local size_t cap[2] = {0, 0};
+    if (slot < 0 || slot > 1) return nullptr;
+    if (cap[slot] < bytes) {
+        size_t want = std::max<size_t>(bytes, 4096);
+        void* p = nullptr;
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+        if (buf[slot]) (void)hipHostFree(buf[slot]);
+        buf[slot] = p;
+        cap[slot] = want;
+    }
+    return buf[slot];
 }
 
 }  // namespace mnrt
@@ -76,12 +174,13 @@ extern "C" int mnrt_tree_collapse(mnrt_tree* tree, uint32_t max_refs, void* stre
     hipStream_t st = (hipStream_t)stream;
     if (!tree->d_qnodes) {
         void* p = nullptr;
-        if (hipMalloc(&p, (size_t)tree->num_nodes * 8) != hipSuccess) return MNRT_ERR_NOMEM;
+        if (mnrt::pool_alloc(&p, (size_t)tree->num_nodes * 8, st) != hipSuccess) return MNRT_ERR_NOMEM;
         tree->d_qnodes = (uint32_t*)p;
         tree->owns_qnodes = 1;
     }
     hipLaunchKernelGGL(mnrt::k_collapse, dim3((tree->num_nodes + 255) / 256), dim3(256), 0, st,
-                       (const uint2*)tree->d_nodes, tree->num_nodes, max_refs, (uint2*)tree->d_qnodes);
+                       (const uint2*)tree->d_nodes, tree->num_nodes, (const uint32_t*)nullptr, 1u, max_refs,
+                       (uint2*)tree->d_qnodes);
     MNRT_HIP_CHECK(hipGetLastError());
     tree->qleaf_max = max_refs;
     return MNRT_OK;
@@ -92,8 +191,7 @@ static int ensure_recs(mnrt_tree* t, size_t floats_per_ref, hipStream_t st) {
     size_t bytes = (size_t)t->num_refs * floats_per_ref * sizeof(float);
     if (bytes == 0) bytes = 16;
     void* p = nullptr;
-    (void)st;
-    if (hipMalloc(&p, bytes) != hipSuccess) return MNRT_ERR_NOMEM;
+    if (mnrt::pool_alloc(&p, bytes, st) != hipSuccess) return MNRT_ERR_NOMEM;
     t->d_recs = (float*)p;
     t->owns_recs = 1;
     return MNRT_OK;
@@ -131,15 +229,136 @@ extern "C" int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_p
     return mnrt_tree_collapse(tree, 64, stream);
 }
 
+// ---- batches: trees[s] view shared arrays owned by trees[0] ----
+
+namespace {
+
+struct BatchTables {
+    std::vector<uint32_t> h;  // nbase (S+1), rbase (S+1), ebase (S+1)
+    uint32_t total_nodes = 0, total_refs = 0;
+};
+
+int batch_tables(const mnrt_tree* t, uint32_t ns, const uint32_t* h_offsets, uint32_t kind, BatchTables& B) {
+    if (!t || ns == 0 || !h_offsets || !t[0].d_nodes || !t[0].owns_nodes) return MNRT_ERR_ARG;
+    B.h.assign(3ull * (ns + 1), 0u);
+    uint32_t* nb = B.h.data();
+    uint32_t* rb = nb + ns + 1;
+    uint32_t* eb = rb + ns + 1;
+    for (uint32_t s = 0; s < ns; ++s) {
+        if (t[s].kind != kind) return MNRT_ERR_STATE;
+        const ptrdiff_t dn = t[s].d_nodes - t[0].d_nodes, dr = t[s].d_refs - t[0].d_refs;
+        if (dn != 2 * (ptrdiff_t)B.total_nodes || dr != (ptrdiff_t)B.total_refs) return MNRT_ERR_ARG;
+        if (h_offsets[s + 1] < h_offsets[s] || h_offsets[s + 1] - h_offsets[s] != t[s].num_elems) return MNRT_ERR_ARG;
+        nb[s] = B.total_nodes;
+        rb[s] = B.total_refs;
+        eb[s] = h_offsets[s];
+        B.total_nodes += t[s].num_nodes;
+        B.total_refs += t[s].num_refs;
+    }
+    nb[ns] = B.total_nodes;
+    rb[ns] = B.total_refs;
+    eb[ns] = h_offsets[ns];
+    return MNRT_OK;
+}
+
+int upload(const BatchTables& B, uint32_t** d, hipStream_t st) {
+    if (mnrt::pool_alloc((void**)d, B.h.size() * 4, st) != hipSuccess) return MNRT_ERR_NOMEM;
+    MNRT_HIP_CHECK(hipMemcpyAsync(*d, B.h.data(), B.h.size() * 4, hipMemcpyHostToDevice, st));
+    return MNRT_OK;
+}
+
+}  // namespace
+
+extern "C" int mnrt_tree_attach_tri_records_batch(mnrt_tree* trees, uint32_t num_scenes, const float* d_v0,
+                                                  const float* d_v1, const float* d_v2,
+                                                  const uint32_t* h_offsets, void* stream) {
+    BatchTables B;
+    int rc = batch_tables(trees, num_scenes, h_offsets, MNRT_KIND_TRI, B);
+    if (rc != MNRT_OK) return rc;
+    if (B.total_refs > 0 && (!d_v0 || !d_v1 || !d_v2)) return MNRT_ERR_ARG;
+    for (uint32_t s = 0; s < num_scenes; ++s)
+        if (trees[s].d_recs) return MNRT_ERR_STATE;
+    hipStream_t st = (hipStream_t)stream;
+    float* recs = nullptr;
+    if (mnrt::pool_alloc((void**)&recs, (size_t)B.total_refs * 48, st) != hipSuccess) return MNRT_ERR_NOMEM;
+    uint32_t* d_tab = nullptr;
+    rc = upload(B, &d_tab, st);
+    if (rc == MNRT_OK && B.total_refs > 0) {
+        const uint32_t ns = num_scenes;
+        hipLaunchKernelGGL(mnrt::k_tri_records_batch, dim3((B.total_refs + 255) / 256), dim3(256), 0, st,
+                           trees[0].d_refs, B.total_refs, d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
+                           (const float4*)d_v0, (const float4*)d_v1, (const float4*)d_v2, (float4*)recs);
+        if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
+    }
+    if (d_tab) (void)mnrt::pool_free(d_tab, st);
+    if (hipStreamSynchronize(st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
+    if (rc != MNRT_OK) {
+        (void)mnrt::pool_free(recs, st);
+        return rc;
+    }
+    for (uint32_t s = 0; s < num_scenes; ++s) {
+        trees[s].d_recs = recs + 12ull * B.h[num_scenes + 1 + s];
+        trees[s].owns_recs = s == 0 ? 1u : 0u;
+    }
+    return MNRT_OK;
+}
+
+extern "C" int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t num_scenes, const float* d_pos4,
+                                                     const float* d_pow4, const float* d_dir4,
+                                                     const uint32_t* h_offsets, void* stream) {
+    BatchTables B;
+    int rc = batch_tables(trees, num_scenes, h_offsets, MNRT_KIND_POINT, B);
+    if (rc != MNRT_OK) return rc;
+    if (B.total_refs > 0 && (!d_pos4 || !d_pow4 || !d_dir4)) return MNRT_ERR_ARG;
+    for (uint32_t s = 0; s < num_scenes; ++s)
+        if (trees[s].d_recs || trees[s].d_qnodes) return MNRT_ERR_STATE;
+    hipStream_t st = (hipStream_t)stream;
+    float* recs = nullptr;
+    uint32_t* qn = nullptr;
+    if (mnrt::pool_alloc((void**)&recs, (size_t)B.total_refs * 32, st) != hipSuccess) return MNRT_ERR_NOMEM;
+    if (mnrt::pool_alloc((void**)&qn, (size_t)B.total_nodes * 8, st) != hipSuccess) {
+        (void)mnrt::pool_free(recs, st);
+        return MNRT_ERR_NOMEM;
+    }
+    uint32_t* d_tab = nullptr;
+    rc = upload(B, &d_tab, st);
+    const uint32_t ns = num_scenes;
+    if (rc == MNRT_OK && B.total_refs > 0)
+        hipLaunchKernelGGL(mnrt::k_photon_records_batch, dim3((B.total_refs + 255) / 256), dim3(256), 0, st,
+                           trees[0].d_refs, B.total_refs, d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
+                           (const float4*)d_pos4, (const float4*)d_pow4, (const float4*)d_dir4, (float4*)recs);
+    if (rc == MNRT_OK)
+        hipLaunchKernelGGL(mnrt::k_collapse, dim3((B.total_nodes + 255) / 256), dim3(256), 0, st,
+                           (const uint2*)trees[0].d_nodes, B.total_nodes, (const uint32_t*)d_tab, ns, 64u, (uint2*)qn);
+    if (rc == MNRT_OK && hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
+    if (d_tab) (void)mnrt::pool_free(d_tab, st);
+    if (hipStreamSynchronize(st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
+    if (rc != MNRT_OK) {
+        (void)mnrt::pool_free(recs, st);
+        (void)mnrt::pool_free(qn, st);
+        return rc;
+    }
+    for (uint32_t s = 0; s < num_scenes; ++s) {
+        trees[s].d_recs = recs + 8ull * B.h[ns + 1 + s];
+        trees[s].d_qnodes = qn + 2ull * B.h[s];
+        trees[s].qleaf_max = 64;
+        trees[s].owns_recs = trees[s].owns_qnodes = s == 0 ? 1u : 0u;
+    }
+    return MNRT_OK;
+}
+
 extern "C" int mnrt_tree_release(mnrt_tree* tree) {
     if (!tree) return MNRT_ERR_ARG;
+    // library arrays come from the stream-ordered pool; the null stream orders the frees
+    // after all work queued on blocking streams
     int rc = MNRT_OK;
-    if (tree->owns_recs && tree->d_recs && hipFree(tree->d_recs) != hipSuccess) rc = MNRT_ERR_HIP;
+    if (tree->owns_recs && tree->d_recs && mnrt::pool_free(tree->d_recs, nullptr) != hipSuccess) rc = MNRT_ERR_HIP;
     if (tree->owns_nodes) {
-        if (tree->d_nodes && hipFree(tree->d_nodes) != hipSuccess) rc = MNRT_ERR_HIP;
-        if (tree->d_refs && hipFree(tree->d_refs) != hipSuccess) rc = MNRT_ERR_HIP;
+        if (tree->d_nodes && mnrt::pool_free(tree->d_nodes, nullptr) != hipSuccess) rc = MNRT_ERR_HIP;
+        if (tree->d_refs && mnrt::pool_free(tree->d_refs, nullptr) != hipSuccess) rc = MNRT_ERR_HIP;
     }
-    if (tree->owns_qnodes && tree->d_qnodes && hipFree(tree->d_qnodes) != hipSuccess) rc = MNRT_ERR_HIP;
+    if (tree->owns_qnodes && tree->d_qnodes && mnrt::pool_free(tree->d_qnodes, nullptr) != hipSuccess)
+        rc = MNRT_ERR_HIP;
     if (tree->owns_qnodes) tree->d_qnodes = nullptr;
     tree->owns_qnodes = 0;
     if (tree->owns_recs) tree->d_recs = nullptr;

@@ -60,6 +60,45 @@ class Tree(ctypes.Structure):
     ]
 
 
+class BatchDesc(ctypes.Structure):
+    """mnrt_batch_desc."""
+
+    _fields_ = [
+        ("num_scenes", u32),
+        ("tri_trees", ctypes.POINTER(Tree)),
+        ("photon_maps", ctypes.POINTER(Tree)),
+        ("tri_offsets", ctypes.POINTER(u32)),
+        ("ray_offsets", ctypes.POINTER(u32)),
+        ("eps", ctypes.POINTER(f32)),
+        ("r_max", ctypes.POINTER(f32)),
+        ("k", u32),
+    ]
+
+
+class BatchTuning(ctypes.Structure):
+    """mnrt_batch_tuning."""
+
+    _fields_ = [(n, u32) for n in ("trace_block", "trace_tile", "trace_grid", "trace_stack", "knn_waves",
+                                   "knn_chunk", "knn_grid", "knn_cap", "parts")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+BATCH_IO_FIELDS = ("d_orig4", "d_dir4", "d_v0", "d_v1", "d_v2", "d_albedo4", "d_infl4", "d_hit_tri", "d_hit_t",
+                   "d_hit_bary2", "d_sp_pos4", "d_sp_nrm4", "d_sp_dir4", "d_sp_clr4", "d_sp_infl4", "d_sp_src",
+                   "d_sp_count", "d_io_rad4", "d_out_r2", "d_out_nbr", "d_out_count")
+
+
+class BatchIO(ctypes.Structure):
+    """mnrt_batch_io (device pointers)."""
+
+    _fields_ = [(n, vp) for n in BATCH_IO_FIELDS]
+
+
+PHASE_TRACE, PHASE_SHADE, PHASE_KNN = 1, 2, 4
+
+
 class MnrtError(RuntimeError):
     pass
 
@@ -82,6 +121,22 @@ def load():
     TP = ctypes.POINTER(Tree)
     L.mnrt_build_tri.argtypes = [vp, vp, vp, u32, ctypes.POINTER(BuildParams), TP, vp]
     L.mnrt_build_points.argtypes = [vp, u32, ctypes.POINTER(BuildParams), TP, vp]
+    L.mnrt_build_tri_batch.argtypes = [vp, vp, vp, ctypes.POINTER(u32), u32, ctypes.POINTER(BuildParams),
+                                       ctypes.POINTER(f32), TP, vp]
+    L.mnrt_build_points_batch.argtypes = [vp, ctypes.POINTER(u32), u32, ctypes.POINTER(BuildParams),
+                                          ctypes.POINTER(f32), ctypes.POINTER(f32), TP, vp]
+    L.mnrt_tree_attach_tri_records_batch.argtypes = [TP, u32, vp, vp, vp, ctypes.POINTER(u32), vp]
+    L.mnrt_tree_attach_photon_records_batch.argtypes = [TP, u32, vp, vp, vp, ctypes.POINTER(u32), vp]
+    BP = ctypes.POINTER(ctypes.c_void_p)
+    L.mnrt_batch_create.argtypes = [ctypes.POINTER(BatchDesc), BP]
+    L.mnrt_batch_destroy.argtypes = [vp]
+    L.mnrt_batch_get_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+    L.mnrt_batch_set_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+    L.mnrt_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchIO), u32, vp]
+    L.mnrt_batch_run.argtypes = [vp, ctypes.POINTER(BatchIO), u32, vp]
+    L.mnrt_batch_update_trees.argtypes = [vp, TP, TP]
+    L.mnrt_batch_capture.argtypes = [vp, ctypes.POINTER(BatchIO), u32, vp]
+    L.mnrt_batch_replay.argtypes = [vp, u32, vp]
     L.mnrt_tree_attach_tri_records.argtypes = [TP, vp, vp, vp, vp]
     L.mnrt_tree_attach_photon_records.argtypes = [TP, vp, vp, vp, vp]
     L.mnrt_tree_release.argtypes = [TP]
@@ -96,7 +151,11 @@ def load():
     L.mnrt_shading_points.argtypes = [u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(u32), vp]
     L.mnrt_bounce_rays.argtypes = [u32, vp, vp, vp, vp, u32, vp, vp, vp, vp]
-    for name in ("mnrt_knn_gather_bwd", "mnrt_tree_collapse", "mnrt_photons_generate", "mnrt_shading_points", "mnrt_bounce_rays", "mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
+    L.mnrt_primary_rays.argtypes = [u32, u32, u32, ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp]
+    for name in ("mnrt_primary_rays", "mnrt_batch_update_trees", "mnrt_batch_create", "mnrt_batch_destroy", "mnrt_batch_get_tuning", "mnrt_batch_set_tuning",
+                 "mnrt_batch_autotune", "mnrt_batch_run", "mnrt_batch_capture", "mnrt_batch_replay",
+                 "mnrt_build_tri_batch", "mnrt_build_points_batch", "mnrt_tree_attach_tri_records_batch",
+                 "mnrt_tree_attach_photon_records_batch", "mnrt_knn_gather_bwd", "mnrt_tree_collapse", "mnrt_photons_generate", "mnrt_shading_points", "mnrt_bounce_rays", "mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
                  "mnrt_tree_attach_photon_records", "mnrt_tree_release", "mnrt_tree_cost",
                  "mnrt_trace_closest", "mnrt_knn_gather", "mnrt_knn_radii", "mnrt_device_init"):
         getattr(L, name).restype = ctypes.c_int

@@ -235,11 +235,9 @@ def scene_sized(num_tris: int, seed: int, num_lights: int = 1) -> Scene:
     return _assemble(parts, albs, lights, f"sized-{num_tris}-{seed}")
 
 
-def camera_rays(width: int, height: int, eye=(0.0, 0.0, 0.9), target=(0.0, 0.0, 0.0), vfov_deg=90.0,
-                tile: int = 8):
-    """Pinhole primary rays through pixel centres (raygen.cu:171-240), tile-ordered.
-
-    Returns (origins (n,4), dirs (n,4), pixel index (n,) uint32)."""
+def camera_basis(width: int, height: int, eye=(0.0, 0.0, 0.9), target=(0.0, 0.0, 0.0), vfov_deg=90.0):
+    """Camera frame of camera_rays as 14 doubles: eye, unit forward, unit right, up,
+    tan(vfov / 2), aspect (the h_cam argument of mnrt_primary_rays)."""
     eye = np.asarray(eye, np.float64)
     fwd = np.asarray(target, np.float64) - eye
     fwd /= np.linalg.norm(fwd)
@@ -248,7 +246,17 @@ def camera_rays(width: int, height: int, eye=(0.0, 0.0, 0.9), target=(0.0, 0.0, 
     right /= np.linalg.norm(right)
     up = np.cross(right, fwd)
     th = np.tan(np.radians(vfov_deg) / 2.0)
-    aspect = width / height
+    return np.concatenate([eye, fwd, right, up, [th, width / height]]).astype(np.float64)
+
+
+def camera_rays(width: int, height: int, eye=(0.0, 0.0, 0.9), target=(0.0, 0.0, 0.0), vfov_deg=90.0,
+                tile: int = 8):
+    """Pinhole primary rays through pixel centres (raygen.cu:171-240), tile-ordered.
+
+    Returns (origins (n,4), dirs (n,4), pixel index (n,) uint32)."""
+    cam = camera_basis(width, height, eye, target, vfov_deg)
+    eye, fwd, right, up = cam[0:3], cam[3:6], cam[6:9], cam[9:12]
+    th, aspect = cam[12], cam[13]
     ty, tx = np.divmod(np.arange(((height + tile - 1) // tile) * ((width + tile - 1) // tile)),
                        (width + tile - 1) // tile)
     iy, ix = np.divmod(np.arange(tile * tile), tile)
