@@ -166,6 +166,27 @@ int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_pos4, const 
  * do not depend on it.  mnrt_tree_attach_photon_records applies max_refs = 64. */
 int mnrt_tree_collapse(mnrt_tree* tree, uint32_t max_refs, void* stream);
 
+/* Reference KDTreeData layout of a tree (KDTreeData, KDKernelDefs.h:76-194, as
+ * kernel_TraversalDownPath writes it, kdtree.cu:1707-1770), for consumers that read the
+ * reference's preorder stream.  Node indices are preorder indices.  Per node (num_nodes
+ * entries): element count (a leaf's count; an inner node's leaf references below it),
+ * word address in the stream, extent (centre xyz, half diagonal) of the node's spatial
+ * box, left / right child (0 / 0 for leaves).  The stream:
+ *   inner: [node] [axis << 30 | custom bits << 28 | address of the right child] [split bits]
+ *   leaf : [node | 0x80000000] [count] [element indices]
+ * The right-child address keeps 28 bits (the reference masks 24, kdtree.cu:1742); a stream
+ * of 2^28 words or more is refused (MNRT_ERR_CAPACITY).  mnrt_tree_preorder_size gives
+ * the stream length (3 per inner node + 2 per leaf + references). */
+int mnrt_tree_preorder_size(const mnrt_tree* tree, uint64_t* out_words);
+int mnrt_tree_export_preorder(const mnrt_tree* tree, uint32_t* d_tree, uint64_t capacity_words,
+                              uint32_t* d_num_elems, uint32_t* d_node_addr, float* d_node_extent4,
+                              uint32_t* d_child_left, uint32_t* d_child_right, void* stream);
+/* KDTreeGPU::SetCustomBits / kernel_SetCustomBit (KDTreeGPU.cpp:1373-1380,
+ * kdtree.cu:1879-1908): bit 0 or 1 of every inner node's parent word in an exported
+ * stream = d_values[node] & 1; leaves are left alone. */
+int mnrt_preorder_set_custom_bits(uint32_t* d_tree, const uint32_t* d_node_addr, uint32_t num_nodes,
+                                  uint32_t bit, const uint32_t* d_values, void* stream);
+
 /* KDTreeGPU::Destroy() (KDTreeGPU.cpp:237-245): frees what the library allocated. */
 int mnrt_tree_release(mnrt_tree* tree);
 

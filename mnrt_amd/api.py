@@ -219,6 +219,29 @@ def tree_from_host(kind, nodes: np.ndarray, refs: np.ndarray, root6, num_elems: 
     return T
 
 
+def export_preorder(T: DeviceTree):
+    """Reference KDTreeData arrays of a tree (mnrt_tree_export_preorder), as device tensors."""
+    torch = _torch()
+    size = ctypes.c_uint64(0)
+    N.check(N.load().mnrt_tree_preorder_size(T.ref(), ctypes.byref(size)), "mnrt_tree_preorder_size")
+    n = T.num_nodes
+    i32 = dict(dtype=torch.int32, device="cuda")
+    out = dict(tree=torch.empty(max(size.value, 1), **i32), num_elems=torch.empty(n, **i32),
+               addr=torch.empty(n, **i32), extent=torch.empty((n, 4), dtype=torch.float32, device="cuda"),
+               left=torch.empty(n, **i32), right=torch.empty(n, **i32))
+    N.check(N.load().mnrt_tree_export_preorder(T.ref(), N.ptr(out["tree"]), size.value, N.ptr(out["num_elems"]),
+                                               N.ptr(out["addr"]), N.ptr(out["extent"]), N.ptr(out["left"]),
+                                               N.ptr(out["right"]), N.stream_handle()), "mnrt_tree_export_preorder")
+    out["tree"] = out["tree"][:size.value]
+    return out
+
+
+def set_custom_bits(X: dict, bit: int, values):
+    values = dev(np.asarray(values, np.uint32))
+    N.check(N.load().mnrt_preorder_set_custom_bits(N.ptr(X["tree"]), N.ptr(X["addr"]), X["addr"].shape[0], bit,
+                                                   N.ptr(values), N.stream_handle()), "mnrt_preorder_set_custom_bits")
+
+
 def attach_tri_records(T: DeviceTree, v0, v1, v2):
     v0, v1, v2 = dev(v0), dev(v1), dev(v2)
     T._keep += [v0, v1, v2]

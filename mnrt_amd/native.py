@@ -151,8 +151,12 @@ def load():
     L.mnrt_shading_points.argtypes = [u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(u32), vp]
     L.mnrt_bounce_rays.argtypes = [u32, vp, vp, vp, vp, u32, vp, vp, vp, vp]
+    L.mnrt_tree_preorder_size.argtypes = [TP, ctypes.POINTER(ctypes.c_uint64)]
+    L.mnrt_tree_export_preorder.argtypes = [TP, vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
+    L.mnrt_preorder_set_custom_bits.argtypes = [vp, vp, u32, u32, vp, vp]
     L.mnrt_primary_rays.argtypes = [u32, u32, u32, ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp]
-    for name in ("mnrt_primary_rays", "mnrt_batch_update_trees", "mnrt_batch_create", "mnrt_batch_destroy", "mnrt_batch_get_tuning", "mnrt_batch_set_tuning",
+    for name in ("mnrt_tree_preorder_size", "mnrt_tree_export_preorder", "mnrt_preorder_set_custom_bits",
+                 "mnrt_primary_rays", "mnrt_batch_update_trees", "mnrt_batch_create", "mnrt_batch_destroy", "mnrt_batch_get_tuning", "mnrt_batch_set_tuning",
                  "mnrt_batch_autotune", "mnrt_batch_run", "mnrt_batch_capture", "mnrt_batch_replay",
                  "mnrt_build_tri_batch", "mnrt_build_points_batch", "mnrt_tree_attach_tri_records_batch",
                  "mnrt_tree_attach_photon_records_batch", "mnrt_knn_gather_bwd", "mnrt_tree_collapse", "mnrt_photons_generate", "mnrt_shading_points", "mnrt_bounce_rays", "mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",

@@ -1476,4 +1476,69 @@ int orc_ray_tri(const float* a, const float* b, const float* c, const float* o, 
     return hit ? 1 : 0;
 }
 
+// Reference KDTreeData layout of a packed tree (kernel_TraversalDownPath, kdtree.cu:1707-1770,
+// format KDKernelDefs.h:76-117): recursive preorder walk, word addresses assigned top-down
+// (left child 3 words after its parent, right child after the left subtree), node boxes
+// carried down the walk.  Returns the stream length in words.
+namespace {
+struct ExportCtx {
+    const uint32_t* nodes;
+    const uint32_t* refs;
+    uint32_t* tree;
+    uint32_t* num_elems;
+    uint32_t* addr;
+    float* extent;
+    uint32_t* left;
+    uint32_t* right;
+};
+
+uint32_t export_words(const uint32_t* nodes, uint32_t i) {
+    uint32_t w1 = nodes[2 * i + 1];
+    if ((w1 >> 30) == 3u) return 2 + nodes[2 * i];
+    return 3 + export_words(nodes, i + 1) + export_words(nodes, w1 & 0x3FFFFFFFu);
+}
+
+// returns the subtree's reference count
+uint32_t export_node(const ExportCtx& X, uint32_t i, uint32_t a, const float* lo, const float* hi) {
+    const uint32_t w0 = X.nodes[2 * i], w1 = X.nodes[2 * i + 1];
+    X.addr[i] = a;
+    float d[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    X.extent[4 * i + 0] = lo[0] + 0.5f * d[0];
+    X.extent[4 * i + 1] = lo[1] + 0.5f * d[1];
+    X.extent[4 * i + 2] = lo[2] + 0.5f * d[2];
+    X.extent[4 * i + 3] = 0.5f * std::sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    if ((w1 >> 30) == 3u) {
+        X.tree[a] = i | 0x80000000u;
+        X.tree[a + 1] = w0;
+        for (uint32_t k = 0; k < w0; ++k) X.tree[a + 2 + k] = X.refs[(w1 & 0x3FFFFFFFu) + k];
+        X.num_elems[i] = w0;
+        X.left[i] = X.right[i] = 0;
+        return w0;
+    }
+    const uint32_t ax = w1 >> 30, r = w1 & 0x3FFFFFFFu;
+    float s;
+    std::memcpy(&s, &w0, 4);
+    const uint32_t al = a + 3, ar = al + export_words(X.nodes, i + 1);
+    X.tree[a] = i;
+    X.tree[a + 1] = (ax << 30) | (ar & 0x0FFFFFFFu);
+    X.tree[a + 2] = w0;
+    X.left[i] = i + 1;
+    X.right[i] = r;
+    float lhi[3] = {hi[0], hi[1], hi[2]}, rlo[3] = {lo[0], lo[1], lo[2]};
+    lhi[ax] = s;
+    rlo[ax] = s;
+    uint32_t c = export_node(X, i + 1, al, lo, lhi);
+    c += export_node(X, r, ar, rlo, hi);
+    X.num_elems[i] = c;
+    return c;
+}
+}  // namespace
+
+uint32_t orc_export_preorder(const uint32_t* nodes, const uint32_t* refs, const float* root6, uint32_t* tree,
+                             uint32_t* num_elems, uint32_t* addr, float* extent4, uint32_t* left, uint32_t* right) {
+    ExportCtx X{nodes, refs, tree, num_elems, addr, extent4, left, right};
+    export_node(X, 0, 0, root6, root6 + 3);
+    return export_words(nodes, 0);
+}
+
 }  // extern "C"

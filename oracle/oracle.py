@@ -72,6 +72,8 @@ def lib():
                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
     L.orc_radiance_grad.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_float, f32p, f32p, u32p, ctypes.c_uint32,
                                     f32p, f32p, f32p, f32p, vp, ctypes.c_uint32, f32p, f64p, f64p]
+    L.orc_export_preorder.restype = ctypes.c_uint32
+    L.orc_export_preorder.argtypes = [u32p, u32p, f32p, u32p, u32p, u32p, f32p, u32p, u32p]
     L.orc_ray_tri.restype = ctypes.c_int
     L.orc_ray_tri.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
     _lib = L
@@ -132,6 +134,20 @@ def build(kind, v0, v1=None, v2=None, ratio=None, small_max=None, max_qr=1.0, ro
 def tree_cost(nodes, root, kind, max_qr=1.0) -> float:
     nodes = _c(nodes, np.uint32)
     return lib().orc_tree_cost(nodes, nodes.size // 2, _c(root).ravel(), kind, max_qr)
+
+
+def export_preorder(nodes, refs, root, num_leaves):
+    """Reference KDTreeData layout of a packed tree (kernel_TraversalDownPath)."""
+    nodes = _c(nodes, np.uint32)
+    n = nodes.size // 2
+    refs_c = _c(refs, np.uint32) if len(refs) else np.zeros(1, np.uint32)
+    size = 3 * (n - num_leaves) + 2 * num_leaves + len(refs)
+    tree = np.zeros(max(size, 1), np.uint32)
+    ne, addr, left, right = (np.zeros(n, np.uint32) for _ in range(4))
+    ext = np.zeros((n, 4), np.float32)
+    got = lib().orc_export_preorder(nodes, refs_c, _c(root).ravel(), tree, ne, addr, ext, left, right)
+    assert got == size, (got, size)
+    return dict(tree=tree[:size], num_elems=ne, addr=addr, extent=ext, left=left, right=right)
 
 
 def check_structure(nodes, refs, num_elems) -> int:
