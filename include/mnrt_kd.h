@@ -211,25 +211,31 @@ int mnrt_trace_closest(const mnrt_tree* tree, uint32_t n, const float* d_orig4,
 
 /* KernelPMPhotonGather (photon_knn.cu:487-552, 833-844) with the range search of
  * dev_RangeSearch (photon_knn.cu:113-235) replaced by the exact k nearest photons
- * (ties by photon index) among those closer than r_max:
+ * (ties by photon index) among those closer than the query's radius cap:
  *   E = sum_{front-facing of the set} power / (pi * r2),  r2 = d_k^2 when k were found,
- *   else r_max^2;  E = 0 when fewer than 8 were found and d_clr4 is given;
- *   radiance[idx] += alpha * rho * (1/pi) * E * influence.
+ *   else cap^2;  E = 0 when fewer than 8 were found and d_clr4 is given;
+ *   radiance[slot] += alpha * rho * (1/pi) * E * influence.
  * Queries: position, shading normal, source ray direction (normal flipped toward it),
  * influence (float4 each, n entries).  d_clr4: diffuse colour with alpha in w, or NULL
  * (photon visualisation: colour 1, no 8-photon cut).  d_idx_sp: output slot per query
- * or NULL (slot = query).  Optional outputs: d_out_r2 (n floats: the r2 used),
- * d_out_irr4 (n float4: E), d_out_nbr (n*k photon indices, sorted by (d2, index),
- * 0xFFFFFFFF past the count), d_out_count (n: neighbours found). */
+ * or NULL (slot = query); several queries may share a slot (final gathering), and
+ * queries of one slot that are consecutive are summed in query order.
+ * d_query_radius: per-query cap (the reference's d_queryRadius) or NULL (r_max for all).
+ * d_io_inv_dist (with d_orig4, the source ray origins): slot += 1 / |position - origin|
+ * (the reference's d_ioInvDistAccu), or NULL.  Optional outputs per query: d_out_r2
+ * (the r2 used), d_out_irr4 (E), d_out_nbr (n*k photon indices sorted by (d2, index),
+ * 0xFFFFFFFF past the count), d_out_count (neighbours found). */
 int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                     const float* d_pos4, const float* d_nrm4, const float* d_dir4,
                     const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,
-                    float* d_io_rad4, float* d_out_r2, float* d_out_irr4, uint32_t* d_out_nbr,
+                    const float* d_query_radius, const float* d_orig4, float* d_io_rad4,
+                    float* d_io_inv_dist, float* d_out_r2, float* d_out_irr4, uint32_t* d_out_nbr,
                     uint32_t* d_out_count, void* stream);
 
 /* Adjoint of mnrt_knn_gather (no reference counterpart): given dLoss/dradiance per query
  * (float4, w ignored) and the forward's r2 and neighbour counts (d_out_r2, d_out_count),
- * recomputes every query's neighbour set (identical to the forward's) and writes
+ * recomputes every query's neighbour set (identical to the forward's: same r_max and
+ * d_query_radius) and writes
  *   d_grad_pow4[p] = dLoss/dpower of photon p (original photon index, num_elems entries),
  *   d_grad_clr4[q] = dLoss/d(diffuse colour rgb) of query q (optional).
  * Per-photon sums use 64-bit fixed-point atomic adds (scale a power of two fixed from the
@@ -237,7 +243,8 @@ int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
  * supported here (radiance per query). */
 int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                         const float* d_pos4, const float* d_nrm4, const float* d_dir4,
-                        const float* d_infl4, const float* d_clr4, const float* d_grad_rad4,
+                        const float* d_infl4, const float* d_clr4, const float* d_query_radius,
+                        const float* d_grad_rad4,
                         const float* d_r2, const uint32_t* d_count, float* d_grad_pow4,
                         float* d_grad_clr4, void* stream);
 

@@ -301,11 +301,14 @@ def trace_closest(T: DeviceTree, orig, dirs, eps: float, want_pt=False):
 
 
 def knn_gather(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, clr=None, idx_sp=None, io_rad=None,
-               want_nbr=False, want_r2=False, want_irr=False, want_count=False):
+               want_nbr=False, want_r2=False, want_irr=False, want_count=False, query_radius=None, orig=None,
+               io_inv_dist=None):
     torch = _torch()
     pos, nrm, dirs, infl = dev(pos), dev(nrm), dev(dirs), dev(infl)
     clr = dev(clr) if clr is not None else None
     idx_sp = dev(idx_sp) if idx_sp is not None else None
+    query_radius = dev(query_radius) if query_radius is not None else None
+    orig = dev(orig) if orig is not None else None
     n = pos.shape[0]
     if io_rad is None:
         io_rad = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
@@ -314,13 +317,15 @@ def knn_gather(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, clr=No
     irr = torch.empty((n, 4), dtype=torch.float32, device="cuda") if want_irr else None
     cnt = torch.empty(n, dtype=torch.int32, device="cuda") if want_count else None
     N.check(N.load().mnrt_knn_gather(P.ref(), k, r_max, n, N.ptr(pos), N.ptr(nrm), N.ptr(dirs), N.ptr(infl),
-                                     N.ptr(clr), N.ptr(idx_sp), N.ptr(io_rad), N.ptr(r2), N.ptr(irr), N.ptr(nbr),
-                                     N.ptr(cnt), N.stream_handle()), "mnrt_knn_gather")
-    return dict(rad=io_rad, nbr=nbr.view(n, k) if nbr is not None else None, r2=r2, irr=irr, count=cnt)
+                                     N.ptr(clr), N.ptr(idx_sp), N.ptr(query_radius), N.ptr(orig), N.ptr(io_rad),
+                                     N.ptr(io_inv_dist), N.ptr(r2), N.ptr(irr), N.ptr(nbr), N.ptr(cnt),
+                                     N.stream_handle()), "mnrt_knn_gather")
+    return dict(rad=io_rad, nbr=nbr.view(n, k) if nbr is not None else None, r2=r2, irr=irr, count=cnt,
+                inv_dist=io_inv_dist)
 
 
 def knn_gather_bwd(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, clr, grad_rad, r2, count,
-                   want_grad_clr=True):
+                   want_grad_clr=True, query_radius=None):
     """Adjoint of knn_gather (neighbour sets recomputed): (dLoss/dpower per photon, dLoss/dcolour per query)."""
     torch = _torch()
     n = pos.shape[0]
@@ -328,6 +333,7 @@ def knn_gather_bwd(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, cl
     gc = torch.empty((n, 4), dtype=torch.float32, device="cuda") if want_grad_clr else None
     N.check(N.load().mnrt_knn_gather_bwd(P.ref(), k, r_max, n, N.ptr(dev(pos)), N.ptr(dev(nrm)), N.ptr(dev(dirs)),
                                          N.ptr(dev(infl)), N.ptr(dev(clr) if clr is not None else None),
+                                         N.ptr(dev(query_radius) if query_radius is not None else None),
                                          N.ptr(grad_rad.contiguous()), N.ptr(r2), N.ptr(count), N.ptr(gp), N.ptr(gc),
                                          N.stream_handle()), "mnrt_knn_gather_bwd")
     return gp, gc

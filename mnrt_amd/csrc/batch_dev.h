@@ -71,6 +71,10 @@ struct KnnArgs {
     uint32_t* out_nbr;
     uint32_t* out_count;
     float* out_radius;
+    const float* r_query;  // per-query radius cap (null: r_max)
+    const float4* orig;    // source ray origins (inverse-distance sums)
+    float* inv_dist;       // 1 / |hit - origin| per query (null: not wanted)
+    uint32_t inv_accumulate;
     // backward
     const float4* grad;            // dLoss/dradiance per query
     unsigned long long* acc;       // 3 fixed-point accumulators per photon (original index)

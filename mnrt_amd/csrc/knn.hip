@@ -68,7 +68,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
     const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
     const f3 qp = mk3(qx, qy, qz);
     const uint32_t k = A.k;
-    const float r2max = A.r2max;
+    const float rq = A.r_query ? uniformf(A.r_query[q]) : A.r_max;
+    const float r2max = A.r_query ? rq * rq : A.r2max;
 
     uint32_t cnt = 0;        // candidates in the buffer
     bool full = false;       // the buffer's first k are the k best so far
@@ -183,7 +184,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
             A.out_nbr[(size_t)q * k + i] = (i < found) ? (uint32_t)(keys[i] & 0xFFFFFFFFull) : 0xFFFFFFFFu;
     }
     if (kMode == kModeRadii) {
-        if (A.out_radius && lane == 0) A.out_radius[q] = (found >= k && k > 0) ? sqrtf(r2) : A.r_max;
+        if (A.out_radius && lane == 0) A.out_radius[q] = (found >= k && k > 0) ? sqrtf(r2) : rq;
         if (A.out_r2 && lane == 0) A.out_r2[q] = r2;
         return;
     }
@@ -255,6 +256,12 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
     float ry = (((c4.w * c4.y) * kInvPi) * iy) * in4.y;
     float rz = (((c4.w * c4.z) * kInvPi) * iz) * in4.z;
     if (lane == 0) {
+        if (A.inv_dist) {
+            // reciprocal distance from the source ray's origin (photon_knn.cu:529-535)
+            f3 od = sub3(qp, xyz(A.orig[q]));
+            float v = 1.0f / sqrtf(dot3(od, od));
+            A.inv_dist[q] = A.inv_accumulate ? A.inv_dist[q] + v : v;
+        }
         if (A.out_r2) A.out_r2[q] = r2;
         if (A.out_irr) A.out_irr[q] = make_float4(ix, iy, iz, 0.0f);
         if (A.accumulate) {
@@ -330,22 +337,25 @@ __global__ __launch_bounds__(64 * kWaves) void k_knn_batch(KnnArgs G, const KnnS
 // summed in query order by the run's first query, so sorted slot indices (as final
 // gathering produces them) give a deterministic result.
 __global__ __launch_bounds__(256) void k_scatter_runs(const float4* __restrict__ per_query,
+                                                      const float* __restrict__ per_query_inv,
                                                       const uint32_t* __restrict__ idx_sp, uint32_t n,
-                                                      float4* __restrict__ io_rad) {
+                                                      float4* __restrict__ io_rad, float* __restrict__ io_inv) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t slot = idx_sp[i];
     if (i > 0 && idx_sp[i - 1] == slot) return;
-    float x = 0.0f, y = 0.0f, z = 0.0f;
+    float x = 0.0f, y = 0.0f, z = 0.0f, w = 0.0f;
     for (uint32_t j = i; j < n && idx_sp[j] == slot; ++j) {
         float4 r = per_query[j];
         x = x + r.x;
         y = y + r.y;
         z = z + r.z;
+        if (per_query_inv) w = w + per_query_inv[j];
     }
     atomicAdd(&io_rad[slot].x, x);
     atomicAdd(&io_rad[slot].y, y);
     atomicAdd(&io_rad[slot].z, z);
+    if (per_query_inv) atomicAdd(&io_inv[slot], w);
 }
 
 // Bound on |weight| per query for the fixed-point scale of the backward pass, from the
@@ -434,12 +444,14 @@ static int knn_check(const mnrt_tree* pmap, uint32_t k) {
 extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                                const float* d_pos4, const float* d_nrm4, const float* d_dir4,
                                const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,
-                               float* d_io_rad4, float* d_out_r2, float* d_out_irr4,
-                               uint32_t* d_out_nbr, uint32_t* d_out_count, void* stream) {
+                               const float* d_query_radius, const float* d_orig4, float* d_io_rad4,
+                               float* d_io_inv_dist, float* d_out_r2, float* d_out_irr4, uint32_t* d_out_nbr,
+                               uint32_t* d_out_count, void* stream) {
     int rc = knn_check(pmap, k);
     if (rc != MNRT_OK) return rc;
     if (n == 0) return MNRT_OK;
     if (!d_pos4 || !d_nrm4 || !d_dir4 || !d_infl4 || !d_io_rad4 || !(r_max > 0.0f)) return MNRT_ERR_ARG;
+    if (d_io_inv_dist && !d_orig4) return MNRT_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     mnrt::KnnArgs A{};
     A.nodes = (const uint2*)(pmap->d_qnodes ? pmap->d_qnodes : pmap->d_nodes);
@@ -453,23 +465,31 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
     A.dir = (const float4*)d_dir4;
     A.infl = (const float4*)d_infl4;
     A.clr = (const float4*)d_clr4;
+    A.r_query = d_query_radius;
+    A.orig = (const float4*)d_orig4;
     A.out_r2 = d_out_r2;
     A.out_irr = (float4*)d_out_irr4;
     A.out_nbr = d_out_nbr;
     A.out_count = d_out_count;
     float4* tmp = nullptr;
     if (d_idx_sp) {
-        MNRT_HIP_CHECK(mnrt::pool_alloc((void**)&tmp, (size_t)n * sizeof(float4), st));
+        // per-query values first, then summed into their slots run by run
+        MNRT_HIP_CHECK(mnrt::pool_alloc((void**)&tmp, (size_t)n * (sizeof(float4) + sizeof(float)), st));
         A.rad_out = tmp;
         A.accumulate = 0;
+        A.inv_dist = d_io_inv_dist ? (float*)(tmp + n) : nullptr;
+        A.inv_accumulate = 0;
     } else {
         A.rad_out = (float4*)d_io_rad4;
         A.accumulate = 1;
+        A.inv_dist = d_io_inv_dist;
+        A.inv_accumulate = 1;
     }
     rc = mnrt::launch_knn<mnrt::kModeRadiance>(A, st);
     if (rc == MNRT_OK && d_idx_sp) {
-        hipLaunchKernelGGL(mnrt::k_scatter_runs, dim3((n + 255) / 256), dim3(256), 0, st, tmp, d_idx_sp,
-                           n, (float4*)d_io_rad4);
+        hipLaunchKernelGGL(mnrt::k_scatter_runs, dim3((n + 255) / 256), dim3(256), 0, st, tmp,
+                           d_io_inv_dist ? (const float*)(tmp + n) : (const float*)nullptr, d_idx_sp, n,
+                           (float4*)d_io_rad4, d_io_inv_dist);
         if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
     }
     if (tmp && mnrt::pool_free(tmp, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
@@ -496,7 +516,8 @@ extern "C" int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, ui
 
 extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                                    const float* d_pos4, const float* d_nrm4, const float* d_dir4,
-                                   const float* d_infl4, const float* d_clr4, const float* d_grad_rad4,
+                                   const float* d_infl4, const float* d_clr4, const float* d_query_radius,
+                                   const float* d_grad_rad4,
                                    const float* d_r2, const uint32_t* d_count, float* d_grad_pow4,
                                    float* d_grad_clr4, void* stream) {
     int rc = knn_check(pmap, k);
@@ -548,6 +569,7 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
         A.dir = (const float4*)d_dir4;
         A.infl = (const float4*)d_infl4;
         A.clr = (const float4*)d_clr4;
+        A.r_query = d_query_radius;
         A.grad = (const float4*)d_grad_rad4;
         A.acc = acc;
         A.fix_scale = scale;
@@ -567,6 +589,7 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
         A.dir = (const float4*)d_dir4;
         A.infl = (const float4*)d_infl4;
         A.clr = (const float4*)d_clr4;
+        A.r_query = d_query_radius;
         A.grad = (const float4*)d_grad_rad4;
         A.acc = acc;
         A.fix_scale = 0.0;

@@ -143,9 +143,9 @@ def load():
     L.mnrt_tree_collapse.argtypes = [TP, u32, vp]
     L.mnrt_tree_cost.argtypes = [TP, f32, ctypes.POINTER(ctypes.c_double), vp]
     L.mnrt_trace_closest.argtypes = [TP, u32, vp, vp, f32, vp, vp, vp, vp, vp, vp]
-    L.mnrt_knn_gather.argtypes = [TP, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mnrt_knn_gather.argtypes = [TP, u32, f32, u32] + [vp] * 15
     L.mnrt_knn_radii.argtypes = [TP, u32, f32, u32, vp, vp, vp]
-    L.mnrt_knn_gather_bwd.argtypes = [TP, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mnrt_knn_gather_bwd.argtypes = [TP, u32, f32, u32] + [vp] * 12
     L.mnrt_photons_generate.argtypes = [TP, vp, vp, vp, vp, vp, u32, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp,
                                         ctypes.POINTER(u32), ctypes.POINTER(u32), vp]
     L.mnrt_shading_points.argtypes = [u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,

@@ -33,8 +33,8 @@ def test_argument_errors_without_device():
     assert L.mnrt_trace_closest(None, 1, None, None, 0.0, None, None, None, None, None, None) == -1
     assert L.mnrt_tree_release(None) == -1
     t.kind = N.KIND_POINT
-    assert L.mnrt_knn_gather(ctypes.byref(t), 0, 1.0, 1, *([None] * 12)) == -1
-    assert L.mnrt_knn_gather(ctypes.byref(t), 5, 1.0, 1, *([None] * 12)) == -6  # no nodes yet
+    assert L.mnrt_knn_gather(ctypes.byref(t), 0, 1.0, 1, *([None] * 15)) == -1
+    assert L.mnrt_knn_gather(ctypes.byref(t), 5, 1.0, 1, *([None] * 15)) == -6  # no nodes yet
     t.kind = N.KIND_TRI
     assert L.mnrt_trace_closest(ctypes.byref(t), 1, None, None, 0.0, None, None, None, None, None, None) == -1
 

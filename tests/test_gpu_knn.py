@@ -78,3 +78,55 @@ def test_knn_coincident_photons_tie_by_index(orc, cuda):
     q = hit_queries(orc, sc, o, d)
     q["pos"][:50] = ppos[:50]  # queries exactly on photons (d2 = 0)
     _run(orc, cuda, sc, ppos, ppow, pdir, q, 30, sc.r_max())
+
+
+def test_knn_per_query_radius_and_inverse_distance(orc, cuda):
+    """Per-query radius caps equal separate calls at each cap (the oracle pins those);
+    the inverse-distance sums equal fp32 1/|x - o| in the same operation order, and with
+    shared output slots both sums run in query order."""
+    torch = cuda
+    sc = S.scene_small(subdiv=3)
+    ppos, ppow, pdir = S.random_surface_photons(sc, 30000, seed=4)
+    o, d, _ = S.camera_rays(48, 48)
+    q = hit_queries(orc, sc, o, d)
+    n = q["pos"].shape[0]
+    P = orc.build(1, ppos, max_qr=sc.r_max(), root=np.concatenate(sc.bounds()))
+    DP = api.tree_from_host(1, P.nodes, P.refs, P.root, ppos.shape[0], P.max_level)
+    api.attach_photon_records(DP, ppos, ppow, pdir)
+    caps = np.where(np.arange(n) % 3 == 0, 0.02, 0.15).astype(np.float32)
+    args = (q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"])
+    orig = q["pos"].copy()
+    orig[:, :3] -= q["dir"][:, :3] * 0.5
+    inv = torch.zeros(n, dtype=torch.float32, device="cuda")
+    per = api.knn_gather(DP, 50, 0.15, *args, query_radius=caps, orig=orig, io_inv_dist=inv, want_nbr=True)
+    cuda.cuda.synchronize()
+    for cap in (0.02, 0.15):
+        sel = np.nonzero(caps == np.float32(cap))[0]
+        one = api.knn_gather(DP, 50, float(cap), *(a[sel] for a in args), want_nbr=True)
+        nbr, _, _ = orc.knn(P, ppos, q["pos"][sel], 50, float(cap))
+        assert np.array_equal(one["nbr"].cpu().numpy().view(np.uint32), nbr)
+        assert torch.equal(per["nbr"][sel], one["nbr"])
+        assert torch.equal(per["rad"][sel], one["rad"])
+    od = (q["pos"][:, :3] - orig[:, :3]).astype(np.float32)
+    d2 = (od[:, 0] * od[:, 0] + od[:, 1] * od[:, 1]) + od[:, 2] * od[:, 2]
+    want = np.float32(1.0) / np.sqrt(d2)
+    assert np.array_equal(inv.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    # shared slots (sorted, as final gathering produces them): run sums in query order
+    slots = (np.arange(n) // 4).astype(np.uint32)
+    rad = torch.zeros((n // 4 + 1, 4), dtype=torch.float32, device="cuda")
+    inv2 = torch.zeros(n // 4 + 1, dtype=torch.float32, device="cuda")
+    api.knn_gather(DP, 50, 0.15, *args, idx_sp=slots, io_rad=rad, query_radius=caps, orig=orig, io_inv_dist=inv2)
+    cuda.cuda.synchronize()
+    pr = per["rad"].cpu().numpy()
+    er = np.zeros((n // 4 + 1, 4), np.float32)
+    ei = np.zeros(n // 4 + 1, np.float32)
+    for s in range(n // 4 + 1):
+        acc = np.zeros(3, np.float32)
+        ai = np.float32(0)
+        for j in np.nonzero(slots == s)[0]:
+            acc = acc + pr[j, :3]
+            ai = ai + want[j]
+        er[s, :3] = np.float32(0) + acc
+        ei[s] = np.float32(0) + ai
+    assert np.array_equal(rad.cpu().numpy().view(np.uint32), er.view(np.uint32))
+    assert np.array_equal(inv2.cpu().numpy().view(np.uint32), ei.view(np.uint32))
