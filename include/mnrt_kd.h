@@ -275,6 +275,25 @@ int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, const float*
                           uint32_t* d_out_meta, uint32_t* d_out_path, uint32_t* d_out_path_tris,
                           uint32_t* out_count, uint32_t* out_paths, void* stream);
 
+/* PhotonData in the reference layout (KernelDefs.h:537-630, photon_dev.h:71-116) keeps the
+ * incoming direction as spherical angles: azimuth phi in position.w, polar theta in
+ * power.w.  These convert between that and direction vectors (dev_Spherical2Direction /
+ * dev_Direction2Spherical, mncudautil_dev.h:345-375, with HIP's sinf/cosf/atan2f/acosf). */
+int mnrt_photons_spherical_to_dir(uint32_t n, const float* d_pos4, const float* d_pow4, float* d_out_dir4,
+                                  void* stream);
+int mnrt_photons_dir_to_spherical(uint32_t n, const float* d_dir4, float* d_io_pos4, float* d_io_pow4,
+                                  void* stream);
+/* PhotonData::Merge (KernelDefs.cpp:363-384): appends the entries i with d_valid[i] != 0,
+ * in order, to d_dst_* (pointers at the first free entry); *out_added = count.  If more
+ * than dst_capacity are valid nothing is written and MNRT_ERR_CAPACITY is returned (the
+ * reference stops with MNFatal).  Synchronous. */
+int mnrt_photons_append_valid(uint32_t n, const uint32_t* d_valid, const float* d_src_pos4,
+                              const float* d_src_pow4, float* d_dst_pos4, float* d_dst_pow4,
+                              uint32_t dst_capacity, uint32_t* out_added, void* stream);
+/* PhotonMap::ScaleFlux / KernelPMScaleFlux (PhotonMap.cpp:186-193): power.rgb *= scale,
+ * w (the polar angle) untouched. */
+int mnrt_photons_scale_flux(uint32_t n, float* d_io_pow4, float scale, void* stream);
+
 /* --- hits to queries -------------------------------------------------------------------- */
 
 /* Compaction of valid hits (RTCore::TraceRays, RTCore.cpp:416-429) into shading points:

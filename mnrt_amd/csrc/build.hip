@@ -1071,17 +1071,50 @@ namespace {
 
 using namespace mnrt;
 
-// Stream-ordered device allocations released together.
+// Build workspace: one device block per host thread and device, kept across builds and
+// grown to the largest build seen, so a build in steady state allocates nothing but its
+// output arrays (allocation calls that must grow device memory cost milliseconds each).
+// Buffers are carved bump-wise and not reused within a build; a build that outgrows the
+// block takes the rest from the stream-ordered pool, and the block grows to fit when that
+// build ends.  The blocks live until the process ends.
+constexpr int kArenaDevices = 64;
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+};
+thread_local Arena t_arena[kArenaDevices];
+
 class Pool {
    public:
-    explicit Pool(hipStream_t st) : st_(st) {}
+    explicit Pool(hipStream_t st) : st_(st) {
+        int dev = -1;
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kArenaDevices) arena_ = &t_arena[dev];
+    }
     ~Pool() {
+        // nothing may still use the workspace when the next build on this thread reuses it
+        (void)hipStreamSynchronize(st_);
         for (void* p : ptrs_) (void)pool_free(p, st_);
+        if (arena_ && total_ > arena_->cap) {
+            (void)hipStreamSynchronize(st_);
+            void* nb = nullptr;
+            const size_t ncap = total_ + total_ / 4 + (size_t(64) << 20);
+            if (hipMalloc(&nb, ncap) == hipSuccess) {
+                if (arena_->base) (void)hipFree(arena_->base);
+                arena_->base = (char*)nb;
+                arena_->cap = ncap;
+            }
+        }
     }
     template <typename T>
     T* get(size_t count) {
+        const size_t bytes = (std::max<size_t>(count * sizeof(T), 16) + 255) & ~size_t(255);
+        total_ += bytes;
+        if (arena_ && top_ + bytes <= arena_->cap) {
+            void* p = arena_->base + top_;
+            top_ += bytes;
+            return (T*)p;
+        }
         void* p = nullptr;
-        size_t bytes = std::max<size_t>(count * sizeof(T), 16);
         if (pool_alloc(&p, bytes, st_) != hipSuccess) {
             ok = false;
             return nullptr;
@@ -1089,6 +1122,8 @@ class Pool {
         ptrs_.push_back(p);
         return (T*)p;
     }
+    // Returns a buffer early: pool buffers go back to the pool, workspace space stays
+    // carved until the build ends.
     void release(void* p) {
         if (!p) return;
         auto it = std::find(ptrs_.begin(), ptrs_.end(), p);
@@ -1101,11 +1136,10 @@ class Pool {
 
    private:
     hipStream_t st_;
+    Arena* arena_ = nullptr;
+    size_t top_ = 0;
+    size_t total_ = 0;
     std::vector<void*> ptrs_;
-};
-
-struct Grow {
-    uint32_t cap = 0;
 };
 
 inline dim3 blocks_for(uint64_t n, uint32_t bs = 256) { return dim3((uint32_t)std::max<uint64_t>(1, (n + bs - 1) / bs)); }

@@ -256,3 +256,114 @@ extern "C" int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, c
     *out_paths = shot;
     return MNRT_OK;
 }
+
+// ---- PhotonData plumbing of the reference layout (KernelDefs.h:537-630) ---------------
+
+namespace mnrt {
+
+// dev_Spherical2Direction (mncudautil_dev.h:345-353): phi in position.w, theta in power.w.
+__global__ __launch_bounds__(256) void k_sph_to_dir(uint32_t n, const float4* __restrict__ pos,
+                                                    const float4* __restrict__ pw, float4* __restrict__ dir) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float phi = pos[i].w, theta = pw[i].w;
+    float st = sinf(theta);
+    dir[i] = make_float4(st * cosf(phi), st * sinf(phi), cosf(theta), 0.0f);
+}
+
+// dev_Direction2Spherical (mncudautil_dev.h:367-375): phi = atan2(y, x) in [0, 2 pi), theta = acos(z).
+__global__ __launch_bounds__(256) void k_dir_to_sph(uint32_t n, const float4* __restrict__ dir,
+                                                    float4* __restrict__ pos, float4* __restrict__ pw) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 d = dir[i];
+    float phi = atan2f(d.y, d.x);
+    if (phi < 0.0f) phi += 2.0f * kPi;
+    pos[i].w = phi;
+    pw[i].w = acosf(d.z);
+}
+
+__global__ __launch_bounds__(256) void k_flags_of(uint32_t n, const uint32_t* __restrict__ valid, uint32_t* __restrict__ f) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = valid[i] ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_append_valid(uint32_t n, const uint32_t* __restrict__ valid,
+                                                      const uint32_t* __restrict__ pos, const float4* __restrict__ a,
+                                                      const float4* __restrict__ b, float4* __restrict__ da,
+                                                      float4* __restrict__ db, uint32_t cap) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !valid[i]) return;
+    uint32_t d = pos[i];
+    if (d >= cap) return;
+    da[d] = a[i];
+    db[d] = b[i];
+}
+
+}  // namespace mnrt
+
+extern "C" int mnrt_photons_spherical_to_dir(uint32_t n, const float* d_pos4, const float* d_pow4, float* d_out_dir4,
+                                             void* stream) {
+    if (n == 0) return MNRT_OK;
+    if (!d_pos4 || !d_pow4 || !d_out_dir4) return MNRT_ERR_ARG;
+    hipLaunchKernelGGL(mnrt::k_sph_to_dir, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                       (const float4*)d_pos4, (const float4*)d_pow4, (float4*)d_out_dir4);
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
+
+extern "C" int mnrt_photons_dir_to_spherical(uint32_t n, const float* d_dir4, float* d_io_pos4, float* d_io_pow4,
+                                             void* stream) {
+    if (n == 0) return MNRT_OK;
+    if (!d_dir4 || !d_io_pos4 || !d_io_pow4) return MNRT_ERR_ARG;
+    hipLaunchKernelGGL(mnrt::k_dir_to_sph, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                       (const float4*)d_dir4, (float4*)d_io_pos4, (float4*)d_io_pow4);
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
+
+extern "C" int mnrt_photons_append_valid(uint32_t n, const uint32_t* d_valid, const float* d_src_pos4,
+                                         const float* d_src_pow4, float* d_dst_pos4, float* d_dst_pow4,
+                                         uint32_t dst_capacity, uint32_t* out_added, void* stream) {
+    using namespace mnrt;
+    if (!out_added) return MNRT_ERR_ARG;
+    *out_added = 0;
+    if (n == 0) return MNRT_OK;
+    if (!d_valid || !d_src_pos4 || !d_src_pow4 || (dst_capacity > 0 && (!d_dst_pos4 || !d_dst_pow4)))
+        return MNRT_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    void* buf = nullptr;
+    const size_t words = 2ull * n + 4 + scan_scratch_words(n);
+    MNRT_HIP_CHECK(pool_alloc(&buf, words * 4, st));
+    uint32_t* f = (uint32_t*)buf;
+    uint32_t* pos = f + n;
+    uint32_t* tot = pos + n;
+    uint32_t* scr = tot + 4;
+    uint32_t* h = (uint32_t*)host_staging(1, 4);
+    int rc = h ? MNRT_OK : MNRT_ERR_NOMEM;
+    if (rc == MNRT_OK) {
+        hipLaunchKernelGGL(k_flags_of, dim3((n + 255) / 256), dim3(256), 0, st, n, d_valid, f);
+        if (scan_exclusive(f, pos, n, tot, scr, st) != hipSuccess ||
+            hipMemcpyAsync(h, tot, 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            rc = MNRT_ERR_HIP;
+    }
+    if (rc == MNRT_OK && *h > dst_capacity) rc = MNRT_ERR_CAPACITY;  // nothing written
+    if (rc == MNRT_OK && *h > 0) {
+        hipLaunchKernelGGL(k_append_valid, dim3((n + 255) / 256), dim3(256), 0, st, n, d_valid, pos,
+                           (const float4*)d_src_pos4, (const float4*)d_src_pow4, (float4*)d_dst_pos4,
+                           (float4*)d_dst_pow4, dst_capacity);
+        if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
+    }
+    if (rc == MNRT_OK) *out_added = *h;
+    (void)pool_free(buf, st);
+    return rc;
+}
+
+extern "C" int mnrt_photons_scale_flux(uint32_t n, float* d_io_pow4, float scale, void* stream) {
+    if (n == 0) return MNRT_OK;
+    if (!d_io_pow4) return MNRT_ERR_ARG;
+    hipLaunchKernelGGL(mnrt::k_scale_power, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                       (float4*)d_io_pow4, scale);
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
