@@ -116,8 +116,8 @@ struct ShadeBatchArgs {
 int launch_trace_batch(const TraceBatchArgs& B, int stack, uint32_t grid, uint32_t block, hipStream_t st);
 int launch_knn_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* slot_begin, uint32_t ns,
                      const uint32_t* count, uint32_t chunk, uint32_t num_parts, const uint32_t* part_begin,
-                     const uint32_t* part_end, uint32_t* work, uint32_t cap, uint32_t waves, uint32_t grid,
-                     hipStream_t st);
+                     const uint32_t* part_end, uint32_t* work, uint32_t cap, uint32_t waves, uint32_t sel,
+                     uint32_t grid, hipStream_t st);
 int launch_shade_batch(const ShadeBatchArgs& A, uint32_t* flags, uint32_t* scan_tmp, hipStream_t st);
 
 }  // namespace mnrt

@@ -60,8 +60,59 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t v) {
     return p;
 }
 
+// Selection of the k best candidates when the buffer fills up:
+//   kSelSort  : wave-wide bitonic sort of the buffer, keep the first k;
+//   kSelRadix : the k-th smallest key by a wave-wide binary search over its bits (count of
+//               keys below a probe: one ballot per key slot), then compaction of the keys
+//               up to it, unsorted; the final set is sorted once at the end.
+enum : int { kSelSort = 0, kSelRadix = 1 };
+
+// k-th smallest (1-based) of the keys held R per lane (~0 for empty slots), wave-uniform;
+// the largest v with count(key < v) < k.
+template <int R>
+__device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint32_t k) {
+    // high word (d2 bits) first, then the low word (photon index) among the ties
+    uint32_t hi = 0;
+    for (int b = 31; b >= 0; --b) {
+        const uint32_t probe = hi | (1u << b);
+        uint32_t c = 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t) c += (uint32_t)__popcll(__ballot((uint32_t)(key[t] >> 32) < probe));
+        if (c < k) hi = probe;
+    }
+    uint32_t below = 0, tie = 0;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const uint32_t h = (uint32_t)(key[t] >> 32);
+        below += (uint32_t)__popcll(__ballot(h < hi));
+        tie += (uint32_t)__popcll(__ballot(h == hi));
+    }
+    const uint32_t need = k - below;  // 1 <= need <= tie
+    if (need == tie) {
+        // every tie is kept: the threshold is the largest tie
+        uint32_t lo = 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            uint32_t v = ((uint32_t)(key[t] >> 32) == hi) ? (uint32_t)key[t] : 0u;
+            for (uint32_t o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+            lo = max(lo, v);
+        }
+        return ((uint64_t)hi << 32) | uniform(lo);
+    }
+    uint32_t lo = 0;
+    for (int b = 31; b >= 0; --b) {
+        const uint32_t probe = lo | (1u << b);
+        uint32_t c = 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+            c += (uint32_t)__popcll(__ballot((uint32_t)(key[t] >> 32) == hi && (uint32_t)key[t] < probe));
+        if (c < need) lo = probe;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // One query by one wave; keys / kpos: CAP LDS entries, snode / soff: kKnnStack.
-template <int CAP, int kMode>
+template <int CAP, int kMode, int kSel = kSelSort>
 __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t lane, uint64_t* keys, uint32_t* kpos,
                                           uint32_t* snode, float4* soff) {
     const float4 qp4 = A.pos[q];
@@ -80,7 +131,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
     // per-axis distance from the query to the current node's box (0 inside)
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
 
-    auto select = [&](uint32_t keep) {
+    // sorts the buffer and keeps the first `keep`
+    auto sort_keep = [&](uint32_t keep) {
         uint32_t size = pow2_at_least(cnt);
         for (uint32_t t = cnt + lane; t < size; t += 64) {
             keys[t] = ~0ull;
@@ -95,9 +147,45 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
             thr_d2 = __uint_as_float((uint32_t)(thr_key >> 32));
         }
     };
+    // keeps the k best, unsorted (cnt > k)
+    auto radix_keep = [&]() {
+        constexpr int R = CAP / 64;
+        uint64_t key[R];
+        uint32_t pos[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const uint32_t i = lane + 64u * t;
+            key[t] = i < cnt ? keys[i] : ~0ull;
+            pos[t] = i < cnt ? kpos[i] : 0u;
+        }
+        const uint64_t kth = wave_kth_key<R>(key, k);
+        wave_sync();
+        uint32_t base = 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const bool keepit = key[t] <= kth;
+            const uint64_t m = __ballot(keepit);
+            if (keepit) {
+                const uint32_t slot = base + mask_prefix(m);
+                keys[slot] = key[t];
+                kpos[slot] = pos[t];
+            }
+            base += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        cnt = k;
+        full = true;
+        thr_key = kth;
+        thr_d2 = __uint_as_float((uint32_t)(kth >> 32));
+    };
+    auto select = [&](uint32_t keep) {
+        if (kSel == kSelRadix && cnt > keep) radix_keep();
+        else sort_keep(keep);
+    };
 
     for (;;) {
-        uint2 nd = A.nodes[node];
+        node = uniform(node);
+        uint2 nd = ld_uniform(A.nodes, node);
         nd.x = uniform(nd.x);
         nd.y = uniform(nd.y);
         while (!is_leaf(nd.y)) {
@@ -121,8 +209,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
                 }
                 ++sp;
             }
-            node = nearc;
-            nd = A.nodes[node];
+            node = uniform(nearc);
+            nd = ld_uniform(A.nodes, node);
             nd.x = uniform(nd.x);
             nd.y = uniform(nd.y);
         }
@@ -160,19 +248,20 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
         bool found = false;
         while (sp > 0) {
             --sp;
-            float4 f = soff[sp];
-            if (full && f.w > thr_d2) continue;
-            node = snode[sp];
-            ox = f.x;
-            oy = f.y;
-            oz = f.z;
+            const float4 f = soff[sp];
+            if (full && uniformf(f.w) > thr_d2) continue;
+            node = uniform(snode[sp]);
+            ox = uniformf(f.x);
+            oy = uniformf(f.y);
+            oz = uniformf(f.z);
             found = true;
             break;
         }
         wave_sync();
         if (!found) break;
     }
-    select(k);
+    if (kSel == kSelRadix && cnt > k) radix_keep();
+    sort_keep(k);
     const uint32_t found = cnt;  // min(#photons closer than r_max, k)
 
     float r2 = r2max;
@@ -273,7 +362,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
     }
 }
 
-template <int CAP, int kMode>
+template <int CAP, int kMode, int kSel>
 __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
     __shared__ uint64_t s_key[kWavesPerBlock][CAP];
     __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
@@ -283,14 +372,14 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t q = blockIdx.x * kWavesPerBlock + wave;
     if (q >= A.n) return;  // uniform per wave
-    knn_query<CAP, kMode>(A, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
+    knn_query<CAP, kMode, kSel>(A, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
 }
 
 // Batched radiance: waves take chunks of `chunk` consecutive query slots from a work
 // counter (one counter per XCD share, as k_trace_batch), slot g belonging to the scene
 // whose slot range holds it.  Slots at or past a scene's count (count[s], device side:
 // the number of shading points the batched compaction wrote) are left untouched.
-template <int CAP, int kWaves>
+template <int CAP, int kWaves, int kSel>
 __global__ __launch_bounds__(64 * kWaves) void k_knn_batch(KnnArgs G, const KnnScene* __restrict__ scenes,
                                                            const uint32_t* __restrict__ slot_begin, uint32_t ns,
                                                            const uint32_t* __restrict__ count, uint32_t chunk,
@@ -314,18 +403,31 @@ __global__ __launch_bounds__(64 * kWaves) void k_knn_batch(KnnArgs G, const KnnS
             const uint64_t g0 = (uint64_t)pb + (uint64_t)c * chunk;
             if (g0 >= pe) break;
             const uint32_t g1 = (uint32_t)min<uint64_t>(g0 + chunk, pe);
+            // the scene of the chunk's slots, refreshed only when a slot crosses into the next
             uint32_t s = uniform(seg_of(slot_begin, ns, (uint32_t)g0));
+            uint32_t b = 0, e = 0, m = 0;
+            KnnArgs A = G;
+            auto enter = [&]() {
+                b = ld_uniform(slot_begin, s);
+                e = ld_uniform(slot_begin, s + 1);
+                m = count ? ld_uniform(count, s) : e - b;
+                const KnnScene sc = ld_uniform(scenes, s);
+                A.nodes = sc.nodes;
+                A.recs = sc.recs;
+                A.r2max = sc.r2max;
+                A.r_max = sc.r_max;
+            };
+            enter();
             for (uint32_t g = (uint32_t)g0; g < g1; ++g) {
-                while (g >= slot_begin[s + 1]) ++s;
-                const uint32_t b = slot_begin[s];
-                const uint32_t m = count ? count[s] : slot_begin[s + 1] - b;
+                if (g >= e) {
+                    do {
+                        ++s;
+                    } while (g >= ld_uniform(slot_begin, s + 1));
+                    enter();
+                }
                 if (g - b >= m) continue;
-                KnnArgs A = G;
-                A.nodes = scenes[s].nodes;
-                A.recs = scenes[s].recs;
-                A.r2max = scenes[s].r2max;
-                A.r_max = scenes[s].r_max;
-                knn_query<CAP, kModeRadiance>(A, g, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
+                knn_query<CAP, kModeRadiance, kSel>(A, g, lane, s_key[wave], s_pos[wave], s_snode[wave],
+                                                    s_soff[wave]);
                 wave_sync();
             }
         }
@@ -393,39 +495,51 @@ __global__ __launch_bounds__(256) void k_fix_to_float(uint32_t np, const unsigne
                          (float)((double)(long long)acc[3ull * p + 2] * inv_scale), 0.0f);
 }
 
+// Selection strategy of the single-tree launches.
+constexpr int kKnnSelDefault = kSelRadix;
+
 template <int kMode>
 int launch_knn(const KnnArgs& A, hipStream_t st) {
     dim3 block(64 * kWavesPerBlock), grid((A.n + kWavesPerBlock - 1) / kWavesPerBlock);
     uint32_t need = A.k + 64;
     if (need <= 128)
-        hipLaunchKernelGGL((k_knn<128, kMode>), grid, block, 0, st, A);
+        hipLaunchKernelGGL((k_knn<128, kMode, kKnnSelDefault>), grid, block, 0, st, A);
     else if (need <= 256)
-        hipLaunchKernelGGL((k_knn<256, kMode>), grid, block, 0, st, A);
+        hipLaunchKernelGGL((k_knn<256, kMode, kKnnSelDefault>), grid, block, 0, st, A);
     else
-        hipLaunchKernelGGL((k_knn<512, kMode>), grid, block, 0, st, A);
+        hipLaunchKernelGGL((k_knn<512, kMode, kKnnSelDefault>), grid, block, 0, st, A);
     MNRT_HIP_CHECK(hipGetLastError());
     return MNRT_OK;
 }
 
-
 int launch_knn_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* slot_begin, uint32_t ns,
                      const uint32_t* count, uint32_t chunk, uint32_t num_parts, const uint32_t* part_begin,
-                     const uint32_t* part_end, uint32_t* work, uint32_t cap, uint32_t waves, uint32_t grid,
-                     hipStream_t st) {
+                     const uint32_t* part_end, uint32_t* work, uint32_t cap, uint32_t waves, uint32_t sel,
+                     uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, work, num_parts, 0u);
-#define MNRT_KB(C, W)                                                                                        \
-    hipLaunchKernelGGL((k_knn_batch<C, W>), dim3(grid), dim3(64 * W), 0, st, G, scenes, slot_begin, ns, count, \
-                       chunk, num_parts, part_begin, part_end, work)
-    if (cap == 128 && waves == 1) MNRT_KB(128, 1);
-    else if (cap == 128 && waves == 2) MNRT_KB(128, 2);
-    else if (cap == 128 && waves == 4) MNRT_KB(128, 4);
-    else if (cap == 256 && waves == 1) MNRT_KB(256, 1);
-    else if (cap == 256 && waves == 2) MNRT_KB(256, 2);
-    else if (cap == 256 && waves == 4) MNRT_KB(256, 4);
-    else if (cap == 512 && waves == 1) MNRT_KB(512, 1);
-    else if (cap == 512 && waves == 2) MNRT_KB(512, 2);
-    else if (cap == 512 && waves == 4) MNRT_KB(512, 4);
-    else return MNRT_ERR_ARG;
+#define MNRT_KB(C, W, S)                                                                                    \
+    hipLaunchKernelGGL((k_knn_batch<C, W, S>), dim3(grid), dim3(64 * W), 0, st, G, scenes, slot_begin, ns, \
+                       count, chunk, num_parts, part_begin, part_end, work)
+#define MNRT_KB_W(C, S)                   \
+    do {                                  \
+        if (waves == 1) MNRT_KB(C, 1, S); \
+        else if (waves == 2) MNRT_KB(C, 2, S); \
+        else MNRT_KB(C, 4, S);            \
+    } while (0)
+    if (waves != 1 && waves != 2 && waves != 4) return MNRT_ERR_ARG;
+    if (cap == 128) {
+        if (sel) MNRT_KB_W(128, kSelRadix);
+        else MNRT_KB_W(128, kSelSort);
+    } else if (cap == 256) {
+        if (sel) MNRT_KB_W(256, kSelRadix);
+        else MNRT_KB_W(256, kSelSort);
+    } else if (cap == 512) {
+        if (sel) MNRT_KB_W(512, kSelRadix);
+        else MNRT_KB_W(512, kSelSort);
+    } else {
+        return MNRT_ERR_ARG;
+    }
+#undef MNRT_KB_W
 #undef MNRT_KB
     MNRT_HIP_CHECK(hipGetLastError());
     return MNRT_OK;

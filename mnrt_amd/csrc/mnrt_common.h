@@ -113,6 +113,19 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Load of read-only data at a wave-uniform address through the constant address space, so
+// it becomes a scalar load (scalar cache, result in SGPRs).  Only for data no kernel of the
+// launch writes (tree node words, descriptor tables).
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __attribute__((address_space(4))) T* c = (const __attribute__((address_space(4))) T*)p;
+    return c[i];
+#else
+    return p[i];
+#endif
+}
 __device__ __forceinline__ float uniformf(float v) {
     return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
 }

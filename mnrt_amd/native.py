@@ -79,7 +79,7 @@ class BatchTuning(ctypes.Structure):
     """mnrt_batch_tuning."""
 
     _fields_ = [(n, u32) for n in ("trace_block", "trace_tile", "trace_grid", "trace_stack", "knn_waves",
-                                   "knn_chunk", "knn_grid", "knn_cap", "parts")]
+                                   "knn_chunk", "knn_grid", "knn_cap", "parts", "knn_select")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
