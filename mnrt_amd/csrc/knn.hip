@@ -16,9 +16,10 @@
 //     with one coalesced 16-byte load each;
 //   * candidates (key = d2 bits << 32 | photon index, so integer order is (d2, index)
 //     order) are appended to a per-wave LDS buffer with ballot + mbcnt compaction; when
-//     k are known or the buffer would overflow, a wave-wide bitonic sort keeps the k
-//     best and fixes the pruning threshold;
-//   * the estimate sums the front-facing powers of the final set in (d2, index) order.
+//     k are known or the buffer would overflow, the k best are kept (radix selection of
+//     the k-th key, or a bitonic sort) and the pruning threshold tightens;
+//   * the final set is sorted once; its records are fetched by all lanes at once and the
+//     front-facing powers summed in (d2, index) order.
 #include <cmath>
 #include <cstring>
 
@@ -111,16 +112,24 @@ __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint3
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The tree a query walks: from the launch arguments (one tree) or the query's scene.
+struct KnnTree {
+    const uint2* nodes;
+    const float4* recs;
+    float r2max;
+    float r_max;
+};
+
 // One query by one wave; keys / kpos: CAP LDS entries, snode / soff: kKnnStack.
 template <int CAP, int kMode, int kSel = kSelSort>
-__device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t lane, uint64_t* keys, uint32_t* kpos,
-                                          uint32_t* snode, float4* soff) {
+__device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, uint32_t q, uint32_t lane,
+                                          uint64_t* keys, uint32_t* kpos, uint32_t* snode, float4* soff) {
     const float4 qp4 = A.pos[q];
     const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
     const f3 qp = mk3(qx, qy, qz);
     const uint32_t k = A.k;
-    const float rq = A.r_query ? uniformf(A.r_query[q]) : A.r_max;
-    const float r2max = A.r_query ? rq * rq : A.r2max;
+    const float rq = A.r_query ? uniformf(A.r_query[q]) : Tr.r_max;
+    const float r2max = A.r_query ? rq * rq : Tr.r2max;
 
     uint32_t cnt = 0;        // candidates in the buffer
     bool full = false;       // the buffer's first k are the k best so far
@@ -185,7 +194,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
 
     for (;;) {
         node = uniform(node);
-        uint2 nd = ld_uniform(A.nodes, node);
+        uint2 nd = ld_uniform(Tr.nodes, node);
         nd.x = uniform(nd.x);
         nd.y = uniform(nd.y);
         while (!is_leaf(nd.y)) {
@@ -210,7 +219,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
                 ++sp;
             }
             node = uniform(nearc);
-            nd = ld_uniform(A.nodes, node);
+            nd = ld_uniform(Tr.nodes, node);
             nd.x = uniform(nd.x);
             nd.y = uniform(nd.y);
         }
@@ -221,7 +230,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
             bool take = false;
             uint64_t key = 0;
             if (j < count) {
-                float4 p = A.recs[2ull * (begin + j)];
+                float4 p = Tr.recs[2ull * (begin + j)];
                 f3 df = sub3(xyz(p), qp);
                 float d2 = dot3(df, df);
                 key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
@@ -284,16 +293,33 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
     f3 nrd = mk3(-rd.x, -rd.y, -rd.z);
     if (dot3(nrd, nq) < 0.0f) nq = mk3(nq.x * -1.0f, nq.y * -1.0f, nq.z * -1.0f);
 
+    // Front-facing powers summed in (d2, index) order.  Lanes fetch and test 64 neighbours
+    // at a time into LDS (the traversal stack's box offsets, free now), then the sum runs
+    // through them in order.
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (uint32_t i = 0; i < found; ++i) {
-        uint32_t p = kpos[i];
-        float4 pw = A.recs[2ull * p + 1];
-        f3 pd = oct_decode(__float_as_uint(pw.w));
-        if (dot3(nq, pd) < 0.0f) {
-            sx = sx + pw.x;
-            sy = sy + pw.y;
-            sz = sz + pw.z;
+    static_assert(kKnnStack >= 64, "the stage holds one entry per lane");
+    float4* stage = soff;
+    for (uint32_t base = 0; base < found; base += 64) {
+        const uint32_t i = base + lane;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < found) {
+            const float4 pw = Tr.recs[2ull * kpos[i] + 1];
+            const f3 pd = oct_decode(__float_as_uint(pw.w));
+            v = make_float4(pw.x, pw.y, pw.z, dot3(nq, pd) < 0.0f ? 1.0f : 0.0f);
         }
+        wave_sync();
+        stage[lane] = v;
+        wave_sync();
+        const uint32_t m = min(64u, found - base);
+        for (uint32_t j = 0; j < m; ++j) {
+            const float4 e = stage[j];
+            if (e.w != 0.0f) {
+                sx = sx + e.x;
+                sy = sy + e.y;
+                sz = sz + e.z;
+            }
+        }
+        wave_sync();
     }
     float ix = 0.0f, iy = 0.0f, iz = 0.0f;
     if (r2 > 0.0f) {
@@ -329,7 +355,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, uint32_t q, uint32_t
         long long fz = __double2ll_rn((double)wz * A.fix_scale);
         if (fx != 0 || fy != 0 || fz != 0) {
             for (uint32_t i = lane; i < found; i += 64) {
-                float4 pw = A.recs[2ull * kpos[i] + 1];
+                float4 pw = Tr.recs[2ull * kpos[i] + 1];
                 f3 pd = oct_decode(__float_as_uint(pw.w));
                 if (dot3(nq, pd) < 0.0f) {
                     size_t p = (size_t)(keys[i] & 0xFFFFFFFFull);
@@ -372,7 +398,8 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t q = blockIdx.x * kWavesPerBlock + wave;
     if (q >= A.n) return;  // uniform per wave
-    knn_query<CAP, kMode, kSel>(A, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
+    const KnnTree Tr{A.nodes, A.recs, A.r2max, A.r_max};
+    knn_query<CAP, kMode, kSel>(A, Tr, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
 }
 
 // Batched radiance: waves take chunks of `chunk` consecutive query slots from a work
@@ -380,7 +407,7 @@ __global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
 // whose slot range holds it.  Slots at or past a scene's count (count[s], device side:
 // the number of shading points the batched compaction wrote) are left untouched.
 template <int CAP, int kWaves, int kSel>
-__global__ __launch_bounds__(64 * kWaves) void k_knn_batch(KnnArgs G, const KnnScene* __restrict__ scenes,
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_knn_batch(KnnArgs G, const KnnScene* __restrict__ scenes,
                                                            const uint32_t* __restrict__ slot_begin, uint32_t ns,
                                                            const uint32_t* __restrict__ count, uint32_t chunk,
                                                            uint32_t num_parts, const uint32_t* __restrict__ part_begin,
@@ -406,16 +433,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_knn_batch(KnnArgs G, const KnnS
             // the scene of the chunk's slots, refreshed only when a slot crosses into the next
             uint32_t s = uniform(seg_of(slot_begin, ns, (uint32_t)g0));
             uint32_t b = 0, e = 0, m = 0;
-            KnnArgs A = G;
+            KnnTree Tr{};
             auto enter = [&]() {
                 b = ld_uniform(slot_begin, s);
                 e = ld_uniform(slot_begin, s + 1);
                 m = count ? ld_uniform(count, s) : e - b;
                 const KnnScene sc = ld_uniform(scenes, s);
-                A.nodes = sc.nodes;
-                A.recs = sc.recs;
-                A.r2max = sc.r2max;
-                A.r_max = sc.r_max;
+                Tr = KnnTree{sc.nodes, sc.recs, sc.r2max, sc.r_max};
             };
             enter();
             for (uint32_t g = (uint32_t)g0; g < g1; ++g) {
@@ -426,7 +450,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_knn_batch(KnnArgs G, const KnnS
                     enter();
                 }
                 if (g - b >= m) continue;
-                knn_query<CAP, kModeRadiance, kSel>(A, g, lane, s_key[wave], s_pos[wave], s_snode[wave],
+                knn_query<CAP, kModeRadiance, kSel>(G, Tr, g, lane, s_key[wave], s_pos[wave], s_snode[wave],
                                                     s_soff[wave]);
                 wave_sync();
             }
