@@ -246,6 +246,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one 256 MiB device-to-device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -259,6 +261,13 @@ def main():
 
     native.check(native.load().mnrt_device_init(0), "mnrt_device_init")
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4}))
+        return
 
     t_setup = time.time()
     if args.config == "large":

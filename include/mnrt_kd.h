@@ -358,7 +358,8 @@ typedef struct mnrt_batch_tuning {
     uint32_t knn_grid;    /* persistent workgroups */
     uint32_t knn_cap;     /* LDS candidate entries per wave: 128, 256, 512; >= k + 64 */
     uint32_t parts;       /* work-counter shares: 1, or 8 (one per XCD) */
-    uint32_t knn_select;  /* candidate selection: 0 bitonic sort, 1 radix select of the k-th key */
+    uint32_t knn_select;  /* candidate selection: 0 bitonic sort, 1 radix select of the k-th key,
+                             2 threshold window, exact selection at the end */
 } mnrt_batch_tuning;
 
 /* Buffers of the phases (slot-indexed unless stated). */

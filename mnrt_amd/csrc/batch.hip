@@ -68,7 +68,7 @@ bool valid_tuning(const mnrt_batch& b, const mnrt_batch_tuning& t) {
     if (b.has_knn && t.knn_cap < b.k + 64) return false;
     if (t.knn_cap == 512 && t.knn_waves > 4) return false;
     if (t.parts != 1 && t.parts != kMaxParts) return false;
-    if (t.knn_select > 1) return false;
+    if (t.knn_select > 2) return false;
     return true;
 }
 
@@ -340,7 +340,7 @@ extern "C" int mnrt_batch_create(const mnrt_batch_desc* d, mnrt_batch** out) {
         t.knn_grid = 8 * b->cus;
         t.knn_cap = knn ? min_cap(b->k) : 128;
         t.parts = kMaxParts;
-        t.knn_select = 1;
+        t.knn_select = 2;
         rc = upload_tables(*b);
     }
     if (rc == MNRT_OK && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) rc = MNRT_ERR_HIP;
@@ -470,7 +470,7 @@ extern "C" int mnrt_batch_autotune(mnrt_batch* b, const mnrt_batch_io* io, uint3
     search(MNRT_PHASE_TRACE, &mnrt_batch_tuning::trace_grid, {2 * C, 4 * C, 8 * C, 16 * C, 32 * C});
     if (b->has_knn) {
         const uint32_t c0 = min_cap(b->k);
-        search(MNRT_PHASE_KNN, &mnrt_batch_tuning::knn_select, {0, 1});
+        search(MNRT_PHASE_KNN, &mnrt_batch_tuning::knn_select, {0, 1, 2});
         search(MNRT_PHASE_KNN, &mnrt_batch_tuning::knn_cap, {c0, 2 * c0});
         search(MNRT_PHASE_KNN, &mnrt_batch_tuning::knn_waves, {1, 2, 4});
         search(MNRT_PHASE_KNN, &mnrt_batch_tuning::knn_chunk, {1, 4, 16, 64});

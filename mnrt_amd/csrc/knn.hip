@@ -66,7 +66,10 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t v) {
 //   kSelRadix : the k-th smallest key by a wave-wide binary search over its bits (count of
 //               keys below a probe: one ballot per key slot), then compaction of the keys
 //               up to it, unsorted; the final set is sorted once at the end.
-enum : int { kSelSort = 0, kSelRadix = 1 };
+//   kSelWindow: intermediate selections keep any set of at least k and at most CAP - 64
+//               keys below a high-word threshold (a short binary search); the exact k
+//               best are selected once, at the end, as kSelRadix does.
+enum : int { kSelSort = 0, kSelRadix = 1, kSelWindow = 2 };
 
 // k-th smallest (1-based) of the keys held R per lane (~0 for empty slots), wave-uniform;
 // the largest v with count(key < v) < k.
@@ -110,6 +113,41 @@ __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint3
         if (c < need) lo = probe;
     }
     return ((uint64_t)hi << 32) | lo;
+}
+
+// A pruning threshold on the high word (d2 bits) that keeps between k and hi_k of the cnt
+// keys held R per lane (~0 for empty slots): *upper = count(h <= T).  Binary search over
+// the high word that stops as soon as the keys at or below the range top number at most
+// hi_k (far fewer steps than the exact k-th); *upper > hi_k means ties made it impossible.
+template <int R>
+__device__ __forceinline__ uint32_t wave_window_hi(const uint64_t (&key)[R], uint32_t k, uint32_t hi_k, uint32_t cnt,
+                                                   uint32_t* upper_out) {
+    if (cnt <= hi_k) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const uint32_t h = (uint32_t)(key[t] >> 32);
+            uint32_t v = h != 0xFFFFFFFFu ? h : 0u;
+            for (uint32_t o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+            m = max(m, v);
+        }
+        *upper_out = cnt;
+        return uniform(m);
+    }
+    // [prefix, prefix + width) holds the k-th high word; upper = count(h < prefix + width)
+    uint32_t prefix = 0, upper = cnt;
+    uint64_t width = 1ull << 32;
+    while (width > 1 && upper > hi_k) {
+        width >>= 1;
+        const uint32_t probe = prefix + (uint32_t)width;
+        uint32_t c = 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t) c += (uint32_t)__popcll(__ballot((uint32_t)(key[t] >> 32) < probe));
+        if (c >= k) upper = c;
+        else prefix = probe;
+    }
+    *upper_out = upper;
+    return prefix + (uint32_t)(width - 1);
 }
 
 // The tree a query walks: from the launch arguments (one tree) or the query's scene.
@@ -187,8 +225,45 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         thr_key = kth;
         thr_d2 = __uint_as_float((uint32_t)(kth >> 32));
     };
+    // keeps at least k and at most CAP - 64 candidates under a high-word threshold
+    auto window_keep = [&]() {
+        constexpr int R = CAP / 64;
+        uint64_t key[R];
+        uint32_t pos[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const uint32_t i = lane + 64u * t;
+            key[t] = i < cnt ? keys[i] : ~0ull;
+            pos[t] = i < cnt ? kpos[i] : 0u;
+        }
+        uint32_t upper = 0;
+        const uint32_t T = wave_window_hi<R>(key, k, (uint32_t)CAP - 64u, cnt, &upper);
+        if (upper > (uint32_t)CAP - 64u) {  // equal distances: only the exact selection fits
+            radix_keep();
+            return;
+        }
+        wave_sync();
+        uint32_t base = 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const bool keepit = (uint32_t)(key[t] >> 32) <= T && key[t] != ~0ull;
+            const uint64_t m = __ballot(keepit);
+            if (keepit) {
+                const uint32_t slot = base + mask_prefix(m);
+                keys[slot] = key[t];
+                kpos[slot] = pos[t];
+            }
+            base += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        cnt = upper;
+        full = true;
+        thr_key = ((uint64_t)T << 32) | 0xFFFFFFFFull;
+        thr_d2 = __uint_as_float(T);
+    };
     auto select = [&](uint32_t keep) {
-        if (kSel == kSelRadix && cnt > keep) radix_keep();
+        if (kSel == kSelWindow) window_keep();
+        else if (kSel == kSelRadix && cnt > keep) radix_keep();
         else sort_keep(keep);
     };
 
@@ -269,7 +344,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         wave_sync();
         if (!found) break;
     }
-    if (kSel == kSelRadix && cnt > k) radix_keep();
+    if (kSel != kSelSort && cnt > k) radix_keep();
     sort_keep(k);
     const uint32_t found = cnt;  // min(#photons closer than r_max, k)
 
@@ -305,7 +380,9 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         if (i < found) {
             const float4 pw = Tr.recs[2ull * kpos[i] + 1];
             const f3 pd = oct_decode(__float_as_uint(pw.w));
-            v = make_float4(pw.x, pw.y, pw.z, dot3(nq, pd) < 0.0f ? 1.0f : 0.0f);
+            // back-facing photons add +0: the running sums start at +0 and are never -0,
+            // so s + 0 == s and the sums equal the conditional ones bit for bit
+            if (dot3(nq, pd) < 0.0f) v = make_float4(pw.x, pw.y, pw.z, 0.0f);
         }
         wave_sync();
         stage[lane] = v;
@@ -313,11 +390,9 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         const uint32_t m = min(64u, found - base);
         for (uint32_t j = 0; j < m; ++j) {
             const float4 e = stage[j];
-            if (e.w != 0.0f) {
-                sx = sx + e.x;
-                sy = sy + e.y;
-                sz = sz + e.z;
-            }
+            sx = sx + e.x;
+            sy = sy + e.y;
+            sz = sz + e.z;
         }
         wave_sync();
     }
@@ -520,7 +595,7 @@ __global__ __launch_bounds__(256) void k_fix_to_float(uint32_t np, const unsigne
 }
 
 // Selection strategy of the single-tree launches.
-constexpr int kKnnSelDefault = kSelRadix;
+constexpr int kKnnSelDefault = kSelWindow;
 
 template <int kMode>
 int launch_knn(const KnnArgs& A, hipStream_t st) {
@@ -552,13 +627,16 @@ int launch_knn_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* s
     } while (0)
     if (waves != 1 && waves != 2 && waves != 4) return MNRT_ERR_ARG;
     if (cap == 128) {
-        if (sel) MNRT_KB_W(128, kSelRadix);
+        if (sel == 2) MNRT_KB_W(128, kSelWindow);
+        else if (sel == 1) MNRT_KB_W(128, kSelRadix);
         else MNRT_KB_W(128, kSelSort);
     } else if (cap == 256) {
-        if (sel) MNRT_KB_W(256, kSelRadix);
+        if (sel == 2) MNRT_KB_W(256, kSelWindow);
+        else if (sel == 1) MNRT_KB_W(256, kSelRadix);
         else MNRT_KB_W(256, kSelSort);
     } else if (cap == 512) {
-        if (sel) MNRT_KB_W(512, kSelRadix);
+        if (sel == 2) MNRT_KB_W(512, kSelWindow);
+        else if (sel == 1) MNRT_KB_W(512, kSelRadix);
         else MNRT_KB_W(512, kSelSort);
     } else {
         return MNRT_ERR_ARG;
