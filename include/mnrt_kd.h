@@ -142,7 +142,7 @@ int mnrt_build_points_batch(const float* d_pos4, const uint32_t* h_offsets, uint
                             const float* h_max_query_radius, mnrt_tree* out, void* stream);
 /* Records of a batch (trees from one batched build, inputs concatenated as for the
  * build): one shared record array owned by trees[0]; photon batches are collapsed with
- * max_refs = 64 like mnrt_tree_attach_photon_records. */
+ * max_refs = 256 like mnrt_tree_attach_photon_records. */
 int mnrt_tree_attach_tri_records_batch(mnrt_tree* trees, uint32_t num_scenes, const float* d_v0,
                                        const float* d_v1, const float* d_v2,
                                        const uint32_t* h_offsets, void* stream);
@@ -163,7 +163,7 @@ int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_pos4, const 
  * references becomes a leaf over that contiguous reference range (a subtree's leaves are
  * consecutive in preorder).  A wave then tests such a subtree's elements in one step.
  * The build's node words (d_nodes) are unchanged; queries read d_qnodes.  Query results
- * do not depend on it.  mnrt_tree_attach_photon_records applies max_refs = 64. */
+ * do not depend on it.  mnrt_tree_attach_photon_records applies max_refs = 256. */
 int mnrt_tree_collapse(mnrt_tree* tree, uint32_t max_refs, void* stream);
 
 /* Reference KDTreeData layout of a tree (KDTreeData, KDKernelDefs.h:76-194, as

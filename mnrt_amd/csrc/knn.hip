@@ -8,7 +8,7 @@
 // Design (one wave64 per query):
 //   * the point tree is walked by the whole wave with wave-uniform control flow (node
 //     words are scalar loads) over the query-time node words, in which every subtree of
-//     at most 64 photons is one leaf; far children go onto a per-wave LDS stack with
+//     at most kQueryLeafRefs photons is one leaf; far children go onto a per-wave LDS stack with
 //     their box distance (dev_RangeSearch pushes them by plane distance,
 //     photon_knn.cu:146-171).  Rounding is monotone, so the fp32 box distance never
 //     exceeds the fp32 distance of a photon inside the box and pruning by it is exact;

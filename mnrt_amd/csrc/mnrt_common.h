@@ -135,6 +135,11 @@ __device__ __forceinline__ uint32_t mask_prefix(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Query-time leaves of photon trees: subtrees with at most this many references (the kNN
+// kernel tests a leaf 64 photons per step; measured on 1M photons, k = 50 / 100: 16 and 32
+// are slowest, 256 and 512 fastest, 2048 slower again).
+constexpr uint32_t kQueryLeafRefs = 256;
+
 // Device memory of the library: a stream-ordered pool per device that keeps freed blocks
 // for reuse (release threshold: never), so repeated builds and queries do not map and
 // unmap pages.  Tree arrays come from it too; mnrt_tree_release returns them on the

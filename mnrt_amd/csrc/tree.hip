@@ -226,7 +226,7 @@ extern "C" int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_p
                        tree->d_refs, tree->num_refs, (const float4*)d_pos4, (const float4*)d_pow4,
                        (const float4*)d_dir4, (float4*)tree->d_recs);
     MNRT_HIP_CHECK(hipGetLastError());
-    return mnrt_tree_collapse(tree, 64, stream);
+    return mnrt_tree_collapse(tree, mnrt::kQueryLeafRefs, stream);
 }
 
 // ---- batches: trees[s] view shared arrays owned by trees[0] ----
@@ -329,7 +329,8 @@ extern "C" int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t 
                            (const float4*)d_pos4, (const float4*)d_pow4, (const float4*)d_dir4, (float4*)recs);
     if (rc == MNRT_OK)
         hipLaunchKernelGGL(mnrt::k_collapse, dim3((B.total_nodes + 255) / 256), dim3(256), 0, st,
-                           (const uint2*)trees[0].d_nodes, B.total_nodes, (const uint32_t*)d_tab, ns, 64u, (uint2*)qn);
+                           (const uint2*)trees[0].d_nodes, B.total_nodes, (const uint32_t*)d_tab, ns, mnrt::kQueryLeafRefs,
+                           (uint2*)qn);
     if (rc == MNRT_OK && hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
     if (d_tab) (void)mnrt::pool_free(d_tab, st);
     if (hipStreamSynchronize(st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
@@ -341,7 +342,7 @@ extern "C" int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t 
     for (uint32_t s = 0; s < num_scenes; ++s) {
         trees[s].d_recs = recs + 8ull * B.h[ns + 1 + s];
         trees[s].d_qnodes = qn + 2ull * B.h[s];
-        trees[s].qleaf_max = 64;
+        trees[s].qleaf_max = mnrt::kQueryLeafRefs;
         trees[s].owns_recs = trees[s].owns_qnodes = s == 0 ? 1u : 0u;
     }
     return MNRT_OK;
