@@ -20,6 +20,7 @@
 //     the k-th key, or a bitonic sort) and the pruning threshold tightens;
 //   * the final set is sorted once; its records are fetched by all lanes at once and the
 //     front-facing powers summed in (d2, index) order.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -158,10 +159,21 @@ struct KnnTree {
     float r_max;
 };
 
+// Backward mode, pair output: instead of one atomic per (query, neighbour) the wave appends
+// (photon << 8 | slot) to a per-wave LDS list and keeps the query's fixed-point weights in
+// wtab[slot]; the caller merges a chunk of consecutive queries before touching memory.
+struct PairSink {
+    uint64_t* pairs;
+    uint32_t* npairs;  // LDS word, wave-uniform
+    long long* wtab;   // 3 per slot
+    uint32_t slot;
+};
+
 // One query by one wave; keys / kpos: CAP LDS entries, snode / soff: kKnnStack.
-template <int CAP, int kMode, int kSel = kSelSort>
+template <int CAP, int kMode, int kSel = kSelSort, bool kPairs = false>
 __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, uint32_t q, uint32_t lane,
-                                          uint64_t* keys, uint32_t* kpos, uint32_t* snode, float4* soff) {
+                                          uint64_t* keys, uint32_t* kpos, uint32_t* snode, float4* soff,
+                                          const PairSink* sink = nullptr) {
     const float4 qp4 = A.pos[q];
     const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
     const f3 qp = mk3(qx, qy, qz);
@@ -428,6 +440,32 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         long long fx = __double2ll_rn((double)wx * A.fix_scale);
         long long fy = __double2ll_rn((double)wy * A.fix_scale);
         long long fz = __double2ll_rn((double)wz * A.fix_scale);
+        if (kPairs) {
+            if (fx == 0 && fy == 0 && fz == 0) return;
+            if (lane == 0) {
+                sink->wtab[3 * sink->slot + 0] = fx;
+                sink->wtab[3 * sink->slot + 1] = fy;
+                sink->wtab[3 * sink->slot + 2] = fz;
+            }
+            uint32_t np = uniform(*sink->npairs);
+            for (uint32_t base = 0; base < found; base += 64) {
+                const uint32_t i = base + lane;
+                bool front = false;
+                uint64_t key = 0;
+                if (i < found) {
+                    const float4 pw = Tr.recs[2ull * kpos[i] + 1];
+                    front = dot3(nq, oct_decode(__float_as_uint(pw.w))) < 0.0f;
+                    key = ((keys[i] & 0xFFFFFFFFull) << 8) | sink->slot;
+                }
+                const uint64_t m = __ballot(front);
+                if (front) sink->pairs[np + mask_prefix(m)] = key;
+                np += (uint32_t)__popcll(m);
+            }
+            wave_sync();
+            if (lane == 0) *sink->npairs = np;
+            wave_sync();
+            return;
+        }
         if (fx != 0 || fy != 0 || fz != 0) {
             for (uint32_t i = lane; i < found; i += 64) {
                 float4 pw = Tr.recs[2ull * kpos[i] + 1];
@@ -533,6 +571,76 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     }
 }
 
+// Backward pass over chunks of `chunk` consecutive queries per wave (chunk x k <= kPairs):
+// the chunk's (photon, query) pairs are sorted by photon in LDS and each photon's
+// fixed-point contributions summed before one atomic per channel.  Consecutive queries
+// (neighbouring pixels) share most of their neighbours, so this issues a fraction of the
+// per-pair atomics; the integer sums are the same in any order.
+constexpr int kPairCap = 256;
+
+template <int CAP, int kSel>
+__global__ __launch_bounds__(256) void k_knn_bwd_chunks(KnnArgs A, uint32_t chunk) {
+    __shared__ uint64_t s_key[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_snode[kWavesPerBlock][kKnnStack];
+    __shared__ float4 s_soff[kWavesPerBlock][kKnnStack];
+    __shared__ uint64_t s_pairs[kWavesPerBlock][kPairCap];
+    __shared__ long long s_w[kWavesPerBlock][3 * (kPairCap / 8)];
+    __shared__ uint32_t s_np[kWavesPerBlock];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t c = blockIdx.x * kWavesPerBlock + wave;
+    const uint64_t q0 = (uint64_t)c * chunk;
+    if (q0 >= A.n) return;  // uniform per wave
+    const uint32_t q1 = (uint32_t)min<uint64_t>(q0 + chunk, A.n);
+    uint64_t* pairs = s_pairs[wave];
+    if (lane == 0) s_np[wave] = 0;
+    wave_sync();
+    const KnnTree Tr{A.nodes, A.recs, A.r2max, A.r_max};
+    for (uint32_t q = (uint32_t)q0; q < q1; ++q) {
+        const PairSink sink{pairs, &s_np[wave], s_w[wave], q - (uint32_t)q0};
+        knn_query<CAP, kModeBackward, kSel, true>(A, Tr, q, lane, s_key[wave], s_pos[wave], s_snode[wave],
+                                                  s_soff[wave], &sink);
+        wave_sync();
+    }
+    const uint32_t np = uniform(s_np[wave]);
+    if (np == 0) return;
+    // sort the pairs by (photon, slot); pad to a power of two with keys above every pair
+    uint32_t size = 64;
+    while (size < np) size <<= 1;
+    for (uint32_t t = np + lane; t < size; t += 64) pairs[t] = ~0ull;
+    wave_sync();
+    for (uint32_t k2 = 2; k2 <= size; k2 <<= 1) {
+        for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = lane; t < (size >> 1); t += 64) {
+                const uint32_t i = 2 * j * (t / j) + (t % j), l = i + j;
+                const uint64_t a = pairs[i], b = pairs[l];
+                if ((a > b) == ((i & k2) == 0)) {
+                    pairs[i] = b;
+                    pairs[l] = a;
+                }
+            }
+            wave_sync();
+        }
+    }
+    // one lane per run of equal photons: sum its queries' weights, one atomic per channel
+    const long long* w = s_w[wave];
+    for (uint32_t i = lane; i < np; i += 64) {
+        const uint64_t p = pairs[i] >> 8;
+        if (i > 0 && (pairs[i - 1] >> 8) == p) continue;
+        long long sx = 0, sy = 0, sz = 0;
+        for (uint32_t j = i; j < np && (pairs[j] >> 8) == p; ++j) {
+            const uint32_t sl = (uint32_t)(pairs[j] & 0xFFu);
+            sx += w[3 * sl + 0];
+            sy += w[3 * sl + 1];
+            sz += w[3 * sl + 2];
+        }
+        if (sx) atomicAdd(&A.acc[3 * p + 0], (unsigned long long)sx);
+        if (sy) atomicAdd(&A.acc[3 * p + 1], (unsigned long long)sy);
+        if (sz) atomicAdd(&A.acc[3 * p + 2], (unsigned long long)sz);
+    }
+}
+
 // Adds per-query radiance into slots given by a slot index per query (final gathering,
 // PhotonMap::GatherFor, PhotonMap.cpp:234-270).  A run of equal consecutive slots is
 // summed in query order by the run's first query, so sorted slot indices (as final
@@ -596,6 +704,22 @@ __global__ __launch_bounds__(256) void k_fix_to_float(uint32_t np, const unsigne
 
 // Selection strategy of the single-tree launches.
 constexpr int kKnnSelDefault = kSelWindow;
+
+// Backward launches: queries in chunks of floor(kPairCap / k) per wave (at most 8).
+int launch_knn_bwd(const KnnArgs& A, hipStream_t st) {
+    const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(8u, (uint32_t)kPairCap / A.k));
+    const uint32_t chunks = (A.n + chunk - 1) / chunk;
+    dim3 block(64 * kWavesPerBlock), grid((chunks + kWavesPerBlock - 1) / kWavesPerBlock);
+    const uint32_t need = A.k + 64;
+    if (need <= 128)
+        hipLaunchKernelGGL((k_knn_bwd_chunks<128, kKnnSelDefault>), grid, block, 0, st, A, chunk);
+    else if (need <= 256)
+        hipLaunchKernelGGL((k_knn_bwd_chunks<256, kKnnSelDefault>), grid, block, 0, st, A, chunk);
+    else
+        hipLaunchKernelGGL((k_knn_bwd_chunks<512, kKnnSelDefault>), grid, block, 0, st, A, chunk);
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
 
 template <int kMode>
 int launch_knn(const KnnArgs& A, hipStream_t st) {
@@ -790,7 +914,7 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
         A.acc = acc;
         A.fix_scale = scale;
         A.out_grad_clr = (float4*)d_grad_clr4;
-        rc = mnrt::launch_knn<mnrt::kModeBackward>(A, st);
+        rc = mnrt::launch_knn_bwd(A, st);
     } else if (rc == MNRT_OK && d_grad_clr4) {
         // all weights zero: dLoss/drho still needs the recomputed irradiance
         mnrt::KnnArgs A{};
@@ -810,7 +934,7 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
         A.acc = acc;
         A.fix_scale = 0.0;
         A.out_grad_clr = (float4*)d_grad_clr4;
-        rc = mnrt::launch_knn<mnrt::kModeBackward>(A, st);
+        rc = mnrt::launch_knn_bwd(A, st);
     }
     if (rc == MNRT_OK) {
         hipLaunchKernelGGL(mnrt::k_fix_to_float, dim3((np + 255) / 256), dim3(256), 0, st, np, acc,
