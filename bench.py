@@ -50,6 +50,66 @@ def knn_bytes(counts, n_q, n_set):
     return 36.0 * n_q + 8.0 * (inner + leaves) + 4.0 * refs + 12.0 * tests + 16.0 * float(n_set)
 
 
+def shadow_bytes(counts, n_rays):
+    """Any-hit algorithmic bytes: 16 B point in + 4 B flag out per ray, then as traversal."""
+    inner, leaves, refs, tests = [float(c) for c in counts]
+    return 20.0 * n_rays + 8.0 * inner + 8.0 * leaves + 4.0 * refs + 36.0 * tests
+
+
+def shadow_leg(torch, api, N, O, T, ttree, sc, pts, eps, sample, ncpu, reps=5):
+    """Shadow rays from the primary hits of the large workload (mnrt_trace_shadow): one
+    to a point light just below the ceiling light, one to a stratified sample of the
+    ceiling area light (mnrt_sample_area_light); median of `reps` launches each.  Bytes
+    per ray from the oracle's any-hit replay of a 1-in-`sample` subset of the point-light
+    segments on the same tree."""
+    lq = sc.lights[0]
+    centre = (lq[0] + 0.5 * lq[1] + 0.5 * lq[2] - np.array([0.0, 0.01, 0.0], np.float32)).astype(np.float32)
+    point = api.make_light(N.LIGHT_POINT, centre)
+    rect = api.make_light(N.LIGHT_AREA_RECT, lq[0] - np.array([0.0, 0.001, 0.0], np.float32), (0.0, -1.0, 0.0),
+                          v1=lq[1], v2=lq[2])
+    n = pts.shape[0]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    r1 = torch.rand(n, device="cuda", generator=g)
+    r2 = torch.rand(n, device="cuda", generator=g)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            r = fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return r, float(np.median(ms))
+
+    vis_p, ms_p = timed(lambda: api.trace_shadow(T, point, pts, eps))
+    smp, ms_s = timed(lambda: api.sample_area_light(rect, r1, r2, 0.0, 1.0, 0.0, 1.0))
+    vis_a, ms_a = timed(lambda: api.trace_shadow(T, rect, pts, eps, smp))
+    # algorithmic bytes: oracle any-hit walk of the point-light segments on a sample
+    p = pts[::sample].cpu().numpy()
+    v = centre[None, :] - p[:, :3]
+    dist = np.sqrt(np.sum(v * v, axis=1)).astype(np.float32)
+    d = np.zeros_like(p)
+    d[:, :3] = v / dist[:, None]
+    _, ctr = O.trace_any(ttree, ttree.root, sc.v0, sc.v1, sc.v2, p, d, eps, dist, nthreads=ncpu)
+    per_ray = shadow_bytes(ctr, len(p)) / len(p)
+    gbs = per_ray * n / (ms_p * 1e-3) / 1e9
+    return {"workload": f"shadow rays from {n} primary hits of the large workload to a point light and to "
+                        "ceiling area-light samples",
+            "rays": int(n), "point_ms": round(ms_p, 4), "area_ms": round(ms_a, 4), "sample_ms": round(ms_s, 4),
+            "point_mrays_per_s": round(n / (ms_p * 1e-3) / 1e6, 1),
+            "area_mrays_per_s": round(n / (ms_a * 1e-3) / 1e6, 1),
+            "unoccluded_point": round(float(vis_p.float().mean()), 4),
+            "unoccluded_area": round(float(vis_a.float().mean()), 4),
+            "roofline": {"kernel": "k_any_hit<kSegPoint>", "alg_bytes_per_ray": round(per_ray, 1),
+                         "achieved": round(gbs, 2), "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)}}
+
+
 def _offsets(counts):
     off = np.zeros(len(counts) + 1, np.uint32)
     off[1:] = np.cumsum(np.asarray(counts, np.int64))
@@ -244,6 +304,7 @@ def main():
     ap.add_argument("--sample", type=int, default=64, help="1-in-N subsample for the oracle replay")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-ray leg")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -409,6 +470,9 @@ def main():
     _, _, _, tctr = O.trace(ttree, sc.v0, sc.v1, sc.v2, ro, rd, eps, nthreads=ncpu)
     cpu_trace_s = time.time() - tt
     per_ray = trace_bytes(tctr, len(ro)) / len(ro)
+    if not args.no_shadow:
+        out["shadow"] = shadow_leg(torch, api, native, O, T, ttree, sc, rec["queries"][0]["pos"], eps, args.sample,
+                                   ncpu)
     T.release()
     P.release()
 

@@ -9,6 +9,8 @@
  *                  KDTreePoint (KDTreePoint.h:65-181), KDTreeListener (KDTreeListener.h:61-86)
  *   photon map     PhotonMap (PhotonMap.h:89-257)
  *   kernel glue    RTUpdateKernelData / KernelRTTraceRays (RTCore.cpp:49-76),
+ *                  KernelRTSampleAreaLight / KernelRTTraceShadowRaysDelta / _Area
+ *                  (raytracing.cu:963-1114),
  *                  PMSetPhotonMap / PMUnsetPhotonMap / KernelPMPhotonGather
  *                  (PhotonMap.cpp:40-59)
  * Differences, all deliberate:
@@ -290,18 +292,38 @@ class PhotonMap : public KDTreeListener {
 
 /* ---- kernel glue (RTCore.cpp:49-76, PhotonMap.cpp:40-59) --------------------------- */
 
-struct LightData;
+/* LightType / LightData (KernelDefs.h:89-101, 307-323). */
+enum LightType { Light_Point = 0, Light_Directional = 1, Light_AreaDisc = 2, Light_AreaRect = 3 };
+struct LightData {
+    LightType type;
+    float3 position;   /* point / disc centre / rectangle corner */
+    float3 direction;  /* directional: travel direction; area: surface normal */
+    float3 L_emit;
+    float3 areaV1, areaV2;
+    float areaRadius;
+};
 class MaterialData;
 
-/* Binds the triangle tree that owns kdData (a KDTreeTri's GetData()) and the ray
- * epsilon for KernelRTTraceRays on this host thread.  Lights and materials are not read
- * by the traversal (the reference binds them for shading kernels). */
+/* Binds the triangle tree that owns kdData (a KDTreeTri's GetData()), the light and the
+ * ray epsilon for KernelRTTraceRays and the shadow-ray functions on this host thread.
+ * Materials are not read (the reference binds them for shading kernels). */
 void RTUpdateKernelData(const LightData& lights, const TriangleData& tris, const MaterialData& mats,
                         const KDTreeData& kdData, float fRayEpsilon);
 /* Closest hits of rayChunk's rays: outInters.d_idxTris (-1: miss), d_ptInter, d_baryHit
  * and d_pixels per ray (numPoints = numRays), d_outIsValid[i] = 1 on a hit.  Normals are
  * not written (the reference computes them later, KernelBSDFGetNormalsAtHit). */
 void KernelRTTraceRays(const RayChunk& rayChunk, ShadingPoints& outInters, uint* d_outIsValid);
+/* kernel_SampleAreaLight (raytracing.cu:963-1000) for the bound area light: one sample
+ * point per shading point from the stratified uniforms (idx + rnd) * inv. */
+void KernelRTSampleAreaLight(uint numShadingPts, float idxSampleX, float invNumSamplesX, float idxSampleY,
+                             float invNumSamplesY, const float* d_randoms1, const float* d_randoms2,
+                             float4* d_outSamplePts);
+/* kernel_TraceShadowRaysDelta / kernel_TraceShadowRaysArea (raytracing.cu:1019-1114) for
+ * the bound tree and light: d_outShadowRayResult[i] = 1 iff the light (point /
+ * directional) or d_samplePts[i] (area lights) is visible from shadingPts.d_ptInter[i]. */
+void KernelRTTraceShadowRaysDelta(const ShadingPoints& shadingPts, uint* d_outShadowRayResult);
+void KernelRTTraceShadowRaysArea(const ShadingPoints& shadingPts, const float4* d_samplePts,
+                                 uint* d_outShadowRayResult);
 /* Binds the photon map whose tree owns kdData, and k, for KernelPMPhotonGather. */
 void PMSetPhotonMap(const KDTreeData& pmap, const PhotonData& photons, uint knnTargetCount);
 void PMUnsetPhotonMap();

@@ -207,6 +207,51 @@ int mnrt_trace_closest(const mnrt_tree* tree, uint32_t n, const float* d_orig4,
                        float* d_out_bary2, float* d_out_pt4, uint32_t* d_out_valid,
                        void* stream);
 
+/* dev_FindNextIntersectionKDWhileWhile<false> (raytracing.cu:503-645): any-hit test of
+ * n ray segments.  d_out_hit[i] = 1 iff some triangle is hit at lambda with
+ * eps < lambda < min(t_exit, tmax_i) + eps and lambda < tmax_i, where t_exit is where the
+ * ray leaves the root box and tmax_i = d_tmax[i] (NULL: no limit).  The result does not
+ * depend on which triangle is met first. */
+int mnrt_trace_any(const mnrt_tree* tree, uint32_t n, const float* d_orig4, const float* d_dir4,
+                   const float* d_tmax, float eps, uint32_t* d_out_hit, void* stream);
+
+/* LightData / LightType (KernelDefs.h:89-101, 307-323). */
+enum {
+    MNRT_LIGHT_POINT = 0,
+    MNRT_LIGHT_DIRECTIONAL = 1,
+    MNRT_LIGHT_AREA_DISC = 2,
+    MNRT_LIGHT_AREA_RECT = 3
+};
+typedef struct mnrt_light {
+    uint32_t type;         /* MNRT_LIGHT_* */
+    float position[3];     /* point / disc centre / rectangle corner */
+    float direction[3];    /* directional: travel direction; area: surface normal */
+    float L_emit[3];       /* emitted radiance (intensity for point lights) */
+    float area_v1[3];      /* rectangle edges */
+    float area_v2[3];
+    float area_radius;     /* disc radius */
+} mnrt_light;
+
+/* kernel_SampleAreaLight (raytracing.cu:963-1000): one point on the area light per
+ * entry, with the stratified uniforms (idx_x + rnd1[i]) * inv_x, (idx_y + rnd2[i]) * inv_y:
+ * rectangle: position + u*v1 + v*v2; disc: concentric disc sample (Shirley-Chiu) in the
+ * plane through position with normal direction (basis of dev_BuildCoordSystem,
+ * bsdf_dev.h:158-177).  Output float4 (w = 0). */
+int mnrt_sample_area_light(const mnrt_light* light, uint32_t n, float idx_x, float inv_x, float idx_y,
+                           float inv_y, const float* d_rnd1, const float* d_rnd2, float* d_out_pts4,
+                           void* stream);
+
+/* kernel_TraceShadowRaysDelta / kernel_TraceShadowRaysArea (raytracing.cu:1019-1114):
+ * d_out_unoccluded[i] = 1 iff the light is visible from shading point d_pts4[i].
+ *   point light: segment to position, occluded by a hit with lambda < distance;
+ *   directional: direction -light.direction, unlimited;
+ *   area lights: segment to d_samples4[i] (mnrt_sample_area_light); 0 when the sample
+ *     faces away (dot(-dir, light.direction) <= 0), else occluded by a hit with
+ *     lambda + eps < distance.
+ * A point at zero distance from its light sample is unoccluded. */
+int mnrt_trace_shadow(const mnrt_tree* tree, const mnrt_light* light, uint32_t n, const float* d_pts4,
+                      const float* d_samples4, float eps, uint32_t* d_out_unoccluded, void* stream);
+
 /* --- photon-map kNN radiance --------------------------------------------------------- */
 
 /* KernelPMPhotonGather (photon_knn.cu:487-552, 833-844) with the range search of

@@ -300,6 +300,56 @@ def trace_closest(T: DeviceTree, orig, dirs, eps: float, want_pt=False):
     return tri, t, bary, pt
 
 
+def trace_any(T: DeviceTree, orig, dirs, eps: float, tmax=None):
+    """Any-hit test of ray segments (mnrt_trace_any): uint32 1 where a triangle is hit at
+    eps < lambda < tmax (tmax None: unlimited)."""
+    torch = _torch()
+    orig, dirs = dev(orig), dev(dirs)
+    tmax = dev(tmax) if tmax is not None else None
+    n = orig.shape[0]
+    hit = torch.empty(n, dtype=torch.int32, device="cuda")
+    N.check(N.load().mnrt_trace_any(T.ref(), n, N.ptr(orig), N.ptr(dirs), N.ptr(tmax), eps, N.ptr(hit),
+                                    N.stream_handle()), "mnrt_trace_any")
+    return hit
+
+
+def make_light(kind: int, position=(0.0, 0.0, 0.0), direction=(0.0, 0.0, -1.0), L_emit=(1.0, 1.0, 1.0),
+               v1=(0.0, 0.0, 0.0), v2=(0.0, 0.0, 0.0), radius=0.0):
+    """mnrt_light of one of N.LIGHT_POINT / LIGHT_DIRECTIONAL / LIGHT_AREA_DISC / LIGHT_AREA_RECT."""
+    L = N.Light()
+    L.type = kind
+    for name, val in (("position", position), ("direction", direction), ("L_emit", L_emit), ("area_v1", v1),
+                      ("area_v2", v2)):
+        arr = getattr(L, name)
+        for a in range(3):
+            arr[a] = float(val[a])
+    L.area_radius = float(radius)
+    return L
+
+
+def sample_area_light(light, rnd1, rnd2, idx_x=0.0, inv_x=1.0, idx_y=0.0, inv_y=1.0):
+    """Points on an area light (mnrt_sample_area_light): (n, 4) float32."""
+    torch = _torch()
+    rnd1, rnd2 = dev(rnd1), dev(rnd2)
+    n = rnd1.shape[0]
+    out = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    N.check(N.load().mnrt_sample_area_light(ctypes.byref(light), n, idx_x, inv_x, idx_y, inv_y, N.ptr(rnd1),
+                                            N.ptr(rnd2), N.ptr(out), N.stream_handle()), "mnrt_sample_area_light")
+    return out
+
+
+def trace_shadow(T: DeviceTree, light, pts, eps: float, samples=None):
+    """Shadow rays (mnrt_trace_shadow): uint32 1 where the light (or its sample) is visible."""
+    torch = _torch()
+    pts = dev(pts)
+    samples = dev(samples) if samples is not None else None
+    n = pts.shape[0]
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    N.check(N.load().mnrt_trace_shadow(T.ref(), ctypes.byref(light), n, N.ptr(pts), N.ptr(samples), eps,
+                                       N.ptr(out), N.stream_handle()), "mnrt_trace_shadow")
+    return out
+
+
 def knn_gather(P: DeviceTree, k: int, r_max: float, pos, nrm, dirs, infl, clr=None, idx_sp=None, io_rad=None,
                want_nbr=False, want_r2=False, want_irr=False, want_count=False, query_radius=None, orig=None,
                io_inv_dist=None):

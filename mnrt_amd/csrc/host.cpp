@@ -67,6 +67,7 @@ Bound lookup(const KDTreeData* d, const char* what) {
 // __constant__ memory and texture references; here per host thread).
 thread_local const mnrt_tree* t_tri_tree = nullptr;
 thread_local float t_ray_eps = 0.0f;
+thread_local mnrt_light t_light = {};
 thread_local const mnrt_tree* t_photon_tree = nullptr;
 thread_local float t_photon_qr = 0.0f;
 thread_local uint t_knn_k = 0;
@@ -479,12 +480,50 @@ void PhotonMap::OnFinalNodeList(KDFinalNodeList*) {
 
 // ---- kernel glue -------------------------------------------------------------------------
 
-void RTUpdateKernelData(const LightData&, const TriangleData&, const MaterialData&, const KDTreeData& kdData,
+void RTUpdateKernelData(const LightData& lights, const TriangleData&, const MaterialData&, const KDTreeData& kdData,
                         float fRayEpsilon) {
     Bound b = lookup(&kdData, "RTUpdateKernelData");
     if (b.tree->kind != MNRT_KIND_TRI) throw mnrt_error("RTUpdateKernelData: not a triangle tree", MNRT_ERR_STATE);
     t_tri_tree = b.tree;
     t_ray_eps = fRayEpsilon;
+    mnrt_light L{};
+    L.type = (uint32_t)lights.type;
+    const float3* src[5] = {&lights.position, &lights.direction, &lights.L_emit, &lights.areaV1, &lights.areaV2};
+    float* dst[5] = {L.position, L.direction, L.L_emit, L.area_v1, L.area_v2};
+    for (int k = 0; k < 5; ++k) {
+        dst[k][0] = src[k]->x;
+        dst[k][1] = src[k]->y;
+        dst[k][2] = src[k]->z;
+    }
+    L.area_radius = lights.areaRadius;
+    t_light = L;
+}
+
+void KernelRTSampleAreaLight(uint numShadingPts, float idxSampleX, float invNumSamplesX, float idxSampleY,
+                             float invNumSamplesY, const float* d_randoms1, const float* d_randoms2,
+                             float4* d_outSamplePts) {
+    check(mnrt_sample_area_light(&t_light, numShadingPts, idxSampleX, invNumSamplesX, idxSampleY, invNumSamplesY,
+                                 d_randoms1, d_randoms2, (float*)d_outSamplePts, nullptr),
+          "KernelRTSampleAreaLight");
+}
+
+void KernelRTTraceShadowRaysDelta(const ShadingPoints& shadingPts, uint* d_outShadowRayResult) {
+    if (!t_tri_tree) throw mnrt_error("KernelRTTraceShadowRaysDelta: no tree bound (RTUpdateKernelData)", MNRT_ERR_STATE);
+    if (t_light.type != MNRT_LIGHT_POINT && t_light.type != MNRT_LIGHT_DIRECTIONAL)
+        throw mnrt_error("KernelRTTraceShadowRaysDelta: the bound light is not a delta light", MNRT_ERR_ARG);
+    check(mnrt_trace_shadow(t_tri_tree, &t_light, shadingPts.numPoints, (const float*)shadingPts.d_ptInter, nullptr,
+                            t_ray_eps, d_outShadowRayResult, nullptr),
+          "KernelRTTraceShadowRaysDelta");
+}
+
+void KernelRTTraceShadowRaysArea(const ShadingPoints& shadingPts, const float4* d_samplePts,
+                                 uint* d_outShadowRayResult) {
+    if (!t_tri_tree) throw mnrt_error("KernelRTTraceShadowRaysArea: no tree bound (RTUpdateKernelData)", MNRT_ERR_STATE);
+    if (t_light.type != MNRT_LIGHT_AREA_DISC && t_light.type != MNRT_LIGHT_AREA_RECT)
+        throw mnrt_error("KernelRTTraceShadowRaysArea: the bound light is not an area light", MNRT_ERR_ARG);
+    check(mnrt_trace_shadow(t_tri_tree, &t_light, shadingPts.numPoints, (const float*)shadingPts.d_ptInter,
+                            (const float*)d_samplePts, t_ray_eps, d_outShadowRayResult, nullptr),
+          "KernelRTTraceShadowRaysArea");
 }
 
 void KernelRTTraceRays(const RayChunk& rayChunk, ShadingPoints& outInters, uint* d_outIsValid) {

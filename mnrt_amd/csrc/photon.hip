@@ -50,6 +50,7 @@ struct PhotonArgs {
 };
 
 __global__ __launch_bounds__(256) void k_photon_paths(PhotonArgs A) {
+    extern __shared__ uint32_t s_stack[];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.num_paths) return;
     const uint32_t path = A.path_begin + i;
@@ -69,7 +70,8 @@ __global__ __launch_bounds__(256) void k_photon_paths(PhotonArgs A) {
     uint32_t tris[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
 
     for (uint32_t b = 0; b < slots; ++b) {
-        Hit h = trace_ray(A.nodes, A.recs, A.root, o, d, A.eps);
+        Hit h = trace_ray<kTraceStack, kTraceLds>(A.nodes, A.recs, A.root, o, d, A.eps, s_stack + threadIdx.x,
+                                                  blockDim.x);
         if (h.tri < 0) break;
         float4 alb = A.albedo[h.tri];
         if (!(alb.x > 0.0f || alb.y > 0.0f || alb.z > 0.0f)) break;
@@ -206,7 +208,7 @@ extern "C" int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, c
         uint32_t* scr = tot + 4;
         A.path_begin = shot;
         A.num_paths = np;
-        hipLaunchKernelGGL(k_photon_paths, dim3((np + 255) / 256), dim3(256), 0, st, A);
+        hipLaunchKernelGGL(k_photon_paths, dim3((np + 255) / 256), dim3(256), trace_lds_bytes(kTraceLds, 256), st, A);
         if (scan_exclusive(A.valid, pos, (uint32_t)nslots, tot, scr, st) != hipSuccess) {
             rc = MNRT_ERR_HIP;
             break;

@@ -12,7 +12,7 @@
 //     the lower triangle index, so the hit equals the brute-force closest hit over all
 //     triangles whatever the tree;
 //   * the traversal stack is bounded by the build's level cap instead of KD_MAX_HEIGHT
-//     local-memory arrays with no overflow check.
+//     local-memory arrays with no overflow check, and its top entries live in LDS.
 #include "batch_dev.h"
 #include "trace_dev.h"
 
@@ -23,11 +23,12 @@ __global__ __launch_bounds__(256) void k_trace_closest(
     const float4* __restrict__ orig, const float4* __restrict__ dir, float eps,
     int32_t* __restrict__ out_tri, float* __restrict__ out_t, float2* __restrict__ out_bary,
     float4* __restrict__ out_pt, uint32_t* __restrict__ out_valid) {
+    extern __shared__ uint32_t s_stack[];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     f3 o = xyz(orig[i]);
     f3 d = xyz(dir[i]);
-    Hit h = trace_ray(nodes, recs, root, o, d, eps);
+    Hit h = trace_ray<kTraceStack, kTraceLds>(nodes, recs, root, o, d, eps, s_stack + threadIdx.x, blockDim.x);
     out_tri[i] = h.tri;
     if (out_t) out_t[i] = h.t;
     if (out_bary) out_bary[i] = make_float2(h.u, h.v);
@@ -52,7 +53,8 @@ extern "C" int mnrt_trace_closest(const mnrt_tree* tree, uint32_t n, const float
         rb.hi[a] = tree->root_max[a];
     }
     dim3 block(256), grid((n + 255) / 256);
-    hipLaunchKernelGGL(mnrt::k_trace_closest, grid, block, 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(mnrt::k_trace_closest, grid, block, mnrt::trace_lds_bytes(mnrt::kTraceLds, 256),
+                       (hipStream_t)stream,
                        (const uint2*)tree->d_nodes, (const float4*)tree->d_recs, rb, n,
                        (const float4*)d_orig4, (const float4*)d_dir4, eps, d_out_tri, d_out_t,
                        (float2*)d_out_bary2, (float4*)d_out_pt4, d_out_valid);
@@ -72,6 +74,7 @@ namespace mnrt {
 template <int kStack>
 __global__ __launch_bounds__(256) void k_trace_batch(TraceBatchArgs B) {
     __shared__ uint32_t s_tile;
+    extern __shared__ uint32_t s_stack[];
     const uint32_t xcd = blockIdx.x % B.num_parts;
     for (uint32_t part = 0; part < B.num_parts; ++part) {
         const uint32_t p = (xcd + part) % B.num_parts;
@@ -94,7 +97,8 @@ __global__ __launch_bounds__(256) void k_trace_batch(TraceBatchArgs B) {
             for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
                 f3 o = xyz(B.orig[i]);
                 f3 dd = xyz(B.dir[i]);
-                Hit h = trace_ray<kStack>(d.nodes, d.recs, root, o, dd, d.eps);
+                Hit h = trace_ray<kStack, kTraceLds>(d.nodes, d.recs, root, o, dd, d.eps, s_stack + threadIdx.x,
+                                                     blockDim.x);
                 B.out_tri[i] = h.tri;
                 if (B.out_t) B.out_t[i] = h.t;
                 if (B.out_bary) B.out_bary[i] = make_float2(h.u, h.v);
@@ -105,11 +109,12 @@ __global__ __launch_bounds__(256) void k_trace_batch(TraceBatchArgs B) {
 
 int launch_trace_batch(const TraceBatchArgs& B, int stack, uint32_t grid, uint32_t block, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, B.work, B.num_parts, 0u);
+    const uint32_t lds = trace_lds_bytes(kTraceLds, block);
     switch (stack) {
-        case 16: hipLaunchKernelGGL(k_trace_batch<16>, dim3(grid), dim3(block), 0, st, B); break;
-        case 24: hipLaunchKernelGGL(k_trace_batch<24>, dim3(grid), dim3(block), 0, st, B); break;
-        case 32: hipLaunchKernelGGL(k_trace_batch<32>, dim3(grid), dim3(block), 0, st, B); break;
-        case kTraceStack: hipLaunchKernelGGL(k_trace_batch<kTraceStack>, dim3(grid), dim3(block), 0, st, B); break;
+        case 16: hipLaunchKernelGGL(k_trace_batch<16>, dim3(grid), dim3(block), lds, st, B); break;
+        case 24: hipLaunchKernelGGL(k_trace_batch<24>, dim3(grid), dim3(block), lds, st, B); break;
+        case 32: hipLaunchKernelGGL(k_trace_batch<32>, dim3(grid), dim3(block), lds, st, B); break;
+        case kTraceStack: hipLaunchKernelGGL(k_trace_batch<kTraceStack>, dim3(grid), dim3(block), lds, st, B); break;
         default: return MNRT_ERR_ARG;
     }
     MNRT_HIP_CHECK(hipGetLastError());

@@ -37,6 +37,23 @@ class BuildParams(ctypes.Structure):
     ]
 
 
+class Light(ctypes.Structure):
+    """mnrt_light (LightData, KernelDefs.h:307-323)."""
+
+    _fields_ = [
+        ("type", u32),
+        ("position", f32 * 3),
+        ("direction", f32 * 3),
+        ("L_emit", f32 * 3),
+        ("area_v1", f32 * 3),
+        ("area_v2", f32 * 3),
+        ("area_radius", f32),
+    ]
+
+
+LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_AREA_DISC, LIGHT_AREA_RECT = 0, 1, 2, 3
+
+
 class Tree(ctypes.Structure):
     """mnrt_tree: device-resident packed preorder kd-tree."""
 
@@ -99,6 +116,10 @@ class BatchIO(ctypes.Structure):
 PHASE_TRACE, PHASE_SHADE, PHASE_KNN = 1, 2, 4
 
 
+# status codes (the MNRT_OK / MNRT_ERR_* values of mnrt_kd.h)
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_CAPACITY, ERR_STATE = 0, -1, -2, -3, -4, -5, -6
+
+
 class MnrtError(RuntimeError):
     pass
 
@@ -143,6 +164,10 @@ def load():
     L.mnrt_tree_collapse.argtypes = [TP, u32, vp]
     L.mnrt_tree_cost.argtypes = [TP, f32, ctypes.POINTER(ctypes.c_double), vp]
     L.mnrt_trace_closest.argtypes = [TP, u32, vp, vp, f32, vp, vp, vp, vp, vp, vp]
+    LP = ctypes.POINTER(Light)
+    L.mnrt_trace_any.argtypes = [TP, u32, vp, vp, vp, f32, vp, vp]
+    L.mnrt_sample_area_light.argtypes = [LP, u32, f32, f32, f32, f32, vp, vp, vp, vp]
+    L.mnrt_trace_shadow.argtypes = [TP, LP, u32, vp, vp, f32, vp, vp]
     L.mnrt_knn_gather.argtypes = [TP, u32, f32, u32] + [vp] * 15
     L.mnrt_knn_radii.argtypes = [TP, u32, f32, u32, vp, vp, vp]
     L.mnrt_knn_gather_bwd.argtypes = [TP, u32, f32, u32] + [vp] * 12
@@ -161,7 +186,7 @@ def load():
                  "mnrt_build_tri_batch", "mnrt_build_points_batch", "mnrt_tree_attach_tri_records_batch",
                  "mnrt_tree_attach_photon_records_batch", "mnrt_knn_gather_bwd", "mnrt_tree_collapse", "mnrt_photons_generate", "mnrt_shading_points", "mnrt_bounce_rays", "mnrt_build_tri", "mnrt_build_points", "mnrt_tree_attach_tri_records",
                  "mnrt_tree_attach_photon_records", "mnrt_tree_release", "mnrt_tree_cost",
-                 "mnrt_trace_closest", "mnrt_knn_gather", "mnrt_knn_radii", "mnrt_device_init"):
+                 "mnrt_trace_closest", "mnrt_trace_any", "mnrt_sample_area_light", "mnrt_trace_shadow", "mnrt_knn_gather", "mnrt_knn_radii", "mnrt_device_init"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -739,6 +739,129 @@ void trace_one(const uint32_t* nodes, const uint32_t* refs, const float* root, c
 }
 
 // ---------------------------------------------------------------------------------
+// Any-hit segments (dev_FindNextIntersectionKDWhileWhile<false>, raytracing.cu:503-645,
+// as the shadow-ray kernels use it, raytracing.cu:1019-1114).  A triangle occludes when
+// it is hit at lambda with
+//     lambda > eps,  lambda < min(t_exit, dist) + eps,  lambda + slack < dist,
+// t_exit being where the ray leaves the root box.  The reference tests only the first
+// hit found against the light distance; here any occluding triangle counts, so the
+// tree walk must agree with the test over all triangles (any_brute).
+// ---------------------------------------------------------------------------------
+
+inline bool occludes(const TriSoup& S, uint32_t t, const V3& o, const V3& d, float eps, float tlim,
+                     float dist, float slack) {
+    V3 a = ld4(S.v0, t), b = ld4(S.v1, t), c = ld4(S.v2, t);
+    float lam, u, v;
+    return ray_tri(a, sub(b, a), sub(c, a), o, d, lam, u, v) && lam > eps && lam < tlim &&
+           lam + slack < dist;
+}
+
+bool any_one(const uint32_t* nodes, const uint32_t* refs, const float* root, const TriSoup& S,
+             const V3& o, const V3& d, float eps, float dist, float slack, TraceCounts& cnt) {
+    V3 inv = mk(1.0f / d.v[0], 1.0f / d.v[1], 1.0f / d.v[2]);
+    float t0, t1;
+    if (!ray_box(root, root + 3, o, inv, eps, kInf, t0, t1)) return false;
+    const float tlim = std::fmin(t1, dist) + eps;
+    struct Ent { uint32_t node; float tmin, tmax; };
+    std::vector<Ent> stack;
+    stack.reserve(64);
+    uint32_t node = 0;
+    float tmin = t0, tmax = tlim;
+    for (;;) {
+        for (;;) {
+            uint32_t w1 = nodes[2 * node + 1];
+            if ((w1 >> 30) == kLeafTag) break;
+            cnt.inner++;
+            int a = (int)(w1 >> 30);
+            float s = bitsf(nodes[2 * node]);
+            uint32_t left = node + 1, right = w1 & kIndexMask;
+            float oa = o.v[a], da = d.v[a];
+            float tsplit = (s - oa) * inv.v[a];
+            bool below = (oa < s) || (oa == s && da > 0.0f);
+            uint32_t first = below ? left : right, second = below ? right : left;
+            if (tsplit != tsplit) {
+                stack.push_back({second, tmin, tmax});
+                node = first;
+            } else if (tsplit < 0.0f || tsplit > tmax * kTravTolHi) {
+                node = first;
+            } else if (tsplit < tmin * kTravTolLo) {
+                node = second;
+            } else {
+                stack.push_back({second, std::fmax(tmin, tsplit * kTravTolLo), tmax});
+                tmax = std::fmin(tmax, tsplit * kTravTolHi);
+                node = first;
+            }
+        }
+        cnt.leaves++;
+        uint32_t count = nodes[2 * node], begin = nodes[2 * node + 1] & kIndexMask;
+        for (uint32_t i = 0; i < count; ++i) {
+            cnt.refs++;
+            cnt.tests++;
+            if (occludes(S, refs[begin + i], o, d, eps, tlim, dist, slack)) return true;
+        }
+        if (stack.empty()) return false;
+        node = stack.back().node;
+        tmin = stack.back().tmin;
+        tmax = stack.back().tmax;
+        stack.pop_back();
+    }
+}
+
+bool any_brute(const float* root, const TriSoup& S, uint32_t ntri, const V3& o, const V3& d, float eps,
+               float dist, float slack) {
+    V3 inv = mk(1.0f / d.v[0], 1.0f / d.v[1], 1.0f / d.v[2]);
+    float t0, t1;
+    if (!ray_box(root, root + 3, o, inv, eps, kInf, t0, t1)) return false;
+    const float tlim = std::fmin(t1, dist) + eps;
+    for (uint32_t t = 0; t < ntri; ++t)
+        if (occludes(S, t, o, d, eps, tlim, dist, slack)) return true;
+    return false;
+}
+
+// Light (LightData, KernelDefs.h:307-323) as 16 floats: type, position[3], direction[3],
+// L_emit[3], v1[3], v2[3].  radius is passed separately.
+struct OLight {
+    int type;
+    V3 pos, dir, v1, v2;
+    float radius;
+};
+
+inline OLight olight(const float* L16, float radius) {
+    OLight L;
+    L.type = (int)L16[0];
+    L.pos = mk(L16[1], L16[2], L16[3]);
+    L.dir = mk(L16[4], L16[5], L16[6]);
+    L.v1 = mk(L16[10], L16[11], L16[12]);
+    L.v2 = mk(L16[13], L16[14], L16[15]);
+    L.radius = radius;
+    return L;
+}
+
+// Shadow segment of kernel_TraceShadowRaysDelta / _Area (raytracing.cu:1019-1114).
+// Returns 0 / 1 when the answer is fixed without tracing (area sample facing away or
+// at zero distance: 0; point light at zero distance: 1), else -1 with the segment set.
+int shadow_segment(const OLight& L, const V3& p, const float* samples4, size_t i, V3& d, float& dist,
+                   float& slack, float eps) {
+    slack = 0.0f;
+    if (L.type == 1) {
+        d = mk(-L.dir.v[0], -L.dir.v[1], -L.dir.v[2]);
+        dist = kInf;
+        return -1;
+    }
+    const bool area = L.type >= 2;
+    V3 target = area ? ld4(samples4, i) : L.pos;
+    V3 v = sub(target, p);
+    dist = std::sqrt(dot(v, v));
+    if (!(dist > 0.0f)) return area ? 0 : 1;
+    d = mk(v.v[0] / dist, v.v[1] / dist, v.v[2] / dist);
+    if (area) {
+        slack = eps;
+        if (!(dot(mk(-d.v[0], -d.v[1], -d.v[2]), L.dir) > 0.0f)) return 0;
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------------
 // Exact kNN (replaces the Zhou radius estimate + range search, kdtree_point.cu:229-267,
 // photon_knn.cu:113-235).  Depth-first as dev_RangeSearch: near child where the query
 // lies (p > split goes right), far child pushed when its plane distance^2 < r_max^2;
@@ -1279,6 +1402,120 @@ void orc_trace_brute(const float* root6, const float* v0, const float* v1, const
             }
         }
     });
+}
+
+// Any-hit segments: out_hit[i] = 1 iff a triangle occludes ray i within tmax[i] (NULL:
+// unlimited).  ntri > 0 tests every triangle instead of walking the tree (nodes / refs
+// unused).  counters[4] as orc_trace (tree walk only).
+void orc_trace_any(const uint32_t* nodes, const uint32_t* refs, const float* root6, const float* v0,
+                   const float* v1, const float* v2, uint32_t ntri, const float* orig4, const float* dir4,
+                   const float* tmax, uint32_t n, float eps, uint32_t* out_hit, uint64_t* counters,
+                   int nthreads) {
+    TriSoup S{v0, v1, v2};
+    std::vector<TraceCounts> per(std::max(1, nthreads));
+    std::atomic<int> slot{0};
+    parallel_for(n, nthreads, [&](uint32_t b, uint32_t e) {
+        TraceCounts& c = per[slot++];
+        for (uint32_t i = b; i < e; ++i) {
+            const V3 o = ld4(orig4, i), d = ld4(dir4, i);
+            const float dist = tmax ? tmax[i] : kInf;
+            out_hit[i] = (ntri ? any_brute(root6, S, ntri, o, d, eps, dist, 0.0f)
+                               : any_one(nodes, refs, root6, S, o, d, eps, dist, 0.0f, c))
+                             ? 1u : 0u;
+        }
+    });
+    if (counters) {
+        for (auto& c : per) {
+            counters[0] += c.inner;
+            counters[1] += c.leaves;
+            counters[2] += c.refs;
+            counters[3] += c.tests;
+        }
+    }
+}
+
+// Shadow rays (kernel_TraceShadowRaysDelta / _Area, raytracing.cu:1019-1114):
+// out_unocc[i] = 1 iff the light is visible from pts4[i].  light16 / radius as OLight;
+// samples4: area-light sample per point (area lights only).  ntri > 0: brute force.
+void orc_trace_shadow(const uint32_t* nodes, const uint32_t* refs, const float* root6, const float* v0,
+                      const float* v1, const float* v2, uint32_t ntri, const float* light16, float radius,
+                      const float* pts4, const float* samples4, uint32_t n, float eps, uint32_t* out_unocc,
+                      int nthreads) {
+    TriSoup S{v0, v1, v2};
+    const OLight L = olight(light16, radius);
+    parallel_for(n, nthreads, [&](uint32_t b, uint32_t e) {
+        TraceCounts c;
+        for (uint32_t i = b; i < e; ++i) {
+            const V3 p = ld4(pts4, i);
+            V3 d;
+            float dist, slack;
+            const int fixed = shadow_segment(L, p, samples4, i, d, dist, slack, eps);
+            if (fixed >= 0) {
+                out_unocc[i] = (uint32_t)fixed;
+                continue;
+            }
+            const bool occ = ntri ? any_brute(root6, S, ntri, p, d, eps, dist, slack)
+                                  : any_one(nodes, refs, root6, S, p, d, eps, dist, slack, c);
+            out_unocc[i] = occ ? 0u : 1u;
+        }
+    });
+}
+
+// kernel_SampleAreaLight (raytracing.cu:963-1000): rectangle position + u v1 + v v2;
+// disc: Shirley-Chiu concentric map (dev_ConcentricSampleDisk, sample_dev.h:100-155) in
+// the basis of dev_BuildCoordSystem (bsdf_dev.h:158-177), IEEE 1/sqrt.  Uses the host's
+// cosf / sinf, so disc samples agree with the device to a few ulps, not bit for bit.
+void orc_sample_area_light(const float* light16, float radius, uint32_t n, float ix, float inv_x, float iy,
+                           float inv_y, const float* rnd1, const float* rnd2, float* out4) {
+    const OLight L = olight(light16, radius);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float u = (ix + rnd1[i]) * inv_x, v = (iy + rnd2[i]) * inv_y;
+        V3 p;
+        if (L.type == 2) {
+            const V3& nn = L.dir;
+            V3 s;
+            if (std::fabs(nn.v[0]) > std::fabs(nn.v[1])) {
+                const float il = 1.0f / std::sqrt(nn.v[0] * nn.v[0] + nn.v[2] * nn.v[2]);
+                s = mk(-nn.v[2] * il, 0.0f, nn.v[0] * il);
+            } else {
+                const float il = 1.0f / std::sqrt(nn.v[1] * nn.v[1] + nn.v[2] * nn.v[2]);
+                s = mk(0.0f, nn.v[2] * il, -nn.v[1] * il);
+            }
+            const V3 t = cross(nn, s);
+            // square [-1,1]^2 -> disk, angle in units of pi/4 per wedge
+            const float a = 2.0f * u - 1.0f, b = 2.0f * v - 1.0f;
+            float x = 0.0f, y = 0.0f;
+            if (!(a == 0.0f && b == 0.0f)) {
+                float r, q;
+                if (a >= -b && a > b) {
+                    r = a;
+                    q = (b > 0.0f) ? b / r : 8.0f + b / r;
+                } else if (a >= -b) {
+                    r = b;
+                    q = 2.0f - a / r;
+                } else if (a <= b) {
+                    r = -a;
+                    q = 4.0f - b / r;
+                } else {
+                    r = -b;
+                    q = 6.0f + a / r;
+                }
+                const float ang = q * (kPi / 4.0f);
+                x = r * std::cos(ang);
+                y = r * std::sin(ang);
+            }
+            p = mk(L.pos.v[0] + L.radius * (x * s.v[0] + y * t.v[0]),
+                   L.pos.v[1] + L.radius * (x * s.v[1] + y * t.v[1]),
+                   L.pos.v[2] + L.radius * (x * s.v[2] + y * t.v[2]));
+        } else {
+            p = mk(L.pos.v[0] + u * L.v1.v[0] + v * L.v2.v[0], L.pos.v[1] + u * L.v1.v[1] + v * L.v2.v[1],
+                   L.pos.v[2] + u * L.v1.v[2] + v * L.v2.v[2]);
+        }
+        out4[4 * i] = p.v[0];
+        out4[4 * i + 1] = p.v[1];
+        out4[4 * i + 2] = p.v[2];
+        out4[4 * i + 3] = 0.0f;
+    }
 }
 
 // Exact kNN through the point tree.  out_nbr: nq*k indices sorted by (d2, index),

@@ -76,6 +76,12 @@ def lib():
     L.orc_export_preorder.argtypes = [u32p, u32p, f32p, u32p, u32p, u32p, f32p, u32p, u32p]
     L.orc_ray_tri.restype = ctypes.c_int
     L.orc_ray_tri.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
+    L.orc_trace_any.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, f32p, f32p,
+                                ctypes.c_uint32, ctypes.c_float, u32p, u64p, ctypes.c_int]
+    L.orc_trace_shadow.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, ctypes.c_float,
+                                   f32p, f32p, ctypes.c_uint32, ctypes.c_float, u32p, ctypes.c_int]
+    L.orc_sample_area_light.argtypes = [f32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_float, f32p, f32p, f32p]
     _lib = L
     return L
 
@@ -182,6 +188,53 @@ def trace_brute(root, v0, v1, v2, orig, dirs, eps, nthreads=0):
     lib().orc_trace_brute(_c(root).ravel(), _c(v0), _c(v1), _c(v2), v0.shape[0], _c(orig), _c(dirs), n, eps, tri,
                           t, uv, _threads(nthreads))
     return tri, t, uv.reshape(n, 2)
+
+
+FLT_MAX = float(np.finfo(np.float32).max)
+_EMPTY_TREE = (np.array([0, 3 << 30], np.uint32), np.zeros(1, np.uint32))
+
+
+def _tree_arrays(tree):
+    if tree is None:
+        return _EMPTY_TREE
+    return tree.nodes, (tree.refs if tree.refs.size else np.zeros(1, np.uint32))
+
+
+def trace_any(tree, root, v0, v1, v2, orig, dirs, eps, tmax=None, brute=False, nthreads=0):
+    """Any-hit segments (orc_trace_any): uint32 1 where a triangle occludes; brute=True
+    tests every triangle instead of walking `tree`.  Returns (hit, counters)."""
+    n = orig.shape[0]
+    out = np.zeros(n, np.uint32)
+    cnt = np.zeros(4, np.uint64)
+    tm = _c(tmax) if tmax is not None else np.full(n, FLT_MAX, np.float32)
+    nodes, refs = _tree_arrays(None if brute else tree)
+    lib().orc_trace_any(nodes, refs, _c(root).ravel(), _c(v0), _c(v1), _c(v2), v0.shape[0] if brute else 0,
+                        _c(orig), _c(dirs), tm, n, eps, out, cnt, _threads(nthreads))
+    return out, cnt
+
+
+def light16(kind, position=(0, 0, 0), direction=(0, 0, -1), L_emit=(1, 1, 1), v1=(0, 0, 0), v2=(0, 0, 0)):
+    """LightData as the oracle's 16 floats: type, position, direction, L_emit, v1, v2."""
+    return np.array([kind, *position, *direction, *L_emit, *v1, *v2], np.float32)
+
+
+def trace_shadow(tree, root, v0, v1, v2, light, radius, pts, eps, samples=None, brute=False, nthreads=0):
+    """Shadow rays (orc_trace_shadow): uint32 1 where the light is visible."""
+    n = pts.shape[0]
+    out = np.zeros(n, np.uint32)
+    smp = _c(samples) if samples is not None else np.zeros(4, np.float32)
+    nodes, refs = _tree_arrays(None if brute else tree)
+    lib().orc_trace_shadow(nodes, refs, _c(root).ravel(), _c(v0), _c(v1), _c(v2), v0.shape[0] if brute else 0,
+                           _c(light), radius, _c(pts), smp, n, eps, out, _threads(nthreads))
+    return out
+
+
+def sample_area_light(light, radius, rnd1, rnd2, ix=0.0, inv_x=1.0, iy=0.0, inv_y=1.0):
+    """Area-light sample points (orc_sample_area_light): (n, 4) float32."""
+    n = rnd1.shape[0]
+    out = np.zeros(4 * n, np.float32)
+    lib().orc_sample_area_light(_c(light), radius, n, ix, inv_x, iy, inv_y, _c(rnd1), _c(rnd2), out)
+    return out.reshape(n, 4)
 
 
 def knn(tree: Tree, ppos, qpos, k, rmax, nthreads=0):

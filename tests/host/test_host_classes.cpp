@@ -10,9 +10,6 @@
 
 #include "mnrt_host.h"
 
-struct LightData {
-    int unused;
-};
 class MaterialData {
    public:
     int unused;
@@ -199,6 +196,79 @@ int main() {
         expect(false, "RTUpdateKernelData refuses a KDTreeData of no tree");
     } catch (const mnrt_error&) {
         expect(true, "RTUpdateKernelData refuses a KDTreeData of no tree");
+    }
+
+    // ---- shadow rays: KernelRTTraceShadowRaysDelta / _Area, KernelRTSampleAreaLight ----
+    {
+        uint *d_res = nullptr, *d_ref = nullptr;
+        HIPOK(hipMalloc((void**)&d_res, nr * 4));
+        HIPOK(hipMalloc((void**)&d_ref, nr * 4));
+        LightData pl{};
+        pl.type = Light_Point;
+        pl.position = make_float3(0.1f, 0.9f, 0.0f);
+        RTUpdateKernelData(pl, td, mats, *kdd, eps);
+        KernelRTTraceShadowRaysDelta(sp, d_res);
+        mnrt_light ml{};
+        ml.type = MNRT_LIGHT_POINT;
+        ml.position[0] = 0.1f;
+        ml.position[1] = 0.9f;
+        ABIOK(mnrt_trace_shadow(T, &ml, nr, (const float*)sp.d_ptInter, nullptr, eps, d_ref, nullptr));
+        HIPOK(hipDeviceSynchronize());
+        auto a = download(d_res, nr), b = download(d_ref, nr);
+        uint32_t lit = 0;
+        for (uint32_t i = 0; i < nr; ++i) lit += a[i];
+        expect(a == b && lit > 0 && lit < nr, "KernelRTTraceShadowRaysDelta == mnrt_trace_shadow (point light)");
+
+        LightData al{};
+        al.type = Light_AreaRect;
+        al.position = make_float3(-0.25f, 0.998f, -0.25f);
+        al.direction = make_float3(0.0f, -1.0f, 0.0f);
+        al.areaV1 = make_float3(0.5f, 0.0f, 0.0f);
+        al.areaV2 = make_float3(0.0f, 0.0f, 0.5f);
+        RTUpdateKernelData(al, td, mats, *kdd, eps);
+        std::vector<float> r1(nr), r2(nr);
+        for (uint32_t i = 0; i < nr; ++i) {
+            r1[i] = frand();
+            r2[i] = frand();
+        }
+        float *d_r1 = upload(r1), *d_r2 = upload(r2);
+        float4 *d_smp = nullptr, *d_smp2 = nullptr;
+        HIPOK(hipMalloc((void**)&d_smp, nr * 16));
+        HIPOK(hipMalloc((void**)&d_smp2, nr * 16));
+        KernelRTSampleAreaLight(nr, 1.0f, 0.5f, 0.0f, 0.5f, d_r1, d_r2, d_smp);
+        mnrt_light ma{};
+        ma.type = MNRT_LIGHT_AREA_RECT;
+        ma.position[0] = -0.25f;
+        ma.position[1] = 0.998f;
+        ma.position[2] = -0.25f;
+        ma.direction[1] = -1.0f;
+        ma.area_v1[0] = 0.5f;
+        ma.area_v2[2] = 0.5f;
+        ABIOK(mnrt_sample_area_light(&ma, nr, 1.0f, 0.5f, 0.0f, 0.5f, d_r1, d_r2, (float*)d_smp2, nullptr));
+        KernelRTTraceShadowRaysArea(sp, d_smp, d_res);
+        ABIOK(mnrt_trace_shadow(T, &ma, nr, (const float*)sp.d_ptInter, (const float*)d_smp2, eps, d_ref, nullptr));
+        HIPOK(hipDeviceSynchronize());
+        auto s1 = download(d_smp, nr), s2 = download(d_smp2, nr);
+        a = download(d_res, nr);
+        b = download(d_ref, nr);
+        lit = 0;
+        for (uint32_t i = 0; i < nr; ++i) lit += a[i];
+        expect(same_bytes(s1.data(), s2.data(), nr * 16), "KernelRTSampleAreaLight == mnrt_sample_area_light");
+        expect(a == b && lit > 0 && lit < nr, "KernelRTTraceShadowRaysArea == mnrt_trace_shadow (area light)");
+        bool threw_kind = false;
+        try {
+            KernelRTTraceShadowRaysDelta(sp, d_res);
+        } catch (const mnrt_error&) {
+            threw_kind = true;
+        }
+        expect(threw_kind, "KernelRTTraceShadowRaysDelta refuses a bound area light");
+        RTUpdateKernelData(lights, td, mats, *kdd, eps);
+        HIPOK(hipFree(d_res));
+        HIPOK(hipFree(d_ref));
+        HIPOK(hipFree(d_r1));
+        HIPOK(hipFree(d_r2));
+        HIPOK(hipFree(d_smp));
+        HIPOK(hipFree(d_smp2));
     }
 
     // ---- photons into the reference PhotonData layout ----

@@ -37,6 +37,26 @@ def test_argument_errors_without_device():
     assert L.mnrt_knn_gather(ctypes.byref(t), 5, 1.0, 1, *([None] * 15)) == -6  # no nodes yet
     t.kind = N.KIND_TRI
     assert L.mnrt_trace_closest(ctypes.byref(t), 1, None, None, 0.0, None, None, None, None, None, None) == -1
+    # any-hit / shadow entry points: argument and state errors are found before any device call
+    fake = 256  # never dereferenced: every call below returns before launching
+    light = N.Light()
+    light.type = N.LIGHT_POINT
+    assert L.mnrt_trace_any(None, 1, fake, fake, None, 0.0, fake, None) == N.ERR_ARG
+    assert L.mnrt_trace_any(ctypes.byref(t), 1, None, fake, None, 0.0, fake, None) == N.ERR_ARG
+    assert L.mnrt_trace_any(ctypes.byref(t), 1, fake, fake, None, 0.0, fake, None) == N.ERR_STATE
+    assert L.mnrt_trace_shadow(ctypes.byref(t), None, 1, fake, None, 0.0, fake, None) == N.ERR_ARG
+    assert L.mnrt_trace_shadow(ctypes.byref(t), ctypes.byref(light), 1, fake, None, 0.0, fake, None) == N.ERR_STATE
+    light.type = N.LIGHT_AREA_DISC
+    assert L.mnrt_trace_shadow(ctypes.byref(t), ctypes.byref(light), 1, fake, None, 0.0, fake, None) == N.ERR_ARG
+    light.type = 9
+    assert L.mnrt_trace_shadow(ctypes.byref(t), ctypes.byref(light), 1, fake, fake, 0.0, fake, None) == N.ERR_ARG
+    light.type = N.LIGHT_POINT
+    assert L.mnrt_sample_area_light(ctypes.byref(light), 1, 0.0, 1.0, 0.0, 1.0, fake, fake, fake, None) == N.ERR_ARG
+    light.type = N.LIGHT_AREA_RECT
+    assert L.mnrt_sample_area_light(ctypes.byref(light), 1, 0.0, 1.0, 0.0, 1.0, None, fake, fake, None) == N.ERR_ARG
+    t.kind = N.KIND_POINT
+    light.type = N.LIGHT_POINT
+    assert L.mnrt_trace_shadow(ctypes.byref(t), ctypes.byref(light), 1, fake, None, 0.0, fake, None) == N.ERR_STATE
 
 
 def test_product_fails_loudly_without_library(monkeypatch, tmp_path):

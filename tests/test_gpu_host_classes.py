@@ -1,5 +1,6 @@
 """The C++ call surface (include/mnrt_host.h: KDTreeTri, KDTreePoint, PhotonMap,
-RTUpdateKernelData / KernelRTTraceRays, PMSetPhotonMap / KernelPMPhotonGather) runs on the
+RTUpdateKernelData / KernelRTTraceRays, KernelRTSampleAreaLight / KernelRTTraceShadowRays*,
+PMSetPhotonMap / KernelPMPhotonGather) runs on the
 GPU and agrees with the C ABI it wraps; see tests/host/test_host_classes.cpp."""
 
 import os
