@@ -719,6 +719,7 @@ struct SmallStackEnt {
 struct SmallOut {
     uint32_t* nsize;  // pass 1: nodes per small subtree
     uint32_t* rsize;  // pass 1: references per small subtree
+    uint32_t* mlvl;   // pass 1: deepest level in the small subtree
     const uint32_t* bn_addr;  // pass 2: subtree start address / reference base
     const uint32_t* bn_rbase;
     uint32_t* nodes;
@@ -728,8 +729,6 @@ struct SmallOut {
     uint32_t max_refs;
     const uint32_t* node_base;  // per scene: first node / reference of its tree
     const uint32_t* ref_base;
-    uint32_t* scene_leaves;     // per scene statistics
-    uint32_t* scene_maxlvl;
 };
 
 template <bool kEmit>
@@ -802,7 +801,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         mR[c] = mr;
     }
 
-    uint32_t addr = 0, rbase = 0, leaves = 0, maxlvl = 0;
+    uint32_t addr = 0, rbase = 0, maxlvl = 0;
     const uint32_t scene = S.scene[r];
     // local addresses go into the node words; nodes / refs point at the scene's tree
     uint32_t* nodes = kEmit ? O.nodes + 2ull * O.node_base[scene] : nullptr;
@@ -903,7 +902,6 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
             }
             ++addr;
             rbase += cnt;
-            ++leaves;
             continue;
         }
         const uint32_t a = (best % per) / n;
@@ -931,22 +929,19 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         ++addr;
         wave_sync();
     }
-    if (lane == 0) {
-        if (kEmit) {
-            atomicAdd(&O.scene_leaves[scene], leaves);
-            atomicMax(&O.scene_maxlvl[scene], maxlvl);
-        } else {
-            O.nsize[r] = addr;
-            O.rsize[r] = rbase;
-        }
+    if (!kEmit && lane == 0) {
+        O.nsize[r] = addr;
+        O.rsize[r] = rbase;
+        O.mlvl[r] = maxlvl;
     }
 }
 
 // ---- layout: subtree sizes bottom-up, addresses top-down, one workgroup ------------------
 
 struct LayoutView {
-    uint32_t* size;
-    uint32_t* rsize;
+    uint32_t* size;   // nodes in the subtree
+    uint32_t* rsize;  // references in the subtree
+    uint32_t* depth;  // deepest level in the subtree
     uint32_t* addr;
     uint32_t* rbase;
 };
@@ -971,22 +966,44 @@ __global__ __launch_bounds__(256) void k_level_scatter(uint32_t nbn, BnView B, u
 __global__ __launch_bounds__(1024) void k_layout(uint32_t maxlvl, const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ list, BnView B,
                                                  const uint32_t* __restrict__ s_nsize,
-                                                 const uint32_t* __restrict__ s_rsize, LayoutView Lv,
+                                                 const uint32_t* __restrict__ s_rsize,
+                                                 const uint32_t* __restrict__ s_mlvl, LayoutView Lv,
                                                  uint32_t* __restrict__ counters) {
+    // Each thread takes kPer entries of a level at once, so their dependent loads (list
+    // entry, node columns, children's sizes) are in flight together.
+    constexpr int kPer = 4;
     for (int lvl = (int)maxlvl; lvl >= 0; --lvl) {
-        for (uint32_t j = off[lvl] + threadIdx.x; j < off[lvl + 1]; j += blockDim.x) {
-            uint32_t i = list[j];
-            uint32_t t = B.type[i];
-            if (t <= 2) {
-                uint32_t l = B.left[i], r = B.right[i];
-                Lv.size[i] = 1 + Lv.size[l] + Lv.size[r];
-                Lv.rsize[i] = Lv.rsize[l] + Lv.rsize[r];
-            } else if (t == BN_LEAF) {
-                Lv.size[i] = 1;
-                Lv.rsize[i] = B.cnt[i];
-            } else if (t == BN_SMALL) {
-                Lv.size[i] = s_nsize[B.aux[i]];
-                Lv.rsize[i] = s_rsize[B.aux[i]];
+        const uint32_t end = off[lvl + 1];
+        for (uint32_t j0 = off[lvl] + threadIdx.x; j0 < end; j0 += kPer * blockDim.x) {
+            uint32_t idx[kPer], typ[kPer], l[kPer], r[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const uint32_t j = j0 + k * blockDim.x;
+                idx[k] = (j < end) ? list[j] : kNone;
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                typ[k] = (idx[k] != kNone) ? B.type[idx[k]] : BN_PENDING;
+                l[k] = (typ[k] <= 2) ? B.left[idx[k]] : 0u;
+                r[k] = (typ[k] <= 2) ? B.right[idx[k]] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const uint32_t i = idx[k], t = typ[k];
+                if (t <= 2) {
+                    Lv.size[i] = 1 + Lv.size[l[k]] + Lv.size[r[k]];
+                    Lv.rsize[i] = Lv.rsize[l[k]] + Lv.rsize[r[k]];
+                    Lv.depth[i] = max(Lv.depth[l[k]], Lv.depth[r[k]]);
+                } else if (t == BN_LEAF) {
+                    Lv.size[i] = 1;
+                    Lv.rsize[i] = B.cnt[i];
+                    Lv.depth[i] = B.level[i];
+                } else if (t == BN_SMALL) {
+                    const uint32_t x = B.aux[i];
+                    Lv.size[i] = s_nsize[x];
+                    Lv.rsize[i] = s_rsize[x];
+                    Lv.depth[i] = s_mlvl[x];
+                }
             }
         }
         __syncthreads();
@@ -999,15 +1016,31 @@ __global__ __launch_bounds__(1024) void k_layout(uint32_t maxlvl, const uint32_t
     }
     __syncthreads();
     for (uint32_t lvl = 0; lvl <= maxlvl; ++lvl) {
-        for (uint32_t j = off[lvl] + threadIdx.x; j < off[lvl + 1]; j += blockDim.x) {
-            uint32_t i = list[j];
-            if (B.type[i] > 2) continue;
-            uint32_t l = B.left[i], r = B.right[i];
-            uint32_t a = Lv.addr[i], rb = Lv.rbase[i];
-            Lv.addr[l] = a + 1;
-            Lv.rbase[l] = rb;
-            Lv.addr[r] = a + 1 + Lv.size[l];
-            Lv.rbase[r] = rb + Lv.rsize[l];
+        const uint32_t end = off[lvl + 1];
+        for (uint32_t j0 = off[lvl] + threadIdx.x; j0 < end; j0 += kPer * blockDim.x) {
+            uint32_t idx[kPer], l[kPer], r[kPer];
+            bool inner[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const uint32_t j = j0 + k * blockDim.x;
+                idx[k] = (j < end) ? list[j] : kNone;
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                inner[k] = idx[k] != kNone && B.type[idx[k]] <= 2;
+                l[k] = inner[k] ? B.left[idx[k]] : 0u;
+                r[k] = inner[k] ? B.right[idx[k]] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                if (!inner[k]) continue;
+                const uint32_t i = idx[k];
+                const uint32_t a = Lv.addr[i], rb = Lv.rbase[i];
+                Lv.addr[l[k]] = a + 1;
+                Lv.rbase[l[k]] = rb;
+                Lv.addr[r[k]] = a + 1 + Lv.size[l[k]];
+                Lv.rbase[r[k]] = rb + Lv.rsize[l[k]];
+            }
         }
         __syncthreads();
     }
@@ -1017,9 +1050,7 @@ __global__ __launch_bounds__(256) void k_emit_large(uint32_t nbn, BnView B, Layo
                                                     uint32_t* __restrict__ nodes_all, uint32_t* __restrict__ refs_all,
                                                     uint32_t* __restrict__ counters, uint32_t max_nodes_all,
                                                     uint32_t max_refs_all, const uint32_t* __restrict__ node_base,
-                                                    const uint32_t* __restrict__ ref_base,
-                                                    uint32_t* __restrict__ scene_leaves,
-                                                    uint32_t* __restrict__ scene_maxlvl) {
+                                                    const uint32_t* __restrict__ ref_base) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nbn) return;
     uint32_t t = B.type[i];
@@ -1037,14 +1068,11 @@ __global__ __launch_bounds__(256) void k_emit_large(uint32_t nbn, BnView B, Layo
         uint32_t a = Lv.addr[i];
         nodes[2 * a] = B.split[i];
         nodes[2 * a + 1] = (t << 30) | Lv.addr[B.right[i]];
-        atomicMax(&scene_maxlvl[sc], B.level[i]);
     } else if (t == BN_LEAF) {
         uint32_t a = Lv.addr[i], rb = Lv.rbase[i], m = B.cnt[i];
         nodes[2 * a] = m;
         nodes[2 * a + 1] = (MNRT_LEAF_TAG << 30) | rb;
         for (uint32_t k = 0; k < m; ++k) refs[rb + k] = ll_idx[B.aux[i] + k];
-        atomicAdd(&scene_leaves[sc], 1u);
-        atomicMax(&scene_maxlvl[sc], B.level[i]);
     }
 }
 
@@ -1054,12 +1082,17 @@ __global__ __launch_bounds__(256) void k_scene_counts(uint32_t nscenes, const ui
     if (s < nscenes) cnt[s] = base[s + 1] - base[s];
 }
 
+// Per scene (root = build node s): nodes, references, leaves and max level.  Every inner
+// node has two children, so leaves = (nodes + 1) / 2.
 __global__ __launch_bounds__(256) void k_scene_sizes(uint32_t nscenes, LayoutView Lv, uint32_t* __restrict__ nodes,
-                                                     uint32_t* __restrict__ refs) {
+                                                     uint32_t* __restrict__ refs, uint32_t* __restrict__ leaves,
+                                                     uint32_t* __restrict__ maxlvl) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nscenes) return;
     nodes[s] = Lv.size[s];
     refs[s] = Lv.rsize[s];
+    leaves[s] = (Lv.size[s] + 1) / 2;
+    maxlvl[s] = Lv.depth[s];
 }
 
 }  // namespace mnrt
@@ -1521,10 +1554,12 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     // ---- small-node stage, pass 1: subtree sizes ----
     uint32_t* s_nsize = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
     uint32_t* s_rsize = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
+    uint32_t* s_mlvl = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
     if (!pool.ok) return MNRT_ERR_NOMEM;
     SmallOut O{};
     O.nsize = s_nsize;
     O.rsize = s_rsize;
+    O.mlvl = s_mlvl;
     O.counters = counters;
     if (nsmall > 0)
         hipLaunchKernelGGL((k_small<false>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
@@ -1533,6 +1568,7 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     LayoutView Lv{};
     Lv.size = pool.get<uint32_t>(nbn);
     Lv.rsize = pool.get<uint32_t>(nbn);
+    Lv.depth = pool.get<uint32_t>(nbn);
     Lv.addr = pool.get<uint32_t>(nbn);
     Lv.rbase = pool.get<uint32_t>(nbn);
     uint32_t* lvb = pool.get<uint32_t>(64 + 65 + 64 + nbn + scan_scratch_words(std::max<uint32_t>(64, S_ + 1)));
@@ -1549,9 +1585,10 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         MNRT_HIP_CHECK(hipMemcpyAsync(cursor, off, 64 * 4, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_level_scatter, blocks_for(nbn), dim3(256), 0, st, nbn, B, cursor, list);
         hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, st, std::min<uint32_t>(maxlvl, 63), off, list, B,
-                           s_nsize, s_rsize, Lv, counters);
+                           s_nsize, s_rsize, s_mlvl, Lv, counters);
         // per-scene sizes (root = build node s) and their offsets in the shared arrays
-        hipLaunchKernelGGL(k_scene_sizes, blocks_for(S_), dim3(256), 0, st, S_, Lv, d_snodes, d_srefs);
+        hipLaunchKernelGGL(k_scene_sizes, blocks_for(S_), dim3(256), 0, st, S_, Lv, d_snodes, d_srefs, d_sleaves,
+                           d_smaxlvl);
         MNRT_HIP_CHECK(scan_exclusive(d_snodes, d_nbase, S_, d_nbase + S_, sc, st));
         MNRT_HIP_CHECK(scan_exclusive(d_srefs, d_rbase, S_, d_rbase + S_, sc, st));
         MNRT_HIP_CHECK(hipMemcpyAsync(h_counters + 40, d_nbase + S_, 4, hipMemcpyDeviceToHost, st));
@@ -1570,7 +1607,7 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         return MNRT_ERR_NOMEM;
     }
     hipLaunchKernelGGL(k_emit_large, blocks_for(nbn), dim3(256), 0, st, nbn, B, Lv, ll_idx, d_nodes, d_refs, counters,
-                       total_nodes, total_refs, d_nbase, d_rbase, d_sleaves, d_smaxlvl);
+                       total_nodes, total_refs, d_nbase, d_rbase);
     O.bn_addr = Lv.addr;
     O.bn_rbase = Lv.rbase;
     O.nodes = d_nodes;
@@ -1579,8 +1616,6 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     O.max_refs = total_refs;
     O.node_base = d_nbase;
     O.ref_base = d_rbase;
-    O.scene_leaves = d_sleaves;
-    O.scene_maxlvl = d_smaxlvl;
     if (nsmall > 0)
         hipLaunchKernelGGL((k_small<true>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
                            st, nsmall, SV, Sm, kind, (const float*)d_qr, O);
