@@ -18,8 +18,10 @@
 //     order) are appended to a per-wave LDS buffer with ballot + mbcnt compaction; when
 //     k are known or the buffer would overflow, the k best are kept (radix selection of
 //     the k-th key, or a bitonic sort) and the pruning threshold tightens;
-//   * the final set is sorted once; its records are fetched by all lanes at once and the
-//     front-facing powers summed in (d2, index) order.
+//   * the final set is sorted only when the caller asks for the neighbour lists; each
+//     lane fetches the records of its slots and the front-facing powers are summed per
+//     channel as exact 64-bit fixed-point integers (the reference adds floats in its
+//     range-search order, photon_knn.cu:506-516, so its sums depend on that order).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -54,6 +56,30 @@ __device__ __forceinline__ void wave_bitonic(uint64_t* keys, uint32_t* pos, uint
             wave_sync();
         }
     }
+}
+
+// Exact sum of one float per lane slot (RK slots per lane, unused slots 0) as a 64-bit
+// fixed-point integer: with M the largest |v| (M = f 2^e, f in [0.5, 1)), every term is
+// rounded to a multiple of 2^(e-40) (exact for terms down to 2^-17 M), the integers are
+// added in any order (|sum| < 2^48) and the total is rounded to float once.  Wave-uniform
+// result; the same in every lane.
+template <int RK>
+__device__ __forceinline__ float wave_fixed_sum(const float (&v)[RK]) {
+    float m = 0.0f;
+#pragma unroll
+    for (int t = 0; t < RK; ++t) m = fmaxf(m, fabsf(v[t]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (!(m > 0.0f)) return 0.0f;
+    int ex;
+    (void)frexpf(m, &ex);
+    const int sh = 40 - ex;
+    long long acc = 0;
+#pragma unroll
+    for (int t = 0; t < RK; ++t) acc += __double2ll_rn(ldexp((double)v[t], sh));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    return (float)ldexp((double)acc, -sh);
 }
 
 __device__ __forceinline__ uint32_t pow2_at_least(uint32_t v) {
@@ -357,11 +383,19 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         if (!found) break;
     }
     if (kSel != kSelSort && cnt > k) radix_keep();
-    sort_keep(k);
+    // the buffer now holds exactly the set; it is sorted only for callers of the lists
+    if (kSel == kSelSort || A.out_nbr) sort_keep(k);
     const uint32_t found = cnt;  // min(#photons closer than r_max, k)
 
     float r2 = r2max;
-    if (found >= k && k > 0) r2 = __uint_as_float((uint32_t)(keys[k - 1] >> 32));
+    if (found >= k && k > 0) {
+        // the k-th distance: the largest d2 of the set
+        uint32_t hi = 0;
+        for (uint32_t i = lane; i < found; i += 64) hi = max(hi, (uint32_t)(keys[i] >> 32));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) hi = max(hi, (uint32_t)__shfl_xor((int)hi, o, 64));
+        r2 = __uint_as_float(hi);
+    }
 
     if (A.out_count && lane == 0) A.out_count[q] = found;
     if (A.out_nbr) {
@@ -380,34 +414,24 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     f3 nrd = mk3(-rd.x, -rd.y, -rd.z);
     if (dot3(nrd, nq) < 0.0f) nq = mk3(nq.x * -1.0f, nq.y * -1.0f, nq.z * -1.0f);
 
-    // Front-facing powers summed in (d2, index) order.  Lanes fetch and test 64 neighbours
-    // at a time into LDS (the traversal stack's box offsets, free now), then the sum runs
-    // through them in order.
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    static_assert(kKnnStack >= 64, "the stage holds one entry per lane");
-    float4* stage = soff;
-    for (uint32_t base = 0; base < found; base += 64) {
-        const uint32_t i = base + lane;
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // Front-facing powers: each lane fetches the records of its slots of the set, the
+    // channel sums are exact fixed-point sums (independent of the set's order).
+    constexpr int RK = (int)((MNRT_KNN_MAX_K + 63u) / 64u);
+    float vx[RK], vy[RK], vz[RK];
+#pragma unroll
+    for (int t = 0; t < RK; ++t) {
+        const uint32_t i = lane + 64u * (uint32_t)t;
+        vx[t] = vy[t] = vz[t] = 0.0f;
         if (i < found) {
             const float4 pw = Tr.recs[2ull * kpos[i] + 1];
-            const f3 pd = oct_decode(__float_as_uint(pw.w));
-            // back-facing photons add +0: the running sums start at +0 and are never -0,
-            // so s + 0 == s and the sums equal the conditional ones bit for bit
-            if (dot3(nq, pd) < 0.0f) v = make_float4(pw.x, pw.y, pw.z, 0.0f);
+            if (dot3(nq, oct_decode(__float_as_uint(pw.w))) < 0.0f) {
+                vx[t] = pw.x;
+                vy[t] = pw.y;
+                vz[t] = pw.z;
+            }
         }
-        wave_sync();
-        stage[lane] = v;
-        wave_sync();
-        const uint32_t m = min(64u, found - base);
-        for (uint32_t j = 0; j < m; ++j) {
-            const float4 e = stage[j];
-            sx = sx + e.x;
-            sy = sy + e.y;
-            sz = sz + e.z;
-        }
-        wave_sync();
     }
+    const float sx = wave_fixed_sum<RK>(vx), sy = wave_fixed_sum<RK>(vy), sz = wave_fixed_sum<RK>(vz);
     float ix = 0.0f, iy = 0.0f, iz = 0.0f;
     if (r2 > 0.0f) {
         float den = r2 * kPi;

@@ -17,7 +17,9 @@
 //     (the reference races on s_idxMin, kdtree.cu:1458-1463);
 //   * closest hits tie-break equal lambda to the lower triangle index (the reference
 //     keeps the first found, raytracing.cu:627);
-//   * kNN sets are ordered by (d2, photon index).
+//   * kNN sets are ordered by (d2, photon index);
+//   * the fp32 photon power sums are exact fixed-point sums (fixed_sum), so they do not
+//     depend on the order of the set.
 // Every deviation is listed in the repository's design notes.
 //
 // Floating point: compiled with -ffp-contract=off and without fast-math so that every
@@ -952,6 +954,21 @@ struct RadIn {
     const float* qclr;
 };
 
+// Exact sum of the terms as 64-bit fixed-point integers: with M the largest |v|
+// (M = f 2^e, f in [0.5, 1)) every term is rounded to a multiple of 2^(e-40), the integers
+// are added (any order gives the same total) and the total is rounded to float once.
+float fixed_sum(const float* v, uint32_t n) {
+    float m = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) m = std::fmax(m, std::fabs(v[i]));
+    if (!(m > 0.0f)) return 0.0f;
+    int ex;
+    (void)std::frexp(m, &ex);
+    const int sh = 40 - ex;
+    long long acc = 0;
+    for (uint32_t i = 0; i < n; ++i) acc += std::llrint(std::ldexp((double)v[i], sh));
+    return (float)std::ldexp((double)acc, -sh);
+}
+
 void radiance_one(const RadIn& I, uint32_t q, const uint32_t* nbr, uint32_t cnt, uint32_t k,
                   float rmax, float* out32, double* out64, float* irr32, double* irr64,
                   float* r2out) {
@@ -959,18 +976,22 @@ void radiance_one(const RadIn& I, uint32_t q, const uint32_t* nbr, uint32_t cnt,
     V3 nd = mk(-d.v[0], -d.v[1], -d.v[2]);
     if (dot(nd, n) < 0.0f) n = mk(n.v[0] * -1.0f, n.v[1] * -1.0f, n.v[2] * -1.0f);
     float r2max = rmax * rmax;
-    float s[3] = {0.0f, 0.0f, 0.0f};
+    // front-facing powers (photon_knn.cu:506-516): fp32 as an exact fixed-point sum, fp64
+    // directly
     double s64[3] = {0.0, 0.0, 0.0};
+    std::vector<float> terms[3];
     for (uint32_t i = 0; i < cnt && i < k; ++i) {
         uint32_t p = nbr[i];
         V3 pd = oct_decode(I.pdir[p]);
         if (dot(n, pd) < 0.0f) {
             for (int c = 0; c < 3; ++c) {
-                s[c] = s[c] + I.ppow[4 * p + c];
+                terms[c].push_back(I.ppow[4 * p + c]);
                 s64[c] += (double)I.ppow[4 * p + c];
             }
         }
     }
+    float s[3];
+    for (int c = 0; c < 3; ++c) s[c] = fixed_sum(terms[c].data(), (uint32_t)terms[c].size());
     float r2 = r2max;
     double r2d = (double)rmax * (double)rmax;
     if (cnt >= k && k > 0) {
@@ -1689,6 +1710,8 @@ void orc_radiance_grad(const uint32_t* nbr, const uint32_t* count, uint32_t k, f
         if (out_gclr) out_gclr[4 * q + 3] = 0.0;
     }
 }
+
+float orc_fixed_sum(const float* v, uint32_t n) { return fixed_sum(v, n); }
 
 uint32_t orc_oct_encode(float x, float y, float z) { return oct_encode(x, y, z); }
 void orc_oct_encode_n(const float* dir4, uint32_t n, uint32_t* out) {

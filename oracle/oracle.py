@@ -76,6 +76,8 @@ def lib():
     L.orc_export_preorder.argtypes = [u32p, u32p, f32p, u32p, u32p, u32p, f32p, u32p, u32p]
     L.orc_ray_tri.restype = ctypes.c_int
     L.orc_ray_tri.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
+    L.orc_fixed_sum.restype = ctypes.c_float
+    L.orc_fixed_sum.argtypes = [f32p, ctypes.c_uint32]
     L.orc_trace_any.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, f32p, f32p,
                                 ctypes.c_uint32, ctypes.c_float, u32p, u64p, ctypes.c_int]
     L.orc_trace_shadow.argtypes = [u32p, u32p, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, ctypes.c_float,
@@ -188,6 +190,13 @@ def trace_brute(root, v0, v1, v2, orig, dirs, eps, nthreads=0):
     lib().orc_trace_brute(_c(root).ravel(), _c(v0), _c(v1), _c(v2), v0.shape[0], _c(orig), _c(dirs), n, eps, tri,
                           t, uv, _threads(nthreads))
     return tri, t, uv.reshape(n, 2)
+
+
+def fixed_sum(v):
+    """The radiance estimate's fp32 power sum (orc_fixed_sum): exact fixed-point total of
+    the terms rounded to float once."""
+    v = np.ascontiguousarray(v, np.float32).ravel()
+    return np.float32(lib().orc_fixed_sum(v if v.size else np.zeros(1, np.float32), v.size))
 
 
 FLT_MAX = float(np.finfo(np.float32).max)

@@ -161,3 +161,46 @@ def test_oct_code_round_trip(orc):
         for s in (1, -1):
             v = orc.oct_decode(orc.oct_encode(f4([axis * s]))[0])
             assert np.allclose(v / np.linalg.norm(v), axis * s)
+
+
+def test_fixed_sum_is_the_rounded_exact_sum(orc):
+    # terms within 2^17 of the largest are exact multiples of 2^(e-40): the total is the
+    # exact rational sum rounded to float32 once, whatever the order of the terms
+    from fractions import Fraction
+
+    rng = np.random.default_rng(12)
+    for trial in range(200):
+        n = int(rng.integers(1, 257))
+        v = (rng.uniform(0.5, 1.0, size=n) * 2.0 ** rng.integers(-16, 1, size=n)).astype(np.float32)
+        v *= np.float32(2.0 ** rng.integers(-60, 60))
+        if trial % 3 == 0:
+            v[rng.random(n) < 0.3] *= -1
+        exact = sum(Fraction(float(x)) for x in v)
+        want = np.float32(float(exact))  # exact sum fits a double: one rounding to float32
+        got = orc.fixed_sum(v)
+        assert got == want, (trial, got, want)
+        assert orc.fixed_sum(rng.permutation(v)) == got
+
+
+def test_fixed_sum_edge_cases(orc):
+    assert orc.fixed_sum(np.zeros(0, np.float32)) == 0.0
+    assert orc.fixed_sum(np.zeros(5, np.float32)) == 0.0
+    assert orc.fixed_sum([1.0, -1.0, 2.0 ** -10]) == np.float32(2.0 ** -10)
+    # terms below 2^-41 of the largest are rounded to multiples of 2^(e-40)
+    assert orc.fixed_sum([1.0, 2.0 ** -45]) == 1.0
+    assert orc.fixed_sum([1.0] + [2.0 ** -41] * 4) == np.float32(1.0 + 4 * 2.0 ** -41)
+    # denormal and huge magnitudes
+    tiny = np.float32(1e-44)
+    assert orc.fixed_sum([tiny, tiny]) == np.float32(2 * float(tiny))
+    big = np.float32(3e38)
+    e = np.frexp(big)[1]
+    q = 2.0 ** (e - 40)  # a term 2^-28 of the largest is rounded to a multiple of q
+    assert orc.fixed_sum([big, -big, 1e30]) == np.float32(round(float(np.float32(1e30)) / q) * q)
+    # close to a sequential float32 sum for positive terms (within the sequential sum's error)
+    rng = np.random.default_rng(2)
+    v = rng.uniform(0, 1, size=100).astype(np.float32)
+    seq = np.float32(0)
+    for x in v:
+        seq = np.float32(seq + x)
+    assert abs(float(orc.fixed_sum(v)) - float(seq)) <= 100 * np.spacing(seq)
+    assert abs(float(orc.fixed_sum(v)) - float(np.sum(v.astype(np.float64)))) <= np.spacing(np.float32(v.sum()))
