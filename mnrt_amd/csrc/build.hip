@@ -946,103 +946,87 @@ struct LayoutView {
     uint32_t* rbase;
 };
 
-// Build nodes bucketed by level (the order inside a level does not matter).
+// Build nodes bucketed by level (the order inside a level does not matter).  Counts go
+// through a per-workgroup LDS histogram, so each workgroup adds to a level's global
+// counter once.
 __global__ __launch_bounds__(256) void k_level_hist(uint32_t nbn, BnView B, uint32_t* __restrict__ hist) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbn || B.type[i] >= BN_PENDING) return;
-    atomicAdd(&hist[B.level[i]], 1u);
+    __shared__ uint32_t s_hist[64];
+    if (threadIdx.x < 64) s_hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nbn && B.type[i] < BN_PENDING) atomicAdd(&s_hist[B.level[i] & 63u], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64 && s_hist[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_level_scatter(uint32_t nbn, BnView B, uint32_t* __restrict__ cursor,
                                                        uint32_t* __restrict__ list) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbn || B.type[i] >= BN_PENDING) return;
-    list[atomicAdd(&cursor[B.level[i]], 1u)] = i;
+    __shared__ uint32_t s_cnt[64];
+    __shared__ uint32_t s_base[64];
+    if (threadIdx.x < 64) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < nbn && B.type[i] < BN_PENDING;
+    const uint32_t lvl = live ? (B.level[i] & 63u) : 0u;
+    const uint32_t local = live ? atomicAdd(&s_cnt[lvl], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x < 64 && s_cnt[threadIdx.x] != 0u)
+        s_base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], s_cnt[threadIdx.x]);
+    __syncthreads();
+    if (live) list[s_base[lvl] + local] = i;
 }
 
-// Subtree sizes bottom-up then preorder addresses top-down, one workgroup walking the
-// level buckets (KDTreeGPU::PreorderTraversal launches two kernels per level over all
-// nodes, KDTreeGPU.cpp:1156-1209).
-__global__ __launch_bounds__(1024) void k_layout(uint32_t maxlvl, const uint32_t* __restrict__ off,
-                                                 const uint32_t* __restrict__ list, BnView B,
-                                                 const uint32_t* __restrict__ s_nsize,
-                                                 const uint32_t* __restrict__ s_rsize,
-                                                 const uint32_t* __restrict__ s_mlvl, LayoutView Lv,
-                                                 uint32_t* __restrict__ counters) {
-    // Each thread takes kPer entries of a level at once, so their dependent loads (list
-    // entry, node columns, children's sizes) are in flight together.
-    constexpr int kPer = 4;
-    for (int lvl = (int)maxlvl; lvl >= 0; --lvl) {
-        const uint32_t end = off[lvl + 1];
-        for (uint32_t j0 = off[lvl] + threadIdx.x; j0 < end; j0 += kPer * blockDim.x) {
-            uint32_t idx[kPer], typ[kPer], l[kPer], r[kPer];
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const uint32_t j = j0 + k * blockDim.x;
-                idx[k] = (j < end) ? list[j] : kNone;
-            }
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                typ[k] = (idx[k] != kNone) ? B.type[idx[k]] : BN_PENDING;
-                l[k] = (typ[k] <= 2) ? B.left[idx[k]] : 0u;
-                r[k] = (typ[k] <= 2) ? B.right[idx[k]] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const uint32_t i = idx[k], t = typ[k];
-                if (t <= 2) {
-                    Lv.size[i] = 1 + Lv.size[l[k]] + Lv.size[r[k]];
-                    Lv.rsize[i] = Lv.rsize[l[k]] + Lv.rsize[r[k]];
-                    Lv.depth[i] = max(Lv.depth[l[k]], Lv.depth[r[k]]);
-                } else if (t == BN_LEAF) {
-                    Lv.size[i] = 1;
-                    Lv.rsize[i] = B.cnt[i];
-                    Lv.depth[i] = B.level[i];
-                } else if (t == BN_SMALL) {
-                    const uint32_t x = B.aux[i];
-                    Lv.size[i] = s_nsize[x];
-                    Lv.rsize[i] = s_rsize[x];
-                    Lv.depth[i] = s_mlvl[x];
-                }
-            }
+// Subtree sizes bottom-up, then preorder addresses top-down, one launch per level over the
+// level's bucket (KDTreeGPU::PreorderTraversal launches two kernels per level over all
+// nodes, KDTreeGPU.cpp:1156-1209; here only the level's nodes are touched).
+struct LayoutSrc {
+    const uint32_t* off;   // level bucket offsets (65)
+    const uint32_t* list;  // build nodes by level
+    const uint32_t* s_nsize;
+    const uint32_t* s_rsize;
+    const uint32_t* s_mlvl;
+};
+
+__global__ __launch_bounds__(256) void k_layout_up(uint32_t lvl, LayoutSrc Ls, BnView B, LayoutView Lv) {
+    const uint32_t end = Ls.off[lvl + 1];
+    for (uint32_t j = Ls.off[lvl] + blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
+        const uint32_t i = Ls.list[j];
+        const uint32_t t = B.type[i];
+        if (t <= 2) {
+            const uint32_t l = B.left[i], r = B.right[i];
+            Lv.size[i] = 1 + Lv.size[l] + Lv.size[r];
+            Lv.rsize[i] = Lv.rsize[l] + Lv.rsize[r];
+            Lv.depth[i] = max(Lv.depth[l], Lv.depth[r]);
+        } else if (t == BN_LEAF) {
+            Lv.size[i] = 1;
+            Lv.rsize[i] = B.cnt[i];
+            Lv.depth[i] = B.level[i];
+        } else if (t == BN_SMALL) {
+            const uint32_t x = B.aux[i];
+            Lv.size[i] = Ls.s_nsize[x];
+            Lv.rsize[i] = Ls.s_rsize[x];
+            Lv.depth[i] = Ls.s_mlvl[x];
         }
-        __syncthreads();
     }
-    // every scene's root (level 0) starts its own tree at address 0
-    for (uint32_t j = off[0] + threadIdx.x; j < off[1]; j += blockDim.x) {
-        uint32_t i = list[j];
-        Lv.addr[i] = 0;
-        Lv.rbase[i] = 0;
-    }
-    __syncthreads();
-    for (uint32_t lvl = 0; lvl <= maxlvl; ++lvl) {
-        const uint32_t end = off[lvl + 1];
-        for (uint32_t j0 = off[lvl] + threadIdx.x; j0 < end; j0 += kPer * blockDim.x) {
-            uint32_t idx[kPer], l[kPer], r[kPer];
-            bool inner[kPer];
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const uint32_t j = j0 + k * blockDim.x;
-                idx[k] = (j < end) ? list[j] : kNone;
-            }
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                inner[k] = idx[k] != kNone && B.type[idx[k]] <= 2;
-                l[k] = inner[k] ? B.left[idx[k]] : 0u;
-                r[k] = inner[k] ? B.right[idx[k]] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                if (!inner[k]) continue;
-                const uint32_t i = idx[k];
-                const uint32_t a = Lv.addr[i], rb = Lv.rbase[i];
-                Lv.addr[l[k]] = a + 1;
-                Lv.rbase[l[k]] = rb;
-                Lv.addr[r[k]] = a + 1 + Lv.size[l[k]];
-                Lv.rbase[r[k]] = rb + Lv.rsize[l[k]];
-            }
+}
+
+// Level lvl's nodes place their children; every scene's root (level 0) starts its own
+// tree at address 0.
+__global__ __launch_bounds__(256) void k_layout_down(uint32_t lvl, LayoutSrc Ls, BnView B, LayoutView Lv) {
+    const uint32_t end = Ls.off[lvl + 1];
+    for (uint32_t j = Ls.off[lvl] + blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
+        const uint32_t i = Ls.list[j];
+        if (lvl == 0) {
+            Lv.addr[i] = 0;
+            Lv.rbase[i] = 0;
         }
-        __syncthreads();
+        if (B.type[i] > 2) continue;
+        const uint32_t l = B.left[i], r = B.right[i];
+        const uint32_t a = Lv.addr[i], rb = Lv.rbase[i];
+        Lv.addr[l] = a + 1;
+        Lv.rbase[l] = rb;
+        Lv.addr[r] = a + 1 + Lv.size[l];
+        Lv.rbase[r] = rb + Lv.rsize[l];
     }
 }
 
@@ -1584,8 +1568,13 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         MNRT_HIP_CHECK(scan_exclusive(hist, off, 64, off + 64, sc, st));
         MNRT_HIP_CHECK(hipMemcpyAsync(cursor, off, 64 * 4, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_level_scatter, blocks_for(nbn), dim3(256), 0, st, nbn, B, cursor, list);
-        hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, st, std::min<uint32_t>(maxlvl, 63), off, list, B,
-                           s_nsize, s_rsize, s_mlvl, Lv, counters);
+        LayoutSrc Ls{off, list, s_nsize, s_rsize, s_mlvl};
+        const uint32_t top = std::min<uint32_t>(maxlvl, 63);
+        const dim3 lgrid(std::min<uint32_t>(blocks_for(nbn).x, 512u));
+        for (int lvl = (int)top; lvl >= 0; --lvl)
+            hipLaunchKernelGGL(k_layout_up, lgrid, dim3(256), 0, st, (uint32_t)lvl, Ls, B, Lv);
+        for (uint32_t lvl = 0; lvl <= top; ++lvl)
+            hipLaunchKernelGGL(k_layout_down, lgrid, dim3(256), 0, st, lvl, Ls, B, Lv);
         // per-scene sizes (root = build node s) and their offsets in the shared arrays
         hipLaunchKernelGGL(k_scene_sizes, blocks_for(S_), dim3(256), 0, st, S_, Lv, d_snodes, d_srefs, d_sleaves,
                            d_smaxlvl);
