@@ -195,17 +195,35 @@ struct PairSink {
     uint32_t slot;
 };
 
+// Warm start for consecutive queries of one wave on one tree: every one of the previous
+// query's k neighbours lies within d_k(prev) + |prev - q| of q, so that radius (rounded
+// up) bounds this query's k-th distance and the search starts with it instead of r_max.
+struct WarmHint {
+    float x, y, z;
+    float r2;  // previous query's k-th d2; < 0: none (fewer than k found, or first query)
+};
+
 // One query by one wave; keys / kpos: CAP LDS entries, snode / soff: kKnnStack.
 template <int CAP, int kMode, int kSel = kSelSort, bool kPairs = false>
 __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, uint32_t q, uint32_t lane,
                                           uint64_t* keys, uint32_t* kpos, uint32_t* snode, float4* soff,
-                                          const PairSink* sink = nullptr) {
+                                          const PairSink* sink = nullptr, WarmHint* hint = nullptr) {
     const float4 qp4 = A.pos[q];
     const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
     const f3 qp = mk3(qx, qy, qz);
     const uint32_t k = A.k;
     const float rq = A.r_query ? uniformf(A.r_query[q]) : Tr.r_max;
     const float r2max = A.r_query ? rq * rq : Tr.r2max;
+    // search radius: r_max, or the warm-start bound when smaller (margins of 2^-20 cover
+    // the rounding of the fp32 distances; the bound is exclusive, hence nextafter)
+    float r2lim = r2max;
+    if (hint && hint->r2 >= 0.0f && k > 0) {
+        const double dx = (double)qx - (double)hint->x, dy = (double)qy - (double)hint->y,
+                     dz = (double)qz - (double)hint->z;
+        const double h = (sqrt((double)hint->r2) + sqrt(dx * dx + dy * dy + dz * dz)) * (1.0 + 0x1p-20);
+        const double h2 = h * h * (1.0 + 0x1p-20);
+        if (h2 < (double)r2max) r2lim = nextafterf((float)h2, kInf);
+    }
 
     uint32_t cnt = 0;        // candidates in the buffer
     bool full = false;       // the buffer's first k are the k best so far
@@ -305,82 +323,94 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         else sort_keep(keep);
     };
 
-    for (;;) {
-        node = uniform(node);
-        uint2 nd = ld_uniform(Tr.nodes, node);
-        nd.x = uniform(nd.x);
-        nd.y = uniform(nd.y);
-        while (!is_leaf(nd.y)) {
-            uint32_t a = nd.y >> 30;
-            float s = __uint_as_float(nd.x);
-            float qa = comp(qp, a);
-            float diff = qa - s;
-            uint32_t left = node + 1, right = nd.y & MNRT_INDEX_MASK;
-            uint32_t nearc = left, farc = right;
-            if (qa > s) {
-                nearc = right;
-                farc = left;
-            }
-            // far child's box distance: the split plane replaces this axis' offset
-            float fx = a == 0 ? diff : ox, fy = a == 1 ? diff : oy, fz = a == 2 ? diff : oz;
-            float bd2 = fx * fx + fy * fy + fz * fz;
-            if (bd2 < r2max && !(full && bd2 > thr_d2) && sp < kKnnStack) {
-                if (lane == 0) {
-                    snode[sp] = farc;
-                    soff[sp] = make_float4(fx, fy, fz, bd2);
-                }
-                ++sp;
-            }
-            node = uniform(nearc);
-            nd = ld_uniform(Tr.nodes, node);
+    for (;;) {  // a second pass with r_max only if the warm-start bound held fewer than k
+        for (;;) {
+            node = uniform(node);
+            uint2 nd = ld_uniform(Tr.nodes, node);
             nd.x = uniform(nd.x);
             nd.y = uniform(nd.y);
-        }
-        wave_sync();
-        const uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
-        for (uint32_t base = 0; base < count; base += 64) {
-            uint32_t j = base + lane;
-            bool take = false;
-            uint64_t key = 0;
-            if (j < count) {
-                float4 p = Tr.recs[2ull * (begin + j)];
-                f3 df = sub3(xyz(p), qp);
-                float d2 = dot3(df, df);
-                key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-                take = (d2 < r2max) && (key < thr_key);
+            while (!is_leaf(nd.y)) {
+                uint32_t a = nd.y >> 30;
+                float s = __uint_as_float(nd.x);
+                float qa = comp(qp, a);
+                float diff = qa - s;
+                uint32_t left = node + 1, right = nd.y & MNRT_INDEX_MASK;
+                uint32_t nearc = left, farc = right;
+                if (qa > s) {
+                    nearc = right;
+                    farc = left;
+                }
+                // far child's box distance: the split plane replaces this axis' offset
+                float fx = a == 0 ? diff : ox, fy = a == 1 ? diff : oy, fz = a == 2 ? diff : oz;
+                float bd2 = fx * fx + fy * fy + fz * fz;
+                if (bd2 < r2lim && !(full && bd2 > thr_d2) && sp < kKnnStack) {
+                    if (lane == 0) {
+                        snode[sp] = farc;
+                        soff[sp] = make_float4(fx, fy, fz, bd2);
+                    }
+                    ++sp;
+                }
+                node = uniform(nearc);
+                nd = ld_uniform(Tr.nodes, node);
+                nd.x = uniform(nd.x);
+                nd.y = uniform(nd.y);
             }
-            uint64_t m = __ballot(take);
-            uint32_t nm = (uint32_t)__popcll(m);
-            if (nm == 0) continue;
-            if (cnt + nm > (uint32_t)CAP) {
-                select(k);
-                take = take && (key < thr_key);
-                m = __ballot(take);
-                nm = (uint32_t)__popcll(m);
-            }
-            if (take) {
-                uint32_t slot = cnt + mask_prefix(m);
-                keys[slot] = key;
-                kpos[slot] = begin + j;
-            }
-            cnt += nm;
             wave_sync();
+            const uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
+            for (uint32_t base = 0; base < count; base += 64) {
+                uint32_t j = base + lane;
+                bool take = false;
+                uint64_t key = 0;
+                if (j < count) {
+                    float4 p = Tr.recs[2ull * (begin + j)];
+                    f3 df = sub3(xyz(p), qp);
+                    float d2 = dot3(df, df);
+                    key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
+                    take = (d2 < r2lim) && (key < thr_key);
+                }
+                uint64_t m = __ballot(take);
+                uint32_t nm = (uint32_t)__popcll(m);
+                if (nm == 0) continue;
+                if (cnt + nm > (uint32_t)CAP) {
+                    select(k);
+                    take = take && (key < thr_key);
+                    m = __ballot(take);
+                    nm = (uint32_t)__popcll(m);
+                }
+                if (take) {
+                    uint32_t slot = cnt + mask_prefix(m);
+                    keys[slot] = key;
+                    kpos[slot] = begin + j;
+                }
+                cnt += nm;
+                wave_sync();
+            }
+            if (!full && cnt >= k) select(k);
+            bool found = false;
+            while (sp > 0) {
+                --sp;
+                const float4 f = soff[sp];
+                if (full && uniformf(f.w) > thr_d2) continue;
+                node = uniform(snode[sp]);
+                ox = uniformf(f.x);
+                oy = uniformf(f.y);
+                oz = uniformf(f.z);
+                found = true;
+                break;
+            }
+            wave_sync();
+            if (!found) break;
         }
-        if (!full && cnt >= k) select(k);
-        bool found = false;
-        while (sp > 0) {
-            --sp;
-            const float4 f = soff[sp];
-            if (full && uniformf(f.w) > thr_d2) continue;
-            node = uniform(snode[sp]);
-            ox = uniformf(f.x);
-            oy = uniformf(f.y);
-            oz = uniformf(f.z);
-            found = true;
-            break;
-        }
-        wave_sync();
-        if (!found) break;
+        if (cnt >= k || r2lim == r2max) break;
+        // cannot happen when the bound holds; searched again with r_max to stay exact
+        r2lim = r2max;
+        cnt = 0;
+        full = false;
+        thr_key = ~0ull;
+        thr_d2 = kInf;
+        sp = 0;
+        node = 0;
+        ox = oy = oz = 0.0f;
     }
     if (kSel != kSelSort && cnt > k) radix_keep();
     // the buffer now holds exactly the set; it is sorted only for callers of the lists
@@ -397,6 +427,12 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         r2 = __uint_as_float(hi);
     }
 
+    if (hint) {
+        hint->x = qx;
+        hint->y = qy;
+        hint->z = qz;
+        hint->r2 = (found >= k && k > 0) ? r2 : -1.0f;
+    }
     if (A.out_count && lane == 0) A.out_count[q] = found;
     if (A.out_nbr) {
         for (uint32_t i = lane; i < k; i += 64)
@@ -525,18 +561,28 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     }
 }
 
+// Consecutive queries per wave: each after the first starts from the previous one's
+// k-th distance (WarmHint); neighbouring pixels' hits share most of their neighbours.
+constexpr uint32_t kKnnChain = 4;
+
 template <int CAP, int kMode, int kSel>
-__global__ __launch_bounds__(256) void k_knn(KnnArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_knn(KnnArgs A) {
     __shared__ uint64_t s_key[kWavesPerBlock][CAP];
     __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
     __shared__ uint32_t s_snode[kWavesPerBlock][kKnnStack];
     __shared__ float4 s_soff[kWavesPerBlock][kKnnStack];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t q = blockIdx.x * kWavesPerBlock + wave;
-    if (q >= A.n) return;  // uniform per wave
+    const uint64_t q0 = ((uint64_t)blockIdx.x * kWavesPerBlock + wave) * kKnnChain;
+    if (q0 >= A.n) return;  // uniform per wave
+    const uint32_t q1 = (uint32_t)min<uint64_t>(q0 + kKnnChain, A.n);
     const KnnTree Tr{A.nodes, A.recs, A.r2max, A.r_max};
-    knn_query<CAP, kMode, kSel>(A, Tr, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave]);
+    WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
+    for (uint32_t q = (uint32_t)q0; q < q1; ++q) {
+        knn_query<CAP, kMode, kSel>(A, Tr, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave], nullptr,
+                                    &hint);
+        wave_sync();
+    }
 }
 
 // Batched radiance: waves take chunks of `chunk` consecutive query slots from a work
@@ -571,12 +617,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
             uint32_t s = uniform(seg_of(slot_begin, ns, (uint32_t)g0));
             uint32_t b = 0, e = 0, m = 0;
             KnnTree Tr{};
+            WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
             auto enter = [&]() {
                 b = ld_uniform(slot_begin, s);
                 e = ld_uniform(slot_begin, s + 1);
                 m = count ? ld_uniform(count, s) : e - b;
                 const KnnScene sc = ld_uniform(scenes, s);
                 Tr = KnnTree{sc.nodes, sc.recs, sc.r2max, sc.r_max};
+                hint.r2 = -1.0f;  // another scene's photons
             };
             enter();
             for (uint32_t g = (uint32_t)g0; g < g1; ++g) {
@@ -588,7 +636,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
                 }
                 if (g - b >= m) continue;
                 knn_query<CAP, kModeRadiance, kSel>(G, Tr, g, lane, s_key[wave], s_pos[wave], s_snode[wave],
-                                                    s_soff[wave]);
+                                                    s_soff[wave], nullptr, &hint);
                 wave_sync();
             }
         }
@@ -621,10 +669,11 @@ __global__ __launch_bounds__(256) void k_knn_bwd_chunks(KnnArgs A, uint32_t chun
     if (lane == 0) s_np[wave] = 0;
     wave_sync();
     const KnnTree Tr{A.nodes, A.recs, A.r2max, A.r_max};
+    WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
     for (uint32_t q = (uint32_t)q0; q < q1; ++q) {
         const PairSink sink{pairs, &s_np[wave], s_w[wave], q - (uint32_t)q0};
         knn_query<CAP, kModeBackward, kSel, true>(A, Tr, q, lane, s_key[wave], s_pos[wave], s_snode[wave],
-                                                  s_soff[wave], &sink);
+                                                  s_soff[wave], &sink, &hint);
         wave_sync();
     }
     const uint32_t np = uniform(s_np[wave]);
@@ -747,7 +796,8 @@ int launch_knn_bwd(const KnnArgs& A, hipStream_t st) {
 
 template <int kMode>
 int launch_knn(const KnnArgs& A, hipStream_t st) {
-    dim3 block(64 * kWavesPerBlock), grid((A.n + kWavesPerBlock - 1) / kWavesPerBlock);
+    const uint64_t per_block = (uint64_t)kWavesPerBlock * kKnnChain;
+    dim3 block(64 * kWavesPerBlock), grid((uint32_t)((A.n + per_block - 1) / per_block));
     uint32_t need = A.k + 64;
     if (need <= 128)
         hipLaunchKernelGGL((k_knn<128, kMode, kKnnSelDefault>), grid, block, 0, st, A);

@@ -22,10 +22,12 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
     return v;
 }
 
-// Exclusive scan of one 2048-element tile per block; writes the tile total to sums.
+// Exclusive scan of one 2048-element tile per block; writes the tile total to sums, and
+// to *total (may be null) when the scan is one tile.
 __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t* in, uint32_t n,
                                                            uint32_t* out,
-                                                           uint32_t* __restrict__ sums) {
+                                                           uint32_t* __restrict__ sums,
+                                                           uint32_t* __restrict__ total) {
     __shared__ uint32_t s_wave[kScanBlock / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)tid * kScanPerThread;
@@ -49,22 +51,23 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t* in, u
         if (i < n) out[i] = run;
         run += v[j];
     }
-    if (tid == kScanBlock - 1) sums[blockIdx.x] = wave_off + incl;
+    if (tid == kScanBlock - 1) {
+        sums[blockIdx.x] = wave_off + incl;
+        if (total && gridDim.x == 1) *total = wave_off + incl;
+    }
 }
 
+// Adds each tile's offset; block 0 also writes the grand total (may be null).
 __global__ __launch_bounds__(kScanBlock) void k_scan_add(uint32_t* __restrict__ out, uint32_t n,
-                                                         const uint32_t* __restrict__ offs) {
+                                                         const uint32_t* __restrict__ offs,
+                                                         const uint32_t* __restrict__ sums, uint32_t* __restrict__ total) {
     const size_t base = (size_t)blockIdx.x * kScanTile;
     const uint32_t add = offs[blockIdx.x];
+    if (total && blockIdx.x == 0 && threadIdx.x == 0) *total = offs[gridDim.x - 1] + sums[gridDim.x - 1];
     for (uint32_t j = threadIdx.x; j < kScanTile; j += kScanBlock) {
         size_t i = base + j;
         if (i < n) out[i] += add;
     }
-}
-
-__global__ void k_scan_total(const uint32_t* __restrict__ in, const uint32_t* __restrict__ out, uint32_t n,
-                             uint32_t* __restrict__ total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total = (n == 0) ? 0u : out[n - 1] + in[n - 1];
 }
 
 __global__ void k_scan_word(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src) {
@@ -94,15 +97,12 @@ inline hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, 
     uint32_t blocks = (n + kScanTile - 1) / kScanTile;
     uint32_t* sums = scratch;
     uint32_t* offs = scratch + blocks + 1;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(blocks), dim3(kScanBlock), 0, st, in, n, out, sums);
-    if (blocks == 1) {
-        if (total) hipLaunchKernelGGL(k_scan_word, dim3(1), dim3(1), 0, st, total, (const uint32_t*)sums);
-        return hipGetLastError();
-    }
+    hipLaunchKernelGGL(k_scan_tiles, dim3(blocks), dim3(kScanBlock), 0, st, in, n, out, sums, total);
+    if (blocks == 1) return hipGetLastError();
     hipError_t e = scan_exclusive(sums, offs, blocks, nullptr, scratch + 2 * blocks + 2, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(kScanBlock), 0, st, out, n, offs);
-    if (total) hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, sums, offs, blocks, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(kScanBlock), 0, st, out, n, offs, (const uint32_t*)sums,
+                       total);
     return hipGetLastError();
 }
 
