@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <vector>
 
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "scan.h"
@@ -53,6 +55,7 @@ enum : uint32_t {
     C_LEAVES,
     C_NLEAFCOPY,
     C_ERR,
+    C_SMALL_OVF,  // small roots whose subtree outgrew its staging area
     C_COUNT
 };
 
@@ -292,47 +295,69 @@ __global__ __launch_bounds__(256) void k_tight_init(uint32_t n, uint32_t* __rest
     }
 }
 
-// Tight node bounds: per-chunk wave/LDS min/max, one atomic per chunk and component
-// (kernel_GenChunkAABB + two segmented reductions, kdtree.cu:310-445, KDTreeGPU.cpp:526-553).
-// min/max are exact, so the order of the atomics does not matter.
+// Tight node bounds: per-chunk wave/LDS min/max (kernel_GenChunkAABB + two segmented
+// reductions, kdtree.cu:310-445, KDTreeGPU.cpp:526-553).  A workgroup reduces
+// kTightChunks consecutive chunks and issues one atomic per component each time the
+// owning node changes (chunks are ordered by node, so a large node's many chunks cost
+// few atomics).  min/max are exact, so the order of the atomics does not matter.
+constexpr uint32_t kTightChunks = 8;
+
 __global__ __launch_bounds__(kChunk) void k_tight(uint32_t max_chunks, const uint32_t* __restrict__ counters,
                                                   const uint32_t* __restrict__ ch_node,
                                                   const uint32_t* __restrict__ ch_first,
                                                   const uint32_t* __restrict__ ch_cnt, ElemView E, int kind,
                                                   uint32_t* __restrict__ tlo, uint32_t* __restrict__ thi) {
     __shared__ uint32_t s_red[6][kChunk / 64];
-    uint32_t c = blockIdx.x;
-    if (c >= max_chunks || c >= counters[C_NCHUNKS]) return;
-    uint32_t node = ch_node[c], first = ch_first[c], count = ch_cnt[c];
-    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    if (threadIdx.x < count) {
-        float4 lo = E.lo[first + threadIdx.x];
-        float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[first + threadIdx.x] : lo;
-        v[0] = fenc(lo.x);
-        v[1] = fenc(lo.y);
-        v[2] = fenc(lo.z);
-        v[3] = fenc(hi.x);
-        v[4] = fenc(hi.y);
-        v[5] = fenc(hi.z);
-    }
-#pragma unroll
-    for (uint32_t o = 32; o > 0; o >>= 1) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) v[k] = min(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
-#pragma unroll
-        for (int k = 3; k < 6; ++k) v[k] = max(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
-    }
-    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int k = 0; k < 6; ++k) s_red[k][wave] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < kChunk / 64; ++w) {
-            for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_red[k][w]);
-            for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_red[k][w]);
+    const uint32_t nch = min(max_chunks, counters[C_NCHUNKS]);
+    const uint32_t c0 = blockIdx.x * kTightChunks;
+    if (c0 >= nch) return;
+    const uint32_t c1 = min(c0 + kTightChunks, nch);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t cur = kNone;  // node of the running accumulation (thread 0)
+    uint32_t acc[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t node = ch_node[c], first = ch_first[c], count = ch_cnt[c];
+        uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+        if (threadIdx.x < count) {
+            float4 lo = E.lo[first + threadIdx.x];
+            float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[first + threadIdx.x] : lo;
+            v[0] = fenc(lo.x);
+            v[1] = fenc(lo.y);
+            v[2] = fenc(lo.z);
+            v[3] = fenc(hi.x);
+            v[4] = fenc(hi.y);
+            v[5] = fenc(hi.z);
         }
-        for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * node + k], v[k]);
-        for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * node + k], v[3 + k]);
+#pragma unroll
+        for (uint32_t o = 32; o > 0; o >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = min(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
+#pragma unroll
+            for (int k = 3; k < 6; ++k) v[k] = max(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
+        }
+        if (lane == 0)
+            for (int k = 0; k < 6; ++k) s_red[k][wave] = v[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t w = 1; w < kChunk / 64; ++w) {
+                for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_red[k][w]);
+                for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_red[k][w]);
+            }
+            if (node != cur && cur != kNone) {
+                for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * cur + k], acc[k]);
+                for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * cur + k], acc[3 + k]);
+                for (int k = 0; k < 3; ++k) acc[k] = 0xFFFFFFFFu;
+                for (int k = 3; k < 6; ++k) acc[k] = 0u;
+            }
+            cur = node;
+            for (int k = 0; k < 3; ++k) acc[k] = min(acc[k], v[k]);
+            for (int k = 3; k < 6; ++k) acc[k] = max(acc[k], v[k]);
+        }
+        __syncthreads();  // s_red is rewritten by the next chunk
+    }
+    if (threadIdx.x == 0 && cur != kNone) {
+        for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * cur + k], acc[k]);
+        for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * cur + k], acc[3 + k]);
     }
 }
 
@@ -716,11 +741,29 @@ struct SmallStackEnt {
     uint32_t axis;
 };
 
+// Staging area of a small root with n elements: stage_mul * n + stage_add node words and
+// as many references (4n + 64 by default; the environment variable MNRT_SMALL_STAGE_CAP
+// = "mul,add" overrides it, so tests can force the fallback).  The subtree is written
+// there with subtree-local addresses while its size is counted, then moved into place
+// once the layout is known.  A subtree that does not fit (rare) is counted on without
+// being written and emitted by a second pass at its final address.
+struct StageCap {
+    uint32_t mul, add;
+    __host__ __device__ uint32_t of(uint32_t n) const { return mul * n + add; }
+};
+
+enum : int { kSmallCount = 0, kSmallEmit = 1, kSmallStage = 2 };
+
 struct SmallOut {
-    uint32_t* nsize;  // pass 1: nodes per small subtree
-    uint32_t* rsize;  // pass 1: references per small subtree
-    uint32_t* mlvl;   // pass 1: deepest level in the small subtree
-    const uint32_t* bn_addr;  // pass 2: subtree start address / reference base
+    uint32_t* nsize;  // count / stage: nodes per small subtree
+    uint32_t* rsize;  // count / stage: references per small subtree
+    uint32_t* mlvl;   // count / stage: deepest level in the small subtree
+    const uint32_t* st_off;  // stage: first slot of each small root's staging area
+    uint2* st_nodes;         // stage: node words, subtree-local addresses
+    uint32_t* st_refs;       // stage: references, subtree-local
+    uint32_t* st_ovf;        // stage: 1 where the subtree did not fit; emit: only these
+    StageCap st_cap;
+    const uint32_t* bn_addr;  // emit: subtree start address / reference base
     const uint32_t* bn_rbase;
     uint32_t* nodes;
     uint32_t* refs;
@@ -731,7 +774,10 @@ struct SmallOut {
     const uint32_t* ref_base;
 };
 
-template <bool kEmit>
+// kMode kSmallCount: subtree sizes only; kSmallStage: sizes and the subtree written to
+// its staging area; kSmallEmit: written at its final address (only roots with st_ovf set
+// when st_ovf is given).
+template <int kMode>
 __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, SmallView S, ElemView Sm, int kind,
                                                             const float* __restrict__ scene_qr, SmallOut O) {
     __shared__ float s_lo[kSmallWaves][3][64];
@@ -745,6 +791,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t r = blockIdx.x * kSmallWaves + wave;
     if (r >= nsmall) return;  // uniform per wave
+    if (kMode == kSmallEmit && O.st_ovf && O.st_ovf[r] == 0u) return;
     float(*elo)[64] = s_lo[wave];
     float(*ehi)[64] = s_hi[wave];
     uint32_t* eidx = s_idx[wave];
@@ -803,15 +850,25 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
 
     uint32_t addr = 0, rbase = 0, maxlvl = 0;
     const uint32_t scene = S.scene[r];
-    // local addresses go into the node words; nodes / refs point at the scene's tree
-    uint32_t* nodes = kEmit ? O.nodes + 2ull * O.node_base[scene] : nullptr;
-    uint32_t* refs = kEmit ? O.refs + O.ref_base[scene] : nullptr;
-    uint32_t lim_nodes = kEmit ? O.max_nodes - O.node_base[scene] : 0u;
-    uint32_t lim_refs = kEmit ? O.max_refs - O.ref_base[scene] : 0u;
-    if (kEmit) {
+    // emit: scene-local addresses into the scene's tree; stage: subtree-local addresses
+    // into the staging area
+    uint32_t* nodes = nullptr;
+    uint32_t* refs = nullptr;
+    uint32_t lim_nodes = 0u, lim_refs = 0u;
+    bool writing = kMode != kSmallCount;
+    if (kMode == kSmallEmit) {
+        nodes = O.nodes + 2ull * O.node_base[scene];
+        refs = O.refs + O.ref_base[scene];
+        lim_nodes = O.max_nodes - O.node_base[scene];
+        lim_refs = O.max_refs - O.ref_base[scene];
         uint32_t bn = S.bn[r];
         addr = O.bn_addr[bn];
         rbase = O.bn_rbase[bn];
+    } else if (kMode == kSmallStage) {
+        const uint32_t off = O.st_off[r];
+        nodes = (uint32_t*)(O.st_nodes + off);
+        refs = O.st_refs + off;
+        lim_nodes = lim_refs = O.st_cap.of(n);
     }
     int sp = 0;
     if (lane == 0) {
@@ -832,11 +889,14 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         --sp;
         SmallStackEnt e = stack[sp];
         wave_sync();
-        if (kEmit && (addr >= lim_nodes || rbase > lim_refs)) {
-            if (lane == 0) atomicOr(&O.counters[C_ERR], 4u);
-            break;
+        if (writing && (addr >= lim_nodes || rbase > lim_refs)) {
+            if (kMode == kSmallEmit) {
+                if (lane == 0) atomicOr(&O.counters[C_ERR], 4u);
+                break;
+            }
+            writing = false;  // staging area full: counted on, emitted later
         }
-        if (kEmit && e.patch != kNone && lane == 0) nodes[2 * e.patch + 1] = (e.axis << 30) | addr;
+        if (writing && e.patch != kNone && lane == 0) nodes[2 * e.patch + 1] = (e.axis << 30) | addr;
         maxlvl = max(maxlvl, e.level);
         const uint32_t cnt = (uint32_t)__popcll(e.mask);
         float best_cost = kInf;
@@ -886,11 +946,14 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
             }
         }
         if (best == kNone || !(best_cost < (float)cnt)) {
-            if (kEmit && rbase + cnt > lim_refs) {
-                if (lane == 0) atomicOr(&O.counters[C_ERR], 8u);
-                break;
+            if (writing && rbase + cnt > lim_refs) {
+                if (kMode == kSmallEmit) {
+                    if (lane == 0) atomicOr(&O.counters[C_ERR], 8u);
+                    break;
+                }
+                writing = false;
             }
-            if (kEmit) {
+            if (writing) {
                 if (lane == 0) {
                     nodes[2 * addr] = cnt;
                     nodes[2 * addr + 1] = (MNRT_LEAF_TAG << 30) | rbase;
@@ -906,7 +969,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         }
         const uint32_t a = (best % per) / n;
         const float s = cpos[best];
-        if (kEmit && lane == 0) {
+        if (writing && lane == 0) {
             nodes[2 * addr] = __float_as_uint(s);
             nodes[2 * addr + 1] = a << 30;
         }
@@ -929,11 +992,58 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         ++addr;
         wave_sync();
     }
-    if (!kEmit && lane == 0) {
+    if (kMode != kSmallEmit && lane == 0) {
         O.nsize[r] = addr;
         O.rsize[r] = rbase;
         O.mlvl[r] = maxlvl;
+        if (kMode == kSmallStage) {
+            O.st_ovf[r] = writing ? 0u : 1u;
+            if (!writing) atomicAdd(&O.counters[C_SMALL_OVF], 1u);
+        }
     }
+}
+
+__global__ __launch_bounds__(256) void k_small_caps(uint32_t nsmall, const uint32_t* __restrict__ cnt, StageCap cap,
+                                                    uint32_t* __restrict__ caps) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nsmall) caps[r] = cap.of(cnt[r]);
+}
+
+StageCap stage_cap_setting() {
+    StageCap c{4u, 64u};
+    if (const char* e = std::getenv("MNRT_SMALL_STAGE_CAP")) {
+        unsigned m = 0, a = 0;
+        if (std::sscanf(e, "%u,%u", &m, &a) == 2 && m <= 64 && a <= 4096) c = StageCap{m, a};
+    }
+    return c;
+}
+
+// Moves staged small subtrees to their final addresses (one wave per small root):
+// inner words get the subtree's first address added to the right-child index, leaf words
+// its first reference added to the reference offset.
+__global__ __launch_bounds__(64 * kSmallWaves) void k_small_place(uint32_t nsmall, SmallView S, SmallOut O) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t r = blockIdx.x * kSmallWaves + wave;
+    if (r >= nsmall || O.st_ovf[r] != 0u) return;
+    const uint32_t scene = S.scene[r];
+    const uint32_t bn = S.bn[r];
+    const uint32_t a0 = O.bn_addr[bn], r0 = O.bn_rbase[bn];
+    const uint32_t nn = O.nsize[r], nr = O.rsize[r];
+    const uint32_t lim_nodes = O.max_nodes - O.node_base[scene], lim_refs = O.max_refs - O.ref_base[scene];
+    if (a0 + nn > lim_nodes || r0 + nr > lim_refs) {
+        if (lane == 0) atomicOr(&O.counters[C_ERR], 4u);
+        return;
+    }
+    uint2* nodes = (uint2*)(O.nodes + 2ull * O.node_base[scene]) + a0;
+    uint32_t* refs = O.refs + O.ref_base[scene] + r0;
+    const uint32_t off = O.st_off[r];
+    for (uint32_t j = lane; j < nn; j += 64) {
+        uint2 w = O.st_nodes[off + j];
+        const uint32_t tag = w.y >> 30;
+        w.y = (tag << 30) | ((w.y & MNRT_INDEX_MASK) + (tag == MNRT_LEAF_TAG ? r0 : a0));
+        nodes[j] = w;
+    }
+    for (uint32_t j = lane; j < nr; j += 64) refs[j] = O.st_refs[off + j];
 }
 
 // ---- layout: subtree sizes bottom-up, addresses top-down, one workgroup ------------------
@@ -1317,7 +1427,8 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, S_, max_chunks, cc, counters,
                            d_base, cnts, chn, chf, chc);
         hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * S_), dim3(256), 0, st, S_, tlo, thi);
-        hipLaunchKernelGGL(k_tight, dim3(max_chunks), dim3(kChunk), 0, st, max_chunks, counters, chn, chf, chc, Ein,
+        hipLaunchKernelGGL(k_tight, dim3((max_chunks + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st,
+                           max_chunks, counters, chn, chf, chc, Ein,
                            kind, tlo, thi);
     }
     // growable tables
@@ -1494,7 +1605,8 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, na, max_chunks, cc, counters,
                            A.first, A.cnt, chn, chf, chc);
         hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * na), dim3(256), 0, st, na, tlo, thi);
-        hipLaunchKernelGGL(k_tight, dim3(max_chunks), dim3(kChunk), 0, st, max_chunks, counters, chn, chf, chc, E,
+        hipLaunchKernelGGL(k_tight, dim3((max_chunks + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st,
+                           max_chunks, counters, chn, chf, chc, E,
                            kind, tlo, thi);
         hipLaunchKernelGGL((k_plan<false>), blocks_for(na), dim3(256), 0, st, na, A, tlo, thi,
                            prm->empty_space_ratio, P, (const uint32_t*)nullptr, 0u, B, counters);
@@ -1535,19 +1647,37 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     release_elems(Nx);
     pool.release(lv_buf);
 
-    // ---- small-node stage, pass 1: subtree sizes ----
+    // ---- small-node stage: subtree sizes, each subtree written to its staging area ----
     uint32_t* s_nsize = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
     uint32_t* s_rsize = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
     uint32_t* s_mlvl = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
+    uint32_t* st_off = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
+    uint32_t* st_ovf = pool.get<uint32_t>(std::max<uint32_t>(nsmall, 1));
+    uint32_t* st_scr = pool.get<uint32_t>(scan_scratch_words(std::max<uint32_t>(nsmall, 1)));
+    const StageCap st_cap = stage_cap_setting();
+    const uint64_t st_total = (uint64_t)st_cap.mul * nse + (uint64_t)st_cap.add * nsmall;  // sum over the roots
+    if (st_total > 0xFFFFFFFFull) return MNRT_ERR_CAPACITY;
+    uint2* st_nodes = pool.get<uint2>(std::max<uint64_t>(st_total, 1));
+    uint32_t* st_refs = pool.get<uint32_t>(std::max<uint64_t>(st_total, 1));
     if (!pool.ok) return MNRT_ERR_NOMEM;
     SmallOut O{};
     O.nsize = s_nsize;
     O.rsize = s_rsize;
     O.mlvl = s_mlvl;
+    O.st_off = st_off;
+    O.st_nodes = st_nodes;
+    O.st_refs = st_refs;
+    O.st_ovf = st_ovf;
+    O.st_cap = st_cap;
     O.counters = counters;
-    if (nsmall > 0)
-        hipLaunchKernelGGL((k_small<false>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
-                           st, nsmall, SV, Sm, kind, (const float*)d_qr, O);
+    const dim3 small_grid((nsmall + kSmallWaves - 1) / kSmallWaves), small_block(64 * kSmallWaves);
+    if (nsmall > 0) {
+        hipLaunchKernelGGL(k_small_caps, blocks_for(nsmall), dim3(256), 0, st, nsmall, (const uint32_t*)SV.cnt, st_cap,
+                           st_off);
+        MNRT_HIP_CHECK(scan_exclusive(st_off, st_off, nsmall, nullptr, st_scr, st));
+        hipLaunchKernelGGL((k_small<kSmallStage>), small_grid, small_block, 0, st, nsmall, SV, Sm, kind,
+                           (const float*)d_qr, O);
+    }
     // ---- layout ----
     LayoutView Lv{};
     Lv.size = pool.get<uint32_t>(nbn);
@@ -1582,9 +1712,10 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         MNRT_HIP_CHECK(scan_exclusive(d_srefs, d_rbase, S_, d_rbase + S_, sc, st));
         MNRT_HIP_CHECK(hipMemcpyAsync(h_counters + 40, d_nbase + S_, 4, hipMemcpyDeviceToHost, st));
         MNRT_HIP_CHECK(hipMemcpyAsync(h_counters + 41, d_rbase + S_, 4, hipMemcpyDeviceToHost, st));
+        MNRT_HIP_CHECK(hipMemcpyAsync(h_counters + 42, counters + C_SMALL_OVF, 4, hipMemcpyDeviceToHost, st));
         MNRT_HIP_CHECK(hipStreamSynchronize(st));
     }
-    const uint32_t total_nodes = h_counters[40], total_refs = h_counters[41];
+    const uint32_t total_nodes = h_counters[40], total_refs = h_counters[41], small_ovf = h_counters[42];
     if (total_nodes == 0 || total_nodes > MNRT_INDEX_MASK || total_refs > MNRT_INDEX_MASK) return MNRT_ERR_CAPACITY;
 
     // ---- emission into the caller-visible trees ----
@@ -1605,9 +1736,13 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     O.max_refs = total_refs;
     O.node_base = d_nbase;
     O.ref_base = d_rbase;
-    if (nsmall > 0)
-        hipLaunchKernelGGL((k_small<true>), dim3((nsmall + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0,
-                           st, nsmall, SV, Sm, kind, (const float*)d_qr, O);
+    if (nsmall > 0) {
+        hipLaunchKernelGGL(k_small_place, small_grid, small_block, 0, st, nsmall, SV, O);
+        // subtrees that outgrew their staging area are built again at their final address
+        if (small_ovf > 0)
+            hipLaunchKernelGGL((k_small<kSmallEmit>), small_grid, small_block, 0, st, nsmall, SV, Sm, kind,
+                               (const float*)d_qr, O);
+    }
     // per-scene results to the host: counts, bases, leaves, max level, root boxes
     uint32_t* h_res = h_tab;
     int rc = MNRT_OK;

@@ -101,3 +101,29 @@ def test_build_empty_input(orc, cuda):
     dt = api.build_tri(v, v, v)
     cuda.cuda.synchronize()
     assert dt.num_nodes == 1 and dt.num_refs == 0
+
+
+@pytest.mark.parametrize("cap", ["0,0", "1,2", "2,8"])
+def test_build_small_stage_fallback(orc, cuda, monkeypatch, cap):
+    # small subtrees that outgrow their staging area (forced by a tiny capacity: none,
+    # some or most fit) are emitted by the second pass: the trees stay identical
+    monkeypatch.setenv("MNRT_SMALL_STAGE_CAP", cap)
+    print(f"build with small staging capacity {cap}", flush=True)
+    sc = S.scene_small(subdiv=4)
+    _tri(orc, cuda, sc)
+    ppos, _, _ = S.random_surface_photons(sc, 20000, seed=4)
+    root = np.concatenate(sc.bounds()).astype(np.float32)
+    dt = api.build_points(ppos, api.default_params(1, root, sc.r_max()))
+    cuda.cuda.synchronize()
+    ot = orc.build(1, ppos, max_qr=sc.r_max(), root=root)
+    _cmp(orc, 1, dt, ot, sc.r_max())
+    # a batch of scenes through the same stage
+    scenes = [S.scene_sized(n, seed=30 + i) for i, n in enumerate((400, 3000, 90))]
+    v0 = np.concatenate([s.v0 for s in scenes])
+    v1 = np.concatenate([s.v1 for s in scenes])
+    v2 = np.concatenate([s.v2 for s in scenes])
+    off = np.cumsum([0] + [s.num_tris for s in scenes]).astype(np.uint32)
+    TB = api.build_tri_batch(v0, v1, v2, off)
+    cuda.cuda.synchronize()
+    for i, s in enumerate(scenes):
+        _cmp(orc, 0, TB[i], orc.build(0, s.v0, s.v1, s.v2))
