@@ -1552,7 +1552,9 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
             // the columns handed out below, each rounded to 16 bytes
             const uint64_t words = r4(A_) + 3 * r4(C_) + 2 * r4(3 * A_) + 9 * r4(A_) + 2 * r4(4 * A_) +
                                    2 * r4(2 * E_ + 1) + r4(22 * A_) +
-                                   scan_scratch_words((uint32_t)std::max<uint64_t>(2 * E_ + 1, 2 * A_)) + 16;
+                                   std::max<uint64_t>(scan_scratch_words((uint32_t)(2 * E_ + 1)),
+                                                      kScanSetMax * scan_scratch_words((uint32_t)(2 * A_))) +
+                                   16;
             lv_buf = pool.get<uint32_t>(words);
             if (!pool.ok) return MNRT_ERR_NOMEM;
         }
@@ -1617,10 +1619,13 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         MNRT_HIP_CHECK(scan_exclusive(fl, pos, 2 * ne, pos + 2 * ne, scr, st));
         hipLaunchKernelGGL(k_child_plan, blocks_for(2ull * na), dim3(256), 0, st, na, ne, A, P, pos,
                            prm->small_node_max, Cv);
-        MNRT_HIP_CHECK(scan_exclusive(Cv.sflag, Cv.sidx, 2 * na, counters + C_NSMALL_NEW, scr, st));
-        MNRT_HIP_CHECK(scan_exclusive(Cv.lflag, Cv.lidx, 2 * na, counters + C_NNEXT, scr, st));
-        MNRT_HIP_CHECK(scan_exclusive(Cv.scnt, Cv.soff, 2 * na, counters + C_ESMALL_NEW, scr, st));
-        MNRT_HIP_CHECK(scan_exclusive(Cv.lcnt, Cv.loff, 2 * na, counters + C_ENEXT, scr, st));
+        {
+            ScanSet cs{{Cv.sflag, Cv.lflag, Cv.scnt, Cv.lcnt},
+                       {Cv.sidx, Cv.lidx, Cv.soff, Cv.loff},
+                       {},
+                       {counters + C_NSMALL_NEW, counters + C_NNEXT, counters + C_ESMALL_NEW, counters + C_ENEXT}};
+            MNRT_HIP_CHECK(scan_exclusive_set(cs, 4, 2 * na, scr, st));
+        }
         hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, na, A, P, Cv, nsmall, nse, SV, An,
                            B, counters);
         hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, na, A, P, E, ll_idx, B, counters);

@@ -74,6 +74,62 @@ __global__ void k_scan_word(uint32_t* __restrict__ dst, const uint32_t* __restri
     if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src ? *src : 0u;
 }
 
+// Up to kScanSetMax independent scans of n elements each, run by the same launches
+// (blockIdx.y picks the scan).
+constexpr int kScanSetMax = 4;
+struct ScanSet {
+    const uint32_t* in[kScanSetMax];
+    uint32_t* out[kScanSetMax];
+    uint32_t* sums[kScanSetMax];
+    uint32_t* total[kScanSetMax];  // may be null
+};
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles_set(ScanSet S, uint32_t n) {
+    const uint32_t y = blockIdx.y;
+    __shared__ uint32_t s_wave[kScanBlock / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)tid * kScanPerThread;
+    const uint32_t* in = S.in[y];
+    uint32_t* out = S.out[y];
+    uint32_t v[kScanPerThread];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPerThread; ++j) {
+        size_t i = base + j;
+        v[j] = (i < n) ? in[i] : 0u;
+        local += v[j];
+    }
+    uint32_t incl = wave_incl_scan(local, lane);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    for (uint32_t w = 0; w < wave; ++w) wave_off += s_wave[w];
+    uint32_t run = wave_off + incl - local;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPerThread; ++j) {
+        size_t i = base + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+    if (tid == kScanBlock - 1) {
+        S.sums[y][blockIdx.x] = wave_off + incl;
+        if (S.total[y] && gridDim.x == 1) *S.total[y] = wave_off + incl;
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add_set(ScanSet S, uint32_t n, ScanSet offs) {
+    const uint32_t y = blockIdx.y;
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    const uint32_t add = offs.out[y][blockIdx.x];
+    if (S.total[y] && blockIdx.x == 0 && threadIdx.x == 0)
+        *S.total[y] = offs.out[y][gridDim.x - 1] + S.sums[y][gridDim.x - 1];
+    uint32_t* out = S.out[y];
+    for (uint32_t j = threadIdx.x; j < kScanTile; j += kScanBlock) {
+        size_t i = base + j;
+        if (i < n) out[i] += add;
+    }
+}
+
 // Scratch words needed by scan_exclusive for n elements.
 inline size_t scan_scratch_words(uint32_t n) {
     size_t words = 0;
@@ -103,6 +159,36 @@ inline hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(kScanBlock), 0, st, out, n, offs, (const uint32_t*)sums,
                        total);
+    return hipGetLastError();
+}
+
+// m (<= kScanSetMax) independent exclusive scans of n elements: out[y][i] = sum(in[y][0..i)),
+// *total[y] (may be null) = the sum.  scratch: m * scan_scratch_words(n) device words.
+inline hipError_t scan_exclusive_set(const ScanSet& S0, int m, uint32_t n, uint32_t* scratch, hipStream_t st) {
+    if (m <= 0 || m > kScanSetMax) return hipErrorInvalidValue;
+    if (n == 0) {
+        for (int y = 0; y < m; ++y)
+            if (S0.total[y]) hipLaunchKernelGGL(k_scan_word, dim3(1), dim3(1), 0, st, S0.total[y], (const uint32_t*)nullptr);
+        return hipGetLastError();
+    }
+    const uint32_t blocks = (n + kScanTile - 1) / kScanTile;
+    const size_t per = scan_scratch_words(n);
+    ScanSet S = S0, O{};
+    for (int y = 0; y < m; ++y) {
+        uint32_t* sc = scratch + (size_t)y * per;
+        S.sums[y] = sc;
+        O.in[y] = sc;
+        O.out[y] = sc + blocks + 1;
+        O.total[y] = nullptr;
+    }
+    hipLaunchKernelGGL(k_scan_tiles_set, dim3(blocks, m), dim3(kScanBlock), 0, st, S, n);
+    if (blocks == 1) return hipGetLastError();
+    // tile sums: one scan each, through the single-scan path (it has its own scratch)
+    for (int y = 0; y < m; ++y) {
+        hipError_t e = scan_exclusive(O.in[y], O.out[y], blocks, nullptr, scratch + (size_t)y * per + 2 * blocks + 2, st);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_scan_add_set, dim3(blocks, m), dim3(kScanBlock), 0, st, S, n, O);
     return hipGetLastError();
 }
 
