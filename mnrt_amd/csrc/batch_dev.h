@@ -80,6 +80,7 @@ struct KnnArgs {
     unsigned long long* acc;       // 3 fixed-point accumulators per photon (original index)
     double fix_scale;              // accumulator units per unit gradient (a power of two)
     float4* out_grad_clr;          // dLoss/d(diffuse colour) per query
+    const uint32_t* order;         // processing order of the queries (null: 0..n-1)
 };
 
 struct KnnScene {

@@ -130,3 +130,34 @@ def test_knn_per_query_radius_and_inverse_distance(orc, cuda):
         ei[s] = np.float32(0) + ai
     assert np.array_equal(rad.cpu().numpy().view(np.uint32), er.view(np.uint32))
     assert np.array_equal(inv2.cpu().numpy().view(np.uint32), ei.view(np.uint32))
+
+
+def test_knn_many_incoherent_queries(orc, cuda):
+    # more queries than the reordering threshold, in random order (as hits of diffuse
+    # bounce rays come): the launch processes them grouped by spatial cell; every set,
+    # radius and radiance is still bit-exact to the oracle, and radii agree
+    sc = S.scene_small(subdiv=4)
+    ppos, ppow, pdir = S.random_surface_photons(sc, 65536, seed=11)
+    qp, _, qd = S.random_surface_photons(sc, 70000, seed=12)
+    rng = np.random.default_rng(13)
+    nrm = qd.copy()
+    qdir = qd.copy()
+    qdir[:, :3] = -qd[:, :3]
+    infl = np.ones_like(qp)
+    infl[:, 3] = 0
+    clr = np.tile(np.array([0.7, 0.6, 0.5, 1.0], np.float32), (len(qp), 1))
+    q = dict(pos=qp, nrm=nrm, dir=qdir, infl=infl, clr=clr)
+    print(f"knn: {len(qp)} incoherent queries k=100", flush=True)
+    _run(orc, cuda, sc, ppos, ppow, pdir, q, 100, sc.r_max(), brute=False)
+    # radii of the same queries through mnrt_knn_radii
+    P = orc.build(1, ppos, max_qr=sc.r_max(), root=np.concatenate(sc.bounds()))
+    DP = api.tree_from_host(1, P.nodes, P.refs, P.root, ppos.shape[0], P.max_level)
+    api.attach_photon_records(DP, ppos, ppow, pdir)
+    sel = rng.permutation(len(qp))
+    r = api.knn_radii(DP, 100, sc.r_max(), qp[sel]).cpu().numpy()
+    nbr, cnt, _ = orc.knn(P, ppos, qp[sel], 100, sc.r_max())
+    full = cnt >= 100
+    d = ppos[nbr[full, 99], :3] - qp[sel][full, :3]
+    d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    assert np.array_equal(r[full], np.sqrt(d2.astype(np.float32)))
+    assert np.all(r[~full] == np.float32(sc.r_max()))

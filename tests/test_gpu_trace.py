@@ -97,3 +97,22 @@ def test_trace_empty_ray_set(orc, cuda):
     api.attach_tri_records(DT, sc.v0, sc.v1, sc.v2)
     tri, t, uv, _ = api.trace_closest(DT, o, o, sc.ray_epsilon())
     assert tri.numel() == 0
+
+
+def test_trace_misses_and_ragged_counts(orc, cuda):
+    # rays that miss the root box (started and finished at the fetch) mixed with hits,
+    # in counts that are not multiples of a wave or a workgroup; every lane of the
+    # persistent kernel takes several rays
+    sc = S.scene_small(subdiv=3)
+    rng = np.random.default_rng(17)
+    for n in (1, 63, 65, 1000, 70001):
+        o = np.zeros((n, 4), np.float32)
+        d = np.zeros((n, 4), np.float32)
+        inside = rng.random(n) < 0.5
+        o[:, :3] = np.where(inside[:, None], rng.uniform(-0.9, 0.9, size=(n, 3)), rng.uniform(2.0, 3.0, size=(n, 3)))
+        v = rng.normal(size=(n, 3))
+        v = v / np.linalg.norm(v, axis=1, keepdims=True)
+        v[~inside] = np.abs(v[~inside])  # outside rays point away from the box
+        d[:, :3] = v
+        print(f"trace: ragged n={n}", flush=True)
+        _check(orc, sc, o, d, cuda, brute_stride=max(1, n // 2000))
