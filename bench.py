@@ -11,6 +11,10 @@ k = 100 kNN radiance estimate at every hit):
   build the photon kd-tree              mnrt_build_points     (KDTreePoint + BuildTree)
   kNN radiance at all hits              mnrt_knn_gather       (KernelPMPhotonGather)
 
+The photon tree does not depend on the first three stages: by default a second host thread
+builds it on a second stream while they run (--serial: after them, on one stream); the
+kNN stage waits for both.
+
 Inputs are resident in HBM before the timed region (the photon map is traced once during
 set-up with mnrt_photons_generate).  `value` counts rays traced plus kNN queries answered
 per second of step time.  The roofline entry is for the kNN gather kernel; its
@@ -22,6 +26,7 @@ the reference algorithms) on the host cores, on a bounded sample.
 from __future__ import annotations
 
 import argparse
+import concurrent.futures
 import json
 import os
 import sys
@@ -305,6 +310,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-ray leg")
+    ap.add_argument("--serial", action="store_true",
+                    help="build the photon tree after the traces on the same stream (default: concurrently)")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -360,10 +367,25 @@ def main():
     stream = torch.cuda.current_stream()
     phases = ["build_tri", "trace_primary", "shade_bounce", "trace_bounce", "build_points", "knn"]
 
+    # The photon tree does not depend on the triangle tree or the rays: unless --serial, a
+    # second host thread builds it on a second stream while this one builds the triangle
+    # tree and traces (the library keeps its build workspace per host thread).
+    side = torch.cuda.Stream()
+    worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+
+    def photon_tree():
+        with torch.cuda.stream(side):
+            P = api.build_points(ppos, pparams)
+            api.attach_photon_records(P, ppos, ppow, pdir)
+            done = torch.cuda.Event()
+            done.record(side)
+        return P, done
+
     def step(record):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
         kev = []
         ev[0].record(stream)
+        pending = None if args.serial else worker.submit(photon_tree)
         T = api.build_tri(v0, v1, v2)
         ev[1].record(stream)
         tri, t, _, _ = api.trace_closest(T, go, gd, eps)
@@ -382,8 +404,12 @@ def main():
         else:
             ev[3].record(stream)
             ev[4].record(stream)
-        P = api.build_points(ppos, pparams)
-        api.attach_photon_records(P, ppos, ppow, pdir)
+        if pending is None:
+            P = api.build_points(ppos, pparams)
+            api.attach_photon_records(P, ppos, ppow, pdir)
+        else:
+            P, done = pending.result()
+            stream.wait_event(done)
         ev[5].record(stream)
         outs = []
         for q in queries:
@@ -435,6 +461,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene generator, traced photons)",
         "config": {"workload": workload, "triangles": sc.num_tris, "photons": ph["count"], "rays": rays,
+                   "photon_tree": "after the traces, one stream" if args.serial else
+                   "concurrent with the triangle tree and traces (second host thread and stream)",
                    "knn_queries": queries, "k": K, "parallelism": "single GPU"},
         "phase_ms": {p: round(v, 3) for p, v in phase_ms.items()},
         "trace_mrays_per_s": round(rays / (trace_ms * 1e-3) / 1e6, 2),

@@ -752,12 +752,12 @@ struct StageCap {
     __host__ __device__ uint32_t of(uint32_t n) const { return mul * n + add; }
 };
 
-enum : int { kSmallCount = 0, kSmallEmit = 1, kSmallStage = 2 };
+enum : int { kSmallStage = 0, kSmallEmit = 1 };
 
 struct SmallOut {
-    uint32_t* nsize;  // count / stage: nodes per small subtree
-    uint32_t* rsize;  // count / stage: references per small subtree
-    uint32_t* mlvl;   // count / stage: deepest level in the small subtree
+    uint32_t* nsize;  // stage: nodes per small subtree
+    uint32_t* rsize;  // stage: references per small subtree
+    uint32_t* mlvl;   // stage: deepest level in the small subtree
     const uint32_t* st_off;  // stage: first slot of each small root's staging area
     uint2* st_nodes;         // stage: node words, subtree-local addresses
     uint32_t* st_refs;       // stage: references, subtree-local
@@ -774,8 +774,8 @@ struct SmallOut {
     const uint32_t* ref_base;
 };
 
-// kMode kSmallCount: subtree sizes only; kSmallStage: sizes and the subtree written to
-// its staging area; kSmallEmit: written at its final address (only roots with st_ovf set
+// kMode kSmallStage: subtree sizes, and the subtree written to its staging area while it
+// fits; kSmallEmit: written at its final address (only roots with st_ovf set
 // when st_ovf is given).
 template <int kMode>
 __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, SmallView S, ElemView Sm, int kind,
@@ -855,7 +855,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     uint32_t* nodes = nullptr;
     uint32_t* refs = nullptr;
     uint32_t lim_nodes = 0u, lim_refs = 0u;
-    bool writing = kMode != kSmallCount;
+    bool writing = true;
     if (kMode == kSmallEmit) {
         nodes = O.nodes + 2ull * O.node_base[scene];
         refs = O.refs + O.ref_base[scene];
@@ -992,14 +992,12 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         ++addr;
         wave_sync();
     }
-    if (kMode != kSmallEmit && lane == 0) {
+    if (kMode == kSmallStage && lane == 0) {
         O.nsize[r] = addr;
         O.rsize[r] = rbase;
         O.mlvl[r] = maxlvl;
-        if (kMode == kSmallStage) {
-            O.st_ovf[r] = writing ? 0u : 1u;
-            if (!writing) atomicAdd(&O.counters[C_SMALL_OVF], 1u);
-        }
+        O.st_ovf[r] = writing ? 0u : 1u;
+        if (!writing) atomicAdd(&O.counters[C_SMALL_OVF], 1u);
     }
 }
 

@@ -127,3 +127,35 @@ def test_build_small_stage_fallback(orc, cuda, monkeypatch, cap):
     cuda.cuda.synchronize()
     for i, s in enumerate(scenes):
         _cmp(orc, 0, TB[i], orc.build(0, s.v0, s.v1, s.v2))
+
+
+def test_builds_from_two_host_threads(orc, cuda):
+    # a triangle tree and a photon tree built at the same time from two host threads on two
+    # streams (each thread has its own build workspace) equal the trees built one by one
+    import concurrent.futures
+
+    torch = cuda
+    sc = S.scene_small(subdiv=4)
+    ppos, _, _ = S.random_surface_photons(sc, 30000, seed=8)
+    root = np.concatenate(sc.bounds()).astype(np.float32)
+    params = api.default_params(1, root, sc.r_max())
+    v0, v1, v2, pp = (api.dev(a) for a in (sc.v0, sc.v1, sc.v2, ppos))
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+
+    def points():
+        with torch.cuda.stream(side):
+            P = api.build_points(pp, params)
+            side.synchronize()
+        return P
+
+    for _ in range(3):
+        with concurrent.futures.ThreadPoolExecutor(max_workers=1) as ex:
+            fut = ex.submit(points)
+            T = api.build_tri(v0, v1, v2)
+            P = fut.result()
+        torch.cuda.synchronize()
+        _cmp(orc, 0, T, orc.build(0, sc.v0, sc.v1, sc.v2))
+        _cmp(orc, 1, P, orc.build(1, ppos, max_qr=sc.r_max(), root=root), sc.r_max())
+        T.release()
+        P.release()
