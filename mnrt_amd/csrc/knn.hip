@@ -564,7 +564,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
 
 // Consecutive queries per wave: each after the first starts from the previous one's
 // k-th distance (WarmHint); neighbouring pixels' hits share most of their neighbours.
-constexpr uint32_t kKnnChain = 4;
+constexpr uint32_t kKnnChain = 8;
 
 template <int CAP, int kMode, int kSel>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_knn(KnnArgs A) {

@@ -136,9 +136,9 @@ __device__ __forceinline__ uint32_t mask_prefix(uint64_t m) {
 }
 
 // Query-time leaves of photon trees: subtrees with at most this many references (the kNN
-// kernel tests a leaf 64 photons per step; measured on 1M photons, k = 50 / 100: 16 and 32
-// are slowest, 256 and 512 fastest, 2048 slower again).
-constexpr uint32_t kQueryLeafRefs = 256;
+// kernel tests a leaf 64 photons per step; measured on 1M photons, k = 100, with the
+// warm-start chains of eight queries: 256 -> 512 saves 5-9 %, 1024 is slower again).
+constexpr uint32_t kQueryLeafRefs = 512;
 
 // Device memory of the library: a stream-ordered pool per device that keeps freed blocks
 // for reuse (release threshold: never), so repeated builds and queries do not map and
