@@ -777,15 +777,21 @@ struct SmallOut {
 // kMode kSmallStage: subtree sizes, and the subtree written to its staging area while it
 // fits; kSmallEmit: written at its final address (only roots with st_ovf set
 // when st_ovf is given).
-template <int kMode>
-__global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, SmallView S, ElemView Sm, int kind,
+// The element kind is a template argument so that LDS holds what the kind needs: point
+// elements have no upper bounds and 3n candidates (triangles: 6n), which halves the mask
+// arrays and lets more waves share a CU.
+template <int kMode, int kKind>
+__global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, SmallView S, ElemView Sm,
                                                             const float* __restrict__ scene_qr, SmallOut O) {
+    constexpr int kind = kKind;
+    constexpr int kCandMax = (kKind == MNRT_KIND_TRI) ? 384 : 192;
+    constexpr int kHiRows = (kKind == MNRT_KIND_TRI) ? 3 : 1;
     __shared__ float s_lo[kSmallWaves][3][64];
-    __shared__ float s_hi[kSmallWaves][3][64];
+    __shared__ float s_hi[kSmallWaves][kHiRows][64];
     __shared__ uint32_t s_idx[kSmallWaves][64];
-    __shared__ float s_cpos[kSmallWaves][384];
-    __shared__ uint64_t s_mL[kSmallWaves][384];
-    __shared__ uint64_t s_mR[kSmallWaves][384];
+    __shared__ float s_cpos[kSmallWaves][kCandMax];
+    __shared__ uint64_t s_mL[kSmallWaves][kCandMax];
+    __shared__ uint64_t s_mR[kSmallWaves][kCandMax];
     __shared__ SmallStackEnt s_stack[kSmallWaves][kSmallStack];
 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -793,7 +799,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     if (r >= nsmall) return;  // uniform per wave
     if (kMode == kSmallEmit && O.st_ovf && O.st_ovf[r] == 0u) return;
     float(*elo)[64] = s_lo[wave];
-    float(*ehi)[64] = s_hi[wave];
+    float(*ehi)[64] = (kKind == MNRT_KIND_TRI) ? s_hi[wave] : s_lo[wave];  // points: hi = lo
     uint32_t* eidx = s_idx[wave];
     float* cpos = s_cpos[wave];
     uint64_t* mL = s_mL[wave];
@@ -803,9 +809,11 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     const uint32_t first = S.first[r], n = S.cnt[r];
     if (lane < n) {
         float4 lo = Sm.lo[first + lane];
-        float4 hi = (kind == MNRT_KIND_TRI) ? Sm.hi[first + lane] : lo;
         elo[0][lane] = lo.x; elo[1][lane] = lo.y; elo[2][lane] = lo.z;
-        ehi[0][lane] = hi.x; ehi[1][lane] = hi.y; ehi[2][lane] = hi.z;
+        if (kKind == MNRT_KIND_TRI) {
+            float4 hi = Sm.hi[first + lane];
+            ehi[0][lane] = hi.x; ehi[1][lane] = hi.y; ehi[2][lane] = hi.z;
+        }
         eidx[lane] = Sm.idx[first + lane];
     }
     wave_sync();
@@ -1678,8 +1686,12 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         hipLaunchKernelGGL(k_small_caps, blocks_for(nsmall), dim3(256), 0, st, nsmall, (const uint32_t*)SV.cnt, st_cap,
                            st_off);
         MNRT_HIP_CHECK(scan_exclusive(st_off, st_off, nsmall, nullptr, st_scr, st));
-        hipLaunchKernelGGL((k_small<kSmallStage>), small_grid, small_block, 0, st, nsmall, SV, Sm, kind,
-                           (const float*)d_qr, O);
+        if (kind == MNRT_KIND_TRI)
+            hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_TRI>), small_grid, small_block, 0, st, nsmall, SV, Sm,
+                               (const float*)d_qr, O);
+        else
+            hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_POINT>), small_grid, small_block, 0, st, nsmall, SV,
+                               Sm, (const float*)d_qr, O);
     }
     // ---- layout ----
     LayoutView Lv{};
@@ -1742,9 +1754,12 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     if (nsmall > 0) {
         hipLaunchKernelGGL(k_small_place, small_grid, small_block, 0, st, nsmall, SV, O);
         // subtrees that outgrew their staging area are built again at their final address
-        if (small_ovf > 0)
-            hipLaunchKernelGGL((k_small<kSmallEmit>), small_grid, small_block, 0, st, nsmall, SV, Sm, kind,
+        if (small_ovf > 0 && kind == MNRT_KIND_TRI)
+            hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_TRI>), small_grid, small_block, 0, st, nsmall, SV, Sm,
                                (const float*)d_qr, O);
+        else if (small_ovf > 0)
+            hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_POINT>), small_grid, small_block, 0, st, nsmall, SV,
+                               Sm, (const float*)d_qr, O);
     }
     // per-scene results to the host: counts, bases, leaves, max level, root boxes
     uint32_t* h_res = h_tab;

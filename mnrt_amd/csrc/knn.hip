@@ -27,7 +27,7 @@
 #include <cstring>
 
 #include "batch_dev.h"
-#include "scan.h"
+#include "order.h"
 
 namespace mnrt {
 
@@ -782,130 +782,6 @@ __global__ __launch_bounds__(256) void k_fix_to_float(uint32_t np, const unsigne
 constexpr int kKnnSelDefault = kSelWindow;
 
 // Backward launches: queries in chunks of floor(kPairCap / k) per wave (at most 8).
-// ---- query order -----------------------------------------------------------------------
-// Incoherent queries (e.g. hits of diffuse bounce rays) are processed grouped by an 18-bit
-// Morton cell (6 bits per axis) of the photon map's root box, cells in Morton order, so a
-// wave's consecutive queries are close (warm start, shared leaves in cache).  A counting
-// sort; the order inside a cell is whatever the atomics give, which changes no result:
-// every query's set is exact and its sums are order-free.  Queries that are coherent
-// already (fewer than half of the consecutive pairs farther apart than a cell diagonal,
-// e.g. hits of camera rays in tile order) keep their order: within a cell the counting
-// sort would scatter them.
-constexpr uint32_t kOrderAxisBits = 6;
-constexpr uint32_t kOrderCells = 1u << (3 * kOrderAxisBits);
-constexpr uint32_t kOrderMinQueries = 1u << 15;  // smaller launches keep the caller's order
-
-struct OrderBox {
-    float lo[3], scale[3];  // cell coordinate = (p - lo) * scale, clamped to [0, 63]
-    float diag2;            // squared cell diagonal
-};
-
-__device__ __forceinline__ uint32_t spread3_6(uint32_t v) {  // bit i of v -> bit 3i
-    uint32_t r = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < kOrderAxisBits; ++b) r |= ((v >> b) & 1u) << (3 * b);
-    return r;
-}
-
-// Consecutive query pairs farther apart than a cell diagonal; one atomic per workgroup of
-// kFarPerThread * 256 pairs.
-constexpr uint32_t kFarPerThread = 16;
-
-__global__ __launch_bounds__(256) void k_query_far(uint32_t n, const float4* __restrict__ pos, float diag2,
-                                                   uint32_t* __restrict__ far_pairs) {
-    int far = 0;
-    const uint32_t i0 = blockIdx.x * (256u * kFarPerThread) + threadIdx.x;
-    for (uint32_t t = 0; t < kFarPerThread; ++t) {
-        const uint32_t i = i0 + t * 256u;
-        if (i + 1 < n) {
-            const float4 a = pos[i], b = pos[i + 1];
-            const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
-            far += !(dx * dx + dy * dy + dz * dz <= diag2) ? 1 : 0;
-        }
-    }
-    __shared__ int s_far[4];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) far += __shfl_xor(far, o, 64);
-    if ((threadIdx.x & 63u) == 0) s_far[threadIdx.x >> 6] = far;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int tot = s_far[0] + s_far[1] + s_far[2] + s_far[3];
-        if (tot > 0) atomicAdd(far_pairs, (uint32_t)tot);
-    }
-}
-
-__device__ __forceinline__ bool queries_coherent(uint32_t n, const uint32_t* far_pairs) {
-    return 2ull * *far_pairs < (uint64_t)n;
-}
-
-// Cells and their histogram, for incoherent queries only (coherent queries keep their
-// order, and their many queries per cell would make the histogram atomics collide).
-__global__ __launch_bounds__(256) void k_query_cells(uint32_t n, const float4* __restrict__ pos, OrderBox B,
-                                                     const uint32_t* __restrict__ far_pairs,
-                                                     uint32_t* __restrict__ cell, uint32_t* __restrict__ hist) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || queries_coherent(n, far_pairs)) return;
-    const float4 p = pos[i];
-    const float c[3] = {p.x, p.y, p.z};
-    uint32_t code = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        // fmaxf / fminf return the other operand for NaN: a NaN coordinate lands in cell 0
-        const float t = fminf(fmaxf((c[a] - B.lo[a]) * B.scale[a], 0.0f), (float)((1u << kOrderAxisBits) - 1));
-        code |= spread3_6((uint32_t)t) << a;
-    }
-    cell[i] = code;
-    atomicAdd(&hist[code], 1u);
-}
-
-__global__ __launch_bounds__(256) void k_query_place(uint32_t n, const uint32_t* __restrict__ cell,
-                                                     const uint32_t* __restrict__ far_pairs,
-                                                     uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (queries_coherent(n, far_pairs)) order[i] = i;
-    else order[atomicAdd(&cursor[cell[i]], 1u)] = i;
-}
-
-// Processing order for n queries at d_pos4 on the photon map (null below
-// kOrderMinQueries); *buf is the allocation to free with pool_free after the launch.
-int query_order(const mnrt_tree* pmap, uint32_t n, const float* d_pos4, hipStream_t st, const uint32_t** order,
-                void** buf) {
-    *order = nullptr;
-    *buf = nullptr;
-    if (n < kOrderMinQueries) return MNRT_OK;
-    const size_t scan_words = scan_scratch_words(kOrderCells);
-    const size_t words = 2ull * n + 2ull * kOrderCells + 4 + scan_words;
-    MNRT_HIP_CHECK(pool_alloc(buf, words * 4, st));
-    uint32_t* cell = (uint32_t*)*buf;
-    uint32_t* ord = cell + n;
-    uint32_t* hist = ord + n;
-    uint32_t* cursor = hist + kOrderCells;
-    uint32_t* far_pairs = cursor + kOrderCells;
-    uint32_t* scr = far_pairs + 4;
-    OrderBox B;
-    B.diag2 = 0.0f;
-    for (int a = 0; a < 3; ++a) {
-        const float ext = pmap->root_max[a] - pmap->root_min[a];
-        B.lo[a] = pmap->root_min[a];
-        B.scale[a] = ext > 0.0f ? (float)(1u << kOrderAxisBits) / ext : 0.0f;
-        const float c = ext / (float)(1u << kOrderAxisBits);
-        B.diag2 += c * c;
-    }
-    MNRT_HIP_CHECK(hipMemsetAsync(hist, 0, (size_t)kOrderCells * 4, st));
-    MNRT_HIP_CHECK(hipMemsetAsync(far_pairs, 0, 4, st));
-    hipLaunchKernelGGL(k_query_far, dim3((n + 256 * kFarPerThread - 1) / (256 * kFarPerThread)), dim3(256), 0, st, n,
-                       (const float4*)d_pos4, B.diag2, far_pairs);
-    hipLaunchKernelGGL(k_query_cells, dim3((n + 255) / 256), dim3(256), 0, st, n, (const float4*)d_pos4, B,
-                       (const uint32_t*)far_pairs, cell, hist);
-    MNRT_HIP_CHECK(scan_exclusive(hist, cursor, kOrderCells, nullptr, scr, st));
-    hipLaunchKernelGGL(k_query_place, dim3((n + 255) / 256), dim3(256), 0, st, n, (const uint32_t*)cell,
-                       (const uint32_t*)far_pairs, cursor, ord);
-    MNRT_HIP_CHECK(hipGetLastError());
-    *order = ord;
-    return MNRT_OK;
-}
-
 int launch_knn_bwd(const KnnArgs& A, hipStream_t st) {
     const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(8u, (uint32_t)kPairCap / A.k));
     const uint32_t chunks = (A.n + chunk - 1) / chunk;
@@ -1013,7 +889,7 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
     A.out_nbr = d_out_nbr;
     A.out_count = d_out_count;
     void* obuf = nullptr;
-    rc = mnrt::query_order(pmap, n, d_pos4, st, &A.order, &obuf);
+    rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &A.order, &obuf);
     if (rc != MNRT_OK) return rc;
     float4* tmp = nullptr;
     if (d_idx_sp) {
@@ -1058,7 +934,7 @@ extern "C" int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, ui
     A.out_radius = d_out_radius;
     hipStream_t st = (hipStream_t)stream;
     void* obuf = nullptr;
-    rc = mnrt::query_order(pmap, n, d_pos4, st, &A.order, &obuf);
+    rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &A.order, &obuf);
     if (rc != MNRT_OK) return rc;
     rc = mnrt::launch_knn<mnrt::kModeRadii>(A, st);
     if (obuf && mnrt::pool_free(obuf, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
@@ -1109,7 +985,7 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
     }
     const uint32_t* order = nullptr;
     void* obuf = nullptr;
-    if (rc == MNRT_OK) rc = mnrt::query_order(pmap, n, d_pos4, st, &order, &obuf);
+    if (rc == MNRT_OK) rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &order, &obuf);
     if (rc == MNRT_OK && scale > 0.0) {
         mnrt::KnnArgs A{};
         A.order = order;
