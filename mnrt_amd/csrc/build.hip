@@ -792,6 +792,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     __shared__ float s_cpos[kSmallWaves][kCandMax];
     __shared__ uint64_t s_mL[kSmallWaves][kCandMax];
     __shared__ uint64_t s_mR[kSmallWaves][kCandMax];
+    __shared__ uint8_t s_cax[kSmallWaves][kCandMax];
     __shared__ SmallStackEnt s_stack[kSmallWaves][kSmallStack];
 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -804,6 +805,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     float* cpos = s_cpos[wave];
     uint64_t* mL = s_mL[wave];
     uint64_t* mR = s_mR[wave];
+    uint8_t* cax = s_cax[wave];
     SmallStackEnt* stack = s_stack[wave];
 
     const uint32_t first = S.first[r], n = S.cnt[r];
@@ -852,6 +854,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
             if (R) mr |= 1ull << j;
         }
         cpos[c] = sp;
+        cax[c] = (uint8_t)axis;
         mL[c] = ml;
         mR[c] = mr;
     }
@@ -920,22 +923,30 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
                 float area = 2.0f * (ext[0] * ext[1] + ext[0] * ext[2] + ext[1] * ext[2]);
                 denom = 1.0f / area;
             }
+            // the candidate's axis comes from the table and the node's bounds on it by
+            // selects: indexing e.lo / e.hi with a lane-varying axis, or deriving the axis
+            // from c by division, costs more than the cost evaluation itself
             for (uint32_t c = lane; c < ncand; c += 64) {
-                uint32_t a = (c % per) / n;
-                float s = cpos[c];
-                if (!(e.lo[a] < s && s < e.hi[a])) continue;
-                uint32_t nl = (uint32_t)__popcll(mL[c] & e.mask);
-                uint32_t nr = (uint32_t)__popcll(mR[c] & e.mask);
+                const uint32_t a = cax[c];
+                const float s = cpos[c];
+                const float lo_a = a == 0 ? e.lo[0] : (a == 1 ? e.lo[1] : e.lo[2]);
+                const float hi_a = a == 0 ? e.hi[0] : (a == 1 ? e.hi[1] : e.hi[2]);
+                if (!(lo_a < s && s < hi_a)) continue;
+                const uint64_t bl = mL[c] & e.mask, br = mR[c] & e.mask;
+                const uint32_t nl = (uint32_t)__popc((uint32_t)bl) + (uint32_t)__popc((uint32_t)(bl >> 32));
+                const uint32_t nr = (uint32_t)__popc((uint32_t)br) + (uint32_t)__popc((uint32_t)(br >> 32));
                 if (nl == 0 || nr == 0) continue;
-                uint32_t o1 = (a + 1) % 3, o2 = (a + 2) % 3;
+                // the other two axes' extents, (a + 1) % 3 and (a + 2) % 3
+                const float x1 = a == 0 ? ext[1] : (a == 1 ? ext[2] : ext[0]);
+                const float x2 = a == 0 ? ext[2] : (a == 1 ? ext[0] : ext[1]);
                 float cost;
                 if (kind == MNRT_KIND_POINT) {
-                    float vl = (ext[o1] + qr2) * (s - e.lo[a] + qr2) * (ext[o2] + qr2);
-                    float vr = (ext[o1] + qr2) * (e.hi[a] - s + qr2) * (ext[o2] + qr2);
+                    float vl = (x1 + qr2) * (s - lo_a + qr2) * (x2 + qr2);
+                    float vr = (x1 + qr2) * (hi_a - s + qr2) * (x2 + qr2);
                     cost = ((float)nl * vl + (float)nr * vr) * denom + kCostTraverse;
                 } else {
-                    float al = 2.0f * (ext[o1] * ext[o2] + (s - e.lo[a]) * (ext[o1] + ext[o2]));
-                    float ar = 2.0f * (ext[o1] * ext[o2] + (e.hi[a] - s) * (ext[o1] + ext[o2]));
+                    float al = 2.0f * (x1 * x2 + (s - lo_a) * (x1 + x2));
+                    float ar = 2.0f * (x1 * x2 + (hi_a - s) * (x1 + x2));
                     cost = ((float)nl * al + (float)nr * ar) * denom + kCostTraverse;
                 }
                 if (cost < best_cost) {
@@ -975,7 +986,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
             rbase += cnt;
             continue;
         }
-        const uint32_t a = (best % per) / n;
+        const uint32_t a = cax[best];
         const float s = cpos[best];
         if (writing && lane == 0) {
             nodes[2 * addr] = __float_as_uint(s);
@@ -984,12 +995,14 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         if (lane == 0) {
             SmallStackEnt R = e, L = e;
             R.mask = mR[best] & e.mask;
-            R.lo[a] = s;
+            for (uint32_t k = 0; k < 3; ++k) {  // the plane bounds the split axis (selects, not indexing)
+                R.lo[k] = (a == k) ? s : e.lo[k];
+                L.hi[k] = (a == k) ? s : e.hi[k];
+            }
             R.level = e.level + 1;
             R.patch = addr;
             R.axis = a;
             L.mask = mL[best] & e.mask;
-            L.hi[a] = s;
             L.level = e.level + 1;
             L.patch = kNone;
             L.axis = 0;
