@@ -736,9 +736,10 @@ constexpr int kSmallStack = MNRT_MAX_LEVEL + 4;
 struct SmallStackEnt {
     uint64_t mask;
     float lo[3], hi[3];
-    uint32_t level;
-    uint32_t patch;  // node whose right-child word gets this entry's address, or kNone
-    uint32_t axis;
+    uint32_t lvl_ax;  // level | axis << 30 (axis: of the split whose right child this is)
+    uint32_t patch;   // node whose right-child word gets this entry's address, or kNone
+    __device__ uint32_t level() const { return lvl_ax & 0x3FFFFFFFu; }
+    __device__ uint32_t axis() const { return lvl_ax >> 30; }
 };
 
 // Staging area of a small root with n elements: stage_mul * n + stage_add node words and
@@ -774,6 +775,16 @@ struct SmallOut {
     const uint32_t* ref_base;
 };
 
+// Returns `old` with lane `lane` replaced by the wave-uniform `v` (v_writelane_b32, which
+// ignores exec; with an SGPR source the lane select has to be m0).
+__device__ __forceinline__ uint32_t write_lane(uint32_t v, int lane, uint32_t old) {
+    uint32_t r;
+    asm("v_writelane_b32 %0, %1, m0"
+        : "=v"(r)
+        : "s"(__builtin_amdgcn_readfirstlane(v)), "{m0}"(__builtin_amdgcn_readfirstlane(lane)), "0"(old));
+    return r;
+}
+
 // kMode kSmallStage: subtree sizes, and the subtree written to its staging area while it
 // fits; kSmallEmit: written at its final address (only roots with st_ovf set
 // when st_ovf is given).
@@ -785,23 +796,16 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
                                                             const float* __restrict__ scene_qr, SmallOut O) {
     constexpr int kind = kKind;
     constexpr int kCandMax = (kKind == MNRT_KIND_TRI) ? 384 : 192;
-    constexpr int kHiRows = (kKind == MNRT_KIND_TRI) ? 3 : 1;
-    __shared__ float s_lo[kSmallWaves][3][64];
-    __shared__ float s_hi[kSmallWaves][kHiRows][64];
-    __shared__ uint32_t s_idx[kSmallWaves][64];
     __shared__ float s_cpos[kSmallWaves][kCandMax];
     __shared__ uint64_t s_mL[kSmallWaves][kCandMax];
     __shared__ uint64_t s_mR[kSmallWaves][kCandMax];
     __shared__ uint8_t s_cax[kSmallWaves][kCandMax];
     __shared__ SmallStackEnt s_stack[kSmallWaves][kSmallStack];
 
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
     const uint32_t r = blockIdx.x * kSmallWaves + wave;
     if (r >= nsmall) return;  // uniform per wave
     if (kMode == kSmallEmit && O.st_ovf && O.st_ovf[r] == 0u) return;
-    float(*elo)[64] = s_lo[wave];
-    float(*ehi)[64] = (kKind == MNRT_KIND_TRI) ? s_hi[wave] : s_lo[wave];  // points: hi = lo
-    uint32_t* eidx = s_idx[wave];
     float* cpos = s_cpos[wave];
     uint64_t* mL = s_mL[wave];
     uint64_t* mR = s_mR[wave];
@@ -809,54 +813,83 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     SmallStackEnt* stack = s_stack[wave];
 
     const uint32_t first = S.first[r], n = S.cnt[r];
-    if (lane < n) {
-        float4 lo = Sm.lo[first + lane];
-        elo[0][lane] = lo.x; elo[1][lane] = lo.y; elo[2][lane] = lo.z;
-        if (kKind == MNRT_KIND_TRI) {
-            float4 hi = Sm.hi[first + lane];
-            ehi[0][lane] = hi.x; ehi[1][lane] = hi.y; ehi[2][lane] = hi.z;
-        }
-        eidx[lane] = Sm.idx[first + lane];
+    // lane j holds element j (n <= 64)
+    const bool valid = lane < n;
+    float4 mlo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mhi = mlo;
+    uint32_t midx = 0;
+    if (valid) {
+        mlo = Sm.lo[first + lane];
+        mhi = (kKind == MNRT_KIND_TRI) ? Sm.hi[first + lane] : mlo;
+        midx = Sm.idx[first + lane];
     }
-    wave_sync();
+    const bool mleft = plane_goes_left(midx);
     // Candidates axis-major, minimums then maximums (kernel_CreateSplitCandidates,
-    // kdtree.cu:1035-1082); masks as kernel_InitSplitMasks (kdtree.cu:1110-1251).
+    // kdtree.cu:1035-1082), c = isMax * 3n + axis * n + i; masks as kernel_InitSplitMasks
+    // (kdtree.cu:1110-1251).  One candidate per step: every lane compares its element with
+    // the plane, the comparisons' lane masks combine into the candidate's masks (element
+    // i itself, and elements lying in the plane, by the rules below), and lane i keeps
+    // them for its own candidate (writelane); each lane writes its candidates at the end
+    // of the axis.
+    //   triangles: L = min < s, R = s < max; the candidate's own element goes right of
+    //              its minimum and left of its maximum; a flat element in the plane goes
+    //              left if plane_goes_left, else right;
+    //   points:    L = p < s, R = s < p; a point in the plane goes left if
+    //              plane_goes_left, else right.
     const uint32_t per = 3 * n;
     const uint32_t ncand = (kind == MNRT_KIND_TRI) ? 2 * per : per;
-    for (uint32_t c = lane; c < ncand; c += 64) {
-        uint32_t isMax = c / per, rem = c - isMax * per, axis = rem / n, i = rem - axis * n;
-        float sp = isMax ? ehi[axis][i] : elo[axis][i];
-        uint64_t ml = 0, mr = 0;
-        for (uint32_t j = 0; j < n; ++j) {
-            bool L, R;
-            if (kind == MNRT_KIND_TRI) {
-                float mn = elo[axis][j], mx = ehi[axis][j];
-                L = mn < sp;
-                R = sp < mx;
-                if (j == i) {
-                    if (isMax) L = true;
-                    else R = true;
-                } else if (mn == mx && mn == sp) {
-                    if (plane_goes_left(eidx[j])) L = true;
-                    else R = true;
-                }
-            } else {
-                float pv = elo[axis][j];
-                if (plane_goes_left(eidx[j])) {
-                    L = pv <= sp;
-                    R = sp < pv;
+    const uint64_t vmask = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+    const uint64_t leftm = __builtin_amdgcn_ballot_w64(mleft) & vmask;
+    for (uint32_t ax = 0; ax < 3; ++ax) {
+        const float mn = ax == 0 ? mlo.x : (ax == 1 ? mlo.y : mlo.z);
+        const float mx = ax == 0 ? mhi.x : (ax == 1 ? mhi.y : mhi.z);
+        const uint64_t flat = (kind == MNRT_KIND_TRI) ? __builtin_amdgcn_ballot_w64(mn == mx) & vmask : vmask;
+        // this lane's minimum (0) and maximum (1) candidate: masks as 32-bit halves
+        uint32_t l0lo = 0, l0hi = 0, r0lo = 0, r0hi = 0, l1lo = 0, l1hi = 0, r1lo = 0, r1hi = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t self = 1ull << i;
+#pragma unroll
+            for (int isMax = 0; isMax < (kind == MNRT_KIND_TRI ? 2 : 1); ++isMax) {
+                const float sp = __uint_as_float(
+                    __builtin_amdgcn_readlane(__float_as_uint(isMax ? mx : mn), (int)i));
+                const uint64_t lt = __builtin_amdgcn_ballot_w64(mn < sp);
+                const uint64_t gt = __builtin_amdgcn_ballot_w64(sp < mx);
+                const uint64_t eq = __builtin_amdgcn_ballot_w64(mn == sp) & flat;
+                uint64_t ml, mr;
+                if (kind == MNRT_KIND_TRI) {
+                    const uint64_t planar = eq & ~self;
+                    ml = (lt | (planar & leftm) | (isMax ? self : 0ull)) & vmask;
+                    mr = (gt | (planar & ~leftm) | (isMax ? 0ull : self)) & vmask;
                 } else {
-                    L = pv < sp;
-                    R = sp <= pv;
+                    ml = (lt | (eq & leftm)) & vmask;
+                    mr = (gt | (eq & ~leftm)) & vmask;
+                }
+                if (isMax) {
+                    l1lo = write_lane((uint32_t)ml, (int)i, l1lo);
+                    l1hi = write_lane((uint32_t)(ml >> 32), (int)i, l1hi);
+                    r1lo = write_lane((uint32_t)mr, (int)i, r1lo);
+                    r1hi = write_lane((uint32_t)(mr >> 32), (int)i, r1hi);
+                } else {
+                    l0lo = write_lane((uint32_t)ml, (int)i, l0lo);
+                    l0hi = write_lane((uint32_t)(ml >> 32), (int)i, l0hi);
+                    r0lo = write_lane((uint32_t)mr, (int)i, r0lo);
+                    r0hi = write_lane((uint32_t)(mr >> 32), (int)i, r0hi);
                 }
             }
-            if (L) ml |= 1ull << j;
-            if (R) mr |= 1ull << j;
         }
-        cpos[c] = sp;
-        cax[c] = (uint8_t)axis;
-        mL[c] = ml;
-        mR[c] = mr;
+        if (valid) {
+            const uint32_t c0 = ax * n + lane;
+            cpos[c0] = mn;
+            cax[c0] = (uint8_t)ax;
+            mL[c0] = ((uint64_t)l0hi << 32) | l0lo;
+            mR[c0] = ((uint64_t)r0hi << 32) | r0lo;
+            if (kind == MNRT_KIND_TRI) {
+                const uint32_t c1 = per + c0;
+                cpos[c1] = mx;
+                cax[c1] = (uint8_t)ax;
+                mL[c1] = ((uint64_t)l1hi << 32) | l1lo;
+                mR[c1] = ((uint64_t)r1hi << 32) | r1lo;
+            }
+        }
     }
 
     uint32_t addr = 0, rbase = 0, maxlvl = 0;
@@ -888,9 +921,8 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
         float4 lo = S.lo[r], hi = S.hi[r];
         e.lo[0] = lo.x; e.lo[1] = lo.y; e.lo[2] = lo.z;
         e.hi[0] = hi.x; e.hi[1] = hi.y; e.hi[2] = hi.z;
-        e.level = S.level[r];
+        e.lvl_ax = S.level[r];
         e.patch = kNone;
-        e.axis = 0;
         stack[0] = e;
     }
     sp = 1;
@@ -907,12 +939,12 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
             }
             writing = false;  // staging area full: counted on, emitted later
         }
-        if (writing && e.patch != kNone && lane == 0) nodes[2 * e.patch + 1] = (e.axis << 30) | addr;
-        maxlvl = max(maxlvl, e.level);
+        if (writing && e.patch != kNone && lane == 0) nodes[2 * e.patch + 1] = (e.axis() << 30) | addr;
+        maxlvl = max(maxlvl, e.level());
         const uint32_t cnt = (uint32_t)__popcll(e.mask);
         float best_cost = kInf;
         uint32_t best = kNone;
-        if (cnt > 0 && e.level < MNRT_MAX_LEVEL) {
+        if (cnt > 0 && e.level() < MNRT_MAX_LEVEL) {
             float ext[3] = {e.hi[0] - e.lo[0], e.hi[1] - e.lo[1], e.hi[2] - e.lo[2]};
             float denom;
             if (kind == MNRT_KIND_POINT) {
@@ -979,7 +1011,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
                 }
                 if (lane < n && ((e.mask >> lane) & 1ull)) {
                     uint64_t below = (lane == 0) ? 0ull : (e.mask & ((1ull << lane) - 1ull));
-                    refs[rbase + (uint32_t)__popcll(below)] = eidx[lane];
+                    refs[rbase + (uint32_t)__popcll(below)] = midx;
                 }
             }
             ++addr;
@@ -999,13 +1031,11 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
                 R.lo[k] = (a == k) ? s : e.lo[k];
                 L.hi[k] = (a == k) ? s : e.hi[k];
             }
-            R.level = e.level + 1;
+            R.lvl_ax = (e.level() + 1) | (a << 30);
             R.patch = addr;
-            R.axis = a;
             L.mask = mL[best] & e.mask;
-            L.level = e.level + 1;
+            L.lvl_ax = e.level() + 1;
             L.patch = kNone;
-            L.axis = 0;
             stack[sp] = R;
             stack[sp + 1] = L;
         }
