@@ -329,12 +329,9 @@ __global__ __launch_bounds__(kChunk) void k_tight(uint32_t max_chunks, const uin
             v[5] = fenc(hi.z);
         }
 #pragma unroll
-        for (uint32_t o = 32; o > 0; o >>= 1) {
+        for (int k = 0; k < 3; ++k) v[k] = wave_min_u32(v[k]);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) v[k] = min(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
-#pragma unroll
-            for (int k = 3; k < 6; ++k) v[k] = max(v[k], (uint32_t)__shfl_xor(v[k], o, 64));
-        }
+        for (int k = 3; k < 6; ++k) v[k] = wave_max_u32(v[k]);
         if (lane == 0)
             for (int k = 0; k < 6; ++k) s_red[k][wave] = v[k];
         __syncthreads();
@@ -986,14 +983,24 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
                     best = c;
                 }
             }
-#pragma unroll
-            for (uint32_t o = 32; o > 0; o >>= 1) {
-                float oc = __shfl_xor(best_cost, o, 64);
-                uint32_t ob = (uint32_t)__shfl_xor((int)best, o, 64);
+            // lowest cost, ties to the lower candidate (an order-independent minimum)
+            auto take = [&](float oc, uint32_t ob) {
                 if (oc < best_cost || (oc == best_cost && ob < best)) {
                     best_cost = oc;
                     best = ob;
                 }
+            };
+#define MNRT_ARGMIN_DPP(ctrl) take(__uint_as_float(dpp_mov<ctrl>(__float_as_uint(best_cost))), dpp_mov<ctrl>(best))
+            MNRT_ARGMIN_DPP(kDppPairSwap);
+            MNRT_ARGMIN_DPP(kDppQuadSwap);
+            MNRT_ARGMIN_DPP(kDppHalfMirror);
+            MNRT_ARGMIN_DPP(kDppRowMirror);
+#undef MNRT_ARGMIN_DPP
+            {
+                const uint32_t cb = __float_as_uint(best_cost), bb = best;
+                best_cost = __uint_as_float(row_lane(cb, 0));
+                best = row_lane(bb, 0);
+                for (int row = 1; row < 4; ++row) take(__uint_as_float(row_lane(cb, row)), row_lane(bb, row));
             }
         }
         if (best == kNone || !(best_cost < (float)cnt)) {

@@ -69,8 +69,7 @@ __device__ __forceinline__ float wave_fixed_sum(const float (&v)[RK]) {
     float m = 0.0f;
 #pragma unroll
     for (int t = 0; t < RK; ++t) m = fmaxf(m, fabsf(v[t]));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    m = wave_max_f32(m);
     if (!(m > 0.0f)) return 0.0f;
     int ex;
     (void)frexpf(m, &ex);
@@ -78,8 +77,7 @@ __device__ __forceinline__ float wave_fixed_sum(const float (&v)[RK]) {
     long long acc = 0;
 #pragma unroll
     for (int t = 0; t < RK; ++t) acc += __double2ll_rn(ldexp((double)v[t], sh));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    acc = wave_sum_i64(acc);
     return (float)ldexp((double)acc, -sh);
 }
 
@@ -124,12 +122,8 @@ __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint3
         // every tie is kept: the threshold is the largest tie
         uint32_t lo = 0;
 #pragma unroll
-        for (int t = 0; t < R; ++t) {
-            uint32_t v = ((uint32_t)(key[t] >> 32) == hi) ? (uint32_t)key[t] : 0u;
-            for (uint32_t o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-            lo = max(lo, v);
-        }
-        return ((uint64_t)hi << 32) | uniform(lo);
+        for (int t = 0; t < R; ++t) lo = max(lo, ((uint32_t)(key[t] >> 32) == hi) ? (uint32_t)key[t] : 0u);
+        return ((uint64_t)hi << 32) | wave_max_u32(lo);
     }
     uint32_t lo = 0;
     for (int b = 31; b >= 0; --b) {
@@ -155,12 +149,10 @@ __device__ __forceinline__ uint32_t wave_window_hi(const uint64_t (&key)[R], uin
 #pragma unroll
         for (int t = 0; t < R; ++t) {
             const uint32_t h = (uint32_t)(key[t] >> 32);
-            uint32_t v = h != 0xFFFFFFFFu ? h : 0u;
-            for (uint32_t o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-            m = max(m, v);
+            m = max(m, h != 0xFFFFFFFFu ? h : 0u);
         }
         *upper_out = cnt;
-        return uniform(m);
+        return wave_max_u32(m);
     }
     // [prefix, prefix + width) holds the k-th high word; upper = count(h < prefix + width)
     uint32_t prefix = 0, upper = cnt;
@@ -423,9 +415,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         // the k-th distance: the largest d2 of the set
         uint32_t hi = 0;
         for (uint32_t i = lane; i < found; i += 64) hi = max(hi, (uint32_t)(keys[i] >> 32));
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) hi = max(hi, (uint32_t)__shfl_xor((int)hi, o, 64));
-        r2 = __uint_as_float(hi);
+        r2 = __uint_as_float(wave_max_u32(hi));
     }
 
     if (hint) {

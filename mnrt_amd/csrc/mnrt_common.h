@@ -110,6 +110,74 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wave-wide reductions; every lane of the wave must be active.  DPP moves within rows of
+// 16 lanes (swap in pairs, swap pairs in quads, half-row mirror, row mirror) leave the row's
+// reduction in each of its lanes, and the four rows (lanes 0, 16, 32, 48) are combined by
+// readlane, so the result is wave-uniform.  For associative, commutative operations (max,
+// min, integer sums, lexicographic (cost, index) minimum) the result does not depend on
+// the combining order.
+constexpr int kDppPairSwap = 0xB1;    // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuadSwap = 0x4E;    // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t row_lane(uint32_t v, int row) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 16 * row);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    v = max(v, dpp_mov<kDppPairSwap>(v));
+    v = max(v, dpp_mov<kDppQuadSwap>(v));
+    v = max(v, dpp_mov<kDppHalfMirror>(v));
+    v = max(v, dpp_mov<kDppRowMirror>(v));
+    return max(max(row_lane(v, 0), row_lane(v, 1)), max(row_lane(v, 2), row_lane(v, 3)));
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = min(v, dpp_mov<kDppPairSwap>(v));
+    v = min(v, dpp_mov<kDppQuadSwap>(v));
+    v = min(v, dpp_mov<kDppHalfMirror>(v));
+    v = min(v, dpp_mov<kDppRowMirror>(v));
+    return min(min(row_lane(v, 0), row_lane(v, 1)), min(row_lane(v, 2), row_lane(v, 3)));
+}
+
+__device__ __forceinline__ float wave_max_f32(float v) {
+    v = fmaxf(v, __uint_as_float(dpp_mov<kDppPairSwap>(__float_as_uint(v))));
+    v = fmaxf(v, __uint_as_float(dpp_mov<kDppQuadSwap>(__float_as_uint(v))));
+    v = fmaxf(v, __uint_as_float(dpp_mov<kDppHalfMirror>(__float_as_uint(v))));
+    v = fmaxf(v, __uint_as_float(dpp_mov<kDppRowMirror>(__float_as_uint(v))));
+    const uint32_t b = __float_as_uint(v);
+    return fmaxf(fmaxf(__uint_as_float(row_lane(b, 0)), __uint_as_float(row_lane(b, 1))),
+                 fmaxf(__uint_as_float(row_lane(b, 2)), __uint_as_float(row_lane(b, 3))));
+}
+
+template <int kCtrl>
+__device__ __forceinline__ long long dpp_mov64(long long v) {
+    const uint32_t lo = dpp_mov<kCtrl>((uint32_t)(unsigned long long)v);
+    const uint32_t hi = dpp_mov<kCtrl>((uint32_t)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+    v += dpp_mov64<kDppPairSwap>(v);
+    v += dpp_mov64<kDppQuadSwap>(v);
+    v += dpp_mov64<kDppHalfMirror>(v);
+    v += dpp_mov64<kDppRowMirror>(v);
+    long long t = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t lo = row_lane((uint32_t)(unsigned long long)v, r);
+        const uint32_t hi = row_lane((uint32_t)((unsigned long long)v >> 32), r);
+        t += (long long)(((unsigned long long)hi << 32) | lo);
+    }
+    return t;
+}
+
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
