@@ -101,9 +101,23 @@ enum : int { kSelSort = 0, kSelRadix = 1, kSelWindow = 2 };
 // the largest v with count(key < v) < k.
 template <int R>
 __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint32_t k) {
-    // high word (d2 bits) first, then the low word (photon index) among the ties
-    uint32_t hi = 0;
-    for (int b = 31; b >= 0; --b) {
+    // high word (d2 bits) first, then the low word (photon index) among the ties.  Every
+    // key's high word lies in [hmin, hmax], so the answer shares their common leading
+    // bits and the search starts below the highest bit in which they differ.
+    uint32_t hmin = 0xFFFFFFFFu, hmax = 0u;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        if (key[t] != ~0ull) {
+            hmin = min(hmin, (uint32_t)(key[t] >> 32));
+            hmax = max(hmax, (uint32_t)(key[t] >> 32));
+        }
+    }
+    hmin = wave_min_u32(hmin);
+    hmax = wave_max_u32(hmax);
+    const uint32_t differ = hmin ^ hmax;
+    const int top = differ ? 31 - __clz(differ) : -1;
+    uint32_t hi = (top < 0) ? hmin : (top >= 31) ? 0u : (hmin & ~((2u << top) - 1u));
+    for (int b = top; b >= 0; --b) {
         const uint32_t probe = hi | (1u << b);
         uint32_t c = 0;
 #pragma unroll
