@@ -299,6 +299,73 @@ def adjoint_leg(args, torch, api, S):
             "bwd_fwd_ratio": round(b / f, 3)}
 
 
+def inverse_leg(args, torch, api, S):
+    """Batched inverse lighting: 64 scenes (4 lights each; triangles log-U[1k,50k], photons
+    log-U[4k,256k], primary-hit queries log-U[16k,256k], seed 5), target radiance from random
+    per-light RGB emission in [0.5,2]; Adam (lr 1e-2) on per-light emission and per-triangle
+    albedo from emission 1 and the traced albedo.  A step = photon reweighting (torch),
+    record refresh, one batched kNN launch for every scene, one batched adjoint launch, the
+    torch chain to the parameters and the Adam update."""
+    from mnrt_amd.inverse import BatchedInverseLighting
+
+    t0 = time.time()
+    NS, K = args.inverse_scenes, 50
+    rng = np.random.default_rng(5)
+
+    def lu(lo, hi):
+        return np.exp(rng.uniform(np.log(lo), np.log(hi), NS))
+
+    ntri, nph, nq = lu(1000, 50000).astype(int), lu(4096, 262144).astype(int), lu(16384, 262144).astype(int)
+    scs = [S.scene_sized(int(n), seed=1000 + i, num_lights=4) for i, n in enumerate(ntri)]
+    dims = []
+    for q in nq:
+        w = 8 * max(1, int(round(np.sqrt(q) / 8)))
+        dims.append((w, max(1, int(round(q / w)))))
+    P = BatchedInverseLighting(scs, nph, dims, k=K, seed=7)
+    L = int(P.loff[-1])
+    g = torch.Generator(device="cpu").manual_seed(5)
+    e_true = (0.5 + 1.5 * torch.rand((L, 3), generator=g)).cuda()
+    target = P.radiance(e_true, P.albedo0)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    # one step's batched launches timed with events
+    e = torch.ones((L, 3), device="cuda", requires_grad=True)
+    a = P.albedo0.clone().requires_grad_(True)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd, bwd = [], []
+    for it in range(3):
+        P.bufs["io_rad4"].zero_()
+        ev[0].record(stream)
+        P.Q.run(4)
+        ev[1].record(stream)
+        P.Q.run(8)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        if it:
+            fwd.append(ev[0].elapsed_time(ev[1]))
+            bwd.append(ev[1].elapsed_time(ev[2]))
+    torch.cuda.synchronize()
+    t1 = time.time()
+    e_fit, a_fit, losses = P.fit(target, steps=args.inverse_steps, lr=1e-2)
+    torch.cuda.synchronize()
+    fit_s = time.time() - t1
+    err0 = float((1.0 - e_true).abs().mean())
+    err = float((e_fit - e_true).abs().mean())
+    alb_dev = float((a_fit - P.albedo0).abs().mean())
+    out = {"workload": (f"{NS} scenes x 4 lights: triangles log-U[1k,50k], photons log-U[4k,256k], "
+                        "queries log-U[16k,256k] (seed 5), k=50; Adam lr 1e-2 on emission + albedo"),
+           "scenes": NS, "triangles": int(P.toff[-1]), "photons": int(P.poff[-1]), "queries": P.num_queries,
+           "steps": args.inverse_steps, "ms_per_step": round(fit_s / max(1, args.inverse_steps) * 1e3, 3),
+           "knn_fwd_ms": round(float(np.median(fwd)), 3), "knn_bwd_ms": round(float(np.median(bwd)), 3),
+           "bwd_fwd_ratio": round(float(np.median(bwd)) / float(np.median(fwd)), 3),
+           "loss_first": losses[0], "loss_last": losses[-1],
+           "emission_mean_abs_err": {"start": round(err0, 4), "end": round(err, 4)},
+           "albedo_mean_abs_change": round(alb_dev, 5), "setup_s": round(setup_s, 1)}
+    P.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +381,9 @@ def main():
                     help="build the photon tree after the traces on the same stream (default: concurrently)")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=2)
+    ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-lighting leg")
+    ap.add_argument("--inverse-scenes", type=int, default=64)
+    ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 256 MiB device-to-device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -545,6 +615,8 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args, torch, api, S)
         out["bwd_fwd_ratio"] = out["adjoint"]["bwd_fwd_ratio"]
+    if not args.no_inverse:
+        out["inverse"] = inverse_leg(args, torch, api, S)
     print(json.dumps(out))
 
 

@@ -142,7 +142,9 @@ int mnrt_build_points_batch(const float* d_pos4, const uint32_t* h_offsets, uint
                             const float* h_max_query_radius, mnrt_tree* out, void* stream);
 /* Records of a batch (trees from one batched build, inputs concatenated as for the
  * build): one shared record array owned by trees[0]; photon batches are collapsed with
- * max_refs = 256 like mnrt_tree_attach_photon_records. */
+ * max_refs = 256 like mnrt_tree_attach_photon_records.  Called again on photon trees whose
+ * records it attached, it rewrites the records in place (new photon powers, say) and
+ * keeps the arrays, so batches made from these trees stay valid. */
 int mnrt_tree_attach_tri_records_batch(mnrt_tree* trees, uint32_t num_scenes, const float* d_v0,
                                        const float* d_v1, const float* d_v2,
                                        const uint32_t* h_offsets, void* stream);
@@ -429,12 +431,25 @@ typedef struct mnrt_batch_io {
     float* d_io_rad4;       /* kNN: radiance accumulated per shading point */
     float* d_out_r2;        /* optional */
     uint32_t* d_out_nbr;    /* optional, k per slot */
-    uint32_t* d_out_count;  /* optional */
+    uint32_t* d_out_count;  /* optional for MNRT_PHASE_KNN; MNRT_PHASE_KNN_BWD reads it */
+    /* kNN adjoint (MNRT_PHASE_KNN_BWD), the batched mnrt_knn_gather_bwd: */
+    const float* d_grad_rad4; /* dLoss/dradiance per slot (float4, w ignored) */
+    float* d_grad_pow4;       /* out: dLoss/dpower per photon of the concatenation of the
+                                 scenes' photon arrays (scene s from the sum of the photon
+                                 counts of scenes 0..s-1, original photon order) */
+    float* d_grad_clr4;       /* optional out: dLoss/d(diffuse colour) per slot */
 } mnrt_batch_io;
 
 #define MNRT_PHASE_TRACE 1u
 #define MNRT_PHASE_SHADE 2u
 #define MNRT_PHASE_KNN 4u
+/* Adjoint of MNRT_PHASE_KNN for every scene (reads the forward's d_out_r2 / d_out_count and
+ * the shading points; recomputes the neighbour sets).  Per scene, d_grad_pow4 and
+ * d_grad_clr4 equal mnrt_knn_gather_bwd on that scene's d_sp_count[s] shading points bit for
+ * bit: each scene keeps its own fixed-point unit, fixed on the device from its own largest
+ * weight and query count, so the phase needs no host synchronisation and can be captured.
+ * The first run (or capture) allocates the batch's accumulators (24 bytes per photon). */
+#define MNRT_PHASE_KNN_BWD 8u
 
 /* Copies the per-scene tables to the device (the trees must outlive the batch). */
 int mnrt_batch_create(const mnrt_batch_desc* desc, mnrt_batch** out);

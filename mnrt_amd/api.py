@@ -170,7 +170,7 @@ def build_points_batch(pos, offsets, params: N.BuildParams | None = None, root_b
 
 def attach_photon_records_batch(B: TreeBatch, pos, power, dirs):
     pos, power, dirs = dev(pos), dev(power), dev(dirs)
-    B._keep += [pos, power, dirs]
+    B._keep = [pos, power, dirs]  # a second call rewrites the records in place
     N.check(N.load().mnrt_tree_attach_photon_records_batch(B.ref(), B.num_scenes, N.ptr(pos), N.ptr(power),
                                                            N.ptr(dirs), B.offsets_ref(), N.stream_handle()),
             "mnrt_tree_attach_photon_records_batch")
@@ -526,6 +526,14 @@ class QueryBatch:
             self.h, ctypes.cast(self._tri_arr, ctypes.POINTER(N.Tree)),
             ctypes.cast(self._pm_arr, ctypes.POINTER(N.Tree)) if self._pm_arr is not None else None),
             "mnrt_batch_update_trees")
+
+    @property
+    def photon_offsets(self) -> np.ndarray:
+        """(S+1) offsets of the scenes' photons in the concatenated gradient d_grad_pow4."""
+        off = np.zeros(self.S + 1, np.int64)
+        if self._pm_arr is not None:
+            off[1:] = np.cumsum([self._pm_arr[s].num_elems for s in range(self.S)])
+        return off
 
     def make_io(self, **bufs):
         """BatchIO from named device tensors (names as mnrt_batch_io without d_)."""

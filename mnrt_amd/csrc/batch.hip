@@ -46,8 +46,16 @@ struct mnrt_batch {
     uint32_t* d_pos = nullptr;         // total_slots + 1
     uint32_t* d_scan = nullptr;
     size_t tiles_cap = 0;
-    hipGraph_t graph[8] = {};
-    hipGraphExec_t exec[8] = {};
+    // kNN adjoint state (allocated on the first MNRT_PHASE_KNN_BWD run or capture)
+    std::vector<uint32_t> h_poff;  // ns + 1: photon offsets of the scenes
+    void* d_bwd = nullptr;
+    size_t bwd_cap = 0;            // photons the accumulators hold
+    unsigned long long* d_acc = nullptr;
+    uint32_t* d_wmax = nullptr;
+    double* d_scale = nullptr;
+    uint32_t* d_poff = nullptr;
+    hipGraph_t graph[16] = {};
+    hipGraphExec_t exec[16] = {};
     hipStream_t cap_stream = nullptr;
 };
 
@@ -72,8 +80,9 @@ bool valid_tuning(const mnrt_batch& b, const mnrt_batch_tuning& t) {
     return true;
 }
 
-void drop_graphs(mnrt_batch& b) {
-    for (int i = 0; i < 8; ++i) {
+void drop_graphs(mnrt_batch& b, uint32_t with_phase = 0) {
+    for (uint32_t i = 0; i < 16; ++i) {
+        if (with_phase && !(i & with_phase)) continue;
         if (b.exec[i]) (void)hipGraphExecDestroy(b.exec[i]);
         if (b.graph[i]) (void)hipGraphDestroy(b.graph[i]);
         b.exec[i] = nullptr;
@@ -186,11 +195,68 @@ int run_knn(mnrt_batch& b, const mnrt_batch_io& io, hipStream_t st) {
                             b.tun.knn_cap, b.tun.knn_waves, b.tun.knn_select, b.tun.knn_grid, st);
 }
 
+// Accumulators of the adjoint phase for the current photon counts (synchronous; never
+// called while a stream is being captured).  Graphs of the adjoint phase are dropped when
+// the accumulators move.
+int ensure_bwd(mnrt_batch& b) {
+    if (!b.has_knn) return MNRT_ERR_STATE;
+    const uint32_t ns = b.ns;
+    const size_t np = b.h_poff[ns];
+    if (!b.d_bwd || np > b.bwd_cap) {
+        MNRT_HIP_CHECK(hipDeviceSynchronize());
+        drop_graphs(b, MNRT_PHASE_KNN_BWD);
+        if (b.d_bwd) MNRT_HIP_CHECK(hipFree(b.d_bwd));
+        b.d_bwd = nullptr;
+        const size_t cap = std::max<size_t>(np, 1);
+        const size_t o_wmax = (24 * cap + 255) & ~(size_t)255;
+        const size_t o_scale = o_wmax + ((4 * (size_t)ns + 255) & ~(size_t)255);
+        const size_t o_poff = o_scale + ((8 * (size_t)ns + 255) & ~(size_t)255);
+        const size_t bytes = o_poff + 4 * ((size_t)ns + 1);
+        if (hipMalloc(&b.d_bwd, bytes) != hipSuccess) {
+            b.d_bwd = nullptr;
+            b.bwd_cap = 0;
+            return MNRT_ERR_NOMEM;
+        }
+        char* base = (char*)b.d_bwd;
+        b.d_acc = (unsigned long long*)base;
+        b.d_wmax = (uint32_t*)(base + o_wmax);
+        b.d_scale = (double*)(base + o_scale);
+        b.d_poff = (uint32_t*)(base + o_poff);
+        b.bwd_cap = cap;
+    }
+    MNRT_HIP_CHECK(hipMemcpy(b.d_poff, b.h_poff.data(), 4 * ((size_t)ns + 1), hipMemcpyHostToDevice));
+    return MNRT_OK;
+}
+
+int run_knn_bwd(mnrt_batch& b, const mnrt_batch_io& io, hipStream_t st) {
+    if (!b.has_knn) return MNRT_ERR_STATE;
+    if (!b.d_bwd) return MNRT_ERR_STATE;
+    if (!io.d_sp_pos4 || !io.d_sp_nrm4 || !io.d_sp_dir4 || !io.d_sp_infl4 || !io.d_sp_clr4 || !io.d_out_r2 ||
+        !io.d_out_count || !io.d_grad_rad4 || !io.d_grad_pow4)
+        return MNRT_ERR_ARG;
+    KnnArgs G{};
+    G.k = b.k;
+    G.n = b.total_slots;
+    G.pos = (const float4*)io.d_sp_pos4;
+    G.nrm = (const float4*)io.d_sp_nrm4;
+    G.dir = (const float4*)io.d_sp_dir4;
+    G.infl = (const float4*)io.d_sp_infl4;
+    G.clr = (const float4*)io.d_sp_clr4;
+    G.grad = (const float4*)io.d_grad_rad4;
+    G.acc = b.d_acc;
+    G.out_grad_clr = (float4*)io.d_grad_clr4;
+    return launch_knn_bwd_batch(G, b.d_knn, b.d_slot_begin, b.ns, io.d_sp_count, b.d_poff, b.h_poff[b.ns],
+                                io.d_out_r2, io.d_out_count, b.d_wmax, b.d_scale, b.tun.parts,
+                                b.d_tparts + 2 * kMaxParts, b.d_tparts + 3 * kMaxParts, b.d_work + kMaxParts,
+                                b.tun.knn_grid, io.d_grad_pow4, st);
+}
+
 int run_phases(mnrt_batch& b, const mnrt_batch_io& io, uint32_t phases, hipStream_t st) {
     int rc = MNRT_OK;
     if (phases & MNRT_PHASE_TRACE) rc = run_trace(b, io, st);
     if (rc == MNRT_OK && (phases & MNRT_PHASE_SHADE)) rc = run_shade(b, io, st);
     if (rc == MNRT_OK && (phases & MNRT_PHASE_KNN)) rc = run_knn(b, io, st);
+    if (rc == MNRT_OK && (phases & MNRT_PHASE_KNN_BWD)) rc = run_knn_bwd(b, io, st);
     return rc;
 }
 
@@ -231,6 +297,7 @@ int fill_scenes(mnrt_batch& b, const mnrt_tree* tri, const mnrt_tree* pm, const 
     const uint32_t ns = b.ns;
     const bool knn = b.has_knn;
     uint32_t max_level = 0;
+    b.h_poff.assign(ns + 1, 0);
     for (uint32_t s = 0; s < ns; ++s) {
         const mnrt_tree& t = tri[s];
         if (t.kind != MNRT_KIND_TRI || !t.d_nodes || !t.d_recs || t.num_nodes == 0) return MNRT_ERR_STATE;
@@ -260,6 +327,8 @@ int fill_scenes(mnrt_batch& b, const mnrt_tree* tri, const mnrt_tree* pm, const 
             }
             kq.slot_begin = b.h_slot[s];
             kq.slot_end = b.h_slot[s + 1];
+            if ((uint64_t)b.h_poff[s] + p.num_elems > 0xFFFFFFFFull) return MNRT_ERR_CAPACITY;
+            b.h_poff[s + 1] = b.h_poff[s] + p.num_elems;
         }
     }
     b.max_level = max_level;
@@ -358,6 +427,7 @@ extern "C" int mnrt_batch_destroy(mnrt_batch* b) {
     int rc = MNRT_OK;
     if (b->cap_stream && hipStreamDestroy(b->cap_stream) != hipSuccess) rc = MNRT_ERR_HIP;
     if (b->d_buf && hipFree(b->d_buf) != hipSuccess) rc = MNRT_ERR_HIP;
+    if (b->d_bwd && hipFree(b->d_bwd) != hipSuccess) rc = MNRT_ERR_HIP;
     delete b;
     return rc;
 }
@@ -388,25 +458,37 @@ extern "C" int mnrt_batch_update_trees(mnrt_batch* b, const mnrt_tree* tri_trees
     const uint32_t ns = b->ns;
     MNRT_HIP_CHECK(hipMemcpy(b->d_trace, b->h_trace.data(), sizeof(TraceScene) * ns, hipMemcpyHostToDevice));
     MNRT_HIP_CHECK(hipMemcpy(b->d_knn, b->h_knn.data(), sizeof(KnnScene) * ns, hipMemcpyHostToDevice));
+    if (b->d_bwd) return ensure_bwd(*b);  // new photon offsets (and room, if they grew)
     return MNRT_OK;
 }
 
 extern "C" int mnrt_batch_run(mnrt_batch* b, const mnrt_batch_io* io, uint32_t phases, void* stream) {
-    if (!b || !io || phases == 0 || phases > 7) return MNRT_ERR_ARG;
-    if (b->total_slots == 0) return MNRT_OK;
+    if (!b || !io || phases == 0 || phases > 15) return MNRT_ERR_ARG;
+    if ((phases & MNRT_PHASE_KNN_BWD) && !b->d_bwd) {
+        int rc = ensure_bwd(*b);
+        if (rc != MNRT_OK) return rc;
+    }
+    // no slots: only the adjoint has work (zero photon gradients)
+    if (b->total_slots == 0) phases &= MNRT_PHASE_KNN_BWD;
+    if (phases == 0) return MNRT_OK;
     return run_phases(*b, *io, phases, (hipStream_t)stream);
 }
 
 extern "C" int mnrt_batch_capture(mnrt_batch* b, const mnrt_batch_io* io, uint32_t phases, void* stream) {
-    if (!b || !io || phases == 0 || phases > 7) return MNRT_ERR_ARG;
+    if (!b || !io || phases == 0 || phases > 15) return MNRT_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     MNRT_HIP_CHECK(hipStreamSynchronize(st));
+    if ((phases & MNRT_PHASE_KNN_BWD) && !b->d_bwd) {
+        int rc = ensure_bwd(*b);
+        if (rc != MNRT_OK) return rc;
+    }
     if (b->exec[phases]) (void)hipGraphExecDestroy(b->exec[phases]);
     if (b->graph[phases]) (void)hipGraphDestroy(b->graph[phases]);
     b->exec[phases] = nullptr;
     b->graph[phases] = nullptr;
     MNRT_HIP_CHECK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-    int rc = b->total_slots ? run_phases(*b, *io, phases, b->cap_stream) : MNRT_OK;
+    const uint32_t run = b->total_slots ? phases : (phases & MNRT_PHASE_KNN_BWD);
+    int rc = run ? run_phases(*b, *io, run, b->cap_stream) : MNRT_OK;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
     if (rc != MNRT_OK || e != hipSuccess) {
@@ -424,7 +506,7 @@ extern "C" int mnrt_batch_capture(mnrt_batch* b, const mnrt_batch_io* io, uint32
 }
 
 extern "C" int mnrt_batch_replay(mnrt_batch* b, uint32_t phases, void* stream) {
-    if (!b || phases == 0 || phases > 7) return MNRT_ERR_ARG;
+    if (!b || phases == 0 || phases > 15) return MNRT_ERR_ARG;
     if (!b->exec[phases]) return MNRT_ERR_STATE;
     MNRT_HIP_CHECK(hipGraphLaunch(b->exec[phases], (hipStream_t)stream));
     return MNRT_OK;

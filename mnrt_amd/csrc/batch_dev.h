@@ -119,6 +119,11 @@ int launch_knn_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* s
                      const uint32_t* count, uint32_t chunk, uint32_t num_parts, const uint32_t* part_begin,
                      const uint32_t* part_end, uint32_t* work, uint32_t cap, uint32_t waves, uint32_t sel,
                      uint32_t grid, hipStream_t st);
+int launch_knn_bwd_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* slot_begin, uint32_t ns,
+                         const uint32_t* count, const uint32_t* photon_begin, uint32_t total_photons,
+                         const float* fwd_r2, const uint32_t* fwd_count, uint32_t* wmax, double* scale,
+                         uint32_t num_parts, const uint32_t* part_begin, const uint32_t* part_end, uint32_t* work,
+                         uint32_t grid, float* grad_pow4, hipStream_t st);
 int launch_shade_batch(const ShadeBatchArgs& A, uint32_t* flags, uint32_t* scan_tmp, hipStream_t st);
 
 }  // namespace mnrt

@@ -200,6 +200,8 @@ struct PairSink {
     uint32_t* npairs;  // LDS word, wave-uniform
     long long* wtab;   // 3 per slot
     uint32_t slot;
+    uint32_t pbase;    // added to the photon index (offset of the scene's photons in a batch)
+    double fix_scale;  // accumulator units per unit gradient for this query's scene
 };
 
 // Warm start for consecutive queries of one wave on one tree: every one of the previous
@@ -502,9 +504,10 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
                                             g.z * (((c4.w * kInvPi) * iz) * in4.z), 0.0f);
         // fixed-point adds are associative, so the per-photon sums do not depend on the
         // order in which queries arrive
-        long long fx = __double2ll_rn((double)wx * A.fix_scale);
-        long long fy = __double2ll_rn((double)wy * A.fix_scale);
-        long long fz = __double2ll_rn((double)wz * A.fix_scale);
+        const double fs = kPairs ? sink->fix_scale : A.fix_scale;
+        long long fx = __double2ll_rn((double)wx * fs);
+        long long fy = __double2ll_rn((double)wy * fs);
+        long long fz = __double2ll_rn((double)wz * fs);
         if (kPairs) {
             if (fx == 0 && fy == 0 && fz == 0) return;
             if (lane == 0) {
@@ -520,7 +523,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
                 if (i < found) {
                     const float4 pw = Tr.recs[2ull * kpos[i] + 1];
                     front = dot3(nq, oct_decode(__float_as_uint(pw.w))) < 0.0f;
-                    key = ((keys[i] & 0xFFFFFFFFull) << 8) | sink->slot;
+                    key = (((keys[i] & 0xFFFFFFFFull) + sink->pbase) << 8) | sink->slot;
                 }
                 const uint64_t m = __ballot(front);
                 if (front) sink->pairs[np + mask_prefix(m)] = key;
@@ -656,6 +659,46 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
 // per-pair atomics; the integer sums are the same in any order.
 constexpr int kPairCap = 256;
 
+// Sorts a wave's (photon << 8 | slot) pairs by photon in LDS and adds each photon's summed
+// fixed-point weights with one atomic per channel.
+__device__ __forceinline__ void merge_pairs(uint64_t* pairs, uint32_t np, const long long* w,
+                                            unsigned long long* acc, uint32_t lane) {
+    if (np == 0) return;
+    // sort the pairs by (photon, slot); pad to a power of two with keys above every pair
+    uint32_t size = 64;
+    while (size < np) size <<= 1;
+    for (uint32_t t = np + lane; t < size; t += 64) pairs[t] = ~0ull;
+    wave_sync();
+    for (uint32_t k2 = 2; k2 <= size; k2 <<= 1) {
+        for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = lane; t < (size >> 1); t += 64) {
+                const uint32_t i = 2 * j * (t / j) + (t % j), l = i + j;
+                const uint64_t a = pairs[i], b = pairs[l];
+                if ((a > b) == ((i & k2) == 0)) {
+                    pairs[i] = b;
+                    pairs[l] = a;
+                }
+            }
+            wave_sync();
+        }
+    }
+    // one lane per run of equal photons: sum its queries' weights, one atomic per channel
+    for (uint32_t i = lane; i < np; i += 64) {
+        const uint64_t p = pairs[i] >> 8;
+        if (i > 0 && (pairs[i - 1] >> 8) == p) continue;
+        long long sx = 0, sy = 0, sz = 0;
+        for (uint32_t j = i; j < np && (pairs[j] >> 8) == p; ++j) {
+            const uint32_t sl = (uint32_t)(pairs[j] & 0xFFu);
+            sx += w[3 * sl + 0];
+            sy += w[3 * sl + 1];
+            sz += w[3 * sl + 2];
+        }
+        if (sx) atomicAdd(&acc[3 * p + 0], (unsigned long long)sx);
+        if (sy) atomicAdd(&acc[3 * p + 1], (unsigned long long)sy);
+        if (sz) atomicAdd(&acc[3 * p + 2], (unsigned long long)sz);
+    }
+}
+
 template <int CAP, int kSel>
 __global__ __launch_bounds__(256) void k_knn_bwd_chunks(KnnArgs A, uint32_t chunk) {
     __shared__ uint64_t s_key[kWavesPerBlock][CAP];
@@ -678,47 +721,12 @@ __global__ __launch_bounds__(256) void k_knn_bwd_chunks(KnnArgs A, uint32_t chun
     WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
     for (uint32_t j = (uint32_t)j0; j < j1; ++j) {
         const uint32_t q = A.order ? ld_uniform(A.order, j) : j;
-        const PairSink sink{pairs, &s_np[wave], s_w[wave], j - (uint32_t)j0};
+        const PairSink sink{pairs, &s_np[wave], s_w[wave], j - (uint32_t)j0, 0u, A.fix_scale};
         knn_query<CAP, kModeBackward, kSel, true>(A, Tr, q, lane, s_key[wave], s_pos[wave], s_snode[wave],
                                                   s_soff[wave], &sink, &hint);
         wave_sync();
     }
-    const uint32_t np = uniform(s_np[wave]);
-    if (np == 0) return;
-    // sort the pairs by (photon, slot); pad to a power of two with keys above every pair
-    uint32_t size = 64;
-    while (size < np) size <<= 1;
-    for (uint32_t t = np + lane; t < size; t += 64) pairs[t] = ~0ull;
-    wave_sync();
-    for (uint32_t k2 = 2; k2 <= size; k2 <<= 1) {
-        for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
-            for (uint32_t t = lane; t < (size >> 1); t += 64) {
-                const uint32_t i = 2 * j * (t / j) + (t % j), l = i + j;
-                const uint64_t a = pairs[i], b = pairs[l];
-                if ((a > b) == ((i & k2) == 0)) {
-                    pairs[i] = b;
-                    pairs[l] = a;
-                }
-            }
-            wave_sync();
-        }
-    }
-    // one lane per run of equal photons: sum its queries' weights, one atomic per channel
-    const long long* w = s_w[wave];
-    for (uint32_t i = lane; i < np; i += 64) {
-        const uint64_t p = pairs[i] >> 8;
-        if (i > 0 && (pairs[i - 1] >> 8) == p) continue;
-        long long sx = 0, sy = 0, sz = 0;
-        for (uint32_t j = i; j < np && (pairs[j] >> 8) == p; ++j) {
-            const uint32_t sl = (uint32_t)(pairs[j] & 0xFFu);
-            sx += w[3 * sl + 0];
-            sy += w[3 * sl + 1];
-            sz += w[3 * sl + 2];
-        }
-        if (sx) atomicAdd(&A.acc[3 * p + 0], (unsigned long long)sx);
-        if (sy) atomicAdd(&A.acc[3 * p + 1], (unsigned long long)sy);
-        if (sz) atomicAdd(&A.acc[3 * p + 2], (unsigned long long)sz);
-    }
+    merge_pairs(pairs, uniform(s_np[wave]), s_w[wave], A.acc, lane);
 }
 
 // Adds per-query radiance into slots given by a slot index per query (final gathering,
@@ -780,6 +788,158 @@ __global__ __launch_bounds__(256) void k_fix_to_float(uint32_t np, const unsigne
     out[p] = make_float4((float)((double)(long long)acc[3ull * p] * inv_scale),
                          (float)((double)(long long)acc[3ull * p + 1] * inv_scale),
                          (float)((double)(long long)acc[3ull * p + 2] * inv_scale), 0.0f);
+}
+
+// --- batched backward (one launch sequence for every scene of a batch) ---------------
+
+// Per-scene weight bounds: slot g of scene s (g - slot_begin[s] < count[s]) adds its
+// bound to wmax[s]; a wave whose slots lie in one scene reduces first.
+__global__ __launch_bounds__(256) void k_bwd_wmax_batch(uint32_t n, const uint32_t* __restrict__ slot_begin,
+                                                        uint32_t ns, const uint32_t* __restrict__ count,
+                                                        const float4* __restrict__ grad,
+                                                        const float4* __restrict__ clr,
+                                                        const float4* __restrict__ infl,
+                                                        const float* __restrict__ r2,
+                                                        const uint32_t* __restrict__ nbr_count,
+                                                        uint32_t* __restrict__ wmax_bits) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    float m = 0.0f;
+    uint32_t s = 0;
+    if (g < n) {
+        s = seg_of(slot_begin, ns, g);
+        const uint32_t cnt = count ? count[s] : slot_begin[s + 1] - slot_begin[s];
+        if (g - slot_begin[s] < cnt) {
+            float4 gr = grad[g], in4 = infl[g];
+            float4 c4 = clr ? clr[g] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            float rr = r2[g];
+            bool gated = (clr && nbr_count[g] < 8) || !(rr > 0.0f);
+            if (!gated) {
+                float den = rr * kPi;
+                float wx = (gr.x * (((c4.w * c4.x) * kInvPi) * in4.x)) / den;
+                float wy = (gr.y * (((c4.w * c4.y) * kInvPi) * in4.y)) / den;
+                float wz = (gr.z * (((c4.w * c4.z) * kInvPi) * in4.z)) / den;
+                m = fmaxf(fabsf(wx), fmaxf(fabsf(wy), fabsf(wz)));
+                if (!(m <= kInf)) m = kInf;
+            }
+        }
+    }
+    const uint32_t s0 = uniform(s);
+    if (__ballot(s != s0) == 0ull) {
+        for (uint32_t o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63u) == 0 && m > 0.0f) atomicMax(&wmax_bits[s0], __float_as_uint(m));
+    } else if (m > 0.0f) {
+        atomicMax(&wmax_bits[s], __float_as_uint(m));
+    }
+}
+
+// Fixed-point unit per scene from its bound and query count: the largest power of two
+// with count * wmax * unit < 2^62 (the same expression as the single-scene entry point).
+__global__ __launch_bounds__(64) void k_bwd_scale_batch(uint32_t ns, const uint32_t* __restrict__ slot_begin,
+                                                        const uint32_t* __restrict__ count,
+                                                        const uint32_t* __restrict__ wmax_bits,
+                                                        double* __restrict__ scale) {
+    for (uint32_t s = threadIdx.x; s < ns; s += blockDim.x) {
+        const uint32_t n = count ? count[s] : slot_begin[s + 1] - slot_begin[s];
+        const float wm = __uint_as_float(wmax_bits[s]);
+        double sc = 0.0;
+        if (n > 0 && wm > 0.0f && wm <= kInf) {
+            int e = 0;
+            (void)frexp(4.611686018427387904e18 / ((double)n * (double)wm), &e);
+            sc = ldexp(1.0, e - 1);
+        }
+        scale[s] = sc;
+    }
+}
+
+// Batched backward: waves take work items of `chunk` consecutive query slots (chunk x k <=
+// kPairCap) from the work counters, as k_knn_batch does.  A pair carries the photon's index
+// in the concatenation of the scenes' photon arrays (scene offset + original index) and its
+// weight in the scene's own fixed-point unit, so one work item may span scenes.
+template <int CAP, int kSel>
+__global__ __launch_bounds__(256) void k_knn_bwd_batch(KnnArgs G, const KnnScene* __restrict__ scenes,
+                                                       const uint32_t* __restrict__ slot_begin, uint32_t ns,
+                                                       const uint32_t* __restrict__ count,
+                                                       const uint32_t* __restrict__ photon_begin,
+                                                       const double* __restrict__ scale, uint32_t chunk,
+                                                       uint32_t num_parts, const uint32_t* __restrict__ part_begin,
+                                                       const uint32_t* __restrict__ part_end,
+                                                       uint32_t* __restrict__ work) {
+    __shared__ uint64_t s_key[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_pos[kWavesPerBlock][CAP];
+    __shared__ uint32_t s_snode[kWavesPerBlock][kKnnStack];
+    __shared__ float4 s_soff[kWavesPerBlock][kKnnStack];
+    __shared__ uint64_t s_pairs[kWavesPerBlock][kPairCap];
+    __shared__ long long s_w[kWavesPerBlock][3 * (kPairCap / 8)];
+    __shared__ uint32_t s_np[kWavesPerBlock];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t xcd = blockIdx.x % num_parts;
+    uint64_t* pairs = s_pairs[wave];
+    for (uint32_t part = 0; part < num_parts; ++part) {
+        const uint32_t p = (xcd + part) % num_parts;
+        const uint32_t pb = part_begin[p], pe = part_end[p];
+        for (;;) {
+            uint32_t c = 0;
+            if (lane == 0) c = atomicAdd(&work[p], 1u);
+            c = uniform(__shfl(c, 0, 64));
+            const uint64_t g0 = (uint64_t)pb + (uint64_t)c * chunk;
+            if (g0 >= pe) break;
+            const uint32_t g1 = (uint32_t)min<uint64_t>(g0 + chunk, pe);
+            if (lane == 0) s_np[wave] = 0;
+            wave_sync();
+            uint32_t s = uniform(seg_of(slot_begin, ns, (uint32_t)g0));
+            uint32_t b = 0, e = 0, m = 0, poff = 0;
+            double fs = 0.0;
+            KnnTree Tr{};
+            WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
+            auto enter = [&]() {
+                b = ld_uniform(slot_begin, s);
+                e = ld_uniform(slot_begin, s + 1);
+                m = count ? ld_uniform(count, s) : e - b;
+                poff = ld_uniform(photon_begin, s);
+                fs = ld_uniform(scale, s);
+                const KnnScene sc = ld_uniform(scenes, s);
+                Tr = KnnTree{sc.nodes, sc.recs, sc.r2max, sc.r_max};
+                hint.r2 = -1.0f;
+            };
+            enter();
+            for (uint32_t g = (uint32_t)g0; g < g1; ++g) {
+                if (g >= e) {
+                    do {
+                        ++s;
+                    } while (g >= ld_uniform(slot_begin, s + 1));
+                    enter();
+                }
+                if (g - b >= m) continue;
+                const PairSink sink{pairs, &s_np[wave], s_w[wave], g - (uint32_t)g0, poff, fs};
+                knn_query<CAP, kModeBackward, kSel, true>(G, Tr, g, lane, s_key[wave], s_pos[wave], s_snode[wave],
+                                                          s_soff[wave], &sink, &hint);
+                wave_sync();
+            }
+            merge_pairs(pairs, uniform(s_np[wave]), s_w[wave], G.acc, lane);
+            wave_sync();
+        }
+    }
+}
+
+// Photon gradients of every scene from the fixed-point sums (photon p of the
+// concatenation belongs to the scene whose photon range holds it).
+__global__ __launch_bounds__(256) void k_fix_to_float_batch(uint32_t np, const uint32_t* __restrict__ photon_begin,
+                                                            uint32_t ns, const unsigned long long* __restrict__ acc,
+                                                            const double* __restrict__ scale,
+                                                            float4* __restrict__ out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double sc = scale[seg_of(photon_begin, ns, p)];
+    const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+    out[p] = make_float4((float)((double)(long long)acc[3ull * p] * inv),
+                         (float)((double)(long long)acc[3ull * p + 1] * inv),
+                         (float)((double)(long long)acc[3ull * p + 2] * inv), 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
 }
 
 // Selection strategy of the single-tree launches.
@@ -848,6 +1008,41 @@ int launch_knn_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* s
     }
 #undef MNRT_KB_W
 #undef MNRT_KB
+    MNRT_HIP_CHECK(hipGetLastError());
+    return MNRT_OK;
+}
+
+// Backward of a batch: clears the accumulators, bounds and scales the weights per scene
+// (from the forward's r2 and neighbour counts), runs the pair-merging walk and converts
+// the sums.  G.out_r2 / G.out_count / G.out_nbr must be null (the walk would rewrite them).
+int launch_knn_bwd_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_t* slot_begin, uint32_t ns,
+                         const uint32_t* count, const uint32_t* photon_begin, uint32_t total_photons,
+                         const float* fwd_r2, const uint32_t* fwd_count, uint32_t* wmax, double* scale, uint32_t num_parts, const uint32_t* part_begin,
+                         const uint32_t* part_end, uint32_t* work, uint32_t grid, float* grad_pow4, hipStream_t st) {
+    const uint32_t n = G.n;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, work, num_parts, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, wmax, ns, 0u);
+    const uint64_t words = 3ull * total_photons;
+    if (words) {
+        const uint32_t zb = (uint32_t)std::min<uint64_t>((words + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_zero_u64, dim3(zb), dim3(256), 0, st, G.acc, words);
+    }
+    if (n)
+        hipLaunchKernelGGL(k_bwd_wmax_batch, dim3((n + 255) / 256), dim3(256), 0, st, n, slot_begin, ns, count,
+                           G.grad, G.clr, G.infl, fwd_r2, fwd_count, wmax);
+    hipLaunchKernelGGL(k_bwd_scale_batch, dim3(1), dim3(64), 0, st, ns, slot_begin, count, wmax, scale);
+    const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(8u, (uint32_t)kPairCap / G.k));
+    const uint32_t need = G.k + 64;
+#define MNRT_KBB(C) \
+    hipLaunchKernelGGL((k_knn_bwd_batch<C, kKnnSelDefault>), dim3(grid), dim3(64 * kWavesPerBlock), 0, st, G, scenes, \
+                       slot_begin, ns, count, photon_begin, scale, chunk, num_parts, part_begin, part_end, work)
+    if (need <= 128) MNRT_KBB(128);
+    else if (need <= 256) MNRT_KBB(256);
+    else MNRT_KBB(512);
+#undef MNRT_KBB
+    if (total_photons)
+        hipLaunchKernelGGL(k_fix_to_float_batch, dim3((total_photons + 255) / 256), dim3(256), 0, st, total_photons,
+                           photon_begin, ns, G.acc, scale, (float4*)grad_pow4);
     MNRT_HIP_CHECK(hipGetLastError());
     return MNRT_OK;
 }

@@ -310,9 +310,32 @@ extern "C" int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t 
     int rc = batch_tables(trees, num_scenes, h_offsets, MNRT_KIND_POINT, B);
     if (rc != MNRT_OK) return rc;
     if (B.total_refs > 0 && (!d_pos4 || !d_pow4 || !d_dir4)) return MNRT_ERR_ARG;
-    for (uint32_t s = 0; s < num_scenes; ++s)
-        if (trees[s].d_recs || trees[s].d_qnodes) return MNRT_ERR_STATE;
+    const uint32_t ns = num_scenes;
     hipStream_t st = (hipStream_t)stream;
+    bool all = true, none = true;
+    for (uint32_t s = 0; s < ns; ++s) {
+        all = all && trees[s].d_recs && trees[s].d_qnodes;
+        none = none && !trees[s].d_recs && !trees[s].d_qnodes;
+    }
+    if (!all && !none) return MNRT_ERR_STATE;
+    if (all) {
+        // records attached by an earlier call: rewritten in place (new photon powers, say);
+        // the query-time nodes depend on the tree only and stay
+        for (uint32_t s = 0; s < ns; ++s)
+            if (trees[s].d_recs != trees[0].d_recs + 8ull * B.h[ns + 1 + s]) return MNRT_ERR_STATE;
+        uint32_t* d_tab = nullptr;
+        rc = upload(B, &d_tab, st);
+        if (rc == MNRT_OK && B.total_refs > 0) {
+            hipLaunchKernelGGL(mnrt::k_photon_records_batch, dim3((B.total_refs + 255) / 256), dim3(256), 0, st,
+                               trees[0].d_refs, B.total_refs, d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
+                               (const float4*)d_pos4, (const float4*)d_pow4, (const float4*)d_dir4,
+                               (float4*)trees[0].d_recs);
+            if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
+        }
+        if (d_tab) (void)mnrt::pool_free(d_tab, st);
+        if (hipStreamSynchronize(st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
+        return rc;
+    }
     float* recs = nullptr;
     uint32_t* qn = nullptr;
     if (mnrt::pool_alloc((void**)&recs, (size_t)B.total_refs * 32, st) != hipSuccess) return MNRT_ERR_NOMEM;
@@ -322,7 +345,6 @@ extern "C" int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t 
     }
     uint32_t* d_tab = nullptr;
     rc = upload(B, &d_tab, st);
-    const uint32_t ns = num_scenes;
     if (rc == MNRT_OK && B.total_refs > 0)
         hipLaunchKernelGGL(mnrt::k_photon_records_batch, dim3((B.total_refs + 255) / 256), dim3(256), 0, st,
                            trees[0].d_refs, B.total_refs, d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,

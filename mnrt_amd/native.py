@@ -104,7 +104,8 @@ class BatchTuning(ctypes.Structure):
 
 BATCH_IO_FIELDS = ("d_orig4", "d_dir4", "d_v0", "d_v1", "d_v2", "d_albedo4", "d_infl4", "d_hit_tri", "d_hit_t",
                    "d_hit_bary2", "d_sp_pos4", "d_sp_nrm4", "d_sp_dir4", "d_sp_clr4", "d_sp_infl4", "d_sp_src",
-                   "d_sp_count", "d_io_rad4", "d_out_r2", "d_out_nbr", "d_out_count")
+                   "d_sp_count", "d_io_rad4", "d_out_r2", "d_out_nbr", "d_out_count", "d_grad_rad4",
+                   "d_grad_pow4", "d_grad_clr4")
 
 
 class BatchIO(ctypes.Structure):
@@ -113,7 +114,7 @@ class BatchIO(ctypes.Structure):
     _fields_ = [(n, vp) for n in BATCH_IO_FIELDS]
 
 
-PHASE_TRACE, PHASE_SHADE, PHASE_KNN = 1, 2, 4
+PHASE_TRACE, PHASE_SHADE, PHASE_KNN, PHASE_KNN_BWD = 1, 2, 4, 8
 
 
 # status codes (the MNRT_OK / MNRT_ERR_* values of mnrt_kd.h)

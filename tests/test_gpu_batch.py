@@ -267,3 +267,63 @@ def test_query_batch_autotuned_shapes(orc, cuda):
     for n, v in ref.items():
         assert torch.equal(B[n].view(torch.int32), v.view(torch.int32)), n
     Q.close()
+
+
+def test_query_batch_bwd_equals_plain(orc, cuda):
+    """The batched kNN adjoint phase gives, per scene, the single-scene adjoint's photon and
+    colour gradients bit for bit (each scene in its own fixed-point unit), an empty scene
+    gets zero photon gradients, and graph replays equal the eager run."""
+    torch = cuda
+    sizes = [1500, 300, 8000, 2500]
+    photons = [6000, 3000, 30000, 9000]
+    res = [(40, 32), (0, 0), (48, 48), (24, 24)]
+    k = 50
+    X = _query_setup(cuda, sizes, photons, res, seed=90)
+    Q = api.QueryBatch(X["TB"], X["toff"], X["roff"], X["eps"], X["PB"], X["rmax"], k)
+    B = _batch_io(cuda, Q, X, k)
+    n = max(int(X["roff"][-1]), 1)
+    poff = Q.photon_offsets
+    g = torch.Generator(device="cpu").manual_seed(5)
+    # per-scene gradient magnitudes far apart: a shared fixed-point unit would round the small ones
+    scale = torch.ones(n, 1)
+    for s in range(len(sizes)):
+        scale[int(X["roff"][s]):int(X["roff"][s + 1])] = 10.0 ** (-3 * s)
+    B["grad_rad4"] = ((torch.rand((n, 4), generator=g) - 0.3) * scale).cuda()
+    B["out_count"] = torch.empty(n, dtype=torch.int32, device="cuda")
+    B["grad_pow4"] = torch.full((max(int(poff[-1]), 1), 4), 7.0, device="cuda")
+    B["grad_clr4"] = torch.zeros((n, 4), device="cuda")
+    Q.make_io(**B)
+    print("batched forward + adjoint", flush=True)
+    Q.run(7)
+    Q.run(8)
+    cuda.cuda.synchronize()
+    cnt = B["sp_count"].cpu().numpy()
+    for s in range(len(sizes)):
+        a, m = int(X["roff"][s]), int(cnt[s])
+        sl = slice(a, a + m)
+        print(f"scene {s}: {m} queries, {int(poff[s + 1] - poff[s])} photons", flush=True)
+        gp, gc = api.knn_gather_bwd(X["PB"][s], k, float(X["rmax"][s]), B["sp_pos4"][sl].contiguous(),
+                                    B["sp_nrm4"][sl].contiguous(), B["sp_dir4"][sl].contiguous(),
+                                    B["sp_infl4"][sl].contiguous(), B["sp_clr4"][sl].contiguous(),
+                                    B["grad_rad4"][sl].contiguous(), B["out_r2"][sl].contiguous(),
+                                    B["out_count"][sl].contiguous())
+        got = B["grad_pow4"][int(poff[s]):int(poff[s + 1])]
+        assert torch.equal(got.view(torch.int32), gp.view(torch.int32)), s
+        if m:
+            assert torch.equal(B["grad_clr4"][sl].view(torch.int32), gc.view(torch.int32)), s
+            assert bool((gp[:, :3] != 0).any()), s
+        else:
+            assert not bool(got.any()), s
+    ref = {nm: B[nm].clone() for nm in ("grad_pow4", "grad_clr4")}
+    Q.capture(8)
+    for rep in range(3):
+        B["grad_pow4"].fill_(3.0)
+        B["grad_clr4"].fill_(5.0)
+        cuda.cuda.synchronize()
+        Q.replay(8)
+        cuda.cuda.synchronize()
+        assert torch.equal(B["grad_pow4"].view(torch.int32), ref["grad_pow4"].view(torch.int32)), rep
+        for s in range(len(sizes)):
+            sl = slice(int(X["roff"][s]), int(X["roff"][s]) + int(cnt[s]))
+            assert torch.equal(B["grad_clr4"][sl].view(torch.int32), ref["grad_clr4"][sl].view(torch.int32)), rep
+    Q.close()
