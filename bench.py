@@ -61,6 +61,11 @@ def shadow_bytes(counts, n_rays):
     return 20.0 * n_rays + 8.0 * inner + 8.0 * leaves + 4.0 * refs + 36.0 * tests
 
 
+def _progress(msg):
+    """One progress line on stderr (the JSON result line stays alone on stdout)."""
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
 def shadow_leg(torch, api, N, O, T, ttree, sc, pts, eps, sample, ncpu, reps=5):
     """Shadow rays from the primary hits of the large workload (mnrt_trace_shadow): one
     to a point light just below the ceiling light, one to a stratified sample of the
@@ -347,7 +352,8 @@ def inverse_leg(args, torch, api, S):
             bwd.append(ev[1].elapsed_time(ev[2]))
     torch.cuda.synchronize()
     t1 = time.time()
-    e_fit, a_fit, losses = P.fit(target, steps=args.inverse_steps, lr=1e-2)
+    _progress(f"inverse lighting set up in {setup_s:.1f}s: {P.num_queries} queries, {int(P.poff[-1])} photons")
+    e_fit, a_fit, losses = P.fit(target, steps=args.inverse_steps, lr=1e-2, log=_progress)
     torch.cuda.synchronize()
     fit_s = time.time() - t1
     err0 = float((1.0 - e_true).abs().mean())
@@ -509,6 +515,7 @@ def main():
     results = [step(None) for _ in range(args.steps)]
     torch.cuda.synchronize()
     wall = time.time() - t0
+    _progress(f"{args.steps} timed steps: {wall * 1e3 / args.steps:.2f} ms per step")
     ms_step = wall * 1e3 / args.steps
     rays, queries = results[0]["rays"], results[0]["queries"]
     phase_ms = {p: float(np.mean([r["ms"][p] for r in results])) for p in phases}
@@ -550,10 +557,12 @@ def main():
     ppos_h = ppos.cpu().numpy()
     qpos = np.concatenate([q["pos"].cpu().numpy() for q in rec["queries"]])
     qs = qpos[:: args.sample]
-    ncpu = os.cpu_count() or 1
+    # host threads for the oracle: the cores this process may use, capped by OMP_NUM_THREADS
+    ncpu = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16"))))
     tk = time.time()
     nbr, cnt, kctr = O.knn(ptree, ppos_h, qs, K, rmax, nthreads=ncpu)
     cpu_knn_s = time.time() - tk
+    _progress(f"oracle kNN replay of {len(qs)} queries: {cpu_knn_s:.1f}s")
     per_q = knn_bytes(kctr, len(qs), int(np.minimum(cnt, K).sum())) / len(qs)
     tn, tr = T.nodes_host(), T.refs_host()
     ttree = O.Tree(0, tn, tr, T.root, T.max_level, 0, sc.num_tris)
@@ -567,6 +576,7 @@ def main():
     tt = time.time()
     _, _, _, tctr = O.trace(ttree, sc.v0, sc.v1, sc.v2, ro, rd, eps, nthreads=ncpu)
     cpu_trace_s = time.time() - tt
+    _progress(f"oracle traversal replay of {len(ro)} rays: {cpu_trace_s:.1f}s")
     per_ray = trace_bytes(tctr, len(ro)) / len(ro)
     if not args.no_shadow:
         out["shadow"] = shadow_leg(torch, api, native, O, T, ttree, sc, rec["queries"][0]["pos"], eps, args.sample,
@@ -596,6 +606,7 @@ def main():
     if not args.no_cpu:
         tb = time.time()
         O.build(0, sc.v0, sc.v1, sc.v2)
+        _progress(f"oracle triangle build: {time.time() - tb:.1f}s")
         cpu_build_tri = time.time() - tb
         tb = time.time()
         O.build(1, ppos_h, max_qr=rmax, root=root)
@@ -610,12 +621,15 @@ def main():
                        f"scaled to the full step ({cpu_step:.1f}s)"),
             "gpu_speedup": round(cpu_step / (ms_step * 1e-3), 1)}
     if not args.no_batched:
+        _progress("batched leg")
         out["batched"] = batched_leg(args, torch, api, S)
         out["batched_vs_plain"] = out["batched"]["speedup_vs_plain"]
     if not args.no_adjoint:
+        _progress("adjoint leg")
         out["adjoint"] = adjoint_leg(args, torch, api, S)
         out["bwd_fwd_ratio"] = out["adjoint"]["bwd_fwd_ratio"]
     if not args.no_inverse:
+        _progress("inverse-lighting leg")
         out["inverse"] = inverse_leg(args, torch, api, S)
     print(json.dumps(out))
 

@@ -54,8 +54,9 @@ def test_gradient_equals_single_scene_adjoints(cuda):
     total.backward()
     for pm in pms:
         pm.release()
-    print(f"loss {loss:.6e} vs {float(total):.6e}", flush=True)
-    assert abs(loss - float(total)) <= 1e-6 * abs(float(total))
+    tot = float(total.detach())
+    print(f"loss {loss:.6e} vs {tot:.6e}", flush=True)
+    assert abs(loss - tot) <= 1e-6 * abs(tot)
     torch.testing.assert_close(e.grad, e2.grad, rtol=1e-4, atol=1e-6 * float(e2.grad.abs().max()))
     torch.testing.assert_close(a.grad, a2.grad, rtol=1e-4, atol=1e-6 * float(a2.grad.abs().max()))
     assert float(e.grad.abs().sum()) > 0 and float(a.grad.abs().sum()) > 0
