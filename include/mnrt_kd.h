@@ -41,7 +41,9 @@
 extern "C" {
 #endif
 
-#define MNRT_ABI_VERSION 1
+/* Raised whenever a declaration or struct layout here changes (2: mnrt_batch_io gained the
+ * adjoint buffers). */
+#define MNRT_ABI_VERSION 2
 
 enum mnrt_status_code {
     MNRT_OK = 0,

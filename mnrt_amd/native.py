@@ -116,6 +116,9 @@ class BatchIO(ctypes.Structure):
 
 PHASE_TRACE, PHASE_SHADE, PHASE_KNN, PHASE_KNN_BWD = 1, 2, 4, 8
 
+# MNRT_ABI_VERSION of the header these declarations mirror
+ABI_VERSION = 2
+
 
 # status codes (the MNRT_OK / MNRT_ERR_* values of mnrt_kd.h)
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_CAPACITY, ERR_STATE = 0, -1, -2, -3, -4, -5, -6
@@ -139,6 +142,9 @@ def load():
     L.mnrt_status_string.restype = ctypes.c_char_p
     L.mnrt_status_string.argtypes = [ctypes.c_int]
     L.mnrt_abi_version.restype = ctypes.c_int
+    if L.mnrt_abi_version() != ABI_VERSION:
+        raise MnrtError(f"{LIB_PATH} has ABI version {L.mnrt_abi_version()}, these bindings expect {ABI_VERSION}: "
+                        "rebuild it with __graft_entry__.build()")
     L.mnrt_device_init.argtypes = [ctypes.c_int]
     TP = ctypes.POINTER(Tree)
     L.mnrt_build_tri.argtypes = [vp, vp, vp, u32, ctypes.POINTER(BuildParams), TP, vp]
