@@ -389,7 +389,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-lighting leg")
     ap.add_argument("--inverse-scenes", type=int, default=64)
-    ap.add_argument("--inverse-steps", type=int, default=200)
+    ap.add_argument("--inverse-steps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 256 MiB device-to-device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
