@@ -136,20 +136,10 @@ def batched_leg(args, torch, api, S):
     after scene, is timed beside it."""
     t0 = time.time()
     NS, K = args.batched_scenes, 50
-    rng = np.random.default_rng(4)
-
-    def lu(lo, hi):
-        return np.exp(rng.uniform(np.log(lo), np.log(hi), NS))
-
-    ntri, nph, nq = lu(1000, 50000).astype(int), lu(4096, 262144).astype(int), lu(16384, 1 << 20).astype(int)
-    scs = [S.scene_sized(int(n), seed=4 + i) for i, n in enumerate(ntri)]
+    scs, nph, dims = S.batched_workload(NS, seed=4)
     v0, v1, v2, alb = (api.dev(np.concatenate([getattr(sc, a) for sc in scs]).astype(np.float32))
                        for a in ("v0", "v1", "v2", "albedo"))
     toff = _offsets([sc.num_tris for sc in scs])
-    dims = []
-    for q in nq:
-        w = 8 * max(1, int(round(np.sqrt(q) / 8)))
-        dims.append((w, max(1, int(round(q / w)))))
     roff = _offsets([w * h for w, h in dims])
     R = int(roff[-1])
     go = torch.empty((R, 4), dtype=torch.float32, device="cuda")
@@ -304,28 +294,22 @@ def adjoint_leg(args, torch, api, S):
             "bwd_fwd_ratio": round(b / f, 3)}
 
 
-def inverse_leg(args, torch, api, S):
+def inverse_leg(args, torch, api, S, O, ncpu):
     """Batched inverse lighting: 64 scenes (4 lights each; triangles log-U[1k,50k], photons
     log-U[4k,256k], primary-hit queries log-U[16k,256k], seed 5), target radiance from random
-    per-light RGB emission in [0.5,2]; Adam (lr 1e-2) on per-light emission and per-triangle
-    albedo from emission 1 and the traced albedo.  A step = photon reweighting (torch),
-    record refresh, one batched kNN launch for every scene, one batched adjoint launch, the
-    torch chain to the parameters and the Adam update."""
+    per-light RGB emission in [0.5,2]; Adam (lr 1e-2, 200 steps) on per-light emission and
+    per-triangle albedo from emission 1 and the traced albedo.  A step = photon and colour
+    reweighting (mnrt_photons_reweight), record refresh, one batched kNN launch for every
+    scene, the loss, one batched adjoint launch, the parameter adjoint
+    (mnrt_photons_reweight_bwd) and the Adam update, with no host synchronisation.  After the
+    fit every scene is checked against the oracle on a 1-in-`sample` subset of its hits: kNN
+    sets by the oracle's own tree, radiance against its fp32 path (bits) and fp64 path
+    (relative error), photon powers against the fp64 reweighting."""
     from mnrt_amd.inverse import BatchedInverseLighting
 
     t0 = time.time()
     NS, K = args.inverse_scenes, 50
-    rng = np.random.default_rng(5)
-
-    def lu(lo, hi):
-        return np.exp(rng.uniform(np.log(lo), np.log(hi), NS))
-
-    ntri, nph, nq = lu(1000, 50000).astype(int), lu(4096, 262144).astype(int), lu(16384, 262144).astype(int)
-    scs = [S.scene_sized(int(n), seed=1000 + i, num_lights=4) for i, n in enumerate(ntri)]
-    dims = []
-    for q in nq:
-        w = 8 * max(1, int(round(np.sqrt(q) / 8)))
-        dims.append((w, max(1, int(round(q / w)))))
+    scs, nph, dims = S.batched_workload(NS, seed=5, query_hi=262144, num_lights=4, scene_seed=1000)
     P = BatchedInverseLighting(scs, nph, dims, k=K, seed=7)
     L = int(P.loff[-1])
     g = torch.Generator(device="cpu").manual_seed(5)
@@ -333,9 +317,7 @@ def inverse_leg(args, torch, api, S):
     target = P.radiance(e_true, P.albedo0)
     torch.cuda.synchronize()
     setup_s = time.time() - t0
-    # one step's batched launches timed with events
-    e = torch.ones((L, 3), device="cuda", requires_grad=True)
-    a = P.albedo0.clone().requires_grad_(True)
+    # the two batched launches of a step, timed with events
     stream = torch.cuda.current_stream()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     fwd, bwd = [], []
@@ -359,15 +341,51 @@ def inverse_leg(args, torch, api, S):
     err0 = float((1.0 - e_true).abs().mean())
     err = float((e_fit - e_true).abs().mean())
     alb_dev = float((a_fit - P.albedo0).abs().mean())
+    # per-scene check against the oracle at the fitted parameters
+    t2 = time.time()
+    P.radiance(e_fit, a_fit)
+    eh, ah, et = e_fit.cpu().numpy(), a_fit.cpu().numpy(), e_true.cpu().numpy()
+    per = {"rad_rel_err_fp64": [], "rad_fp32_bits_equal": [], "power_rel_err_fp64": [], "emission_mean_abs_err": []}
+    for s in range(NS):
+        A = P.scene_arrays(s)
+        ls, ts = slice(int(P.loff[s]), int(P.loff[s + 1])), slice(int(P.toff[s]), int(P.toff[s + 1]))
+        lt = A["light"].astype(np.int64) - int(P.loff[s])
+        tr = A["tris"].astype(np.int64)
+        tr = np.where(tr >= 0, tr - int(P.toff[s]), -1)
+        p64 = O.reweight(A["base"], lt, tr, eh[ls], ah[ts])
+        # relative to the fp64 power; a zero power (an albedo channel at 0) must be exactly 0
+        rel = np.abs(A["ppow"][:, :3] - p64) / np.where(p64 != 0, np.abs(p64), 1.0)
+        per["power_rel_err_fp64"].append(float(np.max(np.where((p64 == 0) & (A["ppow"][:, :3] != 0), np.inf, rel))))
+        tree = O.build(1, A["ppos"], max_qr=A["r_max"], root=A["root"])
+        sl = slice(None, None, args.sample // 4 or 1)
+        q = {n: A[n][sl] for n in ("qpos", "qnrm", "qdir", "qinfl", "qclr", "rad")}
+        nbr, cnt, _ = O.knn(tree, A["ppos"], q["qpos"], K, A["r_max"], nthreads=ncpu)
+        o32, o64, _, _, _ = O.radiance(nbr, cnt, K, A["r_max"], A["ppos"], A["ppow"], A["pdir"], q["qpos"],
+                                       q["qnrm"], q["qdir"], q["qinfl"], q["qclr"], nthreads=ncpu)
+        per["rad_fp32_bits_equal"].append(bool(np.array_equal(q["rad"].view(np.uint32), o32.view(np.uint32))))
+        scale = np.abs(o64).max(axis=1) + 1e-30
+        per["rad_rel_err_fp64"].append(float(np.max(np.abs(q["rad"].astype(np.float64) - o64).max(axis=1) / scale)))
+        per["emission_mean_abs_err"].append(float(np.abs(eh[ls] - et[ls]).mean()))
+    check_s = time.time() - t2
+    _progress(f"inverse lighting oracle check of {NS} scenes: {check_s:.1f}s")
+    steps = max(1, args.inverse_steps)
     out = {"workload": (f"{NS} scenes x 4 lights: triangles log-U[1k,50k], photons log-U[4k,256k], "
                         "queries log-U[16k,256k] (seed 5), k=50; Adam lr 1e-2 on emission + albedo"),
            "scenes": NS, "triangles": int(P.toff[-1]), "photons": int(P.poff[-1]), "queries": P.num_queries,
-           "steps": args.inverse_steps, "ms_per_step": round(fit_s / max(1, args.inverse_steps) * 1e3, 3),
+           "steps": args.inverse_steps, "ms_per_step": round(fit_s / steps * 1e3, 3),
            "knn_fwd_ms": round(float(np.median(fwd)), 3), "knn_bwd_ms": round(float(np.median(bwd)), 3),
            "bwd_fwd_ratio": round(float(np.median(bwd)) / float(np.median(fwd)), 3),
-           "loss_first": losses[0], "loss_last": losses[-1],
+           "step_vs_knn_fwd_bwd": round(fit_s / steps * 1e3 / (float(np.median(fwd)) + float(np.median(bwd))), 3),
+           "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
            "emission_mean_abs_err": {"start": round(err0, 4), "end": round(err, 4)},
-           "albedo_mean_abs_change": round(alb_dev, 5), "setup_s": round(setup_s, 1)}
+           "albedo_mean_abs_change": round(alb_dev, 5), "setup_s": round(setup_s, 1),
+           "oracle_check": {
+               "sample": f"1 in {args.sample // 4 or 1} hits of every scene, at the fitted parameters",
+               "max_rad_rel_err_fp64": max(per["rad_rel_err_fp64"]),
+               "all_rad_fp32_bits_equal": all(per["rad_fp32_bits_equal"]),
+               "max_power_rel_err_fp64": max(per["power_rel_err_fp64"]),
+               "per_scene": {k_: ([round(v, 9) for v in vs] if isinstance(vs[0], float) else vs)
+                             for k_, vs in per.items()}}}
     P.release()
     return out
 
@@ -389,7 +407,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-lighting leg")
     ap.add_argument("--inverse-scenes", type=int, default=64)
-    ap.add_argument("--inverse-steps", type=int, default=50)
+    ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 256 MiB device-to-device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -630,7 +648,7 @@ def main():
         out["bwd_fwd_ratio"] = out["adjoint"]["bwd_fwd_ratio"]
     if not args.no_inverse:
         _progress("inverse-lighting leg")
-        out["inverse"] = inverse_leg(args, torch, api, S)
+        out["inverse"] = inverse_leg(args, torch, api, S, O, ncpu)
     print(json.dumps(out))
 
 

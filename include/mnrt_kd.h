@@ -43,7 +43,7 @@ extern "C" {
 
 /* Raised whenever a declaration or struct layout here changes (2: mnrt_batch_io gained the
  * adjoint buffers). */
-#define MNRT_ABI_VERSION 2
+#define MNRT_ABI_VERSION 3
 
 enum mnrt_status_code {
     MNRT_OK = 0,
@@ -315,7 +315,9 @@ int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
  *             path p starts on light p mod num_lights with power * num_lights.
  * Outputs (target entries): position, power, incoming direction (float4 each); optional
  * meta (light | bounce << 8), source path index, and the 4 triangles the path scattered
- * from before the photon (0xFFFFFFFF padded) for reweighting by emission and albedo.
+ * from before the photon (0xFFFFFFFF padded) for reweighting by emission and albedo
+ * (d_out_path_tris needs max_bounces <= 4, else MNRT_ERR_ARG: a photon stored at bounce
+ * b has scattered from b triangles).
  * *out_count = photons stored, *out_paths = paths shot. */
 int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, const float* d_v1,
                           const float* d_v2, const float* d_albedo4, const float* d_lights,
@@ -342,6 +344,48 @@ int mnrt_photons_append_valid(uint32_t n, const uint32_t* d_valid, const float* 
 /* PhotonMap::ScaleFlux / KernelPMScaleFlux (PhotonMap.cpp:186-193): power.rgb *= scale,
  * w (the polar angle) untouched. */
 int mnrt_photons_scale_flux(uint32_t n, float* d_io_pow4, float scale, void* stream);
+
+/* Photon powers and hit colours from per-light emission and per-triangle albedo, with the
+ * photon paths and hit triangles held fixed (no counterpart in the reference, which never
+ * fits parameters; the forward restates how RebuildPhotonMaps forms a stored photon's
+ * power along its path: emission of its light, photon_build.cu:79-130, times the albedo of
+ * every triangle it scattered from, photon_build.cu:278-331, RTCore.cpp:556-581):
+ *   power[p].rgb = ((base[p] * emission[light[p]]) * albedo[t0]) * ... (per channel, over
+ *                  the path's triangles t0..t3 of d_path_tris that are not 0xFFFFFFFF);
+ *   clr[q].rgb   = albedo[hit_tri[q]] for hit_tri[q] != 0xFFFFFFFF (clr[q].w kept).
+ * d_emission3 / d_albedo3: 3 floats per light / triangle.  An index out of range gives a
+ * NaN power or colour (never an out-of-range read). */
+int mnrt_photons_reweight(uint32_t np, const float* d_base_pow4, const uint32_t* d_light,
+                          const uint32_t* d_path_tris, uint32_t num_lights, const float* d_emission3,
+                          uint32_t num_tris, const float* d_albedo3, uint32_t nq,
+                          const uint32_t* d_hit_tri, float* d_out_pow4, float* d_io_clr4, void* stream);
+
+/* Adjoint of mnrt_photons_reweight: dLoss/demission (3 per light) and dLoss/dalbedo (3 per
+ * triangle) from dLoss/dpower (d_grad_pow4) and dLoss/dclr (d_grad_clr4, nq entries).
+ * Each output is a sum of 64-bit fixed-point terms in a power-of-two unit fixed from the
+ * largest term and the term count, so the result is the same bits on every run; terms
+ * below about 2^-38 of the largest term are lost.  A non-finite term makes every output
+ * NaN.  Photons with an index out of range contribute nothing. */
+int mnrt_photons_reweight_bwd(uint32_t np, const float* d_base_pow4, const uint32_t* d_light,
+                              const uint32_t* d_path_tris, uint32_t num_lights, const float* d_emission3,
+                              uint32_t num_tris, const float* d_albedo3, uint32_t nq,
+                              const uint32_t* d_hit_tri, const float* d_grad_pow4,
+                              const float* d_grad_clr4, float* d_grad_emission3, float* d_grad_albedo3,
+                              void* stream);
+
+/* --- parallel primitives (32-bit data) -------------------------------------------------- */
+
+/* MNCudaPrimitives::Scan (MNCudaPrimitives.h:167-184, MNCudaPrimitives.cpp:160-177): add-scan
+ * of n uint32 words, exclusive (inclusive = 0) or inclusive; d_out may equal d_in;
+ * *d_total (device word, may be null) = the sum of all n (mod 2^32). */
+int mnrt_scan_u32(const uint32_t* d_in, uint32_t n, int inclusive, uint32_t* d_out, uint32_t* d_total,
+                  void* stream);
+
+/* MNCudaPrimitives::Compact (MNCudaPrimitives.h:187-211, MNCudaPrimitives.cpp:180-190):
+ * the words d_in[i] with d_is_valid[i] != 0, in input order, to d_out; *d_out_count (device
+ * word) = their number.  Not in place (d_out != d_in). */
+int mnrt_compact_u32(const uint32_t* d_in, const uint32_t* d_is_valid, uint32_t n, uint32_t* d_out,
+                     uint32_t* d_out_count, void* stream);
 
 /* --- hits to queries -------------------------------------------------------------------- */
 

@@ -7,9 +7,11 @@ backward: mnrt_knn_gather_bwd recomputes every query's neighbour set (identical 
           query; the per-photon sums are order-independent (fixed-point), so repeated
           backward passes give identical gradients.
 
-Chain rules to light emission and per-triangle albedo are ordinary differentiable torch
-ops on top (photon power = emission of its light x product of the albedos it scattered
-from; query colour = albedo of the hit triangle), see `reweight_photons`.
+Chain rules to light emission and per-triangle albedo: `reweight_photons` gives photon
+powers as a function of per-light emission and per-triangle albedo with the photon paths
+fixed (mnrt_photons_reweight), its backward the parameter gradients as fixed-point sums
+(mnrt_photons_reweight_bwd), so the whole chain is deterministic and needs no global
+torch setting.
 """
 
 from __future__ import annotations
@@ -42,17 +44,49 @@ def knn_radiance(power, clr, pmap, ppos, pdir, q, k, r_max):
     return KnnRadiance.apply(power, clr, pmap, ppos, pdir, q["pos"], q["nrm"], q["dir"], q["infl"], k, r_max)
 
 
+class PhotonReweight(torch.autograd.Function):
+    """power (P, 4) = base * emission[light] * albedo products along each photon's path."""
+
+    @staticmethod
+    def forward(ctx, emission, albedo, base, light, path_tris):
+        light, path_tris = api._i32(light), api._i32(path_tris)
+        ctx.save_for_backward(emission.detach(), albedo.detach(), base, light, path_tris)
+        return api.photons_reweight(base, light, path_tris, emission, albedo)
+
+    @staticmethod
+    def backward(ctx, grad_pow):
+        emission, albedo, base, light, path_tris = ctx.saved_tensors
+        ge, ga = api.photons_reweight_bwd(base, light, path_tris, emission, albedo, grad_pow.contiguous())
+        return ge, ga, None, None, None
+
+
 def reweight_photons(base_power, light, path_tris, emission, albedo):
     """Photon powers as a differentiable function of per-light emission (L, 3) and
-    per-triangle albedo (T, 3): base_power / (emission, albedo products at trace time)
-    times the current ones.  `path_tris` (P, 4) lists the triangles each path scattered
-    from before the photon (-1 padded)."""
-    torch.use_deterministic_algorithms(True, warn_only=False)
-    e = emission[light.long()]  # (P, 3)
-    valid = path_tris >= 0
-    idx = torch.where(valid, path_tris, torch.zeros_like(path_tris)).long()
-    a = albedo[idx]  # (P, 4, 3)
-    a = torch.where(valid[..., None], a, torch.ones_like(a))
-    prod = a.prod(dim=1)
-    p = base_power[:, :3] * e * prod
-    return torch.cat([p, torch.zeros_like(p[:, :1])], dim=1)
+    per-triangle albedo (T, 3): base_power times the emission of the photon's light times
+    the albedo of every triangle in `path_tris` (P, 4; -1 padded), the triangles the path
+    scattered from before the photon."""
+    return PhotonReweight.apply(emission, albedo, base_power.contiguous(), light, path_tris)
+
+
+class HitColour(torch.autograd.Function):
+    """Diffuse colours (Q, 4) of hits: rgb = albedo[hit_tri] where hit_tri >= 0, the rest
+    (w, and rows without a triangle) from clr_in; dLoss/dalbedo by fixed-point sums."""
+
+    @staticmethod
+    def forward(ctx, albedo, hit_tri, clr_in):
+        hit_tri = api._i32(hit_tri)
+        out = clr_in.detach().clone().contiguous()
+        api.hit_colours(albedo, hit_tri, out)
+        ctx.save_for_backward(albedo.detach(), hit_tri)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_clr):
+        albedo, hit_tri = ctx.saved_tensors
+        return api.hit_colours_bwd(albedo, hit_tri, grad_clr.contiguous()), None, None
+
+
+def hit_colours(albedo, hit_tri, clr_in):
+    """Differentiable hit colours: albedo (T, 3) gathered at each hit's triangle."""
+    return HitColour.apply(albedo, hit_tri, clr_in)
+

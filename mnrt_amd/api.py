@@ -430,6 +430,114 @@ def photons_generate(T: DeviceTree, v0, v1, v2, albedo, lights, target: int, max
     return out
 
 
+def _i32(t):
+    """int32 view of an index tensor (0xFFFFFFFF = -1 for "none")."""
+    torch = _torch()
+    return t if t.dtype == torch.int32 and t.is_contiguous() else t.to(torch.int32).contiguous()
+
+
+def photons_reweight(base, light, path_tris, emission, albedo, hit_tri=None, io_clr=None, out_pow=None):
+    """Photon powers (P, 4) = base * emission[light] * albedo products along each path
+    (mnrt_photons_reweight); with hit_tri / io_clr also writes io_clr[q].rgb =
+    albedo[hit_tri[q]] for hit_tri[q] >= 0.  emission (L, 3), albedo (T, 3) float32."""
+    torch = _torch()
+    np_ = base.shape[0]
+    light, path_tris = _i32(light), _i32(path_tris)
+    emission, albedo = emission.detach().float().contiguous(), albedo.detach().float().contiguous()
+    out = out_pow if out_pow is not None else torch.empty((max(np_, 1), 4), dtype=torch.float32, device="cuda")
+    nq = 0 if hit_tri is None else hit_tri.shape[0]
+    ht = _i32(hit_tri) if hit_tri is not None else None
+    N.check(N.load().mnrt_photons_reweight(np_, N.ptr(base), N.ptr(light), N.ptr(path_tris), emission.shape[0],
+                                           N.ptr(emission), albedo.shape[0], N.ptr(albedo), nq, N.ptr(ht),
+                                           N.ptr(out), N.ptr(io_clr) if nq else None, N.stream_handle()),
+            "mnrt_photons_reweight")
+    return out[:np_]
+
+
+def photons_reweight_bwd(base, light, path_tris, emission, albedo, grad_pow, hit_tri=None, grad_clr=None):
+    """(dLoss/demission (L, 3), dLoss/dalbedo (T, 3)) from dLoss/dpower (P, 4) and, with
+    hit_tri, dLoss/dclr (Q, 4) (mnrt_photons_reweight_bwd; fixed-point sums, same bits on
+    every run)."""
+    torch = _torch()
+    np_ = base.shape[0]
+    light, path_tris = _i32(light), _i32(path_tris)
+    emission, albedo = emission.detach().float().contiguous(), albedo.detach().float().contiguous()
+    ge = torch.empty_like(emission)
+    ga = torch.empty_like(albedo)
+    nq = 0 if hit_tri is None else hit_tri.shape[0]
+    ht = _i32(hit_tri) if hit_tri is not None else None
+    gc = grad_clr.contiguous() if nq else None
+    N.check(N.load().mnrt_photons_reweight_bwd(np_, N.ptr(base), N.ptr(light), N.ptr(path_tris),
+                                               emission.shape[0], N.ptr(emission), albedo.shape[0], N.ptr(albedo),
+                                               nq, N.ptr(ht), N.ptr(grad_pow.contiguous()), N.ptr(gc), N.ptr(ge),
+                                               N.ptr(ga), N.stream_handle()), "mnrt_photons_reweight_bwd")
+    return ge, ga
+
+
+def photons_spherical_to_dir(pos, power):
+    """Directions (P, 4) from the reference's PhotonData angles: azimuth phi in pos.w,
+    polar theta in power.w (mnrt_photons_spherical_to_dir)."""
+    torch = _torch()
+    pos, power = dev(pos), dev(power)
+    n = pos.shape[0]
+    out = torch.empty((max(n, 1), 4), dtype=torch.float32, device="cuda")
+    N.check(N.load().mnrt_photons_spherical_to_dir(n, N.ptr(pos), N.ptr(power), N.ptr(out), N.stream_handle()),
+            "mnrt_photons_spherical_to_dir")
+    return out[:n]
+
+
+def photons_dir_to_spherical(dirs, io_pos, io_power):
+    """Writes the angles of unit directions into io_pos.w (phi) and io_power.w (theta)
+    (mnrt_photons_dir_to_spherical)."""
+    dirs = dev(dirs)
+    N.check(N.load().mnrt_photons_dir_to_spherical(dirs.shape[0], N.ptr(dirs), N.ptr(io_pos), N.ptr(io_power),
+                                                   N.stream_handle()), "mnrt_photons_dir_to_spherical")
+
+
+def scan_u32(a, inclusive=False, out=None):
+    """Add-scan of an int32/uint32 tensor (mnrt_scan_u32): (scan, total)."""
+    torch = _torch()
+    a = a.contiguous()
+    out = out if out is not None else torch.empty_like(a)
+    tot = torch.empty(1, dtype=torch.int32, device="cuda")
+    N.check(N.load().mnrt_scan_u32(N.ptr(a) if a.numel() else None, a.numel(), 1 if inclusive else 0,
+                                   N.ptr(out) if a.numel() else None, N.ptr(tot), N.stream_handle()), "mnrt_scan_u32")
+    return out, tot
+
+
+def compact_u32(a, valid):
+    """The words of a where valid != 0, in order (mnrt_compact_u32): (compacted, count)."""
+    torch = _torch()
+    a, valid = a.contiguous(), valid.contiguous()
+    out = torch.empty(max(a.numel(), 1), dtype=a.dtype, device="cuda")
+    cnt = torch.empty(1, dtype=torch.int32, device="cuda")
+    n = a.numel()
+    N.check(N.load().mnrt_compact_u32(N.ptr(a) if n else None, N.ptr(valid) if n else None, n,
+                                      N.ptr(out) if n else None, N.ptr(cnt), N.stream_handle()), "mnrt_compact_u32")
+    return out, cnt
+
+
+def _no_photons():
+    torch = _torch()
+    z4 = torch.empty((0, 4), dtype=torch.float32, device="cuda")
+    zi = torch.empty((0, 4), dtype=torch.int32, device="cuda")
+    return z4, zi[:, 0].contiguous(), zi, torch.empty((0, 3), dtype=torch.float32, device="cuda")
+
+
+def hit_colours(albedo, hit_tri, io_clr):
+    """io_clr[q].rgb = albedo[hit_tri[q]] for hit_tri[q] >= 0 (mnrt_photons_reweight
+    without photons)."""
+    b, l, t, e = _no_photons()
+    photons_reweight(b, l, t, e, albedo, hit_tri=hit_tri, io_clr=io_clr)
+    return io_clr
+
+
+def hit_colours_bwd(albedo, hit_tri, grad_clr):
+    """dLoss/dalbedo (T, 3) from dLoss/dclr (Q, 4) at hits on hit_tri."""
+    b, l, t, e = _no_photons()
+    return photons_reweight_bwd(b, l, t, e, albedo, b, hit_tri=hit_tri, grad_clr=grad_clr)[1]
+
+
 def shading_points(orig, dirs, tri, t, v0, v1, v2, albedo, infl=None, want_src=False):
     """Compacted shading points at the valid hits."""
     torch = _torch()

@@ -50,6 +50,7 @@ struct mnrt_batch {
     std::vector<uint32_t> h_poff;  // ns + 1: photon offsets of the scenes
     void* d_bwd = nullptr;
     size_t bwd_cap = 0;            // photons the accumulators hold
+    size_t bwd_graph_photons = 0;  // total photons baked into captured adjoint graphs
     unsigned long long* d_acc = nullptr;
     uint32_t* d_wmax = nullptr;
     double* d_scale = nullptr;
@@ -197,11 +198,13 @@ int run_knn(mnrt_batch& b, const mnrt_batch_io& io, hipStream_t st) {
 
 // Accumulators of the adjoint phase for the current photon counts (synchronous; never
 // called while a stream is being captured).  Graphs of the adjoint phase are dropped when
-// the accumulators move.
+// the accumulators move or the total photon count changes (a captured adjoint clears and
+// converts exactly that many photons' sums: launch arguments fixed at capture).
 int ensure_bwd(mnrt_batch& b) {
     if (!b.has_knn) return MNRT_ERR_STATE;
     const uint32_t ns = b.ns;
     const size_t np = b.h_poff[ns];
+    if (np != b.bwd_graph_photons) drop_graphs(b, MNRT_PHASE_KNN_BWD);
     if (!b.d_bwd || np > b.bwd_cap) {
         MNRT_HIP_CHECK(hipDeviceSynchronize());
         drop_graphs(b, MNRT_PHASE_KNN_BWD);
@@ -502,6 +505,7 @@ extern "C" int mnrt_batch_capture(mnrt_batch* b, const mnrt_batch_io* io, uint32
     }
     b->graph[phases] = g;
     b->exec[phases] = x;
+    if (phases & MNRT_PHASE_KNN_BWD) b->bwd_graph_photons = b->h_poff[b->ns];
     return MNRT_OK;
 }
 

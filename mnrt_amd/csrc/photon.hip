@@ -148,6 +148,8 @@ extern "C" int mnrt_photons_generate(const mnrt_tree* tree, const float* d_v0, c
     if (!tree || !d_v0 || !d_v1 || !d_v2 || !d_albedo4 || !d_lights || num_lights == 0 || !d_out_pos4 ||
         !d_out_pow4 || !d_out_dir4 || !out_count || !out_paths || max_bounces > 8)
         return MNRT_ERR_ARG;
+    // the record holds 4 triangles: a photon stored at bounce b scattered from b of them
+    if (d_out_path_tris && max_bounces > 4) return MNRT_ERR_ARG;
     if (tree->kind != MNRT_KIND_TRI || !tree->d_nodes || !tree->d_recs) return MNRT_ERR_STATE;
     hipStream_t st = (hipStream_t)stream;
     *out_count = 0;

@@ -117,7 +117,7 @@ class BatchIO(ctypes.Structure):
 PHASE_TRACE, PHASE_SHADE, PHASE_KNN, PHASE_KNN_BWD = 1, 2, 4, 8
 
 # MNRT_ABI_VERSION of the header these declarations mirror
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 # status codes (the MNRT_OK / MNRT_ERR_* values of mnrt_kd.h)
@@ -180,6 +180,12 @@ def load():
     L.mnrt_knn_gather_bwd.argtypes = [TP, u32, f32, u32] + [vp] * 12
     L.mnrt_photons_generate.argtypes = [TP, vp, vp, vp, vp, vp, u32, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp,
                                         ctypes.POINTER(u32), ctypes.POINTER(u32), vp]
+    L.mnrt_photons_reweight.argtypes = [u32, vp, vp, vp, u32, vp, u32, vp, u32, vp, vp, vp, vp]
+    L.mnrt_photons_reweight_bwd.argtypes = [u32, vp, vp, vp, u32, vp, u32, vp, u32, vp, vp, vp, vp, vp, vp]
+    L.mnrt_photons_spherical_to_dir.argtypes = [u32, vp, vp, vp, vp]
+    L.mnrt_photons_dir_to_spherical.argtypes = [u32, vp, vp, vp, vp]
+    L.mnrt_scan_u32.argtypes = [vp, u32, ctypes.c_int, vp, vp, vp]
+    L.mnrt_compact_u32.argtypes = [vp, vp, u32, vp, vp, vp]
     L.mnrt_shading_points.argtypes = [u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(u32), vp]
     L.mnrt_bounce_rays.argtypes = [u32, vp, vp, vp, vp, u32, vp, vp, vp, vp]

@@ -292,3 +292,24 @@ def random_surface_photons(scene: Scene, n: int, seed: int):
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return _f4(pos.astype(np.float32)), _f4(power.astype(np.float32)), _f4(d.astype(np.float32))
+
+
+def batched_workload(num_scenes: int, seed: int, query_hi: int = 1 << 20, num_lights: int = 1,
+                     scene_seed: int = 4):
+    """A many-scene workload: per scene a scene_sized scene with a triangle count
+    log-uniform in [1k, 50k], a photon count log-uniform in [4k, 256k] and a pinhole camera
+    of about a query count log-uniform in [16k, query_hi] pixels (width a multiple of 8).
+    Returns (scenes, photon counts, (W, H) per scene), all drawn from `seed`; scene i uses
+    seed scene_seed + i."""
+    rng = np.random.default_rng(seed)
+
+    def lu(lo, hi):
+        return np.exp(rng.uniform(np.log(lo), np.log(hi), num_scenes))
+
+    ntri, nph, nq = lu(1000, 50000).astype(int), lu(4096, 262144).astype(int), lu(16384, query_hi).astype(int)
+    scenes = [scene_sized(int(n), seed=scene_seed + i, num_lights=num_lights) for i, n in enumerate(ntri)]
+    dims = []
+    for q in nq:
+        w = 8 * max(1, int(round(np.sqrt(q) / 8)))
+        dims.append((w, max(1, int(round(q / w)))))
+    return scenes, [int(p) for p in nph], dims

@@ -92,8 +92,30 @@ def _c(a, dt=np.float32):
     return np.ascontiguousarray(a, dtype=dt)
 
 
+def host_threads() -> int:
+    """Host threads for the oracle: the cores this process may run on, capped by
+    OMP_NUM_THREADS when set (the share of a shared host a job is given)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    """The host CPU's model string (/proc/cpuinfo), for reports."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def _threads(n):
-    return n if n else max(1, os.cpu_count() or 1)
+    return n if n else host_threads()
 
 
 class Tree:
@@ -332,3 +354,61 @@ def photons(tree: Tree, v0, v1, v2, albedo, lights, target, max_bounces=4, seed=
     n = cnt.value
     return dict(pos=pos[:n], power=pw[:n], dir=dr[:n], meta=meta[:n], path=path[:n], tris=tris.reshape(-1, 4)[:n],
                 count=n, paths=shot.value)
+
+
+# --- photon reweighting by emission and albedo (fp64, numpy) ----------------------------
+# Restates the forward and adjoint of mnrt_photons_reweight: with a photon's path fixed its
+# stored power is the emission-only power of its light times the albedo of every triangle it
+# scattered from (the tracer's bookkeeping, photon_build.cu:79-130 and :278-331,
+# RTCore.cpp:545-581); a hit's colour is its triangle's albedo.  Pinned in
+# tests/test_oracle_reweight.py: emission 1 and the trace-time albedo give back the powers
+# the photon tracer stored (an independent code path), and the gradient equals fp64
+# central differences of the forward and a loop over every (photon, path position) pair.
+
+def _path_albedos(tris, albedo):
+    """(P, 4, 3) albedo of each recorded triangle, 1 for the padding (index < 0)."""
+    t = np.asarray(tris).astype(np.int64)
+    t = np.where(t >= 2 ** 31, t - 2 ** 32, t)  # uint32 padding 0xFFFFFFFF = -1
+    a = np.ones(t.shape + (3,), np.float64)
+    ok = t >= 0
+    a[ok] = np.asarray(albedo, np.float64)[t[ok]]
+    return t, a
+
+
+def reweight(base, light, tris, emission, albedo):
+    """fp64 photon powers (P, 3) = base * emission[light] * prod of path albedos."""
+    _, a = _path_albedos(tris, albedo)
+    p = np.asarray(base, np.float64)[:, :3] * np.asarray(emission, np.float64)[np.asarray(light, np.int64)]
+    return p * a.prod(axis=1)
+
+
+def reweight_grad(base, light, tris, emission, albedo, grad_pow, hit_tri=None, grad_clr=None):
+    """fp64 (dLoss/demission (L, 3), dLoss/dalbedo (T, 3)) of the reweighting given
+    dLoss/dpower (P, >=3) and optionally dLoss/dclr at hits on triangles hit_tri (-1: none)."""
+    t, a = _path_albedos(tris, albedo)
+    b = np.asarray(base, np.float64)[:, :3]
+    g = np.asarray(grad_pow, np.float64)[:, :3]
+    lt = np.asarray(light, np.int64)
+    e = np.asarray(emission, np.float64)
+    ge = np.zeros(e.shape, np.float64)
+    ga = np.zeros(np.asarray(albedo).shape[:1] + (3,), np.float64)
+    np.add.at(ge, lt, g * b * a.prod(axis=1))
+    for k in range(t.shape[1]):
+        others = np.prod(np.delete(a, k, axis=1), axis=1)
+        ok = t[:, k] >= 0
+        np.add.at(ga, t[ok, k], (g * b * e[lt] * others)[ok])
+    if hit_tri is not None:
+        h = np.asarray(hit_tri).astype(np.int64)
+        h = np.where(h >= 2 ** 31, h - 2 ** 32, h)
+        ok = h >= 0
+        np.add.at(ga, h[ok], np.asarray(grad_clr, np.float64)[ok, :3])
+    return ge, ga
+
+
+def base_powers(meta, lights, paths):
+    """Emission-only powers (P, 3) of a traced map: light power * light count, doubled per
+    Russian-roulette survival (bounce index > 3), / paths (fp64)."""
+    meta = np.asarray(meta).astype(np.int64)
+    lt, bounce = meta & 0xFF, (meta >> 8) & 0xFF
+    lp = np.asarray(lights, np.float64)[:, 3, :]
+    return lp[lt] * lp.shape[0] * np.exp2(np.maximum(bounce - 3, 0))[:, None] / float(paths)

@@ -20,7 +20,7 @@ def test_status_strings():
     L = N.load()
     assert L.mnrt_status_string(0) == b"ok"
     assert L.mnrt_status_string(-1) == b"invalid argument"
-    assert L.mnrt_abi_version() == N.ABI_VERSION == 2
+    assert L.mnrt_abi_version() == N.ABI_VERSION == 3
 
 
 def test_argument_errors_without_device():
@@ -57,6 +57,36 @@ def test_argument_errors_without_device():
     t.kind = N.KIND_POINT
     light.type = N.LIGHT_POINT
     assert L.mnrt_trace_shadow(ctypes.byref(t), ctypes.byref(light), 1, fake, None, 0.0, fake, None) == N.ERR_STATE
+
+
+def test_reweight_argument_errors_without_device():
+    L = N.load()
+    fake = 256  # never dereferenced: every call below returns before launching
+    # photons but no base / light / path arrays, no emission, or no output
+    assert L.mnrt_photons_reweight(4, None, fake, fake, 1, fake, 1, fake, 0, None, fake, None, None) == N.ERR_ARG
+    assert L.mnrt_photons_reweight(4, fake, fake, fake, 0, fake, 1, fake, 0, None, fake, None, None) == N.ERR_ARG
+    assert L.mnrt_photons_reweight(4, fake, fake, fake, 1, fake, 1, fake, 0, None, None, None, None) == N.ERR_ARG
+    # hits without hit triangles or without a colour buffer
+    assert L.mnrt_photons_reweight(0, None, None, None, 0, None, 1, fake, 3, None, None, fake, None) == N.ERR_ARG
+    assert L.mnrt_photons_reweight(0, None, None, None, 0, None, 1, fake, 3, fake, None, None, None) == N.ERR_ARG
+    assert L.mnrt_photons_reweight(0, None, None, None, 0, None, 0, None, 0, None, None, None, None) == N.OK
+    # adjoint: missing upstream gradients or outputs
+    assert L.mnrt_photons_reweight_bwd(4, fake, fake, fake, 1, fake, 1, fake, 0, None, None, None, fake, fake,
+                                       None) == N.ERR_ARG
+    assert L.mnrt_photons_reweight_bwd(4, fake, fake, fake, 1, fake, 1, fake, 0, None, fake, None, None, fake,
+                                       None) == N.ERR_ARG
+    assert L.mnrt_photons_reweight_bwd(4, fake, fake, fake, 1, fake, 1, fake, 2, fake, fake, None, fake, fake,
+                                       None) == N.ERR_ARG
+
+
+def test_primitive_argument_errors_without_device():
+    L = N.load()
+    fake = 256
+    assert L.mnrt_scan_u32(None, 5, 0, fake, None, None) == N.ERR_ARG
+    assert L.mnrt_scan_u32(fake, 5, 1, None, None, None) == N.ERR_ARG
+    assert L.mnrt_compact_u32(fake, fake, 5, fake, None, None) == N.ERR_ARG
+    assert L.mnrt_compact_u32(fake, fake, 5, fake, fake, None) == N.ERR_ARG  # in place
+    assert L.mnrt_compact_u32(fake, None, 5, 512, fake, None) == N.ERR_ARG
 
 
 def test_product_fails_loudly_without_library(monkeypatch, tmp_path):

@@ -5,7 +5,7 @@ through a batch view give the single-scene results bit for bit."""
 import numpy as np
 import pytest
 
-from mnrt_amd import api, scene as S
+from mnrt_amd import api, native as N, scene as S
 
 pytestmark = pytest.mark.gpu
 
@@ -327,3 +327,58 @@ def test_query_batch_bwd_equals_plain(orc, cuda):
             sl = slice(int(X["roff"][s]), int(X["roff"][s]) + int(cnt[s]))
             assert torch.equal(B["grad_clr4"][sl].view(torch.int32), ref["grad_clr4"][sl].view(torch.int32)), rep
     Q.close()
+
+
+def test_bwd_graph_dropped_when_photon_total_changes(orc, cuda):
+    """A captured adjoint phase clears and converts the photon total it was captured with:
+    after update_trees to maps with fewer photons its replay is refused (MNRT_ERR_STATE)
+    instead of writing past the new gradient buffer; eager runs and a new capture write
+    exactly the new total (guard rows behind it stay untouched)."""
+    torch = cuda
+    sizes, photons, res, k = [1200, 2500, 900], [5000, 9000, 4000], [(24, 24), (32, 24), (16, 16)], 30
+    X = _query_setup(cuda, sizes, photons, res, seed=120)
+    Q = api.QueryBatch(X["TB"], X["toff"], X["roff"], X["eps"], X["PB"], X["rmax"], k)
+    B = _batch_io(cuda, Q, X, k)
+    n = max(int(X["roff"][-1]), 1)
+    B["grad_rad4"] = torch.rand((n, 4), device="cuda")
+    B["out_count"] = torch.empty(n, dtype=torch.int32, device="cuda")
+    B["grad_pow4"] = torch.zeros((int(Q.photon_offsets[-1]), 4), device="cuda")
+    B["grad_clr4"] = torch.zeros((n, 4), device="cuda")
+    Q.make_io(**B)
+    Q.run(7)
+    Q.capture(8)
+    Q.replay(8)
+    cuda.cuda.synchronize()
+    # the same scenes with the first half of each photon map
+    ppos, ppow, pdir = X["photons"]
+    keep = [np.arange(int(X["poff"][s]), int(X["poff"][s]) + (int(X["poff"][s + 1] - X["poff"][s]) // 2))
+            for s in range(len(sizes))]
+    idx = torch.from_numpy(np.concatenate(keep)).cuda()
+    p2 = [t[idx].contiguous() for t in (ppos, ppow, pdir)]
+    poff2 = np.zeros(len(sizes) + 1, np.uint32)
+    poff2[1:] = np.cumsum([len(kk) for kk in keep])
+    boxes = np.stack([np.concatenate(sc.bounds()) for sc in X["scs"]]).astype(np.float32)
+    PB2 = api.build_points_batch(p2[0], poff2, root_boxes=boxes, max_query_radius=X["rmax"])
+    api.attach_photon_records_batch(PB2, *p2)
+    Q.update_trees(X["TB"], PB2)
+    with pytest.raises(N.MnrtError) as ex:
+        Q.replay(8)
+    assert "(-6)" in str(ex.value)
+    total = int(poff2[-1])
+    guard = 16
+    gp = torch.full((total + guard, 4), 7.0, device="cuda")
+    B["grad_pow4"] = gp
+    Q.make_io(**B)
+    Q.run(4)
+    Q.run(8)
+    cuda.cuda.synchronize()
+    assert bool((gp[total:] == 7.0).all())
+    eager = gp[:total].clone()
+    Q.capture(8)
+    gp.fill_(7.0)
+    Q.replay(8)
+    cuda.cuda.synchronize()
+    assert bool((gp[total:] == 7.0).all())
+    assert torch.equal(gp[:total].view(torch.int32), eager.view(torch.int32))
+    Q.close()
+    PB2.release()

@@ -1,0 +1,413 @@
+"""The benchmark workloads at their full sizes, checked against the oracle.
+
+Plain large (1.03M triangles, 1M traced photons, 1920x1080 primary rays plus one bounce
+ray per hit, k = 100): triangle tree, photons and photon tree equal the oracle's word for
+word; closest hits of every primary and bounce ray bit-exact; kNN radiance, r2 and
+neighbour counts at every hit bit-exact to the oracle's fp32 path (and within 1e-5 of its
+fp64 path) through the list-free path the benchmark times; neighbour lists on a sample.
+
+Differentiable (small scene + 3 lights, 1M photons, 1920x1080, k = 50, target from random
+per-light emission in [0.5, 2]): dLoss/dpower, dLoss/dcolour, dLoss/demission and
+dLoss/dalbedo against the oracle's fp64 adjoint over every query; fp64 central
+differences on 1000 coordinates, each over the queries whose sets hold the coordinate.
+
+Batched (256 scenes): after one captured batched step every 16th scene's triangle and
+photon trees equal the oracle's, and a 1-in-64 sample of its rays and kNN queries match
+the oracle bit for bit.
+
+Batched inverse lighting (64 scenes x 4 lights): the batched gradient against the
+oracle's fp64 chain per scene, the radiance at the fitted parameters after 200 Adam steps
+against the oracle per scene (kNN sets do not depend on the powers, so the oracle's sets
+from the first check serve both)."""
+
+import numpy as np
+import pytest
+
+from mnrt_amd import adjoint, api, inverse, scene as S
+from helpers import host, same_tree
+
+pytestmark = pytest.mark.gpu
+
+NONE = 0xFFFFFFFF
+
+
+# --- plain large ----------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def large(orc, cuda):
+    torch = cuda
+    sc = S.scene_large()
+    eps, rmax = sc.ray_epsilon(), sc.r_max()
+    print(f"plain large: {sc.num_tris} triangles", flush=True)
+    v = [api.dev(a) for a in (sc.v0, sc.v1, sc.v2, sc.albedo)]
+    T = api.build_tri(*v[:3])
+    ph = api.photons_generate(T, *v, sc.lights, 1_000_000, 4, 0, eps)
+    root = np.concatenate(sc.bounds()).astype(np.float32)
+    P = api.build_points(ph["pos"], api.default_params(1, root, rmax))
+    api.attach_photon_records(P, ph["pos"], ph["power"], ph["dir"])
+    o, d, _ = api.primary_rays(1920, 1080)
+    tri, t, _, _ = api.trace_closest(T, o, d, eps)
+    q1 = api.shading_points(o, d, tri, t, *v)
+    bo, bd, binfl = api.bounce_rays(q1, 3)
+    tri2, t2, _, _ = api.trace_closest(T, bo, bd, eps)
+    q2 = api.shading_points(bo, bd, tri2, t2, *v, infl=binfl)
+    torch.cuda.synchronize()
+    ot = orc.build(0, sc.v0, sc.v1, sc.v2)
+    X = dict(sc=sc, eps=eps, rmax=rmax, root=root, T=T, P=P, ph=ph, ot=ot,
+             rays=[(o, d, tri, t), (bo, bd, tri2, t2)], q=[q1, q2],
+             ppos=host(ph["pos"]), ppow=host(ph["power"]), pdir=host(ph["dir"]))
+    yield X
+    T.release()
+    P.release()
+
+
+def test_large_trees_and_photons_equal_oracle(orc, large):
+    X = large
+    sc = X["sc"]
+    same_tree(orc, 0, X["T"], X["ot"])
+    oph = orc.photons(X["ot"], sc.v0, sc.v1, sc.v2, sc.albedo, sc.lights, 1_000_000, 4, 0, X["eps"])
+    assert oph["count"] == X["ph"]["count"] and oph["paths"] == X["ph"]["paths"]
+    for k in ("pos", "power", "dir"):
+        assert np.array_equal(X[{"pos": "ppos", "power": "ppow", "dir": "pdir"}[k]].view(np.uint32),
+                              oph[k].view(np.uint32)), k
+    op = orc.build(1, X["ppos"], max_qr=X["rmax"], root=X["root"])
+    same_tree(orc, 1, X["P"], op, X["rmax"])
+    X["op"] = op
+    print(f"trees: {X['T'].num_nodes} / {X['P'].num_nodes} nodes, {oph['count']} photons equal", flush=True)
+
+
+def test_large_hits_equal_oracle(orc, large):
+    X = large
+    sc = X["sc"]
+    for name, (o, d, tri, t) in zip(("primary", "bounce"), X["rays"]):
+        otri, ot_, _, _ = orc.trace(X["ot"], sc.v0, sc.v1, sc.v2, host(o), host(d), X["eps"])
+        gtri, gt = host(tri), host(t)
+        assert np.array_equal(gtri, otri), f"{name}: {(gtri != otri).sum()} of {len(otri)} hit indices differ"
+        hit = otri >= 0
+        assert np.array_equal(gt[hit].view(np.uint32), ot_[hit].view(np.uint32)), f"{name}: distances differ"
+        print(f"{name}: {len(otri)} rays, {int(hit.sum())} hits equal", flush=True)
+
+
+def _oracle_tree(orc, X):
+    if "op" not in X:
+        X["op"] = orc.build(1, X["ppos"], max_qr=X["rmax"], root=X["root"])
+    return X["op"]
+
+
+def test_large_knn_list_free_path_equals_oracle(orc, large):
+    """The timed path: no neighbour lists requested, so the final sort is skipped."""
+    X = large
+    op = _oracle_tree(orc, X)
+    K, rmax = 100, X["rmax"]
+    for name, q in zip(("primary", "bounce"), X["q"]):
+        out = api.knn_gather(X["P"], K, rmax, q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"], want_r2=True,
+                             want_count=True)
+        qh = {k: host(q[k]) for k in ("pos", "nrm", "dir", "infl", "clr")}
+        nbr, cnt, _ = orc.knn(op, X["ppos"], qh["pos"], K, rmax)
+        o32, o64, _, _, r2 = orc.radiance(nbr, cnt, K, rmax, X["ppos"], X["ppow"], X["pdir"], qh["pos"], qh["nrm"],
+                                          qh["dir"], qh["infl"], qh["clr"])
+        gc, gr2, grad = host(out["count"]).view(np.uint32), host(out["r2"]), host(out["rad"])
+        assert np.array_equal(gc, cnt), f"{name}: {(gc != cnt).sum()} counts differ"
+        assert np.array_equal(gr2.view(np.uint32), r2.view(np.uint32)), f"{name}: r2 differs"
+        bad = (grad.view(np.uint32) != o32.view(np.uint32)).any(axis=1)
+        assert not bad.any(), f"{name}: radiance of {bad.sum()} of {len(bad)} queries not bit-exact"
+        scale = np.abs(o64).max(axis=1, keepdims=True) + 1e-30
+        err = (np.abs(grad.astype(np.float64) - o64) / scale).max()
+        assert err <= 1e-5, err
+        print(f"kNN {name}: {len(cnt)} queries k={K} bit-exact (list-free), max rel err vs fp64 {err:.2e}",
+              flush=True)
+
+
+def test_large_knn_lists_on_sample(orc, large):
+    X = large
+    op = _oracle_tree(orc, X)
+    K, rmax = 100, X["rmax"]
+    for name, q in zip(("primary", "bounce"), X["q"]):
+        idx = np.arange(0, q["count"], 64)
+        qs = {k: q[k][idx].contiguous() for k in ("pos", "nrm", "dir", "infl", "clr")}
+        out = api.knn_gather(X["P"], K, rmax, qs["pos"], qs["nrm"], qs["dir"], qs["infl"], qs["clr"], want_nbr=True,
+                             want_count=True)
+        nbr, cnt, _ = orc.knn(op, X["ppos"], host(qs["pos"]), K, rmax)
+        assert np.array_equal(host(out["nbr"]).view(np.uint32), nbr), f"{name}: neighbour lists differ"
+        assert np.array_equal(host(out["count"]).view(np.uint32), cnt)
+        print(f"kNN lists {name}: {len(idx)} sampled queries equal", flush=True)
+
+
+# --- differentiable ---------------------------------------------------------------------------
+
+def test_differentiable_full_size_vs_oracle_and_fd(orc, cuda):
+    torch = cuda
+    K = 50
+    sc = S.scene_small(subdiv=5, num_lights=3)
+    eps, rmax = sc.ray_epsilon(), sc.r_max()
+    v = [api.dev(a) for a in (sc.v0, sc.v1, sc.v2, sc.albedo)]
+    T = api.build_tri(*v[:3])
+    ph = api.photons_generate(T, *v, sc.lights, 1_000_000, 4, 0, eps, want_meta=True)
+    o, d, _ = api.primary_rays(1920, 1080)
+    tri, t, _, _ = api.trace_closest(T, o, d, eps)
+    q = api.shading_points(o, d, tri, t, *v, want_src=True)
+    root = np.concatenate(sc.bounds()).astype(np.float32)
+    P = api.build_points(ph["pos"], api.default_params(1, root, rmax))
+    base = inverse.base_powers(ph["meta"], sc.lights, ph["paths"])
+    light, ptris = (ph["meta"] & 0xFF).contiguous(), ph["tris"].contiguous()
+    g = torch.Generator(device="cpu").manual_seed(11)
+    e_t = (0.5 + 1.5 * torch.rand((3, 3), generator=g)).cuda()
+    alb0 = v[3][:, :3].contiguous()
+    hit = q["tri"]
+    with torch.no_grad():
+        pw_t = adjoint.reweight_photons(base, light, ptris, e_t, alb0)
+        target = adjoint.knn_radiance(pw_t, adjoint.hit_colours(alb0, hit, q["clr"]), P, ph["pos"], ph["dir"], q, K,
+                                      rmax).clone()
+    e = torch.ones((3, 3), device="cuda", requires_grad=True)
+    a = alb0.clone().requires_grad_(True)
+    power = adjoint.reweight_photons(base, light, ptris, e, a)
+    power.retain_grad()
+    clr = adjoint.hit_colours(a, hit, q["clr"])
+    clr.retain_grad()
+    rad = adjoint.knn_radiance(power, clr, P, ph["pos"], ph["dir"], q, K, rmax)
+    loss = ((rad - target)[:, :3].double() ** 2).sum()
+    loss.backward()
+    nq = q["count"]
+    print(f"differentiable: {ph['count']} photons, {nq} queries, loss {float(loss.detach()):.4e}", flush=True)
+    # oracle: sets over every query, fp64 adjoint of the estimate, fp64 chain to the parameters
+    ppos, ppow, pdir = host(ph["pos"]), host(power), host(ph["dir"])
+    qh = {k: host(q[k]) for k in ("pos", "nrm", "dir", "infl")}
+    qh["clr"] = host(clr)
+    op = orc.build(1, ppos, max_qr=rmax, root=root)
+    nbr, cnt, _ = orc.knn(op, ppos, qh["pos"], K, rmax)
+    G = (2.0 * (host(rad).astype(np.float64) - host(target))).astype(np.float32)
+    G[:, 3] = 0
+    ogp, ogc = orc.radiance_grad(nbr, cnt, K, rmax, ppos, ppow, pdir, qh["pos"], qh["nrm"], qh["dir"], qh["infl"],
+                                 qh["clr"], G)
+    mgp, mgc = orc.radiance_grad(nbr, cnt, K, rmax, ppos, ppow, pdir, qh["pos"], qh["nrm"], qh["dir"], qh["infl"],
+                                 qh["clr"], np.abs(G))
+    gp, gc = host(power.grad), host(clr.grad)
+    for name, got, want, mag in (("power", gp, ogp, mgp), ("colour", gc, ogc, mgc)):
+        err = np.abs(got[:, :3].astype(np.float64) - want[:, :3])
+        bound = 1e-5 * mag[:, :3] + 1e-7 * mag[:, :3].max()
+        print(f"dLoss/d{name}: max err / bound {np.max(err / bound):.3f}", flush=True)
+        assert np.all(err <= bound), name
+    hit_h = host(hit)
+    oe, oa = orc.reweight_grad(host(base), host(light), host(ptris), host(e), host(a), ogp, hit_h, ogc)
+    me, ma = orc.reweight_grad(host(base), host(light), host(ptris), host(e), host(a), mgp, hit_h, mgc)
+    for name, got, want, mag in (("emission", host(e.grad), oe, me), ("albedo", host(a.grad), oa, ma)):
+        err = np.abs(got.astype(np.float64) - want)
+        bound = 2e-5 * mag + 1e-7 * mag.max()
+        print(f"dLoss/d{name}: max err / bound {np.max(err / bound):.3f}", flush=True)
+        assert np.all(err <= bound), name
+    # fp64 central differences of a positive linear functional on 1000 coordinates, each
+    # over the queries whose neighbour sets hold it (the others do not depend on it)
+    rng = np.random.default_rng(0)
+    W = torch.from_numpy(rng.uniform(0.5, 1.5, size=(nq, 4)).astype(np.float32)).cuda()
+    W[:, 3] = 0
+    p2 = power.detach().clone().requires_grad_(True)
+    c2 = clr.detach().clone().requires_grad_(True)
+    (adjoint.knn_radiance(p2, c2, P, ph["pos"], ph["dir"], q, K, rmax) * W).sum().backward()
+    gpw, gcw = host(p2.grad), host(c2.grad)
+    Wh = host(W).astype(np.float64)
+    used = np.unique(nbr[nbr != NONE])
+    chosen = np.unique(rng.choice(used, 800, replace=False))
+    rows, cols = np.nonzero(np.isin(nbr, chosen))
+    owner = nbr[rows, cols]
+    order = np.argsort(owner, kind="stable")
+    rows, owner = rows[order], owner[order]
+    starts = np.searchsorted(owner, chosen)
+    ends = np.searchsorted(owner, chosen, side="right")
+
+    def loss_sub(qrows, pw, cl):
+        nb = nbr[qrows]
+        u, inv = np.unique(nb[nb != NONE], return_inverse=True)
+        local = np.full(nb.shape, NONE, np.uint32)
+        local[nb != NONE] = inv.astype(np.uint32)
+        _, o64, _, _, _ = orc.radiance(local, cnt[qrows], K, rmax, ppos[u], pw[u], pdir[u], qh["pos"][qrows],
+                                       qh["nrm"][qrows], qh["dir"][qrows], qh["infl"][qrows], cl[qrows])
+        return float((o64[:, :3] * Wh[qrows, :3]).sum())
+
+    def step(x, typical):
+        return float(2.0 ** (np.floor(np.log2(max(abs(float(x)), typical))) - 10))
+
+    p_scale = float(np.median(ppow[:, :3]))
+    worst, worst_at = 0.0, None
+    scale_p, scale_c = np.abs(gpw).max(), np.abs(gcw).max()
+    for i, p in enumerate(chosen):
+        qrows = rows[starts[i]:ends[i]]
+        c = int(rng.integers(0, 3))
+        h = step(ppow[p, c], p_scale)
+        up, dn = ppow.copy(), ppow.copy()
+        up[p, c] += np.float32(h)
+        dn[p, c] -= np.float32(h)
+        fd = (loss_sub(qrows, up, qh["clr"]) - loss_sub(qrows, dn, qh["clr"])) / (float(up[p, c]) - float(dn[p, c]))
+        err = abs(gpw[p, c] - fd) / (abs(fd) + 1e-6 * scale_p)
+        if err > worst:
+            worst, worst_at = err, ("p", int(p), c, float(gpw[p, c]), fd)
+    for i in rng.choice(nq, 1000 - len(chosen), replace=False):
+        c = int(rng.integers(0, 3))
+        h = step(qh["clr"][i, c], 0.5)
+        up, dn = qh["clr"].copy(), qh["clr"].copy()
+        up[i, c] += np.float32(h)
+        dn[i, c] -= np.float32(h)
+        r = np.array([i])
+        fd = (loss_sub(r, ppow, up) - loss_sub(r, ppow, dn)) / (float(up[i, c]) - float(dn[i, c]))
+        err = abs(gcw[i, c] - fd) / (abs(fd) + 1e-6 * scale_c)
+        if err > worst:
+            worst, worst_at = err, ("c", int(i), c, float(gcw[i, c]), fd)
+    print(f"FD worst relative error over 1000 coordinates: {worst:.2e} at {worst_at}", flush=True)
+    assert worst < 1e-5, worst_at
+    T.release()
+    P.release()
+
+
+# --- batched ----------------------------------------------------------------------------------
+
+def test_batched_256_scenes_sampled_vs_oracle(orc, cuda):
+    torch = cuda
+    NS, K = 256, 50
+    scs, nph, dims = S.batched_workload(NS, seed=4)
+    cat = lambda a: api.dev(np.concatenate([getattr(sc, a) for sc in scs]).astype(np.float32))
+    v0, v1, v2, alb = cat("v0"), cat("v1"), cat("v2"), cat("albedo")
+    toff = np.zeros(NS + 1, np.uint32)
+    toff[1:] = np.cumsum([sc.num_tris for sc in scs])
+    roff = np.zeros(NS + 1, np.uint32)
+    roff[1:] = np.cumsum([w * h for w, h in dims])
+    R = int(roff[-1])
+    go = torch.empty((R, 4), dtype=torch.float32, device="cuda")
+    gd = torch.empty((R, 4), dtype=torch.float32, device="cuda")
+    for s, (w, h) in enumerate(dims):
+        api.primary_rays(w, h, out_orig=go[roff[s]:roff[s + 1]], out_dir=gd[roff[s]:roff[s + 1]])
+    eps = np.array([sc.ray_epsilon() for sc in scs], np.float32)
+    rmax = np.array([sc.r_max() for sc in scs], np.float32)
+    boxes = np.stack([np.concatenate(sc.bounds()) for sc in scs]).astype(np.float32)
+    tsl = [slice(int(toff[s]), int(toff[s + 1])) for s in range(NS)]
+    TB0 = api.build_tri_batch(v0, v1, v2, toff)
+    ph = [api.photons_generate(TB0[s], v0[tsl[s]], v1[tsl[s]], v2[tsl[s]], alb[tsl[s]], scs[s].lights, nph[s], 4, s,
+                               float(eps[s])) for s in range(NS)]
+    poff = np.zeros(NS + 1, np.uint32)
+    poff[1:] = np.cumsum([g["count"] for g in ph])
+    ppos, ppow, pdir = (torch.cat([g[k] for g in ph]).contiguous() for k in ("pos", "power", "dir"))
+    del ph
+    TB0.release()
+    print(f"batched: {NS} scenes, {int(toff[-1])} triangles, {int(poff[-1])} photons, {R} rays", flush=True)
+    # one batched step: both batched builds, then every phase replayed from its graph
+    TB = api.build_tri_batch(v0, v1, v2, toff)
+    PB = api.build_points_batch(ppos, poff, root_boxes=boxes, max_query_radius=rmax)
+    api.attach_photon_records_batch(PB, ppos, ppow, pdir)
+    Q = api.QueryBatch(TB, toff, roff, eps, PB, rmax, K)
+    f4 = lambda: torch.empty((R, 4), dtype=torch.float32, device="cuda")
+    bufs = dict(orig4=go, dir4=gd, v0=v0, v1=v1, v2=v2, albedo4=alb,
+                hit_tri=torch.empty(R, dtype=torch.int32, device="cuda"),
+                hit_t=torch.empty(R, dtype=torch.float32, device="cuda"),
+                sp_pos4=f4(), sp_nrm4=f4(), sp_dir4=f4(), sp_clr4=f4(), sp_infl4=f4(),
+                sp_count=torch.zeros(NS, dtype=torch.int32, device="cuda"), io_rad4=f4())
+    Q.make_io(**bufs)
+    Q.autotune(max_slots=min(R, 8 << 20))
+    for ph_ in (1, 2, 4):
+        Q.capture(ph_)
+    bufs["io_rad4"].zero_()
+    for ph_ in (1, 2, 4):
+        Q.replay(ph_)
+    torch.cuda.synchronize()
+    cnt = host(bufs["sp_count"])
+    hp = lambda t, sl: host(t[sl])
+    for s in range(0, NS, 16):
+        sc = scs[s]
+        ot = orc.build(0, sc.v0, sc.v1, sc.v2)
+        same_tree(orc, 0, TB[s], ot)
+        ps = slice(int(poff[s]), int(poff[s + 1]))
+        pp, pw, pd = hp(ppos, ps), hp(ppow, ps), hp(pdir, ps)
+        op = orc.build(1, pp, max_qr=float(rmax[s]), root=boxes[s])
+        same_tree(orc, 1, PB[s], op, float(rmax[s]))
+        ridx = np.arange(int(roff[s]), int(roff[s + 1]), 64)
+        otri, ot_, _, _ = orc.trace(ot, sc.v0, sc.v1, sc.v2, host(go[ridx]), host(gd[ridx]), float(eps[s]))
+        assert np.array_equal(host(bufs["hit_tri"][ridx]), otri), s
+        hit = otri >= 0
+        assert np.array_equal(host(bufs["hit_t"][ridx])[hit].view(np.uint32), ot_[hit].view(np.uint32)), s
+        qidx = np.arange(int(roff[s]), int(roff[s]) + int(cnt[s]), 64)
+        qh = {k: host(bufs["sp_" + k + "4"][qidx]) for k in ("pos", "nrm", "dir", "infl", "clr")}
+        nbr, c, _ = orc.knn(op, pp, qh["pos"], K, float(rmax[s]))
+        o32, _, _, _, _ = orc.radiance(nbr, c, K, float(rmax[s]), pp, pw, pd, qh["pos"], qh["nrm"], qh["dir"],
+                                       qh["infl"], qh["clr"])
+        got = host(bufs["io_rad4"][qidx])
+        assert np.array_equal(got.view(np.uint32), o32.view(np.uint32)), s
+        print(f"scene {s}: {sc.num_tris} tris, {ps.stop - ps.start} photons, {len(ridx)} rays and {len(qidx)} "
+              f"queries sampled: equal", flush=True)
+    Q.close()
+    TB.release()
+    PB.release()
+
+
+# --- batched inverse lighting -----------------------------------------------------------------
+
+def test_inverse_lighting_64_scenes_vs_oracle(orc, cuda):
+    torch = cuda
+    NS, K = 64, 50
+    scs, nph, dims = S.batched_workload(NS, seed=5, query_hi=262144, num_lights=4, scene_seed=1000)
+    P = inverse.BatchedInverseLighting(scs, nph, dims, k=K, seed=7)
+    L = int(P.loff[-1])
+    g = torch.Generator(device="cpu").manual_seed(5)
+    e_true = (0.5 + 1.5 * torch.rand((L, 3), generator=g)).cuda()
+    target = P.radiance(e_true, P.albedo0)
+    e0 = torch.ones((L, 3), device="cuda")
+    loss, ge, ga = P.loss_and_grad(e0, P.albedo0, P.full_target(target))
+    torch.cuda.synchronize()
+    print(f"inverse lighting: {NS} scenes, {int(P.poff[-1])} photons, {P.num_queries} queries, "
+          f"loss {float(loss):.4e}", flush=True)
+    Gall = host(P.bufs["grad_rad4"])
+    ogp = np.zeros((int(P.poff[-1]), 4))
+    mgp = np.zeros_like(ogp)
+    R = Gall.shape[0]
+    ogc, mgc = np.zeros((R, 4)), np.zeros((R, 4))
+    sets = []
+    for s in range(NS):
+        A = P.scene_arrays(s)
+        op = orc.build(1, A["ppos"], max_qr=A["r_max"], root=A["root"])
+        nbr, cnt, _ = orc.knn(op, A["ppos"], A["qpos"], K, A["r_max"])
+        sets.append((nbr, cnt))
+        o32, _, _, _, _ = orc.radiance(nbr, cnt, K, A["r_max"], A["ppos"], A["ppow"], A["pdir"], A["qpos"], A["qnrm"],
+                                       A["qdir"], A["qinfl"], A["qclr"])
+        assert np.array_equal(A["rad"].view(np.uint32), o32.view(np.uint32)), f"scene {s}: forward radiance"
+        qs = slice(int(P.roff[s]), int(P.roff[s]) + A["qpos"].shape[0])
+        ps = slice(int(P.poff[s]), int(P.poff[s + 1]))
+        G = Gall[qs]
+        args = (nbr, cnt, K, A["r_max"], A["ppos"], A["ppow"], A["pdir"], A["qpos"], A["qnrm"], A["qdir"],
+                A["qinfl"], A["qclr"])
+        ogp[ps], ogc[qs] = orc.radiance_grad(*args, G)
+        mgp[ps], mgc[qs] = orc.radiance_grad(*args, np.abs(G))
+    ht = host(P.slot_tri)
+    a0 = host(P.albedo0)
+    oe, oa = orc.reweight_grad(host(P.base), host(P.light), host(P.tris), host(e0), a0, ogp, ht, ogc)
+    me, ma = orc.reweight_grad(host(P.base), host(P.light), host(P.tris), host(e0), a0, mgp, ht, mgc)
+    worst = []
+    for s in range(NS):
+        ls, ts = slice(int(P.loff[s]), int(P.loff[s + 1])), slice(int(P.toff[s]), int(P.toff[s + 1]))
+        r = 0.0
+        for got, want, mag in ((host(ge)[ls], oe[ls], me[ls]), (host(ga)[ts], oa[ts], ma[ts])):
+            err = np.abs(got.astype(np.float64) - want)
+            bound = 2e-5 * mag + 1e-7 * mag.max()
+            r = max(r, float(np.max(err / np.maximum(bound, 1e-300))))
+        worst.append(r)
+    print(f"per-scene gradient error / bound vs the fp64 oracle: max {max(worst):.3f}", flush=True)
+    assert max(worst) <= 1.0
+    e_fit, a_fit, losses = P.fit(target, steps=200, lr=1e-2)
+    print(f"200 Adam steps: loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+    assert losses[-1] < 0.2 * losses[0]
+    P.radiance(e_fit, a_fit)
+    errs = []
+    eh, ah = host(e_fit), host(a_fit)
+    for s in range(NS):
+        A = P.scene_arrays(s)
+        nbr, cnt = sets[s]
+        o32, o64, _, _, _ = orc.radiance(nbr, cnt, K, A["r_max"], A["ppos"], A["ppow"], A["pdir"], A["qpos"],
+                                         A["qnrm"], A["qdir"], A["qinfl"], A["qclr"])
+        assert np.array_equal(A["rad"].view(np.uint32), o32.view(np.uint32)), f"scene {s}: fitted radiance"
+        scale = np.abs(o64).max(axis=1) + 1e-30
+        errs.append(float((np.abs(A["rad"].astype(np.float64) - o64).max(axis=1) / scale).max()))
+        lt = A["light"].astype(np.int64) - int(P.loff[s])
+        tr = A["tris"].astype(np.int64)
+        tr = np.where(tr >= 0, tr - int(P.toff[s]), -1)
+        p64 = orc.reweight(A["base"], lt, tr, eh[int(P.loff[s]):int(P.loff[s + 1])],
+                           ah[int(P.toff[s]):int(P.toff[s + 1])])
+        # within fp32 rounding of the fp64 product (exact zeros where an albedo went to 0)
+        assert np.all(np.abs(A["ppow"][:, :3] - p64) <= 5e-7 * np.abs(p64)), s
+    print(f"per-scene radiance rel err vs fp64 at the fitted parameters: max {max(errs):.2e}", flush=True)
+    assert max(errs) <= 1e-5
+    P.release()
