@@ -120,13 +120,69 @@ def shadow_leg(torch, api, N, O, T, ttree, sc, pts, eps, sample, ncpu, reps=5):
                          "achieved": round(gbs, 2), "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)}}
 
 
+def _median_time(fn, reps):
+    """(result of a first, untimed call, median seconds of `reps` more calls); reps = 0 times
+    the single call."""
+    t0 = time.perf_counter()
+    r = fn()
+    first = time.perf_counter() - t0
+    if reps <= 0:
+        return r, first
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return r, float(np.median(ts))
+
+
+def build_leg(args, torch, api, O, v0, v1, v2, ppos, pparams, otri, opts):
+    """The reference's kd-tree construction benchmark (RTCore::BenchmarkKDTreeCon,
+    RTCore.cpp:287-327, as the UI runs it: 10 warm-up and 200 timed KDTreeTri builds,
+    MainFrame.cpp:755-792) on the large workload's triangles, and the same for its photon
+    tree.  Each build is timed by the host around the call and a device synchronisation
+    (as the reference's host timer) and by HIP events on the build stream.  Algorithmic
+    bytes: the oracle build's element streams (oracle.build_bytes) plus, for triangles, the
+    leaf-ordered records the build attaches (per reference: 4 B read, 48 B of vertices read,
+    48 B written)."""
+    stream = torch.cuda.current_stream()
+    res = {}
+    for name, fn, otree, extra in (
+            ("tri", lambda: api.build_tri(v0, v1, v2), otri, 100.0 * otri.refs.size),
+            ("points", lambda: api.build_points(ppos, pparams), opts, 0.0)):
+        host_ms, ev_ms = [], []
+        for i in range(args.build_warmup + args.build_runs):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            a.record(stream)
+            Tb = fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if i >= args.build_warmup:
+                host_ms.append((t1 - t0) * 1e3)
+                ev_ms.append(a.elapsed_time(b))
+            Tb.release()
+        alg = O.build_bytes(otree) + extra
+        ms = float(np.mean(ev_ms))
+        gbs = alg / (ms * 1e-3) / 1e9
+        res[name] = {"elements": int(otree.num_elems), "nodes": int(otree.num_nodes), "runs": args.build_runs,
+                     "warmup": args.build_warmup, "avg_ms_host": round(float(np.mean(host_ms)), 3),
+                     "avg_ms_events": round(ms, 3), "min_ms_events": round(float(np.min(ev_ms)), 3),
+                     "alg_bytes": int(alg),
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 5)}}
+    return res
+
+
 def _offsets(counts):
     off = np.zeros(len(counts) + 1, np.uint32)
     off[1:] = np.cumsum(np.asarray(counts, np.int64))
     return off
 
 
-def batched_leg(args, torch, api, S):
+def batched_leg(args, torch, api, S, O, ncpu):
     """Batched workload: 256 independent scenes like the small one (box, icosphere,
     random triangles, one light), triangle / photon / query counts log-uniform in
     [1k, 50k] / [4k, 256k] / [16k, 1M] (seed 4).  One step builds every triangle tree and
@@ -240,10 +296,31 @@ def batched_leg(args, torch, api, S):
             T.release()
             P.release()
     plain_s = float(np.mean(pwalls))
+    # algorithmic bytes of k_knn_batch: the oracle's replay of 1 in `sample` queries of
+    # every 8th scene (its own photon tree of the scene), weighted by query count
+    cnt = bufs["sp_count"].cpu().numpy().astype(np.int64)
+    tot_b, tot_q = 0.0, 0
+    for s in range(0, NS, 8):
+        if cnt[s] == 0:
+            continue
+        pp = ppos[psl[s]].cpu().numpy()
+        op = O.build(1, pp, max_qr=float(rmax[s]), root=boxes[s])
+        idx = torch.arange(int(roff[s]), int(roff[s]) + int(cnt[s]), args.sample, device="cuda")
+        qp = bufs["sp_pos4"][idx].cpu().numpy()
+        _, c, ctr = O.knn(op, pp, qp, K, float(rmax[s]), nthreads=ncpu)
+        tot_b += knn_bytes(ctr, len(qp), int(np.minimum(c, K).sum())) / len(qp) * float(cnt[s])
+        tot_q += int(cnt[s])
+    per_q = tot_b / max(tot_q, 1)
+    knn_ms = float(np.mean([p["knn"] for p in phase]))
+    gbs = per_q * hits / (knn_ms * 1e-3) / 1e9
     Q.close()
     TB.release()
     PB.release()
     return {
+        "knn_roofline": {"kernel": "k_knn_batch (MNRT_PHASE_KNN)", "bound": "hbm", "achieved": round(gbs, 2),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+                         "alg_bytes_per_query": round(per_q, 1), "phase_ms": round(knn_ms, 3),
+                         "sample": f"1 in {args.sample} queries of every 8th scene"},
         "workload": (f"{NS} scenes: triangles log-U[1k,50k], photons log-U[4k,256k], queries log-U[16k,1M] "
                      "(seed 4), k=50; one step = all tree builds + all traversals + compaction + all kNN"),
         "scenes": NS, "triangles": int(toff[-1]), "photons": int(poff[-1]), "rays": R, "knn_queries": hits,
@@ -254,7 +331,7 @@ def batched_leg(args, torch, api, S):
         "tuning": tuning, "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 1), "steps": steps}
 
 
-def adjoint_leg(args, torch, api, S):
+def adjoint_leg(args, torch, api, S, O, ncpu):
     """Differentiable workload: the small scene with 3 lights, 1M traced photons,
     1920x1080 primary rays, k = 50 at all hits; forward = radiance with r2 and counts
     kept (mnrt_knn_gather), backward = the adjoint (mnrt_knn_gather_bwd: neighbour sets
@@ -286,12 +363,30 @@ def adjoint_leg(args, torch, api, S):
         if it > 0:
             fwd.append(e[0].elapsed_time(e[1]))
             bwd.append(e[1].elapsed_time(e[2]))
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    # algorithmic bytes: the forward's from the oracle's replay of 1 in `sample` queries;
+    # the adjoint's = the forward's + 12 B of gradient writes per neighbour
+    root = np.concatenate(sc.bounds()).astype(np.float32)
+    pp = ph["pos"].cpu().numpy()
+    op = O.build(1, pp, max_qr=rmax, root=root)
+    qp = q["pos"][:: args.sample].cpu().numpy()
+    _, c, ctr = O.knn(op, pp, qp, K, rmax, nthreads=ncpu)
+    found = int(np.minimum(c, K).sum())
+    fwd_q = knn_bytes(ctr, len(qp), found) / len(qp)
+    bwd_q = fwd_q + 12.0 * found / len(qp)
+    nq = q["count"]
+    fg, bg = fwd_q * nq / (f * 1e-3) / 1e9, bwd_q * nq / (b * 1e-3) / 1e9
     T.release()
     P.release()
-    f, b = float(np.median(fwd)), float(np.median(bwd))
     return {"workload": "small scene + 3 lights, 1M photons, 1920x1080 primary hits, k=50",
-            "photons": ph["count"], "queries": q["count"], "fwd_ms": round(f, 3), "bwd_ms": round(b, 3),
-            "bwd_fwd_ratio": round(b / f, 3)}
+            "photons": ph["count"], "queries": nq, "fwd_ms": round(f, 3), "bwd_ms": round(b, 3),
+            "bwd_fwd_ratio": round(b / f, 3),
+            "fwd_roofline": {"kernel": "k_knn (mnrt_knn_gather, r2 and counts kept)", "achieved": round(fg, 2),
+                             "unit": "GB/s", "frac": round(fg / HBM_PEAK_GBS, 5), "alg_bytes_per_query": round(fwd_q, 1)},
+            "bwd_roofline": {"kernel": "k_knn_bwd_chunks (mnrt_knn_gather_bwd)", "bound": "hbm",
+                             "achieved": round(bg, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(bg / HBM_PEAK_GBS, 5), "alg_bytes_per_query": round(bwd_q, 1),
+                             "sample": f"1 in {args.sample} queries"}}
 
 
 def inverse_leg(args, torch, api, S, O, ncpu):
@@ -401,6 +496,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-ray leg")
+    ap.add_argument("--no-build", action="store_true", help="skip the build benchmark leg")
+    ap.add_argument("--build-runs", type=int, default=200)
+    ap.add_argument("--build-warmup", type=int, default=10)
     ap.add_argument("--serial", action="store_true",
                     help="build the photon tree after the traces on the same stream (default: concurrently)")
     ap.add_argument("--batched-scenes", type=int, default=256)
@@ -576,11 +674,10 @@ def main():
     qpos = np.concatenate([q["pos"].cpu().numpy() for q in rec["queries"]])
     qs = qpos[:: args.sample]
     # host threads for the oracle: the cores this process may use, capped by OMP_NUM_THREADS
-    ncpu = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16"))))
-    tk = time.time()
-    nbr, cnt, kctr = O.knn(ptree, ppos_h, qs, K, rmax, nthreads=ncpu)
-    cpu_knn_s = time.time() - tk
-    _progress(f"oracle kNN replay of {len(qs)} queries: {cpu_knn_s:.1f}s")
+    ncpu = O.host_threads()
+    reps = 0 if args.no_cpu else 5
+    (nbr, cnt, kctr), cpu_knn_s = _median_time(lambda: O.knn(ptree, ppos_h, qs, K, rmax, nthreads=ncpu), reps)
+    _progress(f"oracle kNN replay of {len(qs)} queries: {cpu_knn_s:.2f}s")
     per_q = knn_bytes(kctr, len(qs), int(np.minimum(cnt, K).sum())) / len(qs)
     tn, tr = T.nodes_host(), T.refs_host()
     ttree = O.Tree(0, tn, tr, T.root, T.max_level, 0, sc.num_tris)
@@ -591,10 +688,9 @@ def main():
         ray_d.append(rec["bounce_rays"][1].cpu().numpy())
     ro = np.concatenate(ray_o)[:: args.sample]
     rd = np.concatenate(ray_d)[:: args.sample]
-    tt = time.time()
-    _, _, _, tctr = O.trace(ttree, sc.v0, sc.v1, sc.v2, ro, rd, eps, nthreads=ncpu)
-    cpu_trace_s = time.time() - tt
-    _progress(f"oracle traversal replay of {len(ro)} rays: {cpu_trace_s:.1f}s")
+    (_, _, _, tctr), cpu_trace_s = _median_time(lambda: O.trace(ttree, sc.v0, sc.v1, sc.v2, ro, rd, eps,
+                                                                nthreads=ncpu), reps)
+    _progress(f"oracle traversal replay of {len(ro)} rays: {cpu_trace_s:.3f}s")
     per_ray = trace_bytes(tctr, len(ro)) / len(ro)
     if not args.no_shadow:
         out["shadow"] = shadow_leg(torch, api, native, O, T, ttree, sc, rec["queries"][0]["pos"], eps, args.sample,
@@ -621,30 +717,31 @@ def main():
     out["knn_pct_hbm"] = round(100.0 * achieved / HBM_PEAK_GBS, 3)
     out["trace_pct_hbm"] = round(100.0 * trace_gbs / HBM_PEAK_GBS, 3)
 
+    # oracle builds: the CPU baseline's build time and the build roofline's bytes
+    (otri, cpu_build_tri) = _median_time(lambda: O.build(0, sc.v0, sc.v1, sc.v2), reps)
+    _progress(f"oracle triangle build: {cpu_build_tri:.2f}s")
+    (opts, cpu_build_pts) = _median_time(lambda: O.build(1, ppos_h, max_qr=rmax, root=root), reps)
+    if not args.no_build:
+        _progress("build leg")
+        out["build"] = build_leg(args, torch, api, O, v0, v1, v2, ppos, pparams, otri, opts)
     if not args.no_cpu:
-        tb = time.time()
-        O.build(0, sc.v0, sc.v1, sc.v2)
-        _progress(f"oracle triangle build: {time.time() - tb:.1f}s")
-        cpu_build_tri = time.time() - tb
-        tb = time.time()
-        O.build(1, ppos_h, max_qr=rmax, root=root)
-        cpu_build_pts = time.time() - tb
         cpu_step = cpu_build_tri + cpu_build_pts + rays * (cpu_trace_s / len(ro)) + queries * (cpu_knn_s / len(qs))
         out["cpu_baseline"] = {
             "value": round((rays + queries) / cpu_step / 1e6, 4), "unit": "M(rays+kNN queries)/s", "cores": ncpu,
-            "kind": "port",
-            "sample": (f"oracle on the host: both tree builds in full (1 thread: {cpu_build_tri:.2f}s + "
-                       f"{cpu_build_pts:.2f}s), traversal of 1 in {args.sample} rays ({len(ro)} rays, {cpu_trace_s:.2f}s) "
-                       f"and kNN of 1 in {args.sample} queries ({len(qs)}, {cpu_knn_s:.2f}s) on {ncpu} threads, "
-                       f"scaled to the full step ({cpu_step:.1f}s)"),
+            "kind": "port", "cpu_model": O.cpu_model(), "host_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "sample": (f"oracle on the host, each part 1 warm-up + median of 5: both tree builds in full "
+                       f"(1 thread: {cpu_build_tri:.2f}s + {cpu_build_pts:.2f}s), traversal of 1 in {args.sample} "
+                       f"rays ({len(ro)} rays, {cpu_trace_s:.3f}s) and kNN of 1 in {args.sample} queries "
+                       f"({len(qs)}, {cpu_knn_s:.3f}s) on {ncpu} threads, scaled to the full step ({cpu_step:.1f}s)"),
             "gpu_speedup": round(cpu_step / (ms_step * 1e-3), 1)}
     if not args.no_batched:
         _progress("batched leg")
-        out["batched"] = batched_leg(args, torch, api, S)
+        out["batched"] = batched_leg(args, torch, api, S, O, ncpu)
         out["batched_vs_plain"] = out["batched"]["speedup_vs_plain"]
     if not args.no_adjoint:
         _progress("adjoint leg")
-        out["adjoint"] = adjoint_leg(args, torch, api, S)
+        out["adjoint"] = adjoint_leg(args, torch, api, S, O, ncpu)
         out["bwd_fwd_ratio"] = out["adjoint"]["bwd_fwd_ratio"]
     if not args.no_inverse:
         _progress("inverse-lighting leg")

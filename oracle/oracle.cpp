@@ -182,6 +182,10 @@ struct BuildCtx {
     float max_qr;
     std::vector<BNode> T;
     uint32_t max_level = 0;
+    // element-stream counts for the build's algorithmic bytes: elements of active large
+    // nodes per level, child element records written, triangle clips (one per straddling
+    // triangle and side), elements entering the small stage
+    uint64_t large_elems = 0, child_elems = 0, clips = 0, small_elems = 0;
 };
 
 // Deterministic side of an element lying in the split plane (replaces the MT uniforms
@@ -373,6 +377,7 @@ void small_recurse(BuildCtx& C, const SmallRoot& R, const std::vector<Cand>& can
 
 void small_stage(BuildCtx& C, const SmallRoot& R) {
     uint32_t n = (uint32_t)R.el.size();
+    C.small_elems += n;
     std::vector<Cand> cand;
     int passes = (C.kind == 0) ? 2 : 1;
     for (int isMax = 0; isMax < passes; ++isMax) {
@@ -460,6 +465,7 @@ void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>
                 make_leaf(C, cur, A.el);
                 continue;
             }
+            C.large_elems += A.el.size();
             float ex = A.inh_hi[0] - A.inh_lo[0], ey = A.inh_hi[1] - A.inh_lo[1],
                   ez = A.inh_hi[2] - A.inh_lo[2];
             // Longest axis, ties to the lower axis.  The reference's test
@@ -481,6 +487,7 @@ void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>
                 bool L, R;
                 classify(C.kind, e, axis, s, L, R);
                 if (C.kind == 0 && (e.lo[axis] < s && s < e.hi[axis])) {
+                    C.clips += 2;
                     Elem el = e, er = e;
                     clip_bounds(C, el, Lc.inh_lo, Lc.inh_hi);
                     clip_bounds(C, er, Rc.inh_lo, Rc.inh_hi);
@@ -495,6 +502,7 @@ void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>
                 make_leaf(C, cur, A.el);
                 continue;
             }
+            C.child_elems += Lc.el.size() + Rc.el.size();
             int l = new_node(C), r = new_node(C);
             BNode& n = C.T[cur];
             n.leaf = false;
@@ -534,6 +542,7 @@ struct OTree {
     uint32_t max_level;
     uint32_t num_leaves;
     uint32_t n_elems;
+    uint64_t counts[4];  // BuildCtx large_elems, child_elems, clips, small_elems
 };
 
 uint32_t emit(const BuildCtx& C, int id, OTree& T) {
@@ -623,6 +632,10 @@ OTree* build(int kind, const float* v0, const float* v1, const float* v2, uint32
     }
     emit(C, root.id, *T);
     T->max_level = C.max_level;
+    T->counts[0] = C.large_elems;
+    T->counts[1] = C.child_elems;
+    T->counts[2] = C.clips;
+    T->counts[3] = C.small_elems;
     return T;
 }
 
@@ -1264,6 +1277,7 @@ void orc_tree_copy(void* h, uint32_t* nodes, uint32_t* refs) {
     if (!T->refs.empty()) std::memcpy(refs, T->refs.data(), T->refs.size() * 4);
 }
 void orc_tree_free(void* h) { delete (OTree*)h; }
+void orc_build_counts(void* h, uint64_t* out4) { std::memcpy(out4, ((OTree*)h)->counts, 32); }
 
 double orc_tree_cost(const uint32_t* nodes, uint32_t num_nodes, const float* root6, int kind,
                      float max_qr) {

@@ -47,6 +47,7 @@ def lib():
     L.orc_tree_root.argtypes = [vp, f32p]
     L.orc_tree_copy.argtypes = [vp, u32p, u32p]
     L.orc_tree_free.argtypes = [vp]
+    L.orc_build_counts.argtypes = [vp, u64p]
     L.orc_tree_cost.restype = ctypes.c_double
     L.orc_tree_cost.argtypes = [u32p, ctypes.c_uint32, f32p, ctypes.c_int, ctypes.c_float]
     L.orc_tree_check_structure.restype = ctypes.c_int
@@ -156,9 +157,31 @@ def build(kind, v0, v1=None, v2=None, ratio=None, small_max=None, max_qr=1.0, ro
         L.orc_tree_copy(h, nodes, refs)
         rt = np.zeros(6, np.float32)
         L.orc_tree_root(h, rt)
-        return Tree(kind, nodes, refs[:nr], rt, L.orc_tree_max_level(h), L.orc_tree_num_leaves(h), v0.shape[0])
+        T = Tree(kind, nodes, refs[:nr], rt, L.orc_tree_max_level(h), L.orc_tree_num_leaves(h), v0.shape[0])
+        T.counts = np.zeros(4, np.uint64)
+        L.orc_build_counts(h, T.counts)
+        return T
     finally:
         L.orc_tree_free(h)
+
+
+def build_bytes(tree: Tree) -> float:
+    """Algorithmic bytes of a build (element streams read and written by its stages), from
+    the oracle build's counts: the input (36 B per triangle, 12 B per point) read and one
+    element record (index + bounds: 28 B per triangle, 16 B per point: points keep no upper
+    bound) written; per element of an active large node per level its bounds read for the
+    tight box (24 / 12 B) and its record read for classification (28 / 16 B); every child
+    record written (28 / 16 B); a clipped triangle side reads the vertices (36 B); every
+    small-stage element read once (28 / 16 B); 8 B per node and 4 B per leaf reference
+    written."""
+    large, child, clips, small = (float(c) for c in tree.counts)
+    n = float(tree.num_elems)
+    if tree.kind == 0:
+        inp, rec, bnd = 36.0, 28.0, 24.0
+    else:
+        inp, rec, bnd = 12.0, 16.0, 12.0
+    return (n * (inp + rec) + large * (bnd + rec) + child * rec + clips * 36.0 + small * rec
+            + 8.0 * tree.num_nodes + 4.0 * tree.refs.size)
 
 
 def tree_cost(nodes, root, kind, max_qr=1.0) -> float:

@@ -135,3 +135,23 @@ def test_small_node_max_parameter(orc):
         assert orc.check_structure(T.nodes, T.refs, sc.num_tris) == 0
     with pytest.raises(ValueError):
         orc.build(0, sc.v0, sc.v1, sc.v2, small_max=65)
+
+
+def test_build_stream_counts_invariants(orc):
+    """The element-stream counts behind the build's algorithmic bytes: every distinct point
+    goes to exactly one child per large level and ends in exactly one small root; a
+    small-enough input goes straight to the small stage; triangle clips come in pairs."""
+    rng = np.random.default_rng(3)
+    pts = np.zeros((50000, 4), np.float32)
+    pts[:, :3] = rng.uniform(-1, 1, (50000, 3))
+    P = orc.build(1, pts)
+    large, child, clips, small = (int(c) for c in P.counts)
+    assert large > 50000 and child == large and clips == 0 and small == 50000
+    few = orc.build(1, pts[:20])
+    assert [int(c) for c in few.counts] == [0, 0, 0, 20]
+    sc = S.scene_small(subdiv=3)
+    T = orc.build(0, sc.v0, sc.v1, sc.v2)
+    large, child, clips, small = (int(c) for c in T.counts)
+    assert clips % 2 == 0 and child >= large and small >= sc.num_tris
+    b = orc.build_bytes(T)
+    assert b > sc.num_tris * (36 + 28) + 8 * T.num_nodes
