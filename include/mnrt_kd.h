@@ -281,6 +281,23 @@ int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                     float* d_io_inv_dist, float* d_out_r2, float* d_out_irr4, uint32_t* d_out_nbr,
                     uint32_t* d_out_count, void* stream);
 
+/* mnrt_knn_gather for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation); no counterpart in the
+ * reference.  Same arguments, results bit for bit equal to mnrt_knn_gather's, but no
+ * hipMalloc / pool allocation: the query processing order and, with d_idx_sp, the
+ * per-query values live in one fixed 64 MiB device array of the library (one per device,
+ * kept until the process ends), so n is limited to 8,126,329 queries, or 2,321,808 with
+ * d_idx_sp (MNRT_ERR_CAPACITY beyond, nothing launched).  Every call on one device shares
+ * that array: calls that overlap on a device (concurrent streams or host threads) race and
+ * must be serialised by the caller.  mnrt_device_init looks the array's address up for the
+ * device; a call on a device it was not run for looks it up first. */
+int mnrt_knn_gather_noalloc(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                            const float* d_pos4, const float* d_nrm4, const float* d_dir4,
+                            const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,
+                            const float* d_query_radius, const float* d_orig4, float* d_io_rad4,
+                            float* d_io_inv_dist, float* d_out_r2, float* d_out_irr4,
+                            uint32_t* d_out_nbr, uint32_t* d_out_count, void* stream);
+
 /* Adjoint of mnrt_knn_gather (no reference counterpart): given dLoss/dradiance per query
  * (float4, w ignored) and the forward's r2 and neighbour counts (d_out_r2, d_out_count),
  * recomputes every query's neighbour set (identical to the forward's: same r_max and

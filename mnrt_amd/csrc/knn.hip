@@ -25,9 +25,17 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "batch_dev.h"
 #include "order.h"
+
+// Device scratch of mnrt_knn_gather_noalloc (the query processing order and, with slot
+// indices, the per-query values before their runs are summed): 64 MiB per device, held
+// for the process's lifetime.  Host code never passes this name to a kernel or a copy: it
+// is a host-side stand-in; the device address comes from hipGetSymbolAddress, once per
+// device (g_scratch_addr).
+static __device__ __align__(256) uint32_t scratch[(64u << 20) / 4];
 
 namespace mnrt {
 
@@ -1048,6 +1056,37 @@ int launch_knn_bwd_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_
 }
 }  // namespace mnrt
 
+namespace {
+
+constexpr size_t kScratchWords = sizeof(scratch) / sizeof(uint32_t);
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_addr[kScratchMaxDevices] = {};
+
+// Words of scratch mnrt_knn_gather_noalloc needs for n queries: the processing order, and
+// 5 words per query (radiance + inverse distance) when slot indices are given.
+size_t noalloc_words(uint32_t n, bool slots) {
+    return mnrt::spatial_order_words(n) + (slots ? 5ull * n : 0ull);
+}
+
+}  // namespace
+
+namespace mnrt {
+
+// Looks up this device's address of `scratch` (the device must be current); kept per
+// device, so later calls on it reuse the address.
+int knn_scratch_init(int device) {
+    if (device < 0 || device >= kScratchMaxDevices) return MNRT_ERR_NODEV;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[device]) return MNRT_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MNRT_ERR_HIP;
+    g_scratch_addr[device] = (uint32_t*)p;
+    return MNRT_OK;
+}
+
+}  // namespace mnrt
+
 static int knn_check(const mnrt_tree* pmap, uint32_t k) {
     if (!pmap) return MNRT_ERR_ARG;
     if (k == 0 || k > MNRT_KNN_MAX_K) return MNRT_ERR_ARG;
@@ -1113,6 +1152,78 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
     }
     if (tmp && mnrt::pool_free(tmp, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
     if (obuf && mnrt::pool_free(obuf, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
+    return rc;
+}
+
+extern "C" int mnrt_knn_gather_noalloc(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
+                                       const float* d_pos4, const float* d_nrm4, const float* d_dir4,
+                                       const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,
+                                       const float* d_query_radius, const float* d_orig4, float* d_io_rad4,
+                                       float* d_io_inv_dist, float* d_out_r2, float* d_out_irr4,
+                                       uint32_t* d_out_nbr, uint32_t* d_out_count, void* stream) {
+    int rc = knn_check(pmap, k);
+    if (rc != MNRT_OK) return rc;
+    if (n == 0) return MNRT_OK;
+    if (!d_pos4 || !d_nrm4 || !d_dir4 || !d_infl4 || !d_io_rad4 || !(r_max > 0.0f)) return MNRT_ERR_ARG;
+    if (d_io_inv_dist && !d_orig4) return MNRT_ERR_ARG;
+    if (noalloc_words(n, d_idx_sp != nullptr) > kScratchWords) return MNRT_ERR_CAPACITY;
+    int dev = 0;
+    MNRT_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices) return MNRT_ERR_NODEV;
+    uint32_t* scr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        scr = g_scratch_addr[dev];
+    }
+    if (!scr) {  // mnrt_device_init was not called for this device
+        rc = mnrt::knn_scratch_init(dev);
+        if (rc != MNRT_OK) return rc;
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        scr = g_scratch_addr[dev];
+    }
+    hipStream_t st = (hipStream_t)stream;
+    mnrt::KnnArgs A{};
+    A.nodes = (const uint2*)(pmap->d_qnodes ? pmap->d_qnodes : pmap->d_nodes);
+    A.recs = (const float4*)pmap->d_recs;
+    A.k = k;
+    A.r2max = r_max * r_max;
+    A.r_max = r_max;
+    A.n = n;
+    A.pos = (const float4*)d_pos4;
+    A.nrm = (const float4*)d_nrm4;
+    A.dir = (const float4*)d_dir4;
+    A.infl = (const float4*)d_infl4;
+    A.clr = (const float4*)d_clr4;
+    A.r_query = d_query_radius;
+    A.orig = (const float4*)d_orig4;
+    A.out_r2 = d_out_r2;
+    A.out_irr = (float4*)d_out_irr4;
+    A.out_nbr = d_out_nbr;
+    A.out_count = d_out_count;
+    // scratch layout: [processing order][per-query radiance (4 n)][per-query inverse distance (n)]
+    rc = mnrt::spatial_order_into(pmap->root_min, pmap->root_max, n, d_pos4, st, scr, &A.order);
+    if (rc != MNRT_OK) return rc;
+    float4* tmp = nullptr;
+    if (d_idx_sp) {
+        // every query's value is written before the run sums read it: nothing to clear
+        tmp = (float4*)(scr + mnrt::spatial_order_words(n));
+        A.rad_out = tmp;
+        A.accumulate = 0;
+        A.inv_dist = d_io_inv_dist ? (float*)(tmp + n) : nullptr;
+        A.inv_accumulate = 0;
+    } else {
+        A.rad_out = (float4*)d_io_rad4;
+        A.accumulate = 1;
+        A.inv_dist = d_io_inv_dist;
+        A.inv_accumulate = 1;
+    }
+    rc = mnrt::launch_knn<mnrt::kModeRadiance>(A, st);
+    if (rc == MNRT_OK && d_idx_sp) {
+        hipLaunchKernelGGL(mnrt::k_scatter_runs, dim3((n + 255) / 256), dim3(256), 0, st, tmp,
+                           d_io_inv_dist ? (const float*)(tmp + n) : (const float*)nullptr, d_idx_sp, n,
+                           (float4*)d_io_rad4, d_io_inv_dist);
+        if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
+    }
     return rc;
 }
 

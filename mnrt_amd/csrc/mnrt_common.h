@@ -217,6 +217,9 @@ hipError_t pool_free(void* p, hipStream_t st);
 // Pinned host staging buffer of at least `bytes`, one per host thread and slot, kept for
 // the thread's lifetime.
 void* host_staging(int slot, size_t bytes);
+// Device address of the fixed kNN scratch array on `device` (which must be current),
+// looked up once per device.
+int knn_scratch_init(int device);
 
 }  // namespace mnrt
 

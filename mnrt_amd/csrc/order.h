@@ -91,17 +91,26 @@ __global__ __launch_bounds__(256) void k_order_place(uint32_t n, const uint32_t*
     else order[atomicAdd(&cursor[cell[i]], 1u)] = i;
 }
 
-// Processing order of n points at d_pos4 in the box lo3 .. hi3: *order = null below
-// kOrderMinItems; *buf = the allocation to return with pool_free after the launch that
-// reads the order.
-inline int spatial_order(const float* lo3, const float* hi3, uint32_t n, const float* d_pos4, hipStream_t st,
-                         const uint32_t** order, void** buf) {
+// Zeroes the cell histogram and the far-pair counter (a kernel rather than memset nodes, so
+// a caller may capture the sequence in a graph).
+__global__ __launch_bounds__(256) void k_order_clear(uint32_t* __restrict__ hist, uint32_t* __restrict__ far_pairs) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kOrderCells; i += gridDim.x * blockDim.x) hist[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *far_pairs = 0u;
+}
+
+// Device words spatial_order_into needs for n points (0 below kOrderMinItems: no order).
+inline size_t spatial_order_words(uint32_t n) {
+    return n < kOrderMinItems ? 0 : 2ull * n + 2ull * kOrderCells + 4 + scan_scratch_words(kOrderCells);
+}
+
+// Processing order of n points at d_pos4 in the box lo3 .. hi3, built in buf
+// (spatial_order_words(n) device words): *order = null below kOrderMinItems, else a pointer
+// into buf valid until buf is reused.
+inline int spatial_order_into(const float* lo3, const float* hi3, uint32_t n, const float* d_pos4, hipStream_t st,
+                              uint32_t* buf, const uint32_t** order) {
     *order = nullptr;
-    *buf = nullptr;
     if (n < kOrderMinItems) return MNRT_OK;
-    const size_t words = 2ull * n + 2ull * kOrderCells + 4 + scan_scratch_words(kOrderCells);
-    if (pool_alloc(buf, words * 4, st) != hipSuccess) return MNRT_ERR_NOMEM;
-    uint32_t* cell = (uint32_t*)*buf;
+    uint32_t* cell = buf;
     uint32_t* ord = cell + n;
     uint32_t* hist = ord + n;
     uint32_t* cursor = hist + kOrderCells;
@@ -117,8 +126,7 @@ inline int spatial_order(const float* lo3, const float* hi3, uint32_t n, const f
         B.diag2 += c * c;
     }
     const float4* pos = (const float4*)d_pos4;
-    MNRT_HIP_CHECK(hipMemsetAsync(hist, 0, (size_t)kOrderCells * 4, st));
-    MNRT_HIP_CHECK(hipMemsetAsync(far_pairs, 0, 4, st));
+    hipLaunchKernelGGL(k_order_clear, dim3(kOrderCells / 256 / 16), dim3(256), 0, st, hist, far_pairs);
     hipLaunchKernelGGL(k_order_far, dim3((n + 256 * kFarPerThread - 1) / (256 * kFarPerThread)), dim3(256), 0, st, n,
                        pos, B.diag2, far_pairs);
     hipLaunchKernelGGL(k_order_cells, dim3((n + 255) / 256), dim3(256), 0, st, n, pos, B,
@@ -129,6 +137,18 @@ inline int spatial_order(const float* lo3, const float* hi3, uint32_t n, const f
     MNRT_HIP_CHECK(hipGetLastError());
     *order = ord;
     return MNRT_OK;
+}
+
+// As spatial_order_into with the buffer from the device pool: *buf = the allocation to
+// return with pool_free after the launch that reads the order (null below kOrderMinItems).
+inline int spatial_order(const float* lo3, const float* hi3, uint32_t n, const float* d_pos4, hipStream_t st,
+                         const uint32_t** order, void** buf) {
+    *order = nullptr;
+    *buf = nullptr;
+    const size_t words = spatial_order_words(n);
+    if (words == 0) return MNRT_OK;
+    if (pool_alloc(buf, words * 4, st) != hipSuccess) return MNRT_ERR_NOMEM;
+    return spatial_order_into(lo3, hi3, n, d_pos4, st, (uint32_t*)*buf, order);
 }
 
 }  // namespace

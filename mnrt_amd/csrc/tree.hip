@@ -475,5 +475,5 @@ extern "C" int mnrt_device_init(int device) {
     hipDeviceProp_t prop;
     MNRT_HIP_CHECK(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return MNRT_ERR_NODEV;
-    return MNRT_OK;
+    return mnrt::knn_scratch_init(device);
 }

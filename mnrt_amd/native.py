@@ -176,6 +176,7 @@ def load():
     L.mnrt_sample_area_light.argtypes = [LP, u32, f32, f32, f32, f32, vp, vp, vp, vp]
     L.mnrt_trace_shadow.argtypes = [TP, LP, u32, vp, vp, f32, vp, vp]
     L.mnrt_knn_gather.argtypes = [TP, u32, f32, u32] + [vp] * 15
+    L.mnrt_knn_gather_noalloc.argtypes = [TP, u32, f32, u32] + [vp] * 15
     L.mnrt_knn_radii.argtypes = [TP, u32, f32, u32, vp, vp, vp]
     L.mnrt_knn_gather_bwd.argtypes = [TP, u32, f32, u32] + [vp] * 12
     L.mnrt_photons_generate.argtypes = [TP, vp, vp, vp, vp, vp, u32, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp,
