@@ -718,9 +718,9 @@ def main():
     out["trace_pct_hbm"] = round(100.0 * trace_gbs / HBM_PEAK_GBS, 3)
 
     # oracle builds: the CPU baseline's build time and the build roofline's bytes
-    (otri, cpu_build_tri) = _median_time(lambda: O.build(0, sc.v0, sc.v1, sc.v2), reps)
+    (otri, cpu_build_tri) = _median_time(lambda: O.build(0, sc.v0, sc.v1, sc.v2, nthreads=ncpu), reps)
     _progress(f"oracle triangle build: {cpu_build_tri:.2f}s")
-    (opts, cpu_build_pts) = _median_time(lambda: O.build(1, ppos_h, max_qr=rmax, root=root), reps)
+    (opts, cpu_build_pts) = _median_time(lambda: O.build(1, ppos_h, max_qr=rmax, root=root, nthreads=ncpu), reps)
     if not args.no_build:
         _progress("build leg")
         out["build"] = build_leg(args, torch, api, O, v0, v1, v2, ppos, pparams, otri, opts)
@@ -731,7 +731,8 @@ def main():
             "kind": "port", "cpu_model": O.cpu_model(), "host_cpus": os.cpu_count(),
             "affinity_cpus": len(os.sched_getaffinity(0)),
             "sample": (f"oracle on the host, each part 1 warm-up + median of 5: both tree builds in full "
-                       f"(1 thread: {cpu_build_tri:.2f}s + {cpu_build_pts:.2f}s), traversal of 1 in {args.sample} "
+                       f"(level-parallel large stage, per-root parallel small stage, {ncpu} threads: "
+                       f"{cpu_build_tri:.2f}s + {cpu_build_pts:.2f}s), traversal of 1 in {args.sample} "
                        f"rays ({len(ro)} rays, {cpu_trace_s:.3f}s) and kNN of 1 in {args.sample} queries "
                        f"({len(qs)}, {cpu_knn_s:.3f}s) on {ncpu} threads, scaled to the full step ({cpu_step:.1f}s)"),
             "gpu_speedup": round(cpu_step / (ms_step * 1e-3), 1)}

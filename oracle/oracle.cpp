@@ -428,41 +428,86 @@ struct Active {
 // cutting kdtree.cu:467-612 and 1986-2033; median split kdtree.cu:629-682).  A node whose
 // median split would put every element into both children becomes a leaf (the
 // reference loops forever there, F5 in the survey of the reference).
-void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>& smalls) {
+//
+// Level-parallel (the reference's kernels are level-parallel too): the active nodes'
+// elements are cut into chunks; the tight boxes (min / max: exact in any order) and the
+// classification with clipping run in parallel over chunks, the chunk outputs are joined
+// in element order, and the per-node decisions and node allocation run in node order.
+// The tree is the same for every thread count.
+constexpr uint32_t kLargeChunk = 16384;
+
+void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>& smalls, int nthreads) {
+    struct Chunk {
+        uint32_t node, b, e;
+    };
     while (!active.empty()) {
-        std::vector<Active> next;
-        for (Active& A : active) {
-            C.max_level = std::max(C.max_level, A.level);
-            float tlo[3] = {kInf, kInf, kInf}, thi[3] = {-kInf, -kInf, -kInf};
-            for (const Elem& e : A.el)
+        std::vector<Chunk> chunks;
+        for (uint32_t i = 0; i < active.size(); ++i)
+            for (uint32_t b = 0; b < active[i].el.size(); b += kLargeChunk)
+                chunks.push_back({i, b, (uint32_t)std::min<size_t>(active[i].el.size(), (size_t)b + kLargeChunk)});
+        // tight boxes, per chunk then per node
+        std::vector<float> part(6 * chunks.size());
+        parallel_for((uint32_t)chunks.size(), nthreads, [&](uint32_t cb, uint32_t ce) {
+            for (uint32_t c = cb; c < ce; ++c) {
+                float* t = &part[6 * c];
                 for (int a = 0; a < 3; ++a) {
-                    tlo[a] = std::fmin(tlo[a], e.lo[a]);
-                    thi[a] = std::fmax(thi[a], e.hi[a]);
+                    t[a] = kInf;
+                    t[3 + a] = -kInf;
                 }
-            int cur = A.id;
+                const Active& A = active[chunks[c].node];
+                for (uint32_t k = chunks[c].b; k < chunks[c].e; ++k)
+                    for (int a = 0; a < 3; ++a) {
+                        t[a] = std::fmin(t[a], A.el[k].lo[a]);
+                        t[3 + a] = std::fmax(t[3 + a], A.el[k].hi[a]);
+                    }
+            }
+        });
+        std::vector<float> tight(6 * active.size());
+        for (size_t i = 0; i < active.size(); ++i)
+            for (int a = 0; a < 3; ++a) {
+                tight[6 * i + a] = kInf;
+                tight[6 * i + 3 + a] = -kInf;
+            }
+        for (size_t c = 0; c < chunks.size(); ++c)
+            for (int a = 0; a < 3; ++a) {
+                float* t = &tight[6 * chunks[c].node];
+                t[a] = std::fmin(t[a], part[6 * c + a]);
+                t[3 + a] = std::fmax(t[3 + a], part[6 * c + 3 + a]);
+            }
+        // per node: empty-space cuts, level cap, the median split and the child boxes
+        std::vector<int> cur(active.size()), axis(active.size(), -1);
+        std::vector<float> split(active.size());
+        std::vector<Active> Lc(active.size()), Rc(active.size());
+        for (size_t i = 0; i < active.size(); ++i) {
+            Active& A = active[i];
+            C.max_level = std::max(C.max_level, A.level);
+            const float* tlo = &tight[6 * i];
+            const float* thi = &tight[6 * i + 3];
+            int c = A.id;
             for (int isMax = 0; isMax < 2; ++isMax) {
                 for (int ax = 0; ax < 3; ++ax) {
                     if (A.level >= kMaxLevel) continue;
                     float total = A.inh_hi[ax] - A.inh_lo[ax];
                     float empty = isMax ? (A.inh_hi[ax] - thi[ax]) : (tlo[ax] - A.inh_lo[ax]);
                     if (!(empty > total * C.ratio)) continue;
-                    float s = isMax ? thi[ax] : tlo[ax];
+                    float sp = isMax ? thi[ax] : tlo[ax];
                     int eleaf = new_node(C), cont = new_node(C);
-                    BNode& n = C.T[cur];
+                    BNode& n = C.T[c];
                     n.leaf = false;
                     n.axis = ax;
-                    n.split = s;
+                    n.split = sp;
                     n.left = isMax ? cont : eleaf;
                     n.right = isMax ? eleaf : cont;
-                    if (isMax) A.inh_hi[ax] = s;
-                    else A.inh_lo[ax] = s;
+                    if (isMax) A.inh_hi[ax] = sp;
+                    else A.inh_lo[ax] = sp;
                     A.level += 1;
                     C.max_level = std::max(C.max_level, A.level);
-                    cur = cont;
+                    c = cont;
                 }
             }
+            cur[i] = c;
             if (A.level >= kMaxLevel) {
-                make_leaf(C, cur, A.el);
+                make_leaf(C, c, A.el);
                 continue;
             }
             C.large_elems += A.el.size();
@@ -471,50 +516,78 @@ void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>
             // Longest axis, ties to the lower axis.  The reference's test
             // (kdtree.cu:641-647) falls back to x whenever y and z tie for longest even
             // if x is shorter, which slices cube-like scenes into ever thinner x slabs.
-            int axis = 0;
+            int ax = 0;
             float emax = ex;
-            if (ey > emax) { axis = 1; emax = ey; }
-            if (ez > emax) axis = 2;
-            float s = A.inh_lo[axis] + 0.5f * (A.inh_hi[axis] - A.inh_lo[axis]);
-            Active Lc, Rc;
+            if (ey > emax) { ax = 1; emax = ey; }
+            if (ez > emax) ax = 2;
+            axis[i] = ax;
+            split[i] = A.inh_lo[ax] + 0.5f * (A.inh_hi[ax] - A.inh_lo[ax]);
             for (int a = 0; a < 3; ++a) {
-                Lc.inh_lo[a] = Rc.inh_lo[a] = A.inh_lo[a];
-                Lc.inh_hi[a] = Rc.inh_hi[a] = A.inh_hi[a];
+                Lc[i].inh_lo[a] = Rc[i].inh_lo[a] = A.inh_lo[a];
+                Lc[i].inh_hi[a] = Rc[i].inh_hi[a] = A.inh_hi[a];
             }
-            Lc.inh_hi[axis] = s;
-            Rc.inh_lo[axis] = s;
-            for (const Elem& e : A.el) {
-                bool L, R;
-                classify(C.kind, e, axis, s, L, R);
-                if (C.kind == 0 && (e.lo[axis] < s && s < e.hi[axis])) {
-                    C.clips += 2;
-                    Elem el = e, er = e;
-                    clip_bounds(C, el, Lc.inh_lo, Lc.inh_hi);
-                    clip_bounds(C, er, Rc.inh_lo, Rc.inh_hi);
-                    Lc.el.push_back(el);
-                    Rc.el.push_back(er);
-                } else {
-                    if (L) Lc.el.push_back(e);
-                    if (R) Rc.el.push_back(e);
+            Lc[i].inh_hi[ax] = split[i];
+            Rc[i].inh_lo[ax] = split[i];
+        }
+        // classification and clipping, per chunk, joined in element order
+        std::vector<std::vector<Elem>> lo_part(chunks.size()), hi_part(chunks.size());
+        std::vector<uint64_t> clips(chunks.size(), 0);
+        parallel_for((uint32_t)chunks.size(), nthreads, [&](uint32_t cb, uint32_t ce) {
+            for (uint32_t c = cb; c < ce; ++c) {
+                const uint32_t i = chunks[c].node;
+                if (axis[i] < 0) continue;
+                const Active& A = active[i];
+                const int ax = axis[i];
+                const float sp = split[i];
+                for (uint32_t k = chunks[c].b; k < chunks[c].e; ++k) {
+                    const Elem& e = A.el[k];
+                    bool L, R;
+                    classify(C.kind, e, ax, sp, L, R);
+                    if (C.kind == 0 && (e.lo[ax] < sp && sp < e.hi[ax])) {
+                        clips[c] += 2;
+                        Elem el = e, er = e;
+                        clip_bounds(C, el, Lc[i].inh_lo, Lc[i].inh_hi);
+                        clip_bounds(C, er, Rc[i].inh_lo, Rc[i].inh_hi);
+                        lo_part[c].push_back(el);
+                        hi_part[c].push_back(er);
+                    } else {
+                        if (L) lo_part[c].push_back(e);
+                        if (R) hi_part[c].push_back(e);
+                    }
                 }
             }
-            if (Lc.el.size() == A.el.size() && Rc.el.size() == A.el.size()) {
-                make_leaf(C, cur, A.el);
+        });
+        for (size_t c = 0; c < chunks.size(); ++c) {
+            const uint32_t i = chunks[c].node;
+            if (axis[i] < 0) continue;
+            C.clips += clips[c];
+            Lc[i].el.insert(Lc[i].el.end(), lo_part[c].begin(), lo_part[c].end());
+            Rc[i].el.insert(Rc[i].el.end(), hi_part[c].begin(), hi_part[c].end());
+            std::vector<Elem>().swap(lo_part[c]);
+            std::vector<Elem>().swap(hi_part[c]);
+        }
+        // per node: no-progress leaf, or two children (small roots or next level)
+        std::vector<Active> next;
+        for (size_t i = 0; i < active.size(); ++i) {
+            if (axis[i] < 0) continue;
+            Active& A = active[i];
+            if (Lc[i].el.size() == A.el.size() && Rc[i].el.size() == A.el.size()) {
+                make_leaf(C, cur[i], A.el);
                 continue;
             }
-            C.child_elems += Lc.el.size() + Rc.el.size();
+            C.child_elems += Lc[i].el.size() + Rc[i].el.size();
             int l = new_node(C), r = new_node(C);
-            BNode& n = C.T[cur];
+            BNode& n = C.T[cur[i]];
             n.leaf = false;
-            n.axis = axis;
-            n.split = s;
+            n.axis = axis[i];
+            n.split = split[i];
             n.left = l;
             n.right = r;
-            Lc.level = Rc.level = A.level + 1;
-            C.max_level = std::max(C.max_level, Lc.level);
-            Lc.id = l;
-            Rc.id = r;
-            for (Active* ch : {&Lc, &Rc}) {
+            Lc[i].level = Rc[i].level = A.level + 1;
+            C.max_level = std::max(C.max_level, Lc[i].level);
+            Lc[i].id = l;
+            Rc[i].id = r;
+            for (Active* ch : {&Lc[i], &Rc[i]}) {
                 if (ch->el.size() <= C.small_max) {
                     SmallRoot sr;
                     sr.el = std::move(ch->el);
@@ -529,8 +602,54 @@ void large_stage(BuildCtx& C, std::vector<Active> active, std::vector<SmallRoot>
                     next.push_back(std::move(*ch));
                 }
             }
+            std::vector<Elem>().swap(A.el);
         }
         active.swap(next);
+    }
+}
+
+// Small stage of every small root, in parallel: each subtree is built into a node array
+// of its own (local node 0 = the small root) and spliced into the tree in small-root
+// order.
+void small_stages(BuildCtx& C, const std::vector<SmallRoot>& smalls, int nthreads) {
+    std::vector<BuildCtx> sub(smalls.size());
+    parallel_for((uint32_t)smalls.size(), nthreads, [&](uint32_t b, uint32_t e) {
+        for (uint32_t i = b; i < e; ++i) {
+            BuildCtx& L = sub[i];
+            L.kind = C.kind;
+            L.v0 = C.v0;
+            L.v1 = C.v1;
+            L.v2 = C.v2;
+            L.ratio = C.ratio;
+            L.small_max = C.small_max;
+            L.max_qr = C.max_qr;
+            L.T.emplace_back();
+            SmallRoot R = smalls[i];
+            R.id = 0;
+            small_stage(L, R);
+        }
+    });
+    for (size_t i = 0; i < smalls.size(); ++i) {
+        BuildCtx& L = sub[i];
+        const int off = (int)C.T.size() - 1;  // local node j >= 1 -> off + j
+        auto map = [&](int j) { return j == 0 ? smalls[i].id : off + j; };
+        for (size_t j = 1; j < L.T.size(); ++j) {
+            BNode n = std::move(L.T[j]);
+            if (!n.leaf) {
+                n.left = map(n.left);
+                n.right = map(n.right);
+            }
+            C.T.push_back(std::move(n));
+        }
+        BNode r = std::move(L.T[0]);
+        if (!r.leaf) {
+            r.left = map(r.left);
+            r.right = map(r.right);
+        }
+        C.T[smalls[i].id] = std::move(r);
+        C.max_level = std::max(C.max_level, L.max_level);
+        C.small_elems += L.small_elems;
+        BuildCtx().T.swap(L.T);
     }
 }
 
@@ -565,7 +684,7 @@ uint32_t emit(const BuildCtx& C, int id, OTree& T) {
 }
 
 OTree* build(int kind, const float* v0, const float* v1, const float* v2, uint32_t n, float ratio,
-             uint32_t small_max, float max_qr, int has_root, const float* root6) {
+             uint32_t small_max, float max_qr, int has_root, const float* root6, int nthreads) {
     BuildCtx C;
     C.kind = kind;
     C.v0 = v0;
@@ -619,9 +738,9 @@ OTree* build(int kind, const float* v0, const float* v1, const float* v2, uint32
         sr.id = root.id;
         smalls.push_back(std::move(sr));
     } else {
-        large_stage(C, {root}, smalls);
+        large_stage(C, {root}, smalls, nthreads);
     }
-    for (const SmallRoot& sr : smalls) small_stage(C, sr);
+    small_stages(C, smalls, nthreads);
     OTree* T = new OTree();
     T->kind = kind;
     T->num_leaves = 0;
@@ -1264,7 +1383,12 @@ extern "C" {
 void* orc_build(int kind, const float* v0, const float* v1, const float* v2, uint32_t n,
                 float ratio, uint32_t small_max, float max_qr, int has_root, const float* root6) {
     if (small_max < 1 || small_max > 64) return nullptr;
-    return build(kind, v0, v1, v2, n, ratio, small_max, max_qr, has_root, root6);
+    return build(kind, v0, v1, v2, n, ratio, small_max, max_qr, has_root, root6, 1);
+}
+void* orc_build_mt(int kind, const float* v0, const float* v1, const float* v2, uint32_t n, float ratio,
+                   uint32_t small_max, float max_qr, int has_root, const float* root6, int nthreads) {
+    if (small_max < 1 || small_max > 64) return nullptr;
+    return build(kind, v0, v1, v2, n, ratio, small_max, max_qr, has_root, root6, nthreads);
 }
 uint32_t orc_tree_num_nodes(void* h) { return (uint32_t)(((OTree*)h)->nodes.size() / 2); }
 uint32_t orc_tree_num_refs(void* h) { return (uint32_t)((OTree*)h)->refs.size(); }

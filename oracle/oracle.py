@@ -41,6 +41,8 @@ def lib():
     L.orc_build.restype = vp
     L.orc_build.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32,
                             ctypes.c_float, ctypes.c_int, vp]
+    L.orc_build_mt.restype = vp
+    L.orc_build_mt.argtypes = L.orc_build.argtypes + [ctypes.c_int]
     for name in ("orc_tree_num_nodes", "orc_tree_num_refs", "orc_tree_num_leaves", "orc_tree_max_level"):
         getattr(L, name).restype = ctypes.c_uint32
         getattr(L, name).argtypes = [vp]
@@ -136,7 +138,8 @@ class Tree:
         return self.nodes.size // 2
 
 
-def build(kind, v0, v1=None, v2=None, ratio=None, small_max=None, max_qr=1.0, root=None) -> Tree:
+def build(kind, v0, v1=None, v2=None, ratio=None, small_max=None, max_qr=1.0, root=None, nthreads=0) -> Tree:
+    """Oracle kd-tree (the tree is the same for every thread count; 0 = host_threads())."""
     L = lib()
     if ratio is None:
         ratio = 0.25 if kind == 0 else 0.1
@@ -146,8 +149,8 @@ def build(kind, v0, v1=None, v2=None, ratio=None, small_max=None, max_qr=1.0, ro
     v1 = _c(v1) if v1 is not None else v0
     v2 = _c(v2) if v2 is not None else v0
     r6 = _c(root).ravel() if root is not None else np.zeros(6, np.float32)
-    h = L.orc_build(kind, v0.ctypes.data, v1.ctypes.data, v2.ctypes.data, v0.shape[0], ratio, small_max,
-                    max_qr, 1 if root is not None else 0, r6.ctypes.data)
+    h = L.orc_build_mt(kind, v0.ctypes.data, v1.ctypes.data, v2.ctypes.data, v0.shape[0], ratio, small_max,
+                       max_qr, 1 if root is not None else 0, r6.ctypes.data, _threads(nthreads))
     if not h:
         raise ValueError("oracle build rejected its parameters")
     try:

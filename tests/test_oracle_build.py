@@ -155,3 +155,17 @@ def test_build_stream_counts_invariants(orc):
     assert clips % 2 == 0 and child >= large and small >= sc.num_tris
     b = orc.build_bytes(T)
     assert b > sc.num_tris * (36 + 28) + 8 * T.num_nodes
+
+
+def test_parallel_build_equals_single_thread(orc):
+    """The level-parallel large stage and the per-root parallel small stage give the same
+    tree (node words, references, counts) as one thread, for triangles and points."""
+    sc = S.scene_small(subdiv=4)
+    rng = np.random.default_rng(8)
+    pts = np.zeros((120000, 4), np.float32)
+    pts[:, :3] = rng.uniform(-1, 1, (120000, 3))
+    for kind, args in ((0, (sc.v0, sc.v1, sc.v2)), (1, (pts,))):
+        one = orc.build(kind, *args, nthreads=1)
+        many = orc.build(kind, *args, nthreads=7)
+        assert np.array_equal(one.nodes, many.nodes) and np.array_equal(one.refs, many.refs)
+        assert np.array_equal(one.counts, many.counts) and one.max_level == many.max_level
