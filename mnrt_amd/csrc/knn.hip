@@ -129,15 +129,15 @@ __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint3
         const uint32_t probe = hi | (1u << b);
         uint32_t c = 0;
 #pragma unroll
-        for (int t = 0; t < R; ++t) c += (uint32_t)__popcll(__ballot((uint32_t)(key[t] >> 32) < probe));
+        for (int t = 0; t < R; ++t) c += (uint32_t)__popcll(ballot((uint32_t)(key[t] >> 32) < probe));
         if (c < k) hi = probe;
     }
     uint32_t below = 0, tie = 0;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
         const uint32_t h = (uint32_t)(key[t] >> 32);
-        below += (uint32_t)__popcll(__ballot(h < hi));
-        tie += (uint32_t)__popcll(__ballot(h == hi));
+        below += (uint32_t)__popcll(ballot(h < hi));
+        tie += (uint32_t)__popcll(ballot(h == hi));
     }
     const uint32_t need = k - below;  // 1 <= need <= tie
     if (need == tie) {
@@ -153,7 +153,7 @@ __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint3
         uint32_t c = 0;
 #pragma unroll
         for (int t = 0; t < R; ++t)
-            c += (uint32_t)__popcll(__ballot((uint32_t)(key[t] >> 32) == hi && (uint32_t)key[t] < probe));
+            c += (uint32_t)__popcll(ballot((uint32_t)(key[t] >> 32) == hi && (uint32_t)key[t] < probe));
         if (c < need) lo = probe;
     }
     return ((uint64_t)hi << 32) | lo;
@@ -184,7 +184,7 @@ __device__ __forceinline__ uint32_t wave_window_hi(const uint64_t (&key)[R], uin
         const uint32_t probe = prefix + (uint32_t)width;
         uint32_t c = 0;
 #pragma unroll
-        for (int t = 0; t < R; ++t) c += (uint32_t)__popcll(__ballot((uint32_t)(key[t] >> 32) < probe));
+        for (int t = 0; t < R; ++t) c += (uint32_t)__popcll(ballot((uint32_t)(key[t] >> 32) < probe));
         if (c >= k) upper = c;
         else prefix = probe;
     }
@@ -220,6 +220,18 @@ struct WarmHint {
     float r2;  // previous query's k-th d2; < 0: none (fewer than k found, or first query)
 };
 
+// Warm-start bound at q from a query at h with k-th distance sqrt(h.r2): every one of
+// that query's k neighbours lies within sqrt(h.r2) + |h - q| of q (margins of 2^-20 cover
+// the rounding of the fp32 distances; the bound is exclusive, hence nextafter).  r2max
+// when the hint has no distance or the bound is not below it.
+__device__ __forceinline__ float warm_bound(const WarmHint& h, float qx, float qy, float qz, float r2max) {
+    if (!(h.r2 >= 0.0f)) return r2max;
+    const double dx = (double)qx - (double)h.x, dy = (double)qy - (double)h.y, dz = (double)qz - (double)h.z;
+    const double b = (sqrt((double)h.r2) + sqrt(dx * dx + dy * dy + dz * dz)) * (1.0 + 0x1p-20);
+    const double b2 = b * b * (1.0 + 0x1p-20);
+    return b2 < (double)r2max ? nextafterf((float)b2, kInf) : r2max;
+}
+
 // One query by one wave; keys / kpos: CAP LDS entries, snode / soff: kKnnStack.
 template <int CAP, int kMode, int kSel = kSelSort, bool kPairs = false>
 __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, uint32_t q, uint32_t lane,
@@ -234,13 +246,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     // search radius: r_max, or the warm-start bound when smaller (margins of 2^-20 cover
     // the rounding of the fp32 distances; the bound is exclusive, hence nextafter)
     float r2lim = r2max;
-    if (hint && hint->r2 >= 0.0f && k > 0) {
-        const double dx = (double)qx - (double)hint->x, dy = (double)qy - (double)hint->y,
-                     dz = (double)qz - (double)hint->z;
-        const double h = (sqrt((double)hint->r2) + sqrt(dx * dx + dy * dy + dz * dz)) * (1.0 + 0x1p-20);
-        const double h2 = h * h * (1.0 + 0x1p-20);
-        if (h2 < (double)r2max) r2lim = nextafterf((float)h2, kInf);
-    }
+    if (hint && k > 0) r2lim = warm_bound(*hint, qx, qy, qz, r2max);
 
     uint32_t cnt = 0;        // candidates in the buffer
     bool full = false;       // the buffer's first k are the k best so far
@@ -374,24 +380,23 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
             }
             wave_sync();
             const uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
+            const float4* lrec = Tr.recs + 2ull * begin;  // the leaf's records: 32-bit offsets from here
             for (uint32_t base = 0; base < count; base += 64) {
-                uint32_t j = base + lane;
-                bool take = false;
-                uint64_t key = 0;
-                if (j < count) {
-                    float4 p = Tr.recs[2ull * (begin + j)];
-                    f3 df = sub3(xyz(p), qp);
-                    float d2 = dot3(df, df);
-                    key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-                    take = (d2 < r2lim) && (key < thr_key);
-                }
-                uint64_t m = __ballot(take);
+                const uint32_t j = base + lane;
+                const bool in = j < count;
+                // lanes past the leaf re-read its last record (no branch around the load)
+                const float4 p = lrec[2u * (in ? j : count - 1u)];
+                const f3 df = sub3(xyz(p), qp);
+                const float d2 = dot3(df, df);
+                uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
+                bool take = in && (d2 < r2lim) && (key < thr_key);
+                uint64_t m = ballot(take);
                 uint32_t nm = (uint32_t)__popcll(m);
                 if (nm == 0) continue;
                 if (cnt + nm > (uint32_t)CAP) {
                     select(k);
                     take = take && (key < thr_key);
-                    m = __ballot(take);
+                    m = ballot(take);
                     nm = (uint32_t)__popcll(m);
                 }
                 if (take) {
@@ -466,8 +471,9 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     if (dot3(nrd, nq) < 0.0f) nq = mk3(nq.x * -1.0f, nq.y * -1.0f, nq.z * -1.0f);
 
     // Front-facing powers: each lane fetches the records of its slots of the set, the
-    // channel sums are exact fixed-point sums (independent of the set's order).
-    constexpr int RK = (int)((MNRT_KNN_MAX_K + 63u) / 64u);
+    // channel sums are exact fixed-point sums (independent of the set's order).  The set
+    // has at most k <= CAP - 64 entries (the launch picks CAP >= k + 64).
+    constexpr int RK = std::min<int>((CAP - 64) / 64, (int)((MNRT_KNN_MAX_K + 63u) / 64u));
     float vx[RK], vy[RK], vz[RK];
 #pragma unroll
     for (int t = 0; t < RK; ++t) {

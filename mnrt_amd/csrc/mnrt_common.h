@@ -198,6 +198,11 @@ __device__ __forceinline__ float uniformf(float v) {
     return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
 }
 
+// Mask of the lanes where p holds.  The builtin takes the predicate as a bool, so a
+// comparison's lane mask is used as it is (HIP's int-argument __ballot first turns it into
+// a 0/1 vector value and compares that against 0: two more vector instructions).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // Lanes below this one whose predicate is set (exclusive prefix of a 64-bit ballot).
 __device__ __forceinline__ uint32_t mask_prefix(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
