@@ -186,10 +186,13 @@ def batched_leg(args, torch, api, S, O, ncpu):
     """Batched workload: 256 independent scenes like the small one (box, icosphere,
     random triangles, one light), triangle / photon / query counts log-uniform in
     [1k, 50k] / [4k, 256k] / [16k, 1M] (seed 4).  One step builds every triangle tree and
-    every photon tree (one batched build each), then traces all primary rays, compacts
-    the hits and answers kNN (k = 50) at all hits, each phase one persistent launch
-    replayed from a hipGraph.  The same work through the single-scene entry points, scene
-    after scene, is timed beside it."""
+    every photon tree (one batched build each, device-driven and replayed from a hipGraph:
+    mnrt_build_graph, records and query-time nodes included), then traces all primary
+    rays, compacts the hits and answers kNN (k = 50) at all hits, each phase one
+    persistent launch replayed from a hipGraph.  The same work through the single-scene
+    entry points, scene after scene, is timed beside it."""
+    from mnrt_amd import native as N
+
     t0 = time.time()
     NS, K = args.batched_scenes, 50
     scs, nph, dims = S.batched_workload(NS, seed=4)
@@ -215,14 +218,10 @@ def batched_leg(args, torch, api, S, O, ncpu):
     psl = [slice(int(poff[s]), int(poff[s + 1])) for s in range(NS)]
     del ph
 
-    def build_all():
-        tb = api.build_tri_batch(v0, v1, v2, toff)
-        pb = api.build_points_batch(ppos, poff, root_boxes=boxes, max_query_radius=rmax)
-        api.attach_photon_records_batch(pb, ppos, ppow, pdir)
-        return tb, pb
-
     TB.release()
-    TB, PB = build_all()
+    GT = api.BuildGraph(N.KIND_TRI, toff, v0, v1, v2)
+    GP = api.BuildGraph(N.KIND_POINT, poff, ppos, power=ppow, dirs=pdir, root_boxes=boxes, max_query_radius=rmax)
+    TB, PB = GT.trees, GP.trees
     Q = api.QueryBatch(TB, toff, roff, eps, PB, rmax, K)
     f4 = lambda: torch.empty((R, 4), dtype=torch.float32, device="cuda")
     bufs = dict(orig4=go, dir4=gd, v0=v0, v1=v1, v2=v2, albedo4=alb,
@@ -246,11 +245,9 @@ def batched_leg(args, torch, api, S, O, ncpu):
         torch.cuda.synchronize()
         w0 = time.time()
         ev[0].record(stream)
-        tb = api.build_tri_batch(v0, v1, v2, toff)
+        GT.replay()
         ev[1].record(stream)
-        pb = api.build_points_batch(ppos, poff, root_boxes=boxes, max_query_radius=rmax)
-        api.attach_photon_records_batch(pb, ppos, ppow, pdir)
-        Q.update_trees(tb, pb)
+        GP.replay()
         bufs["io_rad4"].zero_()
         ev[2].record(stream)
         for i, ph_ in enumerate((1, 2, 4)):
@@ -258,7 +255,10 @@ def batched_leg(args, torch, api, S, O, ncpu):
             ev[3 + i].record(stream)
         torch.cuda.synchronize()
         wall = time.time() - w0
-        return (tb, pb), {n: ev[i].elapsed_time(ev[i + 1]) for i, n in enumerate(names)}, wall
+        # the rebuilt trees fit the graphs' buffers (same input: same trees, same addresses)
+        if GT.status() != 0 or GP.status() != 0:
+            raise RuntimeError("a recorded build outgrew its buffers")
+        return {n: ev[i].elapsed_time(ev[i + 1]) for i, n in enumerate(names)}, wall
 
     def plain_step():
         keep = []
@@ -278,13 +278,10 @@ def batched_leg(args, torch, api, S, O, ncpu):
     steps = max(1, args.batched_steps)
     phase, walls = [], []
     for it in range(steps + 1):  # the first is a warm-up
-        trees, ms, wall = batched_step()
+        ms, wall = batched_step()
         if it > 0:
             phase.append(ms)
             walls.append(wall)
-        Q.update_trees(TB, PB)
-        for t_ in trees:
-            t_.release()
     batched_s = float(np.mean(walls))
     hits = int(bufs["sp_count"].sum().item())
     pwalls = []
@@ -314,8 +311,8 @@ def batched_leg(args, torch, api, S, O, ncpu):
     knn_ms = float(np.mean([p["knn"] for p in phase]))
     gbs = per_q * hits / (knn_ms * 1e-3) / 1e9
     Q.close()
-    TB.release()
-    PB.release()
+    GT.close()
+    GP.close()
     return {
         "knn_roofline": {"kernel": "k_knn_batch (MNRT_PHASE_KNN)", "bound": "hbm", "achieved": round(gbs, 2),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
@@ -323,6 +320,8 @@ def batched_leg(args, torch, api, S, O, ncpu):
                          "sample": f"1 in {args.sample} queries of every 8th scene"},
         "workload": (f"{NS} scenes: triangles log-U[1k,50k], photons log-U[4k,256k], queries log-U[16k,1M] "
                      "(seed 4), k=50; one step = all tree builds + all traversals + compaction + all kNN"),
+        "builds": "device-driven, one hipGraph replay per build (mnrt_build_graph: levels, small stage, layout, "
+                  "emission, records, query-time nodes; no host synchronisation)",
         "scenes": NS, "triangles": int(toff[-1]), "photons": int(poff[-1]), "rays": R, "knn_queries": hits,
         "value": round(hits / batched_s / 1e6, 3), "unit": "M kNN queries/s (step incl. builds and traversal)",
         "ms_per_step": round(batched_s * 1e3, 3), "plain_ms_per_step": round(plain_s * 1e3, 3),

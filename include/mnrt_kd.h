@@ -43,7 +43,7 @@ extern "C" {
 
 /* Raised whenever a declaration or struct layout here changes (2: mnrt_batch_io gained the
  * adjoint buffers). */
-#define MNRT_ABI_VERSION 3
+#define MNRT_ABI_VERSION 4
 
 enum mnrt_status_code {
     MNRT_OK = 0,
@@ -144,7 +144,7 @@ int mnrt_build_points_batch(const float* d_pos4, const uint32_t* h_offsets, uint
                             const float* h_max_query_radius, mnrt_tree* out, void* stream);
 /* Records of a batch (trees from one batched build, inputs concatenated as for the
  * build): one shared record array owned by trees[0]; photon batches are collapsed with
- * max_refs = 256 like mnrt_tree_attach_photon_records.  Called again on photon trees whose
+ * max_refs = 512 like mnrt_tree_attach_photon_records.  Called again on photon trees whose
  * records it attached, it rewrites the records in place (new photon powers, say) and
  * keeps the arrays, so batches made from these trees stay valid. */
 int mnrt_tree_attach_tri_records_batch(mnrt_tree* trees, uint32_t num_scenes, const float* d_v0,
@@ -154,6 +154,44 @@ int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t num_scenes,
                                           const float* d_pos4, const float* d_pow4,
                                           const float* d_dir4, const uint32_t* h_offsets,
                                           void* stream);
+
+/* Device-driven batched build recorded into a hipGraph (no reference counterpart: the
+ * reference's level loop reads its counters on the host every level, KDTreeGPU.cpp:247-281).
+ * mnrt_build_graph_create builds the batch once as mnrt_build_*_batch does, sizes fixed
+ * buffers from that build (each quantity the level loop grows a buffer for, times
+ * 1 + margin), and records the whole build into a graph: every level (num_levels of the
+ * first build + 2), the small-node stage, layout, emission, the records (photon trees
+ * when d_power4 / d_dir4 are given) and, for photon trees, the query-time node words.
+ * Counts stay on the device; there is no host synchronisation and no allocation in it.
+ * It then replays the graph once and fills out[num_scenes] with the trees, which equal
+ * mnrt_build_*_batch's word for word.  The trees are views into arrays the graph owns
+ * (owns_* = 0; mnrt_build_graph_destroy frees them).
+ * mnrt_build_graph_replay rebuilds the trees in place from the input addresses given at
+ * creation (their contents may change; the element counts are fixed).  The trees move
+ * only when their sizes change: mnrt_build_graph_status waits for the stream, returns in
+ * *h_flags 0 when the last replay fit its buffers (and then refreshes out[], if given,
+ * with its sizes, root boxes and addresses), nonzero when it did not (the trees are then
+ * invalid until a replay that fits; create a new graph for the new sizes). */
+typedef struct mnrt_build_graph mnrt_build_graph;
+typedef struct mnrt_build_graph_desc {
+    int kind;                         /* MNRT_KIND_TRI or MNRT_KIND_POINT */
+    uint32_t num_scenes;
+    const uint32_t* h_offsets;        /* num_scenes + 1 element offsets (host) */
+    const float* d_v0;                /* triangles: float4 vertices; points: float4 positions */
+    const float* d_v1;
+    const float* d_v2;
+    const float* d_power4;            /* points: photon powers and directions for the */
+    const float* d_dir4;              /* records (both NULL: no records) */
+    const mnrt_build_params* params;
+    const float* h_root_boxes;        /* NULL or 6 floats per scene, as mnrt_build_*_batch */
+    const float* h_max_query_radius;  /* NULL or one per scene */
+    float margin;                     /* buffer room over the first build's sizes (0..16) */
+} mnrt_build_graph_desc;
+int mnrt_build_graph_create(const mnrt_build_graph_desc* desc, mnrt_tree* out, mnrt_build_graph** graph,
+                            void* stream);
+int mnrt_build_graph_replay(mnrt_build_graph* graph, void* stream);
+int mnrt_build_graph_status(mnrt_build_graph* graph, mnrt_tree* out, uint32_t* h_flags, void* stream);
+int mnrt_build_graph_destroy(mnrt_build_graph* graph);
 
 /* Leaf-ordered records for a tree whose nodes/refs exist. */
 int mnrt_tree_attach_tri_records(mnrt_tree* tree, const float* d_v0, const float* d_v1,
@@ -167,7 +205,7 @@ int mnrt_tree_attach_photon_records(mnrt_tree* tree, const float* d_pos4, const 
  * references becomes a leaf over that contiguous reference range (a subtree's leaves are
  * consecutive in preorder).  A wave then tests such a subtree's elements in one step.
  * The build's node words (d_nodes) are unchanged; queries read d_qnodes.  Query results
- * do not depend on it.  mnrt_tree_attach_photon_records applies max_refs = 256. */
+ * do not depend on it.  mnrt_tree_attach_photon_records applies max_refs = 512. */
 int mnrt_tree_collapse(mnrt_tree* tree, uint32_t max_refs, void* stream);
 
 /* Reference KDTreeData layout of a tree (KDTreeData, KDKernelDefs.h:76-194, as

@@ -70,6 +70,27 @@ class DeviceTree:
         _copy_from_ptr(out, self.t.d_refs, n * 4)
         return out.cpu().numpy().view(np.uint32).copy()
 
+    def recs_host(self) -> np.ndarray:
+        """Leaf-ordered records as float32 rows (12 floats per triangle, 8 per photon)."""
+        torch = _torch()
+        per = 12 if self.kind == N.KIND_TRI else 8
+        n = self.t.num_refs * per
+        if n == 0 or not self.t.d_recs:
+            return np.zeros((0, per), np.float32)
+        out = torch.empty(n, dtype=torch.float32, device="cuda")
+        _copy_from_ptr(out, self.t.d_recs, n * 4)
+        return out.cpu().numpy().reshape(-1, per).copy()
+
+    def qnodes_host(self) -> np.ndarray:
+        """Query-time node words (empty when the tree has none)."""
+        torch = _torch()
+        n = 2 * self.t.num_nodes
+        if not self.t.d_qnodes:
+            return np.zeros(0, np.uint32)
+        out = torch.empty(n, dtype=torch.int32, device="cuda")
+        _copy_from_ptr(out, self.t.d_qnodes, n * 4)
+        return out.cpu().numpy().view(np.uint32).copy()
+
     def cost(self, max_query_radius: float = 1.0) -> float:
         c = ctypes.c_double(0.0)
         N.check(N.load().mnrt_tree_cost(self.ref(), max_query_radius, ctypes.byref(c), N.stream_handle()),
@@ -174,6 +195,52 @@ def attach_photon_records_batch(B: TreeBatch, pos, power, dirs):
     N.check(N.load().mnrt_tree_attach_photon_records_batch(B.ref(), B.num_scenes, N.ptr(pos), N.ptr(power),
                                                            N.ptr(dirs), B.offsets_ref(), N.stream_handle()),
             "mnrt_tree_attach_photon_records_batch")
+
+
+class BuildGraph:
+    """A batched build recorded into a hipGraph (mnrt_build_graph_*): `trees` (a TreeBatch
+    over the graph's arrays) is rebuilt in place by replay(), from the input tensors given
+    here (their contents may change, their sizes not).  status() waits for the stream and
+    returns 0 when the last replay fit the graph's buffers, refreshing the trees' sizes,
+    root boxes and addresses; nonzero means the trees are invalid."""
+
+    def __init__(self, kind, offsets, v0, v1=None, v2=None, power=None, dirs=None,
+                 params: N.BuildParams | None = None, root_boxes=None, max_query_radius=None, margin=0.25):
+        ins = [dev(a) if a is not None else None for a in (v0, v1, v2, power, dirs)]
+        self.trees = TreeBatch(kind, offsets)
+        if int(self.trees.offsets[-1]) != ins[0].shape[0]:
+            raise N.MnrtError("offsets do not cover the input")
+        self._keep = [a for a in ins if a is not None]
+        self.params = params or default_params(kind)
+        ns = self.trees.num_scenes
+        self._rb, rbp = _f32_ptr(root_boxes, 6, ns)
+        self._qr, qrp = _f32_ptr(max_query_radius, 1, ns)
+        p = lambda a: N.ptr(a) if a is not None else None
+        d = N.BuildGraphDesc(kind, ns, self.trees.offsets_ref(), p(ins[0]), p(ins[1]), p(ins[2]), p(ins[3]),
+                             p(ins[4]), ctypes.pointer(self.params), rbp, qrp, float(margin))
+        self.h = ctypes.c_void_p()
+        N.check(N.load().mnrt_build_graph_create(ctypes.byref(d), self.trees.ref(), ctypes.byref(self.h),
+                                                 N.stream_handle()), "mnrt_build_graph_create")
+
+    def replay(self, stream=None):
+        N.check(N.load().mnrt_build_graph_replay(self.h, N.stream_handle(stream)), "mnrt_build_graph_replay")
+
+    def status(self, stream=None) -> int:
+        flags = N.u32(0)
+        N.check(N.load().mnrt_build_graph_status(self.h, self.trees.ref(), ctypes.byref(flags),
+                                                 N.stream_handle(stream)), "mnrt_build_graph_status")
+        return int(flags.value)
+
+    def close(self):
+        if self.h:
+            N.check(N.load().mnrt_build_graph_destroy(self.h), "mnrt_build_graph_destroy")
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _copy_from_ptr(dst, src_ptr: int, nbytes: int):

@@ -125,6 +125,11 @@ int launch_knn_bwd_batch(const KnnArgs& G, const KnnScene* scenes, const uint32_
                          uint32_t num_parts, const uint32_t* part_begin, const uint32_t* part_end, uint32_t* work,
                          uint32_t grid, float* grad_pow4, hipStream_t st);
 int launch_shade_batch(const ShadeBatchArgs& A, uint32_t* flags, uint32_t* scan_tmp, hipStream_t st);
+int launch_records_batch(int kind, const uint32_t* refs, DN nrefs, const uint32_t* rbase, const uint32_t* ebase,
+                         uint32_t ns, const float4* a, const float4* b, const float4* c, float4* recs,
+                         const uint32_t* err, hipStream_t st);
+int launch_collapse_batch(const uint2* nodes, DN nnodes, const uint32_t* nbase, uint32_t ns, uint32_t max_refs,
+                          uint2* qnodes, const uint32_t* err, hipStream_t st);
 
 }  // namespace mnrt
 

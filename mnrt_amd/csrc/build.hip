@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "batch_dev.h"
 #include "scan.h"
 
 namespace mnrt {
@@ -237,6 +238,11 @@ __device__ void clip_bounds(const Geometry& G, uint32_t idx, float4& lo, float4&
 // ---------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------
+// Counts come as DN (host value or device word).  Per-item kernels stride over the grid,
+// so any grid is correct: the host-driven build launches one item per thread, the
+// recorded device-driven build launches grids sized from the first build's counts.
+#define MNRT_GRID_STRIDE(i, n) \
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 __global__ __launch_bounds__(256) void k_init_elems(Geometry G, uint32_t n, ElemView E) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -259,37 +265,39 @@ __global__ __launch_bounds__(256) void k_init_elems(Geometry G, uint32_t n, Elem
     E.node[i] = s;
 }
 
-__global__ __launch_bounds__(256) void k_chunk_counts(uint32_t n, const uint32_t* __restrict__ cnt,
+__global__ __launch_bounds__(256) void k_chunk_counts(DN dn_, const uint32_t* __restrict__ cnt,
                                                       uint32_t* __restrict__ cc) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) cc[p] = (cnt[p] + kChunk - 1) / kChunk;
+    const uint32_t n = dn_.get();
+    MNRT_GRID_STRIDE(p, n) cc[p] = (cnt[p] + kChunk - 1) / kChunk;
 }
 
 // Chunk -> node by binary search over the chunk offsets (kernel_GenerateChunks,
 // kdtree_chunklist.cu:110-134, runs one serial thread per node instead).
-__global__ __launch_bounds__(256) void k_chunk_fill(uint32_t n, uint32_t max_chunks, const uint32_t* __restrict__ coff,
+__global__ __launch_bounds__(256) void k_chunk_fill(DN dn_, uint32_t max_chunks, const uint32_t* __restrict__ coff,
                                                     const uint32_t* __restrict__ counters,
                                                     const uint32_t* __restrict__ first,
                                                     const uint32_t* __restrict__ cnt, uint32_t* __restrict__ ch_node,
                                                     uint32_t* __restrict__ ch_first, uint32_t* __restrict__ ch_cnt) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= max_chunks || c >= counters[C_NCHUNKS]) return;
-    uint32_t lo = 0, hi = n;  // last p with coff[p] <= c
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (coff[mid] <= c) lo = mid;
-        else hi = mid;
+    const uint32_t n = dn_.get();
+    const uint32_t nch = min(max_chunks, counters[C_NCHUNKS]);
+    MNRT_GRID_STRIDE(c, nch) {
+        uint32_t lo = 0, hi = n;  // last p with coff[p] <= c
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (coff[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        uint32_t p = lo;
+        uint32_t local = c - coff[p];
+        ch_node[c] = p;
+        ch_first[c] = first[p] + local * kChunk;
+        ch_cnt[c] = min(kChunk, cnt[p] - local * kChunk);
     }
-    uint32_t p = lo;
-    uint32_t local = c - coff[p];
-    ch_node[c] = p;
-    ch_first[c] = first[p] + local * kChunk;
-    ch_cnt[c] = min(kChunk, cnt[p] - local * kChunk);
 }
 
-__global__ __launch_bounds__(256) void k_tight_init(uint32_t n, uint32_t* __restrict__ tlo, uint32_t* __restrict__ thi) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 3 * n) {
+__global__ __launch_bounds__(256) void k_tight_init(DN dn_, uint32_t* __restrict__ tlo, uint32_t* __restrict__ thi) {
+    const uint32_t n3 = 3u * dn_.get();
+    MNRT_GRID_STRIDE(i, n3) {
         tlo[i] = 0xFFFFFFFFu;
         thi[i] = 0u;
     }
@@ -309,52 +317,52 @@ __global__ __launch_bounds__(kChunk) void k_tight(uint32_t max_chunks, const uin
                                                   uint32_t* __restrict__ tlo, uint32_t* __restrict__ thi) {
     __shared__ uint32_t s_red[6][kChunk / 64];
     const uint32_t nch = min(max_chunks, counters[C_NCHUNKS]);
-    const uint32_t c0 = blockIdx.x * kTightChunks;
-    if (c0 >= nch) return;
-    const uint32_t c1 = min(c0 + kTightChunks, nch);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t cur = kNone;  // node of the running accumulation (thread 0)
-    uint32_t acc[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t node = ch_node[c], first = ch_first[c], count = ch_cnt[c];
-        uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-        if (threadIdx.x < count) {
-            float4 lo = E.lo[first + threadIdx.x];
-            float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[first + threadIdx.x] : lo;
-            v[0] = fenc(lo.x);
-            v[1] = fenc(lo.y);
-            v[2] = fenc(lo.z);
-            v[3] = fenc(hi.x);
-            v[4] = fenc(hi.y);
-            v[5] = fenc(hi.z);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) v[k] = wave_min_u32(v[k]);
-#pragma unroll
-        for (int k = 3; k < 6; ++k) v[k] = wave_max_u32(v[k]);
-        if (lane == 0)
-            for (int k = 0; k < 6; ++k) s_red[k][wave] = v[k];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (uint32_t w = 1; w < kChunk / 64; ++w) {
-                for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_red[k][w]);
-                for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_red[k][w]);
+    for (uint32_t c0 = blockIdx.x * kTightChunks; c0 < nch; c0 += gridDim.x * kTightChunks) {  // uniform per block
+        const uint32_t c1 = min(c0 + kTightChunks, nch);
+        uint32_t cur = kNone;  // node of the running accumulation (thread 0)
+        uint32_t acc[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+        for (uint32_t c = c0; c < c1; ++c) {
+            const uint32_t node = ch_node[c], first = ch_first[c], count = ch_cnt[c];
+            uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+            if (threadIdx.x < count) {
+                float4 lo = E.lo[first + threadIdx.x];
+                float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[first + threadIdx.x] : lo;
+                v[0] = fenc(lo.x);
+                v[1] = fenc(lo.y);
+                v[2] = fenc(lo.z);
+                v[3] = fenc(hi.x);
+                v[4] = fenc(hi.y);
+                v[5] = fenc(hi.z);
             }
-            if (node != cur && cur != kNone) {
-                for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * cur + k], acc[k]);
-                for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * cur + k], acc[3 + k]);
-                for (int k = 0; k < 3; ++k) acc[k] = 0xFFFFFFFFu;
-                for (int k = 3; k < 6; ++k) acc[k] = 0u;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = wave_min_u32(v[k]);
+#pragma unroll
+            for (int k = 3; k < 6; ++k) v[k] = wave_max_u32(v[k]);
+            if (lane == 0)
+                for (int k = 0; k < 6; ++k) s_red[k][wave] = v[k];
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (uint32_t w = 1; w < kChunk / 64; ++w) {
+                    for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_red[k][w]);
+                    for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_red[k][w]);
+                }
+                if (node != cur && cur != kNone) {
+                    for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * cur + k], acc[k]);
+                    for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * cur + k], acc[3 + k]);
+                    for (int k = 0; k < 3; ++k) acc[k] = 0xFFFFFFFFu;
+                    for (int k = 3; k < 6; ++k) acc[k] = 0u;
+                }
+                cur = node;
+                for (int k = 0; k < 3; ++k) acc[k] = min(acc[k], v[k]);
+                for (int k = 3; k < 6; ++k) acc[k] = max(acc[k], v[k]);
             }
-            cur = node;
-            for (int k = 0; k < 3; ++k) acc[k] = min(acc[k], v[k]);
-            for (int k = 3; k < 6; ++k) acc[k] = max(acc[k], v[k]);
+            __syncthreads();  // s_red is rewritten by the next chunk
         }
-        __syncthreads();  // s_red is rewritten by the next chunk
-    }
-    if (threadIdx.x == 0 && cur != kNone) {
-        for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * cur + k], acc[k]);
-        for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * cur + k], acc[3 + k]);
+        if (threadIdx.x == 0 && cur != kNone) {
+            for (int k = 0; k < 3; ++k) atomicMin(&tlo[3 * cur + k], acc[k]);
+            for (int k = 0; k < 3; ++k) atomicMax(&thi[3 * cur + k], acc[3 + k]);
+        }
     }
 }
 
@@ -458,12 +466,10 @@ struct PlanView {
 // inner node with an empty leaf and a tighter copy.  The median splits the longest axis
 // of the cut box, ties to the lower axis.
 template <bool kWrite>
-__global__ __launch_bounds__(256) void k_plan(uint32_t n, NodeView A, const uint32_t* __restrict__ tlo,
-                                              const uint32_t* __restrict__ thi, float ratio, PlanView P,
-                                              const uint32_t* __restrict__ base, uint32_t bn_base, BnView B,
-                                              uint32_t* __restrict__ counters) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+__device__ __forceinline__ void plan_node(uint32_t p, const NodeView& A, const uint32_t* __restrict__ tlo,
+                                          const uint32_t* __restrict__ thi, float ratio, const PlanView& P,
+                                          const uint32_t* __restrict__ base, uint32_t bn_base, const BnView& B,
+                                          uint32_t* __restrict__ counters) {
     float lo[3], hi[3], tl[3], th[3];
     float4 l4 = A.lo[p], h4 = A.hi[p];
     lo[0] = l4.x; lo[1] = l4.y; lo[2] = l4.z;
@@ -546,19 +552,29 @@ __global__ __launch_bounds__(256) void k_plan(uint32_t n, NodeView A, const uint
     B.scene[R] = A.scene[p];
 }
 
-__global__ __launch_bounds__(256) void k_classify(uint32_t ne, ElemView E, int kind, PlanView P,
+template <bool kWrite>
+__global__ __launch_bounds__(256) void k_plan(DN dn_, NodeView A, const uint32_t* __restrict__ tlo,
+                                              const uint32_t* __restrict__ thi, float ratio, PlanView P,
+                                              const uint32_t* __restrict__ base, DN bn_base, BnView B,
+                                              uint32_t* __restrict__ counters) {
+    const uint32_t n = dn_.get(), b0 = bn_base.get();
+    MNRT_GRID_STRIDE(p, n) plan_node<kWrite>(p, A, tlo, thi, ratio, P, base, b0, B, counters);
+}
+
+__global__ __launch_bounds__(256) void k_classify(DN dne, ElemView E, int kind, PlanView P,
                                                   uint32_t* __restrict__ fl) {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    uint32_t p = E.node[e];
-    bool L = false, R = false;
-    if (!P.leaf[p]) {
-        float4 lo = E.lo[e];
-        float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[e] : lo;
-        classify(kind, E.idx[e], lo, hi, P.axis[p], P.split[p], L, R);
+    const uint32_t ne = dne.get();
+    MNRT_GRID_STRIDE(e, ne) {
+        uint32_t p = E.node[e];
+        bool L = false, R = false;
+        if (!P.leaf[p]) {
+            float4 lo = E.lo[e];
+            float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[e] : lo;
+            classify(kind, E.idx[e], lo, hi, P.axis[p], P.split[p], L, R);
+        }
+        fl[e] = L ? 1u : 0u;
+        fl[ne + e] = R ? 1u : 0u;
     }
-    fl[e] = L ? 1u : 0u;
-    fl[ne + e] = R ? 1u : 0u;
 }
 
 struct ChildView {
@@ -577,41 +593,41 @@ struct ChildView {
 
 // Child counts from the flag scan, the no-progress test, and the small/large split
 // (kernel_MarkSmallNodes, kdtree.cu:849-875: small iff count <= smallNodeMax).
-__global__ __launch_bounds__(256) void k_child_plan(uint32_t n, uint32_t ne, NodeView A, PlanView P,
+__global__ __launch_bounds__(256) void k_child_plan(DN dn_, DN dne, NodeView A, PlanView P,
                                                     const uint32_t* __restrict__ pos, uint32_t small_max,
                                                     ChildView Cv) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= 2 * n) return;
-    uint32_t side = c >= n ? 1u : 0u;
-    uint32_t p = c - side * n;
-    uint32_t type = BN_DEAD, cnt = 0, fpos = 0;
-    if (!P.leaf[p]) {
-        uint32_t f = A.first[p], m = A.cnt[p];
-        uint32_t nl = pos[f + m] - pos[f];
-        uint32_t nr = pos[ne + f + m] - pos[ne + f];
-        bool noprog = (nl == m && nr == m);
-        if (noprog) {
-            if (side == 0) P.noprog[p] = 1;
-        } else {
-            fpos = pos[side * ne + f];
-            cnt = side ? nr : nl;
-            type = (cnt <= small_max) ? BN_SMALL : BN_PENDING;
+    const uint32_t n = dn_.get(), ne = dne.get();
+    MNRT_GRID_STRIDE(c, 2 * n) {
+        uint32_t side = c >= n ? 1u : 0u;
+        uint32_t p = c - side * n;
+        uint32_t type = BN_DEAD, cnt = 0, fpos = 0;
+        if (!P.leaf[p]) {
+            uint32_t f = A.first[p], m = A.cnt[p];
+            uint32_t nl = pos[f + m] - pos[f];
+            uint32_t nr = pos[ne + f + m] - pos[ne + f];
+            bool noprog = (nl == m && nr == m);
+            if (noprog) {
+                if (side == 0) P.noprog[p] = 1;
+            } else {
+                fpos = pos[side * ne + f];
+                cnt = side ? nr : nl;
+                type = (cnt <= small_max) ? BN_SMALL : BN_PENDING;
+            }
         }
+        Cv.type[c] = type;
+        Cv.cnt[c] = cnt;
+        Cv.fpos[c] = fpos;
+        Cv.sflag[c] = type == BN_SMALL ? 1u : 0u;
+        Cv.lflag[c] = type == BN_PENDING ? 1u : 0u;
+        Cv.scnt[c] = type == BN_SMALL ? cnt : 0u;
+        Cv.lcnt[c] = type == BN_PENDING ? cnt : 0u;
     }
-    Cv.type[c] = type;
-    Cv.cnt[c] = cnt;
-    Cv.fpos[c] = fpos;
-    Cv.sflag[c] = type == BN_SMALL ? 1u : 0u;
-    Cv.lflag[c] = type == BN_PENDING ? 1u : 0u;
-    Cv.scnt[c] = type == BN_SMALL ? cnt : 0u;
-    Cv.lcnt[c] = type == BN_PENDING ? cnt : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_child_write(uint32_t n, NodeView A, PlanView P, ChildView Cv,
-                                                     uint32_t small_base, uint32_t small_elem_base, SmallView S,
-                                                     NodeView Nx, BnView B, uint32_t* __restrict__ counters) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= 2 * n) return;
+__device__ __forceinline__ void child_write_one(uint32_t c, uint32_t n, const NodeView& A, const PlanView& P,
+                                                const ChildView& Cv, uint32_t small_base, uint32_t small_elem_base,
+                                                const SmallView& S, const NodeView& Nx, const BnView& B,
+                                                uint32_t* __restrict__ counters) {
     uint32_t side = c >= n ? 1u : 0u;
     uint32_t p = c - side * n;
     if (P.leaf[p]) return;
@@ -654,31 +670,42 @@ __global__ __launch_bounds__(256) void k_child_write(uint32_t n, NodeView A, Pla
     }
 }
 
+__global__ __launch_bounds__(256) void k_child_write(DN dn_, NodeView A, PlanView P, ChildView Cv, DN small_base,
+                                                     DN small_elem_base, SmallView S, NodeView Nx, BnView B,
+                                                     uint32_t* __restrict__ counters) {
+    const uint32_t n = dn_.get(), sb = small_base.get(), seb = small_elem_base.get();
+    MNRT_GRID_STRIDE(c, 2 * n) child_write_one(c, n, A, P, Cv, sb, seb, S, Nx, B, counters);
+}
+
 // Elements of large-stage leaves (level cap, no progress) are copied out before the
 // level's element arrays are released.
-__global__ __launch_bounds__(256) void k_leaf_copy(uint32_t n, NodeView A, PlanView P, ElemView E,
-                                                   uint32_t* __restrict__ ll_idx, BnView B,
+__global__ __launch_bounds__(256) void k_leaf_copy(DN dn_, NodeView A, PlanView P, ElemView E,
+                                                   uint32_t* __restrict__ ll_idx, uint32_t ll_cap, BnView B,
                                                    uint32_t* __restrict__ counters) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !(P.leaf[p] || P.noprog[p])) return;
-    uint32_t m = A.cnt[p], f = A.first[p];
-    uint32_t start = atomicAdd(&counters[C_LLCOUNT], m);
-    for (uint32_t i = 0; i < m; ++i) ll_idx[start + i] = E.idx[f + i];
-    uint32_t cur = P.cur[p];
-    B.aux[cur] = start;
-    B.cnt[cur] = m;
+    const uint32_t n = dn_.get();
+    MNRT_GRID_STRIDE(p, n) {
+        if (!(P.leaf[p] || P.noprog[p])) continue;
+        uint32_t m = A.cnt[p], f = A.first[p];
+        uint32_t start = atomicAdd(&counters[C_LLCOUNT], m);
+        if ((uint64_t)start + m > ll_cap) {
+            atomicOr(&counters[C_ERR], 32u);
+            continue;
+        }
+        for (uint32_t i = 0; i < m; ++i) ll_idx[start + i] = E.idx[f + i];
+        uint32_t cur = P.cur[p];
+        B.aux[cur] = start;
+        B.cnt[cur] = m;
+    }
 }
 
 // Scatter each element into its children: small roots append to the small-element
 // buffer, large children to the next level's arrays.  Straddling triangles are clipped
 // to each child box (kernel_PerformSplitClipping, kdtree_tri.cu:389-489).
-__global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t ne, ElemView E, Geometry G, NodeView A,
-                                                 PlanView P,
-                                                 ChildView Cv, const uint32_t* __restrict__ pos,
-                                                 uint32_t small_elem_base, ElemView Sm, uint32_t sm_cap, ElemView Nx,
-                                                 uint32_t nx_cap, uint32_t* __restrict__ counters) {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
+__device__ __forceinline__ void scatter_one(uint32_t e, uint32_t n, uint32_t ne, const ElemView& E,
+                                            const Geometry& G, const NodeView& A, const PlanView& P,
+                                            const ChildView& Cv, const uint32_t* __restrict__ pos,
+                                            uint32_t small_elem_base, const ElemView& Sm, uint32_t sm_cap,
+                                            const ElemView& Nx, uint32_t nx_cap, uint32_t* __restrict__ counters) {
     uint32_t p = E.node[e];
     if (P.leaf[p] || P.noprog[p]) return;
     uint32_t idx = E.idx[e];
@@ -723,6 +750,14 @@ __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t ne, ElemVi
             Nx.node[d] = Cv.lidx[c];
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(DN dn_, DN dne, ElemView E, Geometry G, NodeView A, PlanView P,
+                                                 ChildView Cv, const uint32_t* __restrict__ pos, DN small_elem_base,
+                                                 ElemView Sm, uint32_t sm_cap, ElemView Nx, uint32_t nx_cap,
+                                                 uint32_t* __restrict__ counters) {
+    const uint32_t n = dn_.get(), ne = dne.get(), seb = small_elem_base.get();
+    MNRT_GRID_STRIDE(e, ne) scatter_one(e, n, ne, E, G, A, P, Cv, pos, seb, Sm, sm_cap, Nx, nx_cap, counters);
 }
 
 // ---- small-node stage: one wave64 per small root ----------------------------------------
@@ -789,7 +824,7 @@ __device__ __forceinline__ uint32_t write_lane(uint32_t v, int lane, uint32_t ol
 // elements have no upper bounds and 3n candidates (triangles: 6n), which halves the mask
 // arrays and lets more waves share a CU.
 template <int kMode, int kKind>
-__global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, SmallView S, ElemView Sm,
+__global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallView S, ElemView Sm,
                                                             const float* __restrict__ scene_qr, SmallOut O) {
     constexpr int kind = kKind;
     constexpr int kCandMax = (kKind == MNRT_KIND_TRI) ? 384 : 192;
@@ -801,8 +836,9 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
 
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
     const uint32_t r = blockIdx.x * kSmallWaves + wave;
-    if (r >= nsmall) return;  // uniform per wave
+    if (r >= dnsmall.get()) return;  // uniform per wave
     if (kMode == kSmallEmit && O.st_ovf && O.st_ovf[r] == 0u) return;
+    if (kMode == kSmallEmit && O.counters[C_ERR] != 0u) return;  // failed build: nothing is emitted
     float* cpos = s_cpos[wave];
     uint64_t* mL = s_mL[wave];
     uint64_t* mR = s_mR[wave];
@@ -1059,10 +1095,10 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(uint32_t nsmall, Sma
     }
 }
 
-__global__ __launch_bounds__(256) void k_small_caps(uint32_t nsmall, const uint32_t* __restrict__ cnt, StageCap cap,
+__global__ __launch_bounds__(256) void k_small_caps(DN dnsmall, const uint32_t* __restrict__ cnt, StageCap cap,
                                                     uint32_t* __restrict__ caps) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nsmall) caps[r] = cap.of(cnt[r]);
+    const uint32_t nsmall = dnsmall.get();
+    MNRT_GRID_STRIDE(r, nsmall) caps[r] = cap.of(cnt[r]);
 }
 
 StageCap stage_cap_setting() {
@@ -1077,10 +1113,10 @@ StageCap stage_cap_setting() {
 // Moves staged small subtrees to their final addresses (one wave per small root):
 // inner words get the subtree's first address added to the right-child index, leaf words
 // its first reference added to the reference offset.
-__global__ __launch_bounds__(64 * kSmallWaves) void k_small_place(uint32_t nsmall, SmallView S, SmallOut O) {
+__global__ __launch_bounds__(64 * kSmallWaves) void k_small_place(DN dnsmall, SmallView S, SmallOut O) {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t r = blockIdx.x * kSmallWaves + wave;
-    if (r >= nsmall || O.st_ovf[r] != 0u) return;
+    if (r >= dnsmall.get() || O.st_ovf[r] != 0u || O.counters[C_ERR] != 0u) return;
     const uint32_t scene = S.scene[r];
     const uint32_t bn = S.bn[r];
     const uint32_t a0 = O.bn_addr[bn], r0 = O.bn_rbase[bn];
@@ -1115,22 +1151,24 @@ struct LayoutView {
 // Build nodes bucketed by level (the order inside a level does not matter).  Counts go
 // through a per-workgroup LDS histogram, so each workgroup adds to a level's global
 // counter once.
-__global__ __launch_bounds__(256) void k_level_hist(uint32_t nbn, BnView B, uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(256) void k_level_hist(DN dnbn, BnView B, uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_hist[64];
     if (threadIdx.x < 64) s_hist[threadIdx.x] = 0u;
     __syncthreads();
+    const uint32_t nbn = dnbn.get();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nbn && B.type[i] < BN_PENDING) atomicAdd(&s_hist[B.level[i] & 63u], 1u);
     __syncthreads();
     if (threadIdx.x < 64 && s_hist[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(256) void k_level_scatter(uint32_t nbn, BnView B, uint32_t* __restrict__ cursor,
+__global__ __launch_bounds__(256) void k_level_scatter(DN dnbn, BnView B, uint32_t* __restrict__ cursor,
                                                        uint32_t* __restrict__ list) {
     __shared__ uint32_t s_cnt[64];
     __shared__ uint32_t s_base[64];
     if (threadIdx.x < 64) s_cnt[threadIdx.x] = 0u;
     __syncthreads();
+    const uint32_t nbn = dnbn.get();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < nbn && B.type[i] < BN_PENDING;
     const uint32_t lvl = live ? (B.level[i] & 63u) : 0u;
@@ -1196,13 +1234,13 @@ __global__ __launch_bounds__(256) void k_layout_down(uint32_t lvl, LayoutSrc Ls,
     }
 }
 
-__global__ __launch_bounds__(256) void k_emit_large(uint32_t nbn, BnView B, LayoutView Lv, const uint32_t* __restrict__ ll_idx,
+__global__ __launch_bounds__(256) void k_emit_large(DN dnbn, BnView B, LayoutView Lv, const uint32_t* __restrict__ ll_idx,
                                                     uint32_t* __restrict__ nodes_all, uint32_t* __restrict__ refs_all,
                                                     uint32_t* __restrict__ counters, uint32_t max_nodes_all,
                                                     uint32_t max_refs_all, const uint32_t* __restrict__ node_base,
                                                     const uint32_t* __restrict__ ref_base) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbn) return;
+    if (i >= dnbn.get() || counters[C_ERR] != 0u) return;  // failed build: nothing is emitted
     uint32_t t = B.type[i];
     if (t > BN_LEAF) return;
     const uint32_t sc = B.scene[i];
@@ -1327,6 +1365,16 @@ class Pool {
 
 inline dim3 blocks_for(uint64_t n, uint32_t bs = 256) { return dim3((uint32_t)std::max<uint64_t>(1, (n + bs - 1) / bs)); }
 
+// Sizes a host-driven build met, from which a recorded device-driven build sizes its
+// fixed buffers: per level the active nodes / elements, and the largest value of each
+// quantity the level loop grows a buffer for (the same bounds the loop checks).
+struct BuildStats {
+    uint32_t na0 = 0, ne0 = 0, nsmall0 = 0, nse0 = 0;
+    std::vector<uint32_t> level_na, level_ne;
+    uint64_t max_2na = 0, max_ne = 0, max_bn = 0, max_small = 0, max_se = 0, max_ll = 0;
+    uint32_t nbn = 0, nsmall = 0, nse = 0, maxlvl = 0, total_nodes = 0, total_refs = 0;
+};
+
 // Builds the trees of S scenes together: one level loop over the active nodes of all
 // scenes, one small-node launch for all small roots, one layout / emission pass.  Each
 // scene's tree equals the tree a single-scene build of it gives (indices are
@@ -1338,7 +1386,7 @@ inline dim3 blocks_for(uint64_t n, uint32_t bs = 256) { return dim3((uint32_t)st
 //   frees every tree of the batch).
 int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float* d_v2, const uint32_t* h_base,
                      uint32_t nscenes, const mnrt_build_params* prm, const float* h_roots, const float* h_qr,
-                     mnrt_tree* out, hipStream_t st) {
+                     mnrt_tree* out, hipStream_t st, BuildStats* stats = nullptr) {
     if (!out || !prm || !h_base || nscenes == 0) return MNRT_ERR_ARG;
     const uint32_t ntot = h_base[nscenes];
     for (uint32_t s = 0; s < nscenes; ++s)
@@ -1478,11 +1526,11 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         thi = tlo + 3 * S_;
         uint32_t* sc = thi + 3 * S_;
         hipLaunchKernelGGL(k_scene_counts, blocks_for(S_), dim3(256), 0, st, S_, d_base, cnts);
-        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, S_, cnts, cc);
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, hn(S_), cnts, cc);
         MNRT_HIP_CHECK(scan_exclusive(cc, cc, S_, counters + C_NCHUNKS, sc, st));
-        hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, S_, max_chunks, cc, counters,
+        hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, hn(S_), max_chunks, cc, counters,
                            d_base, cnts, chn, chf, chc);
-        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * S_), dim3(256), 0, st, S_, tlo, thi);
+        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * S_), dim3(256), 0, st, hn(S_), tlo, thi);
         hipLaunchKernelGGL(k_tight, dim3((max_chunks + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st,
                            max_chunks, counters, chn, chf, chc, Ein,
                            kind, tlo, thi);
@@ -1533,7 +1581,25 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     uint32_t nx_cap = 0;
     NodeView An{};
     uint32_t an_cap = 0;
+    if (stats) {
+        *stats = BuildStats{};
+        stats->na0 = na;
+        stats->ne0 = ne;
+        stats->nsmall0 = nsmall;
+        stats->nse0 = nse;
+    }
     while (na > 0) {
+        if (stats) {
+            BuildStats& b = *stats;
+            b.level_na.push_back(na);
+            b.level_ne.push_back(ne);
+            b.max_2na = std::max<uint64_t>(b.max_2na, 2ull * na);
+            b.max_ne = std::max<uint64_t>(b.max_ne, ne);
+            b.max_bn = std::max<uint64_t>(b.max_bn, nbn + 14ull * na);
+            b.max_small = std::max<uint64_t>(b.max_small, nsmall + 2ull * na);
+            b.max_se = std::max<uint64_t>(b.max_se, nse + 2ull * ne);
+            b.max_ll = std::max<uint64_t>(b.max_ll, (uint64_t)ll_count + ne);
+        }
         if (nbn + 14ull * na > bn_cap) {
             uint32_t ncap = std::max<uint32_t>(2 * bn_cap, nbn + 14 * na + 1024);
             uint32_t* nb = pool.get<uint32_t>((size_t)kBnCols * ncap);
@@ -1658,22 +1724,22 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
             Cv.soff = cbuf + 9 * m;
             Cv.loff = cbuf + 10 * m;
         }
-        hipLaunchKernelGGL(k_chunk_counts, blocks_for(na), dim3(256), 0, st, na, A.cnt, cc);
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(na), dim3(256), 0, st, hn(na), A.cnt, cc);
         MNRT_HIP_CHECK(scan_exclusive(cc, cc, na, counters + C_NCHUNKS, scr, st));
-        hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, na, max_chunks, cc, counters,
+        hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, hn(na), max_chunks, cc, counters,
                            A.first, A.cnt, chn, chf, chc);
-        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * na), dim3(256), 0, st, na, tlo, thi);
+        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * na), dim3(256), 0, st, hn(na), tlo, thi);
         hipLaunchKernelGGL(k_tight, dim3((max_chunks + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st,
                            max_chunks, counters, chn, chf, chc, E,
                            kind, tlo, thi);
-        hipLaunchKernelGGL((k_plan<false>), blocks_for(na), dim3(256), 0, st, na, A, tlo, thi,
-                           prm->empty_space_ratio, P, (const uint32_t*)nullptr, 0u, B, counters);
+        hipLaunchKernelGGL((k_plan<false>), blocks_for(na), dim3(256), 0, st, hn(na), A, tlo, thi,
+                           prm->empty_space_ratio, P, (const uint32_t*)nullptr, hn(0u), B, counters);
         MNRT_HIP_CHECK(scan_exclusive(P.newcnt, P.newcnt, na, counters + C_NEWNODES, scr, st));
-        hipLaunchKernelGGL((k_plan<true>), blocks_for(na), dim3(256), 0, st, na, A, tlo, thi,
-                           prm->empty_space_ratio, P, (const uint32_t*)P.newcnt, nbn, B, counters);
-        hipLaunchKernelGGL(k_classify, blocks_for(ne), dim3(256), 0, st, ne, E, kind, P, fl);
+        hipLaunchKernelGGL((k_plan<true>), blocks_for(na), dim3(256), 0, st, hn(na), A, tlo, thi,
+                           prm->empty_space_ratio, P, (const uint32_t*)P.newcnt, hn(nbn), B, counters);
+        hipLaunchKernelGGL(k_classify, blocks_for(ne), dim3(256), 0, st, hn(ne), E, kind, P, fl);
         MNRT_HIP_CHECK(scan_exclusive(fl, pos, 2 * ne, pos + 2 * ne, scr, st));
-        hipLaunchKernelGGL(k_child_plan, blocks_for(2ull * na), dim3(256), 0, st, na, ne, A, P, pos,
+        hipLaunchKernelGGL(k_child_plan, blocks_for(2ull * na), dim3(256), 0, st, hn(na), hn(ne), A, P, pos,
                            prm->small_node_max, Cv);
         {
             ScanSet cs{{Cv.sflag, Cv.lflag, Cv.scnt, Cv.lcnt},
@@ -1682,11 +1748,11 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
                        {counters + C_NSMALL_NEW, counters + C_NNEXT, counters + C_ESMALL_NEW, counters + C_ENEXT}};
             MNRT_HIP_CHECK(scan_exclusive_set(cs, 4, 2 * na, scr, st));
         }
-        hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, na, A, P, Cv, nsmall, nse, SV, An,
-                           B, counters);
-        hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, na, A, P, E, ll_idx, B, counters);
-        hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, na, ne, E, G, A, P, Cv, pos, nse, Sm, se_cap,
-                           Nx, nx_cap, counters);
+        hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, hn(na), A, P, Cv, hn(nsmall),
+                           hn(nse), SV, An, B, counters);
+        hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, hn(na), A, P, E, ll_idx, ll_cap, B, counters);
+        hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, hn(na), hn(ne), E, G, A, P, Cv, pos, hn(nse),
+                           Sm, se_cap, Nx, nx_cap, counters);
         MNRT_HIP_CHECK(hipGetLastError());
         int rc = sync_counters();
         if (rc != MNRT_OK) return rc;
@@ -1733,14 +1799,14 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     O.counters = counters;
     const dim3 small_grid((nsmall + kSmallWaves - 1) / kSmallWaves), small_block(64 * kSmallWaves);
     if (nsmall > 0) {
-        hipLaunchKernelGGL(k_small_caps, blocks_for(nsmall), dim3(256), 0, st, nsmall, (const uint32_t*)SV.cnt, st_cap,
-                           st_off);
+        hipLaunchKernelGGL(k_small_caps, blocks_for(nsmall), dim3(256), 0, st, hn(nsmall), (const uint32_t*)SV.cnt,
+                           st_cap, st_off);
         MNRT_HIP_CHECK(scan_exclusive(st_off, st_off, nsmall, nullptr, st_scr, st));
         if (kind == MNRT_KIND_TRI)
-            hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_TRI>), small_grid, small_block, 0, st, nsmall, SV, Sm,
-                               (const float*)d_qr, O);
+            hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_TRI>), small_grid, small_block, 0, st, hn(nsmall), SV,
+                               Sm, (const float*)d_qr, O);
         else
-            hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_POINT>), small_grid, small_block, 0, st, nsmall, SV,
+            hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_POINT>), small_grid, small_block, 0, st, hn(nsmall), SV,
                                Sm, (const float*)d_qr, O);
     }
     // ---- layout ----
@@ -1759,10 +1825,10 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         uint32_t* list = cursor + 64;
         uint32_t* sc = list + nbn;
         MNRT_HIP_CHECK(hipMemsetAsync(hist, 0, 64 * 4, st));
-        hipLaunchKernelGGL(k_level_hist, blocks_for(nbn), dim3(256), 0, st, nbn, B, hist);
+        hipLaunchKernelGGL(k_level_hist, blocks_for(nbn), dim3(256), 0, st, hn(nbn), B, hist);
         MNRT_HIP_CHECK(scan_exclusive(hist, off, 64, off + 64, sc, st));
         MNRT_HIP_CHECK(hipMemcpyAsync(cursor, off, 64 * 4, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_level_scatter, blocks_for(nbn), dim3(256), 0, st, nbn, B, cursor, list);
+        hipLaunchKernelGGL(k_level_scatter, blocks_for(nbn), dim3(256), 0, st, hn(nbn), B, cursor, list);
         LayoutSrc Ls{off, list, s_nsize, s_rsize, s_mlvl};
         const uint32_t top = std::min<uint32_t>(maxlvl, 63);
         const dim3 lgrid(std::min<uint32_t>(blocks_for(nbn).x, 512u));
@@ -1782,6 +1848,15 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     }
     const uint32_t total_nodes = h_counters[40], total_refs = h_counters[41], small_ovf = h_counters[42];
     if (total_nodes == 0 || total_nodes > MNRT_INDEX_MASK || total_refs > MNRT_INDEX_MASK) return MNRT_ERR_CAPACITY;
+    if (stats) {
+        stats->nbn = nbn;
+        stats->nsmall = nsmall;
+        stats->nse = nse;
+        stats->maxlvl = maxlvl;
+        stats->total_nodes = total_nodes;
+        stats->total_refs = total_refs;
+        stats->max_ll = std::max<uint64_t>(stats->max_ll, ll_count);
+    }
 
     // ---- emission into the caller-visible trees ----
     uint32_t* d_nodes = nullptr;
@@ -1791,8 +1866,8 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         (void)pool_free(d_nodes, st);
         return MNRT_ERR_NOMEM;
     }
-    hipLaunchKernelGGL(k_emit_large, blocks_for(nbn), dim3(256), 0, st, nbn, B, Lv, ll_idx, d_nodes, d_refs, counters,
-                       total_nodes, total_refs, d_nbase, d_rbase);
+    hipLaunchKernelGGL(k_emit_large, blocks_for(nbn), dim3(256), 0, st, hn(nbn), B, Lv, ll_idx, d_nodes, d_refs,
+                       counters, total_nodes, total_refs, d_nbase, d_rbase);
     O.bn_addr = Lv.addr;
     O.bn_rbase = Lv.rbase;
     O.nodes = d_nodes;
@@ -1802,13 +1877,13 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
     O.node_base = d_nbase;
     O.ref_base = d_rbase;
     if (nsmall > 0) {
-        hipLaunchKernelGGL(k_small_place, small_grid, small_block, 0, st, nsmall, SV, O);
+        hipLaunchKernelGGL(k_small_place, small_grid, small_block, 0, st, hn(nsmall), SV, O);
         // subtrees that outgrew their staging area are built again at their final address
         if (small_ovf > 0 && kind == MNRT_KIND_TRI)
-            hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_TRI>), small_grid, small_block, 0, st, nsmall, SV, Sm,
-                               (const float*)d_qr, O);
+            hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_TRI>), small_grid, small_block, 0, st, hn(nsmall), SV,
+                               Sm, (const float*)d_qr, O);
         else if (small_ovf > 0)
-            hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_POINT>), small_grid, small_block, 0, st, nsmall, SV,
+            hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_POINT>), small_grid, small_block, 0, st, hn(nsmall), SV,
                                Sm, (const float*)d_qr, O);
     }
     // per-scene results to the host: counts, bases, leaves, max level, root boxes
@@ -1848,6 +1923,448 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         t.d_refs = d_refs + hrb[s];
         t.owns_nodes = (s == 0) ? 1u : 0u;
     }
+    return MNRT_OK;
+}
+
+
+// =========================================================================================
+// device-driven build, recorded once into a hipGraph
+// =========================================================================================
+// The same kernels as build_batch_impl, with every count a device word (DN) and every
+// buffer fixed: the level loop is unrolled to a recorded number of levels whose kernels
+// find the level's active nodes / elements in counters[G_NA], [G_NE]; k_graph_level_end
+// moves the loop on.  Capacities come from a host-driven build of the same scenes (its
+// BuildStats) with a margin; whatever would outgrow them sets an error bit instead.
+
+// State words of the recorded loop, after the counters.
+enum : uint32_t { G_NA = C_COUNT, G_NE, G_NBN, G_NSMALL, G_NSE, G_WORDS };
+
+// counters[C_ERR] bits of a recorded build (1, 2, 4, 8, 16, 32 as in the host-driven build):
+constexpr uint32_t kErrLevelRoom = 64u;    // a level's nodes / elements outgrew the fixed buffers
+constexpr uint32_t kErrLevels = 128u;      // active nodes left after the last recorded level
+constexpr uint32_t kErrOutputRoom = 256u;  // the trees outgrew the output arrays
+
+struct GraphCaps {
+    uint32_t na;           // node arrays of a level (hold twice the active nodes: the next level)
+    uint32_t ne;           // element arrays of a level
+    uint32_t bn;           // build-node table
+    uint32_t small;        // small roots
+    uint32_t small_elems;  // small-root elements
+    uint32_t ll;           // elements of large-stage leaves
+    uint32_t nodes, refs;  // output arrays (all trees)
+};
+
+__global__ __launch_bounds__(64) void k_graph_init(uint32_t* __restrict__ c, uint32_t na, uint32_t ne,
+                                                   uint32_t nbn, uint32_t nsmall, uint32_t nse) {
+    for (uint32_t i = threadIdx.x; i < G_WORDS; i += blockDim.x)
+        c[i] = i == G_NA ? na : i == G_NE ? ne : i == G_NBN ? nbn : i == G_NSMALL ? nsmall : i == G_NSE ? nse : 0u;
+}
+
+// End of a recorded level: the running totals, and the next level's counts after the
+// checks the host-driven loop makes before growing its buffers.  On any error the loop
+// stops and the small stage takes no roots; emission, records and query-time nodes are
+// then skipped (they test counters[C_ERR]).
+__global__ __launch_bounds__(64) void k_graph_level_end(uint32_t* __restrict__ c, GraphCaps cap, uint32_t last) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c[G_NA] == 0u) return;  // G_NA == 0: no level ran
+    const uint32_t nbn = c[G_NBN] + c[C_NEWNODES];
+    const uint32_t nsmall = c[G_NSMALL] + c[C_NSMALL_NEW];
+    const uint32_t nse = c[G_NSE] + c[C_ESMALL_NEW];
+    const uint32_t na = c[C_NNEXT], ne = c[C_ENEXT];
+    c[G_NBN] = nbn;
+    c[G_NSMALL] = nsmall;
+    c[G_NSE] = nse;
+    uint32_t err = c[C_ERR];
+    if (na > 0u) {
+        const bool fits = 2ull * na <= cap.na && ne <= cap.ne && nbn + 14ull * na <= cap.bn &&
+                          nsmall + 2ull * na <= cap.small && nse + 2ull * ne <= cap.small_elems &&
+                          (uint64_t)c[C_LLCOUNT] + ne <= cap.ll;
+        if (!fits) err |= kErrLevelRoom;
+        else if (last) err |= kErrLevels;
+    }
+    if (err != 0u) {
+        c[C_ERR] = err;
+        c[G_NA] = 0u;
+        c[G_NE] = 0u;
+        c[G_NSMALL] = 0u;
+        return;
+    }
+    c[G_NA] = na;
+    c[G_NE] = ne;
+}
+
+__global__ __launch_bounds__(64) void k_graph_copy(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                                   uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(64) void k_graph_check_totals(uint32_t* __restrict__ c, const uint32_t* __restrict__ nodes,
+                                                           const uint32_t* __restrict__ refs, GraphCaps cap) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && (*nodes == 0u || *nodes > cap.nodes || *refs > cap.refs))
+        c[C_ERR] |= kErrOutputRoom;
+}
+
+// Bump carving of one device block: a first pass (base null) measures, the second carves.
+class Carver {
+   public:
+    explicit Carver(char* base) : base_(base) {}
+    template <typename T>
+    T* take(uint64_t count) {
+        const uint64_t bytes = (std::max<uint64_t>(count * sizeof(T), 16) + 255) & ~uint64_t(255);
+        T* p = base_ ? (T*)(base_ + top_) : nullptr;
+        top_ += bytes;
+        return p;
+    }
+    uint64_t size() const { return top_; }
+
+   private:
+    char* base_;
+    uint64_t top_ = 0;
+};
+
+}  // namespace
+
+struct mnrt_build_graph {
+    int kind = 0;
+    uint32_t ns = 0, ntot = 0;
+    std::vector<uint32_t> h_base;      // ns + 1
+    std::vector<uint32_t> h_plan;      // kind / index / offset per scene
+    std::vector<float> h_qr, h_roots;  // per scene (roots: empty when not given)
+    mnrt_build_params prm{};
+    const float *v0 = nullptr, *v1 = nullptr, *v2 = nullptr, *pow4 = nullptr, *dir4 = nullptr;
+    BuildStats stats;
+    GraphCaps cap{};
+    uint32_t levels = 0;
+    float margin = 0.25f;
+    char* dbuf = nullptr;
+    uint64_t dbytes = 0;
+    uint32_t* counters = nullptr;
+    uint32_t* d_tab = nullptr;
+    uint32_t* nodes = nullptr;
+    uint32_t* refs = nullptr;
+    float* recs = nullptr;
+    uint32_t* qnodes = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Per-scene device table of a recorded build, as build_batch_impl lays it out.
+struct GraphTab {
+    uint32_t *base, *pkind, *pindex, *poff;
+    float *qr, *given, *rootbox;
+    uint32_t *snodes, *srefs, *nbase, *rbase, *sleaves, *smaxlvl;
+};
+
+GraphTab graph_tab(uint32_t* t, uint32_t S_) {
+    GraphTab T;
+    T.base = t;
+    T.pkind = T.base + S_ + 1;
+    T.pindex = T.pkind + S_;
+    T.poff = T.pindex + S_;
+    T.qr = (float*)(T.poff + S_);
+    T.given = T.qr + S_;
+    T.rootbox = T.given + 6 * S_;
+    T.snodes = (uint32_t*)(T.rootbox + 6 * S_);
+    T.srefs = T.snodes + S_;
+    T.nbase = T.srefs + S_;
+    T.rbase = T.nbase + S_ + 1;
+    T.sleaves = T.rbase + S_ + 1;
+    T.smaxlvl = T.sleaves + S_;
+    return T;
+}
+
+// Carves g's device block (base null: measures it) and, when st is given, enqueues the
+// whole build on st (which is capturing).  Every launch is a kernel: the sequence has no
+// memset, copy or allocation nodes.
+int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
+    const uint32_t S_ = g.ns, ntot = g.ntot;
+    const int kind = g.kind;
+    const GraphCaps C = g.cap;
+    const uint32_t NA = C.na, NE = C.ne, CH = NE / kChunk + NA;
+    Carver cv(base);
+    uint32_t* counters = cv.take<uint32_t>(G_WORDS);
+    uint32_t* tab = cv.take<uint32_t>(32ull * S_ + 64);
+    const GraphTab T = graph_tab(tab, S_);
+    auto elems = [&](uint64_t cap, bool with_node) {
+        ElemView V{};
+        V.idx = cv.take<uint32_t>(cap);
+        V.lo = cv.take<float4>(cap);
+        V.hi = kind == MNRT_KIND_TRI ? cv.take<float4>(cap) : V.lo;
+        V.node = with_node ? cv.take<uint32_t>(cap) : nullptr;
+        return V;
+    };
+    auto nodesv = [&](uint64_t cap) {
+        NodeView V{};
+        V.lo = cv.take<float4>(cap);
+        V.hi = cv.take<float4>(cap);
+        V.first = cv.take<uint32_t>(cap);
+        V.cnt = cv.take<uint32_t>(cap);
+        V.level = cv.take<uint32_t>(cap);
+        V.bn = cv.take<uint32_t>(cap);
+        V.scene = cv.take<uint32_t>(cap);
+        return V;
+    };
+    ElemView Ein = elems(std::max<uint32_t>(ntot, 1), true);
+    const uint32_t ch0 = ntot / kChunk + S_;
+    uint32_t* tb = cv.take<uint32_t>(3ull * ch0 + 8ull * S_ + scan_scratch_words(S_) + 16);
+    uint32_t* bnb = cv.take<uint32_t>((uint64_t)kBnCols * C.bn);
+    const BnView B{bnb,
+                   bnb + C.bn,
+                   bnb + 2ull * C.bn,
+                   bnb + 3ull * C.bn,
+                   bnb + 4ull * C.bn,
+                   bnb + 5ull * C.bn,
+                   bnb + 6ull * C.bn,
+                   bnb + 7ull * C.bn};
+    SmallView SV{};
+    SV.first = cv.take<uint32_t>(C.small);
+    SV.cnt = cv.take<uint32_t>(C.small);
+    SV.level = cv.take<uint32_t>(C.small);
+    SV.bn = cv.take<uint32_t>(C.small);
+    SV.lo = cv.take<float4>(C.small);
+    SV.hi = cv.take<float4>(C.small);
+    SV.scene = cv.take<uint32_t>(C.small);
+    const ElemView Sm = elems(C.small_elems, false);
+    const NodeView Nv[2] = {nodesv(NA), nodesv(NA)};
+    const ElemView Xv[2] = {elems(NE, true), elems(NE, true)};
+    uint32_t* ll_idx = cv.take<uint32_t>(C.ll);
+    // level buffers (fixed strides: NA nodes, NE elements)
+    uint32_t* cc = cv.take<uint32_t>(NA);
+    uint32_t* chn = cv.take<uint32_t>(CH);
+    uint32_t* chf = cv.take<uint32_t>(CH);
+    uint32_t* chc = cv.take<uint32_t>(CH);
+    uint32_t* tlo = cv.take<uint32_t>(3ull * NA);
+    uint32_t* thi = cv.take<uint32_t>(3ull * NA);
+    PlanView P{};
+    P.newcnt = cv.take<uint32_t>(NA);
+    P.axis = cv.take<uint32_t>(NA);
+    P.split = cv.take<float>(NA);
+    P.lo = cv.take<float4>(NA);
+    P.hi = cv.take<float4>(NA);
+    P.level = cv.take<uint32_t>(NA);
+    P.childL = cv.take<uint32_t>(NA);
+    P.cur = cv.take<uint32_t>(NA);
+    P.leaf = cv.take<uint32_t>(NA);
+    P.noprog = cv.take<uint32_t>(NA);
+    uint32_t* fl = cv.take<uint32_t>(2ull * NE + 1);
+    uint32_t* pos = cv.take<uint32_t>(2ull * NE + 1);
+    uint32_t* cbuf = cv.take<uint32_t>(22ull * NA);
+    ChildView Cv{};
+    {
+        const uint64_t m = 2ull * NA;
+        Cv.type = cbuf;
+        Cv.cnt = cbuf + m;
+        Cv.fpos = cbuf + 2 * m;
+        Cv.sflag = cbuf + 3 * m;
+        Cv.lflag = cbuf + 4 * m;
+        Cv.scnt = cbuf + 5 * m;
+        Cv.lcnt = cbuf + 6 * m;
+        Cv.sidx = cbuf + 7 * m;
+        Cv.lidx = cbuf + 8 * m;
+        Cv.soff = cbuf + 9 * m;
+        Cv.loff = cbuf + 10 * m;
+    }
+    uint32_t* scr = cv.take<uint32_t>(std::max<uint64_t>({(uint64_t)dscan_tiles(NA), (uint64_t)dscan_tiles(2 * NE),
+                                                           4ull * dscan_tiles(2 * NA), (uint64_t)dscan_tiles(C.small)}) +
+                                      16);
+    // small stage
+    const StageCap st_cap = stage_cap_setting();
+    const uint64_t st_total = (uint64_t)st_cap.mul * C.small_elems + (uint64_t)st_cap.add * C.small;
+    if (st_total > 0xFFFFFFFFull) return MNRT_ERR_CAPACITY;
+    SmallOut O{};
+    O.nsize = cv.take<uint32_t>(C.small);
+    O.rsize = cv.take<uint32_t>(C.small);
+    O.mlvl = cv.take<uint32_t>(C.small);
+    uint32_t* st_off = cv.take<uint32_t>(C.small);
+    O.st_off = st_off;
+    O.st_ovf = cv.take<uint32_t>(C.small);
+    O.st_nodes = cv.take<uint2>(st_total);
+    O.st_refs = cv.take<uint32_t>(st_total);
+    O.st_cap = st_cap;
+    O.counters = counters;
+    // layout
+    LayoutView Lv{};
+    Lv.size = cv.take<uint32_t>(C.bn);
+    Lv.rsize = cv.take<uint32_t>(C.bn);
+    Lv.depth = cv.take<uint32_t>(C.bn);
+    Lv.addr = cv.take<uint32_t>(C.bn);
+    Lv.rbase = cv.take<uint32_t>(C.bn);
+    uint32_t* lvb = cv.take<uint32_t>(64 + 65 + 64 + (uint64_t)C.bn + scan_scratch_words(std::max<uint32_t>(64, S_ + 1)));
+    // outputs
+    uint32_t* nodes = cv.take<uint32_t>(2ull * C.nodes);
+    uint32_t* refs = cv.take<uint32_t>(C.refs);
+    float* recs = cv.take<float>((uint64_t)C.refs * (kind == MNRT_KIND_TRI ? 12u : 8u));
+    uint32_t* qnodes = kind == MNRT_KIND_POINT ? cv.take<uint32_t>(2ull * C.nodes) : nullptr;
+    if (!base) {
+        g.dbytes = cv.size();
+        return MNRT_OK;
+    }
+    g.counters = counters;
+    g.d_tab = tab;
+    g.nodes = nodes;
+    g.refs = refs;
+    g.recs = recs;
+    g.qnodes = qnodes;
+    if (!st) return MNRT_OK;
+
+    // ---- the recorded sequence ----
+    const BuildStats& bs = g.stats;
+    const double grow = 1.0 + (double)g.margin;
+    auto planned = [&](uint64_t x, uint64_t slack, uint64_t cap) {
+        return std::min<uint64_t>(cap, (uint64_t)((double)x * grow) + slack);
+    };
+    hipLaunchKernelGGL(k_graph_init, dim3(1), dim3(64), 0, st, counters, bs.na0, bs.ne0, S_, bs.nsmall0, bs.nse0);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, T.sleaves, 2 * S_, 0u);
+    Geometry G{kind, (const float4*)g.v0, g.v1 ? (const float4*)g.v1 : (const float4*)g.v0,
+               g.v2 ? (const float4*)g.v2 : (const float4*)g.v0, T.base, S_};
+    RootPlan RP{T.pkind, T.pindex, T.poff};
+    if (ntot > 0) hipLaunchKernelGGL(k_init_elems, blocks_for(ntot), dim3(256), 0, st, G, ntot, Ein);
+    uint32_t* rtlo = nullptr;
+    uint32_t* rthi = nullptr;
+    if (g.h_roots.empty() && ntot > 0) {
+        uint32_t *rchn = tb, *rchf = rchn + ch0, *rchc = rchf + ch0, *rcc = rchc + ch0;
+        uint32_t* cnts = rcc + S_;
+        rtlo = cnts + S_;
+        rthi = rtlo + 3 * S_;
+        uint32_t* sc = rthi + 3 * S_;
+        hipLaunchKernelGGL(k_scene_counts, blocks_for(S_), dim3(256), 0, st, S_, T.base, cnts);
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, hn(S_), cnts, rcc);
+        MNRT_HIP_CHECK(scan_exclusive(rcc, rcc, S_, counters + C_NCHUNKS, sc, st));
+        hipLaunchKernelGGL(k_chunk_fill, blocks_for(ch0), dim3(256), 0, st, hn(S_), ch0, rcc, counters, T.base, cnts,
+                           rchn, rchf, rchc);
+        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * S_), dim3(256), 0, st, hn(S_), rtlo, rthi);
+        hipLaunchKernelGGL(k_tight, dim3((ch0 + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st, ch0, counters,
+                           rchn, rchf, rchc, Ein, kind, rtlo, rthi);
+    }
+    hipLaunchKernelGGL(k_roots_setup, blocks_for(S_), dim3(256), 0, st, S_, T.base, rtlo, rthi,
+                       g.h_roots.empty() ? (const float*)nullptr : (const float*)T.given, RP, Nv[0], SV, B, T.rootbox);
+    if (ntot > 0)
+        hipLaunchKernelGGL(k_partition_elems, blocks_for(ntot), dim3(256), 0, st, ntot, Ein, kind, RP, Xv[0], Sm);
+    MNRT_HIP_CHECK(hipGetLastError());
+
+    // large-node stage: g.levels recorded levels
+    const DN na = dn(counters + G_NA, NA), ne = dn(counters + G_NE, NE);
+    const DN na2 = dn(counters + G_NA, 2 * NA, 1), ne2 = dn(counters + G_NE, 2 * NE, 1);
+    const DN nbn = dn(counters + G_NBN, C.bn), nsm = dn(counters + G_NSMALL, C.small);
+    const DN nse = dn(counters + G_NSE, C.small_elems);
+    for (uint32_t l = 0; l < g.levels; ++l) {
+        const NodeView& A = Nv[l & 1u];
+        const NodeView& An = Nv[(l + 1) & 1u];
+        const ElemView& E = Xv[l & 1u];
+        const ElemView& Nx = Xv[(l + 1) & 1u];
+        const size_t pl = std::min<size_t>(l, bs.level_na.size() - 1);
+        const uint64_t pna = planned(bs.level_na[pl], 256, NA), pne = planned(bs.level_ne[pl], 4096, NE);
+        const dim3 gna = blocks_for(pna), gna2 = blocks_for(2 * pna), gne = blocks_for(pne);
+        const dim3 gch((uint32_t)std::max<uint64_t>(1, (pne / kChunk + pna + kTightChunks - 1) / kTightChunks));
+        hipLaunchKernelGGL(k_chunk_counts, gna, dim3(256), 0, st, na, A.cnt, cc);
+        MNRT_HIP_CHECK(dscan1(cc, cc, na, counters + C_NCHUNKS, false, scr, st));
+        hipLaunchKernelGGL(k_chunk_fill, blocks_for(pne / kChunk + pna), dim3(256), 0, st, na, CH, cc, counters,
+                           A.first, A.cnt, chn, chf, chc);
+        hipLaunchKernelGGL(k_tight_init, blocks_for(3 * pna), dim3(256), 0, st, na, tlo, thi);
+        hipLaunchKernelGGL(k_tight, gch, dim3(kChunk), 0, st, CH, counters, chn, chf, chc, E, kind, tlo, thi);
+        hipLaunchKernelGGL((k_plan<false>), gna, dim3(256), 0, st, na, A, tlo, thi, g.prm.empty_space_ratio, P,
+                           (const uint32_t*)nullptr, hn(0u), B, counters);
+        MNRT_HIP_CHECK(dscan1(P.newcnt, P.newcnt, na, counters + C_NEWNODES, false, scr, st));
+        hipLaunchKernelGGL((k_plan<true>), gna, dim3(256), 0, st, na, A, tlo, thi, g.prm.empty_space_ratio, P,
+                           (const uint32_t*)P.newcnt, nbn, B, counters);
+        hipLaunchKernelGGL(k_classify, gne, dim3(256), 0, st, ne, E, kind, P, fl);
+        MNRT_HIP_CHECK(dscan1(fl, pos, ne2, nullptr, true, scr, st));
+        hipLaunchKernelGGL(k_child_plan, gna2, dim3(256), 0, st, na, ne, A, P, pos, g.prm.small_node_max, Cv);
+        {
+            DScanSet cs{};
+            const uint32_t* ins[4] = {Cv.sflag, Cv.lflag, Cv.scnt, Cv.lcnt};
+            uint32_t* outs[4] = {Cv.sidx, Cv.lidx, Cv.soff, Cv.loff};
+            uint32_t* tots[4] = {counters + C_NSMALL_NEW, counters + C_NNEXT, counters + C_ESMALL_NEW,
+                                 counters + C_ENEXT};
+            for (int y = 0; y < 4; ++y) {
+                cs.in[y] = ins[y];
+                cs.out[y] = outs[y];
+                cs.total[y] = tots[y];
+            }
+            MNRT_HIP_CHECK(dscan(cs, 4, na2, scr, st));
+        }
+        hipLaunchKernelGGL(k_child_write, gna2, dim3(256), 0, st, na, A, P, Cv, nsm, nse, SV, An, B, counters);
+        hipLaunchKernelGGL(k_leaf_copy, gna, dim3(256), 0, st, na, A, P, E, ll_idx, C.ll, B, counters);
+        hipLaunchKernelGGL(k_scatter, gne, dim3(256), 0, st, na, ne, E, G, A, P, Cv, pos, nse, Sm, C.small_elems, Nx,
+                           NE, counters);
+        hipLaunchKernelGGL(k_graph_level_end, dim3(1), dim3(64), 0, st, counters, C, l + 1 == g.levels ? 1u : 0u);
+        MNRT_HIP_CHECK(hipGetLastError());
+    }
+
+    // small-node stage
+    const dim3 small_grid((C.small + kSmallWaves - 1) / kSmallWaves), small_block(64 * kSmallWaves);
+    hipLaunchKernelGGL(k_small_caps, blocks_for(C.small), dim3(256), 0, st, nsm, (const uint32_t*)SV.cnt, st_cap,
+                       st_off);
+    MNRT_HIP_CHECK(dscan1(st_off, st_off, nsm, nullptr, false, scr, st));
+    if (kind == MNRT_KIND_TRI)
+        hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_TRI>), small_grid, small_block, 0, st, nsm, SV, Sm,
+                           (const float*)T.qr, O);
+    else
+        hipLaunchKernelGGL((k_small<kSmallStage, MNRT_KIND_POINT>), small_grid, small_block, 0, st, nsm, SV, Sm,
+                           (const float*)T.qr, O);
+    // layout
+    {
+        uint32_t* hist = lvb;
+        uint32_t* off = lvb + 64;
+        uint32_t* cursor = off + 65;
+        uint32_t* list = cursor + 64;
+        uint32_t* sc = list + C.bn;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, hist, 64u, 0u);
+        hipLaunchKernelGGL(k_level_hist, blocks_for(C.bn), dim3(256), 0, st, nbn, B, hist);
+        MNRT_HIP_CHECK(scan_exclusive(hist, off, 64, off + 64, sc, st));
+        hipLaunchKernelGGL(k_graph_copy, dim3(1), dim3(64), 0, st, cursor, (const uint32_t*)off, 64u);
+        hipLaunchKernelGGL(k_level_scatter, blocks_for(C.bn), dim3(256), 0, st, nbn, B, cursor, list);
+        LayoutSrc Ls{off, list, O.nsize, O.rsize, O.mlvl};
+        const uint32_t top = std::min<uint32_t>(MNRT_MAX_LEVEL, 63);
+        const dim3 lgrid(std::min<uint32_t>(blocks_for(C.bn).x, 512u));
+        for (int lvl = (int)top; lvl >= 0; --lvl)
+            hipLaunchKernelGGL(k_layout_up, lgrid, dim3(256), 0, st, (uint32_t)lvl, Ls, B, Lv);
+        for (uint32_t lvl = 0; lvl <= top; ++lvl)
+            hipLaunchKernelGGL(k_layout_down, lgrid, dim3(256), 0, st, lvl, Ls, B, Lv);
+        hipLaunchKernelGGL(k_scene_sizes, blocks_for(S_), dim3(256), 0, st, S_, Lv, T.snodes, T.srefs, T.sleaves,
+                           T.smaxlvl);
+        MNRT_HIP_CHECK(scan_exclusive(T.snodes, T.nbase, S_, T.nbase + S_, sc, st));
+        MNRT_HIP_CHECK(scan_exclusive(T.srefs, T.rbase, S_, T.rbase + S_, sc, st));
+        hipLaunchKernelGGL(k_graph_check_totals, dim3(1), dim3(64), 0, st, counters, (const uint32_t*)(T.nbase + S_),
+                           (const uint32_t*)(T.rbase + S_), C);
+    }
+    // emission (each kernel returns at once when counters[C_ERR] is set)
+    hipLaunchKernelGGL(k_emit_large, blocks_for(C.bn), dim3(256), 0, st, nbn, B, Lv, (const uint32_t*)ll_idx, nodes,
+                       refs, counters, C.nodes, C.refs, (const uint32_t*)T.nbase, (const uint32_t*)T.rbase);
+    O.bn_addr = Lv.addr;
+    O.bn_rbase = Lv.rbase;
+    O.nodes = nodes;
+    O.refs = refs;
+    O.max_nodes = C.nodes;
+    O.max_refs = C.refs;
+    O.node_base = T.nbase;
+    O.ref_base = T.rbase;
+    hipLaunchKernelGGL(k_small_place, small_grid, small_block, 0, st, nsm, SV, O);
+    // subtrees that outgrew their staging area, built again at their final address
+    if (kind == MNRT_KIND_TRI)
+        hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_TRI>), small_grid, small_block, 0, st, nsm, SV, Sm,
+                           (const float*)T.qr, O);
+    else
+        hipLaunchKernelGGL((k_small<kSmallEmit, MNRT_KIND_POINT>), small_grid, small_block, 0, st, nsm, SV, Sm,
+                           (const float*)T.qr, O);
+    // records and query-time node words
+    const DN nrefs = dn(T.rbase + S_, C.refs), nnodes = dn(T.nbase + S_, C.nodes);
+    if (kind == MNRT_KIND_TRI) {
+        int rc = launch_records_batch(kind, refs, nrefs, T.rbase, T.base, S_, (const float4*)g.v0, (const float4*)g.v1,
+                                      (const float4*)g.v2, (float4*)recs, counters + C_ERR, st);
+        if (rc != MNRT_OK) return rc;
+    } else if (g.pow4 && g.dir4) {
+        int rc = launch_records_batch(kind, refs, nrefs, T.rbase, T.base, S_, (const float4*)g.v0,
+                                      (const float4*)g.pow4, (const float4*)g.dir4, (float4*)recs, counters + C_ERR,
+                                      st);
+        if (rc == MNRT_OK)
+            rc = launch_collapse_batch((const uint2*)nodes, nnodes, T.nbase, S_, kQueryLeafRefs, (uint2*)qnodes,
+                                       counters + C_ERR, st);
+        if (rc != MNRT_OK) return rc;
+    }
+    MNRT_HIP_CHECK(hipGetLastError());
     return MNRT_OK;
 }
 
@@ -1902,4 +2419,199 @@ extern "C" int mnrt_build_points_batch(const float* d_pos4, const uint32_t* h_of
                                        const float* h_max_query_radius, mnrt_tree* out, void* stream) {
     return build_batch_impl(MNRT_KIND_POINT, d_pos4, nullptr, nullptr, h_offsets, num_scenes, params, h_root_boxes,
                             h_max_query_radius, out, (hipStream_t)stream);
+}
+
+// ---- device-driven batched build (mnrt_build_graph_*) ------------------------------------
+
+extern "C" int mnrt_build_graph_create(const mnrt_build_graph_desc* d, mnrt_tree* out, mnrt_build_graph** g_out,
+                                       void* stream) {
+    using namespace mnrt;
+    if (!d || !out || !g_out || !d->params || !d->h_offsets || d->num_scenes == 0) return MNRT_ERR_ARG;
+    if (d->kind != MNRT_KIND_TRI && d->kind != MNRT_KIND_POINT) return MNRT_ERR_ARG;
+    if (!(d->margin >= 0.0f && d->margin <= 16.0f)) return MNRT_ERR_ARG;
+    *g_out = nullptr;
+    const uint32_t S_ = d->num_scenes;
+    const uint32_t ntot = d->h_offsets[S_];
+    if (ntot > 0 && (!d->d_v0 || (d->kind == MNRT_KIND_TRI && (!d->d_v1 || !d->d_v2)))) return MNRT_ERR_ARG;
+    if (d->kind == MNRT_KIND_POINT && (!d->d_power4) != (!d->d_dir4)) return MNRT_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    auto* g = new (std::nothrow) mnrt_build_graph;
+    if (!g) return MNRT_ERR_NOMEM;
+    auto fail = [&](int rc) {
+        (void)mnrt_build_graph_destroy(g);
+        return rc;
+    };
+    g->kind = d->kind;
+    g->ns = S_;
+    g->ntot = ntot;
+    g->prm = *d->params;
+    g->v0 = d->d_v0;
+    g->v1 = d->d_v1;
+    g->v2 = d->d_v2;
+    g->pow4 = d->d_power4;
+    g->dir4 = d->d_dir4;
+    g->margin = d->margin;
+    g->h_base.assign(d->h_offsets, d->h_offsets + S_ + 1);
+    if (d->h_root_boxes) g->h_roots.assign(d->h_root_boxes, d->h_root_boxes + 6ull * S_);
+    g->h_qr.assign(S_, d->params->max_query_radius);
+    if (d->h_max_query_radius)
+        for (uint32_t s = 0; s < S_; ++s) g->h_qr[s] = d->h_max_query_radius[s];
+
+    // 1. a host-driven build of the same scenes: sizes for the fixed buffers
+    {
+        std::vector<mnrt_tree> tmp(S_);
+        int rc = build_batch_impl(d->kind, d->d_v0, d->d_v1, d->d_v2, d->h_offsets, S_, d->params, d->h_root_boxes,
+                                  d->h_max_query_radius, tmp.data(), st, &g->stats);
+        if (rc != MNRT_OK) return fail(rc);
+        (void)hipStreamSynchronize(st);
+        (void)mnrt_tree_release(&tmp[0]);
+    }
+    const BuildStats& bs = g->stats;
+    const double grow = 1.0 + (double)d->margin;
+    auto cap = [&](uint64_t x, uint64_t slack) -> uint32_t {
+        return (uint32_t)std::min<uint64_t>(MNRT_INDEX_MASK, (uint64_t)((double)x * grow) + slack);
+    };
+    GraphCaps& C = g->cap;
+    C.na = cap(std::max<uint64_t>(bs.max_2na, 2ull * bs.na0), 64);
+    C.ne = cap(std::max<uint64_t>(bs.max_ne, bs.ne0), 1024);
+    C.bn = cap(std::max<uint64_t>({bs.max_bn, (uint64_t)bs.nbn, (uint64_t)S_}), 1024);
+    C.small = cap(std::max<uint64_t>({bs.max_small, (uint64_t)bs.nsmall, (uint64_t)bs.nsmall0}), 256);
+    C.small_elems = cap(std::max<uint64_t>({bs.max_se, (uint64_t)bs.nse, (uint64_t)bs.nse0}), 1024);
+    C.ll = cap(bs.max_ll, 1024);
+    C.nodes = cap(bs.total_nodes, 1024);
+    C.refs = cap(bs.total_refs, 1024);
+    g->levels = bs.level_na.empty() ? 0u
+                                    : std::min<uint32_t>((uint32_t)bs.level_na.size() + 2u, MNRT_MAX_LEVEL + 1u);
+    // the root plan, as build_batch_impl makes it
+    g->h_plan.assign(3ull * S_, 0u);
+    {
+        uint32_t na = 0, nsmall = 0, ne = 0, nse = 0;
+        for (uint32_t s = 0; s < S_; ++s) {
+            const uint32_t cnt = g->h_base[s + 1] - g->h_base[s];
+            if (cnt > g->prm.small_node_max) {
+                g->h_plan[s] = 0;
+                g->h_plan[S_ + s] = na++;
+                g->h_plan[2 * S_ + s] = ne;
+                ne += cnt;
+            } else {
+                g->h_plan[s] = 1;
+                g->h_plan[S_ + s] = nsmall++;
+                g->h_plan[2 * S_ + s] = nse;
+                nse += cnt;
+            }
+        }
+        if (na != bs.na0 || ne != bs.ne0 || nsmall != bs.nsmall0 || nse != bs.nse0) return fail(MNRT_ERR_STATE);
+    }
+
+    // 2. the fixed device block, and the per-scene tables in it
+    int rc = graph_record(*g, nullptr, nullptr);
+    if (rc != MNRT_OK) return fail(rc);
+    if (hipMalloc((void**)&g->dbuf, g->dbytes) != hipSuccess) {
+        g->dbuf = nullptr;
+        return fail(MNRT_ERR_NOMEM);
+    }
+    rc = graph_record(*g, g->dbuf, nullptr);
+    if (rc != MNRT_OK) return fail(rc);
+    {
+        const GraphTab T = graph_tab(g->d_tab, S_);
+        std::vector<float> fq(S_ + 6ull * S_, 0.0f);
+        for (uint32_t s = 0; s < S_; ++s) fq[s] = g->h_qr[s];
+        for (size_t i = 0; i < g->h_roots.size(); ++i) fq[S_ + i] = g->h_roots[i];
+        if (hipMemcpy(T.base, g->h_base.data(), 4ull * (S_ + 1), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(T.pkind, g->h_plan.data(), 12ull * S_, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(T.qr, fq.data(), 4ull * fq.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(MNRT_ERR_HIP);
+    }
+
+    // 3. record the device-driven build
+    if (hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        g->cap_stream = nullptr;
+        return fail(MNRT_ERR_HIP);
+    }
+    if (hipStreamBeginCapture(g->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(MNRT_ERR_HIP);
+    rc = graph_record(*g, g->dbuf, g->cap_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(g->cap_stream, &graph);
+    g->graph = graph;
+    if (rc != MNRT_OK) return fail(rc);
+    if (ce != hipSuccess || !graph) return fail(MNRT_ERR_HIP);
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        g->exec = nullptr;
+        return fail(MNRT_ERR_HIP);
+    }
+
+    // 4. one replay: the trees in out[], which must equal the host-driven build's
+    rc = mnrt_build_graph_replay(g, stream);
+    uint32_t flags = 0;
+    if (rc == MNRT_OK) rc = mnrt_build_graph_status(g, out, &flags, stream);
+    if (rc == MNRT_OK && flags != 0u) rc = MNRT_ERR_STATE;
+    if (rc != MNRT_OK) return fail(rc);
+    *g_out = g;
+    return MNRT_OK;
+}
+
+extern "C" int mnrt_build_graph_replay(mnrt_build_graph* g, void* stream) {
+    if (!g || !g->exec) return MNRT_ERR_ARG;
+    MNRT_HIP_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return MNRT_OK;
+}
+
+extern "C" int mnrt_build_graph_status(mnrt_build_graph* g, mnrt_tree* out, uint32_t* h_flags, void* stream) {
+    using namespace mnrt;
+    if (!g || !g->dbuf || !h_flags) return MNRT_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t S_ = g->ns;
+    const GraphTab T = graph_tab(g->d_tab, S_);
+    std::vector<uint32_t> c(G_WORDS), tab(8ull * S_ + 2);
+    std::vector<float> box(6ull * S_);
+    MNRT_HIP_CHECK(hipStreamSynchronize(st));
+    MNRT_HIP_CHECK(hipMemcpy(c.data(), g->counters, 4ull * G_WORDS, hipMemcpyDeviceToHost));
+    // snodes, srefs, nbase (S+1), rbase (S+1), sleaves, smaxlvl are consecutive
+    MNRT_HIP_CHECK(hipMemcpy(tab.data(), T.snodes, 4ull * (6ull * S_ + 2), hipMemcpyDeviceToHost));
+    MNRT_HIP_CHECK(hipMemcpy(box.data(), T.rootbox, 4ull * 6 * S_, hipMemcpyDeviceToHost));
+    *h_flags = c[C_ERR];
+    if (c[C_ERR] != 0u || !out) return MNRT_OK;
+    const uint32_t* hn_ = tab.data();
+    const uint32_t* hr = hn_ + S_;
+    const uint32_t* hnb = hr + S_;
+    const uint32_t* hrb = hnb + S_ + 1;
+    const uint32_t* hl = hrb + S_ + 1;
+    const uint32_t* hm = hl + S_;
+    const uint32_t per = g->kind == MNRT_KIND_TRI ? 12u : 8u;
+    const bool recs = g->kind == MNRT_KIND_TRI || (g->pow4 && g->dir4);
+    for (uint32_t s = 0; s < S_; ++s) {
+        mnrt_tree& t = out[s];
+        std::memset(&t, 0, sizeof(t));
+        t.kind = (uint32_t)g->kind;
+        t.num_nodes = hn_[s];
+        t.num_refs = hr[s];
+        t.num_elems = g->h_base[s + 1] - g->h_base[s];
+        t.num_leaves = hl[s];
+        t.max_level = hm[s];
+        for (int k = 0; k < 3; ++k) {
+            t.root_min[k] = box[6 * s + k];
+            t.root_max[k] = box[6 * s + 3 + k];
+        }
+        t.d_nodes = g->nodes + 2ull * hnb[s];
+        t.d_refs = g->refs + hrb[s];
+        t.d_recs = recs ? g->recs + (uint64_t)per * hrb[s] : nullptr;
+        if (g->kind == MNRT_KIND_POINT && recs) {
+            t.d_qnodes = g->qnodes + 2ull * hnb[s];
+            t.qleaf_max = kQueryLeafRefs;
+        }
+        // the arrays belong to the graph (mnrt_build_graph_destroy frees them)
+        t.owns_nodes = t.owns_recs = t.owns_qnodes = 0u;
+    }
+    return MNRT_OK;
+}
+
+extern "C" int mnrt_build_graph_destroy(mnrt_build_graph* g) {
+    if (!g) return MNRT_OK;
+    (void)hipDeviceSynchronize();
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
+    if (g->dbuf) (void)hipFree(g->dbuf);
+    delete g;
+    return MNRT_OK;
 }

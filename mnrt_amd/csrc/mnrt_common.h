@@ -24,6 +24,19 @@ struct f3 {
     float x, y, z;
 };
 
+// An element count that the host knows (p null: v), or that a kernel reads from device
+// memory (min(p[0] << sh, v): v is then the capacity the buffers were sized for, so a
+// count that outgrew them is clamped rather than overrun).  Device counts let a recorded
+// launch sequence work on whatever sizes the kernels before it produced.
+struct DN {
+    uint32_t v;
+    const uint32_t* p;
+    uint32_t sh;
+    __device__ __forceinline__ uint32_t get() const { return p ? min(p[0] << sh, v) : v; }
+};
+inline DN hn(uint32_t v) { return DN{v, nullptr, 0u}; }
+inline DN dn(const uint32_t* p, uint32_t cap, uint32_t sh = 0u) { return DN{cap, p, sh}; }
+
 __device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }

@@ -3,6 +3,8 @@
 // through MNCudaPrimitives::Scan (MNCudaPrimitives.cpp:160).
 #pragma once
 
+#include <algorithm>
+
 #include "mnrt_common.h"
 
 namespace mnrt {
@@ -190,6 +192,130 @@ inline hipError_t scan_exclusive_set(const ScanSet& S0, int m, uint32_t n, uint3
     }
     hipLaunchKernelGGL(k_scan_add_set, dim3(blocks, m), dim3(kScanBlock), 0, st, S, n, O);
     return hipGetLastError();
+}
+
+// ---- scans of a device count: n = DN::get(), read by every kernel of the scan, so the
+// three launches below can be recorded once and replayed on other sizes (up to the
+// capacity n.v).  Blocks stride over the tiles; one workgroup per scan scans the tile sums.
+struct DScanSet {
+    const uint32_t* in[kScanSetMax];
+    uint32_t* out[kScanSetMax];
+    uint32_t* total[kScanSetMax];  // may be null
+    uint32_t* sums;                // tiles_cap words per scan
+    uint32_t tiles_cap;
+    uint32_t end_total;            // 1: the total is also written to out[n]
+};
+
+__global__ __launch_bounds__(kScanBlock) void k_dscan_tiles(DScanSet S, DN dn_) {
+    __shared__ uint32_t s_wave[kScanBlock / 64];
+    const uint32_t n = dn_.get();
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    const uint32_t y = blockIdx.y;
+    const uint32_t* in = S.in[y];
+    uint32_t* out = S.out[y];
+    uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {  // uniform per block
+        const size_t base = (size_t)t * kScanTile + (size_t)tid * kScanPerThread;
+        uint32_t v[kScanPerThread];
+        uint32_t local = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kScanPerThread; ++j) {
+            const size_t i = base + j;
+            v[j] = (i < n) ? in[i] : 0u;
+            local += v[j];
+        }
+        const uint32_t incl = wave_incl_scan(local, lane);
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        uint32_t wave_off = 0;
+        for (uint32_t w = 0; w < wave; ++w) wave_off += s_wave[w];
+        uint32_t run = wave_off + incl - local;
+#pragma unroll
+        for (uint32_t j = 0; j < kScanPerThread; ++j) {
+            const size_t i = base + j;
+            if (i < n) out[i] = run;
+            run += v[j];
+        }
+        if (tid == kScanBlock - 1) sums[t] = wave_off + incl;
+        __syncthreads();  // s_wave is rewritten by the next tile
+    }
+}
+
+// One workgroup per scan (blockIdx.x): exclusive scan of the tile sums in place, in
+// rounds of 1024, and the total.
+__global__ __launch_bounds__(1024) void k_dscan_sums(DScanSet S, DN dn_) {
+    __shared__ uint32_t s_wave[1024 / 64];
+    const uint32_t n = dn_.get();
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    const uint32_t y = blockIdx.x;
+    uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < tiles; b += 1024) {
+        const uint32_t i = b + tid;
+        const uint32_t v = i < tiles ? sums[i] : 0u;
+        const uint32_t incl = wave_incl_scan(v, lane);
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        uint32_t wave_off = 0, round = 0;
+        for (uint32_t w = 0; w < 1024 / 64; ++w) {
+            if (w < wave) wave_off += s_wave[w];
+            round += s_wave[w];
+        }
+        if (i < tiles) sums[i] = carry + wave_off + incl - v;
+        carry += round;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (S.total[y]) *S.total[y] = carry;
+        if (S.end_total) S.out[y][n] = carry;
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_dscan_add(DScanSet S, DN dn_) {
+    const uint32_t n = dn_.get();
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    const uint32_t y = blockIdx.y;
+    uint32_t* out = S.out[y];
+    const uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t add = sums[t];
+        if (add == 0u) continue;
+        for (uint32_t j = threadIdx.x; j < kScanTile; j += kScanBlock) {
+            const size_t i = (size_t)t * kScanTile + j;
+            if (i < n) out[i] += add;
+        }
+    }
+}
+
+// Tiles (scratch words per scan) for a device-count scan of capacity cap.
+inline uint32_t dscan_tiles(uint32_t cap) { return std::max<uint32_t>(1u, (cap + kScanTile - 1) / kScanTile); }
+
+// m (<= kScanSetMax) exclusive scans of the device count n (capacity n.v): out[y][i] =
+// sum(in[y][0..i)), *total[y] (may be null) = the sum, and out[y][n] too when end_total.
+// scratch: m * dscan_tiles(n.v) words.  Kernels only: the sequence can be captured.
+inline hipError_t dscan(const DScanSet& S0, int m, DN n, uint32_t* scratch, hipStream_t st) {
+    if (m <= 0 || m > kScanSetMax) return hipErrorInvalidValue;
+    DScanSet S = S0;
+    S.sums = scratch;
+    S.tiles_cap = dscan_tiles(n.v);
+    const uint32_t grid = std::min<uint32_t>(S.tiles_cap, 1024u);
+    hipLaunchKernelGGL(k_dscan_tiles, dim3(grid, m), dim3(kScanBlock), 0, st, S, n);
+    hipLaunchKernelGGL(k_dscan_sums, dim3(m), dim3(1024), 0, st, S, n);
+    hipLaunchKernelGGL(k_dscan_add, dim3(grid, m), dim3(kScanBlock), 0, st, S, n);
+    return hipGetLastError();
+}
+
+// One device-count scan (see dscan).
+inline hipError_t dscan1(const uint32_t* in, uint32_t* out, DN n, uint32_t* total, bool end_total,
+                         uint32_t* scratch, hipStream_t st) {
+    DScanSet S{};
+    S.in[0] = in;
+    S.out[0] = out;
+    S.total[0] = total;
+    S.end_total = end_total ? 1u : 0u;
+    return dscan(S, 1, n, scratch, st);
 }
 
 }  // namespace

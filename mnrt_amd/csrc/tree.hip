@@ -43,16 +43,17 @@ __global__ __launch_bounds__(256) void k_photon_records(const uint32_t* __restri
 }
 
 // Batched records: reference r of the shared array belongs to tree s (rbase), its element
-// to input entry ebase[s] + refs[r].
-__global__ __launch_bounds__(256) void k_tri_records_batch(const uint32_t* __restrict__ refs, uint32_t num_refs,
+// to input entry ebase[s] + refs[r].  err (may be null): a nonzero word there means the
+// build failed and nothing is written.
+__global__ __launch_bounds__(256) void k_tri_records_batch(const uint32_t* __restrict__ refs, DN dnum_refs,
                                                            const uint32_t* __restrict__ rbase,
                                                            const uint32_t* __restrict__ ebase, uint32_t ns,
                                                            const float4* __restrict__ v0,
                                                            const float4* __restrict__ v1,
                                                            const float4* __restrict__ v2,
-                                                           float4* __restrict__ recs) {
+                                                           float4* __restrict__ recs, const uint32_t* __restrict__ err) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= num_refs) return;
+    if (r >= dnum_refs.get() || (err && *err != 0u)) return;
     uint32_t t = refs[r];
     uint32_t g = ebase[seg_of(rbase, ns, r)] + t;
     float4 a = v0[g], b = v1[g], c = v2[g];
@@ -61,15 +62,16 @@ __global__ __launch_bounds__(256) void k_tri_records_batch(const uint32_t* __res
     recs[3ull * r + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_photon_records_batch(const uint32_t* __restrict__ refs, uint32_t num_refs,
+__global__ __launch_bounds__(256) void k_photon_records_batch(const uint32_t* __restrict__ refs, DN dnum_refs,
                                                               const uint32_t* __restrict__ rbase,
                                                               const uint32_t* __restrict__ ebase, uint32_t ns,
                                                               const float4* __restrict__ pos,
                                                               const float4* __restrict__ pw,
                                                               const float4* __restrict__ dir,
-                                                              float4* __restrict__ recs) {
+                                                              float4* __restrict__ recs,
+                                                              const uint32_t* __restrict__ err) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= num_refs) return;
+    if (r >= dnum_refs.get() || (err && *err != 0u)) return;
     uint32_t p = refs[r];
     uint32_t g = ebase[seg_of(rbase, ns, r)] + p;
     float4 x = pos[g], w = pw[g], d = dir[g];
@@ -79,12 +81,13 @@ __global__ __launch_bounds__(256) void k_photon_records_batch(const uint32_t* __
 
 // One lane per node: the subtree's reference range runs from its leftmost leaf's first
 // reference to the end of its rightmost leaf.  nbase (ns + 1 entries, or null for one
-// tree) places the trees of a batch in the shared node array.
-__global__ __launch_bounds__(256) void k_collapse(const uint2* __restrict__ nodes_all, uint32_t num_nodes,
+// tree) places the trees of a batch in the shared node array; err as for the records.
+__global__ __launch_bounds__(256) void k_collapse(const uint2* __restrict__ nodes_all, DN dnum_nodes,
                                                   const uint32_t* __restrict__ nbase, uint32_t ns,
-                                                  uint32_t max_refs, uint2* __restrict__ qnodes_all) {
+                                                  uint32_t max_refs, uint2* __restrict__ qnodes_all,
+                                                  const uint32_t* __restrict__ err) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= num_nodes) return;
+    if (g >= dnum_nodes.get() || (err && *err != 0u)) return;
     const uint32_t b = nbase ? nbase[seg_of(nbase, ns, g)] : 0u;
     const uint2* nodes = nodes_all + b;
     uint2* qnodes = qnodes_all + b;
@@ -112,6 +115,28 @@ __global__ __launch_bounds__(256) void k_collapse(const uint2* __restrict__ node
         qnodes[i] = make_uint2(end - first, (MNRT_LEAF_TAG << 30) | first);
     else
         qnodes[i] = nd;
+}
+
+// Records (and, for photon trees, query-time node words) of the trees of a batch whose
+// sizes are device words (a recorded build): grids cover the capacities nrefs.v / nnodes.v.
+int launch_records_batch(int kind, const uint32_t* refs, DN nrefs, const uint32_t* rbase, const uint32_t* ebase,
+                         uint32_t ns, const float4* a, const float4* b, const float4* c, float4* recs,
+                         const uint32_t* err, hipStream_t st) {
+    const dim3 grid((std::max<uint32_t>(nrefs.v, 1u) + 255) / 256);
+    if (kind == MNRT_KIND_TRI)
+        hipLaunchKernelGGL(k_tri_records_batch, grid, dim3(256), 0, st, refs, nrefs, rbase, ebase, ns, a, b, c, recs,
+                           err);
+    else
+        hipLaunchKernelGGL(k_photon_records_batch, grid, dim3(256), 0, st, refs, nrefs, rbase, ebase, ns, a, b, c,
+                           recs, err);
+    return hipGetLastError() == hipSuccess ? MNRT_OK : MNRT_ERR_HIP;
+}
+
+int launch_collapse_batch(const uint2* nodes, DN nnodes, const uint32_t* nbase, uint32_t ns, uint32_t max_refs,
+                          uint2* qnodes, const uint32_t* err, hipStream_t st) {
+    const dim3 grid((std::max<uint32_t>(nnodes.v, 1u) + 255) / 256);
+    hipLaunchKernelGGL(k_collapse, grid, dim3(256), 0, st, nodes, nnodes, nbase, ns, max_refs, qnodes, err);
+    return hipGetLastError() == hipSuccess ? MNRT_OK : MNRT_ERR_HIP;
 }
 
 }  // namespace mnrt
@@ -179,8 +204,8 @@ extern "C" int mnrt_tree_collapse(mnrt_tree* tree, uint32_t max_refs, void* stre
         tree->owns_qnodes = 1;
     }
     hipLaunchKernelGGL(mnrt::k_collapse, dim3((tree->num_nodes + 255) / 256), dim3(256), 0, st,
-                       (const uint2*)tree->d_nodes, tree->num_nodes, (const uint32_t*)nullptr, 1u, max_refs,
-                       (uint2*)tree->d_qnodes);
+                       (const uint2*)tree->d_nodes, mnrt::hn(tree->num_nodes), (const uint32_t*)nullptr, 1u, max_refs,
+                       (uint2*)tree->d_qnodes, (const uint32_t*)nullptr);
     MNRT_HIP_CHECK(hipGetLastError());
     tree->qleaf_max = max_refs;
     return MNRT_OK;
@@ -239,7 +264,7 @@ struct BatchTables {
 };
 
 int batch_tables(const mnrt_tree* t, uint32_t ns, const uint32_t* h_offsets, uint32_t kind, BatchTables& B) {
-    if (!t || ns == 0 || !h_offsets || !t[0].d_nodes || !t[0].owns_nodes) return MNRT_ERR_ARG;
+    if (!t || ns == 0 || !h_offsets || !t[0].d_nodes) return MNRT_ERR_ARG;
     B.h.assign(3ull * (ns + 1), 0u);
     uint32_t* nb = B.h.data();
     uint32_t* rb = nb + ns + 1;
@@ -286,8 +311,9 @@ extern "C" int mnrt_tree_attach_tri_records_batch(mnrt_tree* trees, uint32_t num
     if (rc == MNRT_OK && B.total_refs > 0) {
         const uint32_t ns = num_scenes;
         hipLaunchKernelGGL(mnrt::k_tri_records_batch, dim3((B.total_refs + 255) / 256), dim3(256), 0, st,
-                           trees[0].d_refs, B.total_refs, d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
-                           (const float4*)d_v0, (const float4*)d_v1, (const float4*)d_v2, (float4*)recs);
+                           trees[0].d_refs, mnrt::hn(B.total_refs), d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
+                           (const float4*)d_v0, (const float4*)d_v1, (const float4*)d_v2, (float4*)recs,
+                           (const uint32_t*)nullptr);
         if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
     }
     if (d_tab) (void)mnrt::pool_free(d_tab, st);
@@ -327,9 +353,9 @@ extern "C" int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t 
         rc = upload(B, &d_tab, st);
         if (rc == MNRT_OK && B.total_refs > 0) {
             hipLaunchKernelGGL(mnrt::k_photon_records_batch, dim3((B.total_refs + 255) / 256), dim3(256), 0, st,
-                               trees[0].d_refs, B.total_refs, d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
+                               trees[0].d_refs, mnrt::hn(B.total_refs), d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
                                (const float4*)d_pos4, (const float4*)d_pow4, (const float4*)d_dir4,
-                               (float4*)trees[0].d_recs);
+                               (float4*)trees[0].d_recs, (const uint32_t*)nullptr);
             if (hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
         }
         if (d_tab) (void)mnrt::pool_free(d_tab, st);
@@ -347,12 +373,13 @@ extern "C" int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t 
     rc = upload(B, &d_tab, st);
     if (rc == MNRT_OK && B.total_refs > 0)
         hipLaunchKernelGGL(mnrt::k_photon_records_batch, dim3((B.total_refs + 255) / 256), dim3(256), 0, st,
-                           trees[0].d_refs, B.total_refs, d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
-                           (const float4*)d_pos4, (const float4*)d_pow4, (const float4*)d_dir4, (float4*)recs);
+                           trees[0].d_refs, mnrt::hn(B.total_refs), d_tab + ns + 1, d_tab + 2 * (ns + 1), ns,
+                           (const float4*)d_pos4, (const float4*)d_pow4, (const float4*)d_dir4, (float4*)recs,
+                           (const uint32_t*)nullptr);
     if (rc == MNRT_OK)
         hipLaunchKernelGGL(mnrt::k_collapse, dim3((B.total_nodes + 255) / 256), dim3(256), 0, st,
-                           (const uint2*)trees[0].d_nodes, B.total_nodes, (const uint32_t*)d_tab, ns, mnrt::kQueryLeafRefs,
-                           (uint2*)qn);
+                           (const uint2*)trees[0].d_nodes, mnrt::hn(B.total_nodes), (const uint32_t*)d_tab, ns,
+                           mnrt::kQueryLeafRefs, (uint2*)qn, (const uint32_t*)nullptr);
     if (rc == MNRT_OK && hipGetLastError() != hipSuccess) rc = MNRT_ERR_HIP;
     if (d_tab) (void)mnrt::pool_free(d_tab, st);
     if (hipStreamSynchronize(st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;

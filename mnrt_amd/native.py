@@ -54,6 +54,25 @@ class Light(ctypes.Structure):
 LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_AREA_DISC, LIGHT_AREA_RECT = 0, 1, 2, 3
 
 
+class BuildGraphDesc(ctypes.Structure):
+    """mnrt_build_graph_desc: a batched build recorded into a hipGraph."""
+
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("num_scenes", u32),
+        ("h_offsets", ctypes.POINTER(u32)),
+        ("d_v0", vp),
+        ("d_v1", vp),
+        ("d_v2", vp),
+        ("d_power4", vp),
+        ("d_dir4", vp),
+        ("params", ctypes.POINTER(BuildParams)),
+        ("h_root_boxes", ctypes.POINTER(f32)),
+        ("h_max_query_radius", ctypes.POINTER(f32)),
+        ("margin", f32),
+    ]
+
+
 class Tree(ctypes.Structure):
     """mnrt_tree: device-resident packed preorder kd-tree."""
 
@@ -117,7 +136,7 @@ class BatchIO(ctypes.Structure):
 PHASE_TRACE, PHASE_SHADE, PHASE_KNN, PHASE_KNN_BWD = 1, 2, 4, 8
 
 # MNRT_ABI_VERSION of the header these declarations mirror
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 # status codes (the MNRT_OK / MNRT_ERR_* values of mnrt_kd.h)
@@ -155,6 +174,10 @@ def load():
                                           ctypes.POINTER(f32), ctypes.POINTER(f32), TP, vp]
     L.mnrt_tree_attach_tri_records_batch.argtypes = [TP, u32, vp, vp, vp, ctypes.POINTER(u32), vp]
     L.mnrt_tree_attach_photon_records_batch.argtypes = [TP, u32, vp, vp, vp, ctypes.POINTER(u32), vp]
+    L.mnrt_build_graph_create.argtypes = [ctypes.POINTER(BuildGraphDesc), TP, ctypes.POINTER(vp), vp]
+    L.mnrt_build_graph_replay.argtypes = [vp, vp]
+    L.mnrt_build_graph_status.argtypes = [vp, TP, ctypes.POINTER(u32), vp]
+    L.mnrt_build_graph_destroy.argtypes = [vp]
     BP = ctypes.POINTER(ctypes.c_void_p)
     L.mnrt_batch_create.argtypes = [ctypes.POINTER(BatchDesc), BP]
     L.mnrt_batch_destroy.argtypes = [vp]

@@ -20,7 +20,7 @@ def test_status_strings():
     L = N.load()
     assert L.mnrt_status_string(0) == b"ok"
     assert L.mnrt_status_string(-1) == b"invalid argument"
-    assert L.mnrt_abi_version() == N.ABI_VERSION == 3
+    assert L.mnrt_abi_version() == N.ABI_VERSION == 4
 
 
 def test_argument_errors_without_device():
@@ -87,6 +87,38 @@ def test_primitive_argument_errors_without_device():
     assert L.mnrt_compact_u32(fake, fake, 5, fake, None, None) == N.ERR_ARG
     assert L.mnrt_compact_u32(fake, fake, 5, fake, fake, None) == N.ERR_ARG  # in place
     assert L.mnrt_compact_u32(fake, None, 5, 512, fake, None) == N.ERR_ARG
+
+
+def test_build_graph_argument_errors_without_device():
+    import ctypes
+
+    L = N.load()
+    fake = 256
+    prm = N.BuildParams(0.25, 64, 1.0, 0)
+    off = (N.u32 * 3)(0, 5, 9)
+    trees = (N.Tree * 2)()
+    g = ctypes.c_void_p()
+
+    def desc(**kw):
+        a = dict(kind=N.KIND_TRI, num_scenes=2, h_offsets=ctypes.cast(off, ctypes.POINTER(N.u32)), d_v0=fake,
+                 d_v1=fake, d_v2=fake, d_power4=None, d_dir4=None, params=ctypes.pointer(prm), h_root_boxes=None,
+                 h_max_query_radius=None, margin=0.25)
+        a.update(kw)
+        return N.BuildGraphDesc(**a)
+
+    create = lambda d: L.mnrt_build_graph_create(ctypes.byref(d), trees, ctypes.byref(g), None)
+    assert L.mnrt_build_graph_create(None, trees, ctypes.byref(g), None) == N.ERR_ARG
+    assert create(desc(kind=7)) == N.ERR_ARG
+    assert create(desc(num_scenes=0)) == N.ERR_ARG
+    assert create(desc(margin=-0.5)) == N.ERR_ARG
+    assert create(desc(margin=float("nan"))) == N.ERR_ARG
+    assert create(desc(d_v1=None)) == N.ERR_ARG  # triangles need three vertex arrays
+    assert create(desc(kind=N.KIND_POINT, d_power4=fake)) == N.ERR_ARG  # powers without directions
+    assert create(desc(params=None)) == N.ERR_ARG
+    assert L.mnrt_build_graph_replay(None, None) == N.ERR_ARG
+    flags = N.u32(0)
+    assert L.mnrt_build_graph_status(None, trees, ctypes.byref(flags), None) == N.ERR_ARG
+    assert L.mnrt_build_graph_destroy(None) == N.OK
 
 
 def test_product_fails_loudly_without_library(monkeypatch, tmp_path):

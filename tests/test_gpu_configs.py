@@ -11,9 +11,9 @@ per-light emission in [0.5, 2]): dLoss/dpower, dLoss/dcolour, dLoss/demission an
 dLoss/dalbedo against the oracle's fp64 adjoint over every query; fp64 central
 differences on 1000 coordinates, each over the queries whose sets hold the coordinate.
 
-Batched (256 scenes): after one captured batched step every 16th scene's triangle and
-photon trees equal the oracle's, and a 1-in-64 sample of its rays and kNN queries match
-the oracle bit for bit.
+Batched (256 scenes): after one batched step replayed from graphs (both device-driven
+builds and every query phase) every 16th scene's triangle and photon trees equal the
+oracle's, and a 1-in-64 sample of its rays and kNN queries match the oracle bit for bit.
 
 Batched inverse lighting (64 scenes x 4 lights): the batched gradient against the
 oracle's fp64 chain per scene, the radiance at the fitted parameters after 200 Adam steps
@@ -23,7 +23,7 @@ from the first check serve both)."""
 import numpy as np
 import pytest
 
-from mnrt_amd import adjoint, api, inverse, scene as S
+from mnrt_amd import adjoint, api, inverse, native as N, scene as S
 from helpers import host, same_tree
 
 pytestmark = pytest.mark.gpu
@@ -287,10 +287,11 @@ def test_batched_256_scenes_sampled_vs_oracle(orc, cuda):
     del ph
     TB0.release()
     print(f"batched: {NS} scenes, {int(toff[-1])} triangles, {int(poff[-1])} photons, {R} rays", flush=True)
-    # one batched step: both batched builds, then every phase replayed from its graph
-    TB = api.build_tri_batch(v0, v1, v2, toff)
-    PB = api.build_points_batch(ppos, poff, root_boxes=boxes, max_query_radius=rmax)
-    api.attach_photon_records_batch(PB, ppos, ppow, pdir)
+    # one batched step: both builds (device-driven, replayed from their graphs), then
+    # every query phase replayed from its graph
+    GT = api.BuildGraph(N.KIND_TRI, toff, v0, v1, v2)
+    GP = api.BuildGraph(N.KIND_POINT, poff, ppos, power=ppow, dirs=pdir, root_boxes=boxes, max_query_radius=rmax)
+    TB, PB = GT.trees, GP.trees
     Q = api.QueryBatch(TB, toff, roff, eps, PB, rmax, K)
     f4 = lambda: torch.empty((R, 4), dtype=torch.float32, device="cuda")
     bufs = dict(orig4=go, dir4=gd, v0=v0, v1=v1, v2=v2, albedo4=alb,
@@ -302,10 +303,13 @@ def test_batched_256_scenes_sampled_vs_oracle(orc, cuda):
     Q.autotune(max_slots=min(R, 8 << 20))
     for ph_ in (1, 2, 4):
         Q.capture(ph_)
+    GT.replay()
+    GP.replay()
     bufs["io_rad4"].zero_()
     for ph_ in (1, 2, 4):
         Q.replay(ph_)
     torch.cuda.synchronize()
+    assert GT.status() == 0 and GP.status() == 0
     cnt = host(bufs["sp_count"])
     hp = lambda t, sl: host(t[sl])
     for s in range(0, NS, 16):
@@ -331,8 +335,8 @@ def test_batched_256_scenes_sampled_vs_oracle(orc, cuda):
         print(f"scene {s}: {sc.num_tris} tris, {ps.stop - ps.start} photons, {len(ridx)} rays and {len(qidx)} "
               f"queries sampled: equal", flush=True)
     Q.close()
-    TB.release()
-    PB.release()
+    GT.close()
+    GP.close()
 
 
 # --- batched inverse lighting -----------------------------------------------------------------
