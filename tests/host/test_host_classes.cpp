@@ -63,6 +63,13 @@ static std::vector<T> download(const T* d, size_t n) {
 static bool same_bytes(const void* a, const void* b, size_t n) { return std::memcmp(a, b, n) == 0; }
 
 int main() {
+    // the structs this program shares with the library (mnrt_tree inside the classes) must
+    // have the layout the library was built with
+    if (mnrt_abi_version() != MNRT_ABI_VERSION) {
+        std::fprintf(stderr, "library ABI %d, this program was built for %d: rebuild it\n", mnrt_abi_version(),
+                     MNRT_ABI_VERSION);
+        return 2;
+    }
     ABIOK(mnrt_device_init(0));
     // scene: the 12 inward faces of [-1,1]^3, a light quad at the ceiling, 3000 small triangles
     std::vector<float4> v0, v1, v2, alb;
