@@ -23,6 +23,7 @@
 //     channel as exact 64-bit fixed-point integers (the reference adds floats in its
 //     range-search order, photon_knn.cu:506-516, so its sums depend on that order).
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -472,23 +473,34 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
 
     // Front-facing powers: each lane fetches the records of its slots of the set, the
     // channel sums are exact fixed-point sums (independent of the set's order).  The set
-    // has at most k <= CAP - 64 entries (the launch picks CAP >= k + 64).
+    // has at most k <= CAP - 64 entries (the launch picks CAP >= k + 64), so ceil(k / 64)
+    // slots per lane hold it; slots beyond would only add zeros.
     constexpr int RK = std::min<int>((CAP - 64) / 64, (int)((MNRT_KNN_MAX_K + 63u) / 64u));
-    float vx[RK], vy[RK], vz[RK];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    auto front_sums = [&](auto slots) {
+        constexpr int S = decltype(slots)::value;
+        float vx[S], vy[S], vz[S];
 #pragma unroll
-    for (int t = 0; t < RK; ++t) {
-        const uint32_t i = lane + 64u * (uint32_t)t;
-        vx[t] = vy[t] = vz[t] = 0.0f;
-        if (i < found) {
-            const float4 pw = Tr.recs[2ull * kpos[i] + 1];
-            if (dot3(nq, oct_decode(__float_as_uint(pw.w))) < 0.0f) {
-                vx[t] = pw.x;
-                vy[t] = pw.y;
-                vz[t] = pw.z;
+        for (int t = 0; t < S; ++t) {
+            const uint32_t i = lane + 64u * (uint32_t)t;
+            vx[t] = vy[t] = vz[t] = 0.0f;
+            if (i < found) {
+                const float4 pw = Tr.recs[2ull * kpos[i] + 1];
+                if (dot3(nq, oct_decode(__float_as_uint(pw.w))) < 0.0f) {
+                    vx[t] = pw.x;
+                    vy[t] = pw.y;
+                    vz[t] = pw.z;
+                }
             }
         }
-    }
-    const float sx = wave_fixed_sum<RK>(vx), sy = wave_fixed_sum<RK>(vy), sz = wave_fixed_sum<RK>(vz);
+        sx = wave_fixed_sum<S>(vx);
+        sy = wave_fixed_sum<S>(vy);
+        sz = wave_fixed_sum<S>(vz);
+    };
+    if (RK >= 2 && k <= 64u) front_sums(std::integral_constant<int, 1>{});
+    else if (RK >= 3 && k <= 128u) front_sums(std::integral_constant<int, (RK >= 3 ? 2 : RK)>{});
+    else if (RK >= 4 && k <= 192u) front_sums(std::integral_constant<int, (RK >= 4 ? 3 : RK)>{});
+    else front_sums(std::integral_constant<int, RK>{});
     float ix = 0.0f, iy = 0.0f, iz = 0.0f;
     if (r2 > 0.0f) {
         float den = r2 * kPi;

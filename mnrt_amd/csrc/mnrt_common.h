@@ -104,9 +104,16 @@ __device__ __forceinline__ uint32_t oct_encode(float x, float y, float z) {
     return ((uint32_t)(uint16_t)(int16_t)iu) | (((uint32_t)(uint16_t)(int16_t)iv) << 16);
 }
 
+// The fp32 quotient q / 32767 of an integer |q| <= 32767 lies at least 2^-39 (relative) from
+// any fp32 rounding boundary (its reduced denominator divides the odd 32767, a boundary's
+// is a power of two), so the fp64 product with the rounded reciprocal (relative error
+// below 2^-52), rounded to fp32, is the correctly rounded quotient: the same value as the
+// IEEE division, in three instructions (checked for all 65535 codes on the host).
+__device__ __forceinline__ float div32767(int32_t q) { return (float)((double)q * (1.0 / 32767.0)); }
+
 __device__ __forceinline__ f3 oct_decode(uint32_t c) {
     int16_t iu = (int16_t)(uint16_t)(c & 0xFFFFu), iv = (int16_t)(uint16_t)(c >> 16);
-    float u = (float)iu / 32767.0f, v = (float)iv / 32767.0f;
+    float u = div32767(iu), v = div32767(iv);
     float z = (1.0f - fabsf(u)) - fabsf(v);
     if (z < 0.0f) {
         float ou = u;

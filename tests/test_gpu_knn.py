@@ -36,6 +36,15 @@ def _run(orc, cuda, sc, ppos, ppow, pdir, q, k, rmax, brute=True, clr=True):
     scale = np.abs(o64).max(axis=1, keepdims=True) + 1e-30
     err = np.abs(grad.astype(np.float64) - o64) / scale
     assert err.max() <= RTOL64, f"max rel err vs fp64 {err.max():.3e}"
+    # the list-free path (no final sort: the selection alone decides the set) gives the same
+    # radiance, r2 and counts
+    free = api.knn_gather(DP, k, rmax, q["pos"], q["nrm"], q["dir"], q["infl"], q["clr"] if clr else None,
+                          want_r2=True, want_count=True)
+    cuda.cuda.synchronize()
+    assert np.array_equal(free["count"].cpu().numpy().view(np.uint32), cnt), "list-free counts differ"
+    assert np.array_equal(free["r2"].cpu().numpy().view(np.uint32), r2.view(np.uint32)), "list-free r2 differs"
+    assert np.array_equal(free["rad"].cpu().numpy().view(np.uint32), o32.view(np.uint32)), \
+        "list-free radiance not bit-exact to fp32 oracle"
     return gn, gc
 
 

@@ -204,3 +204,13 @@ def test_fixed_sum_edge_cases(orc):
         seq = np.float32(seq + x)
     assert abs(float(orc.fixed_sum(v)) - float(seq)) <= 100 * np.spacing(seq)
     assert abs(float(orc.fixed_sum(v)) - float(np.sum(v.astype(np.float64)))) <= np.spacing(np.float32(v.sum()))
+
+
+def test_oct_quotient_by_reciprocal_product():
+    """The kernels decode q / 32767 as the fp64 product with 1/32767 rounded to fp32 (three
+    instructions instead of an IEEE division); for every 16-bit code it is the IEEE quotient
+    the oracle computes, so decoded directions and front-facing tests are unchanged."""
+    q = np.arange(-32767, 32768, dtype=np.int32)
+    ieee = q.astype(np.float32) / np.float32(32767.0)
+    prod = (q.astype(np.float64) * (1.0 / 32767.0)).astype(np.float32)
+    assert np.array_equal(ieee.view(np.uint32), prod.view(np.uint32))
