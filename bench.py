@@ -297,17 +297,50 @@ def batched_leg(args, torch, api, S, O, ncpu):
     # every 8th scene (its own photon tree of the scene), weighted by query count
     cnt = bufs["sp_count"].cpu().numpy().astype(np.int64)
     tot_b, tot_q = 0.0, 0
+    # the CPU baseline times the oracle on the same sample: both tree builds of every 8th
+    # scene, traversal of 1 in `sample` of its rays, kNN sets + estimate at 1 in `sample` of
+    # its hits (all host threads, 1 warm-up + median of 5), each scaled to the whole batch
+    reps = 0 if args.no_cpu else 5
+    ct = dict(build=0.0, elems=0, trace=0.0, rays=0, knn=0.0, queries=0)
     for s in range(0, NS, 8):
         if cnt[s] == 0:
             continue
         pp = ppos[psl[s]].cpu().numpy()
-        op = O.build(1, pp, max_qr=float(rmax[s]), root=boxes[s])
+        op, pb_s = _median_time(lambda: O.build(1, pp, max_qr=float(rmax[s]), root=boxes[s], nthreads=ncpu), reps)
         idx = torch.arange(int(roff[s]), int(roff[s]) + int(cnt[s]), args.sample, device="cuda")
         qp = bufs["sp_pos4"][idx].cpu().numpy()
-        _, c, ctr = O.knn(op, pp, qp, K, float(rmax[s]), nthreads=ncpu)
+        (nbr, c, ctr), knn_s = _median_time(lambda: O.knn(op, pp, qp, K, float(rmax[s]), nthreads=ncpu), reps)
         tot_b += knn_bytes(ctr, len(qp), int(np.minimum(c, K).sum())) / len(qp) * float(cnt[s])
         tot_q += int(cnt[s])
+        if args.no_cpu:
+            continue
+        sc = scs[s]
+        ot, tb_s = _median_time(lambda: O.build(0, sc.v0, sc.v1, sc.v2, nthreads=ncpu), reps)
+        ridx = torch.arange(int(roff[s]), int(roff[s + 1]), args.sample, device="cuda")
+        ro, rd = go[ridx].cpu().numpy(), gd[ridx].cpu().numpy()
+        _, tr_s = _median_time(lambda: O.trace(ot, sc.v0, sc.v1, sc.v2, ro, rd, float(eps[s]), nthreads=ncpu), reps)
+        qh = {k_: bufs["sp_" + k_ + "4"][idx].cpu().numpy() for k_ in ("nrm", "dir", "infl", "clr")}
+        pw, pdr = ppow[psl[s]].cpu().numpy(), pdir[psl[s]].cpu().numpy()
+        _, rad_s = _median_time(lambda: O.radiance(nbr, c, K, float(rmax[s]), pp, pw, pdr, qp, qh["nrm"], qh["dir"],
+                                                   qh["infl"], qh["clr"], nthreads=ncpu), reps)
+        ct["build"] += tb_s + pb_s
+        ct["elems"] += sc.num_tris + len(pp)
+        ct["trace"] += tr_s
+        ct["rays"] += len(ro)
+        ct["knn"] += knn_s + rad_s
+        ct["queries"] += len(qp)
     per_q = tot_b / max(tot_q, 1)
+    cpu = None
+    if not args.no_cpu and ct["queries"] > 0:
+        cpu_s = (ct["build"] * (int(toff[-1]) + int(poff[-1])) / ct["elems"] + ct["trace"] * R / ct["rays"]
+                 + ct["knn"] * hits / ct["queries"])
+        cpu = {"value": round(hits / cpu_s / 1e6, 4), "unit": "M kNN queries/s (step incl. builds and traversal)",
+               "ms_per_step": round(cpu_s * 1e3, 1), "cores": ncpu, "kind": "port",
+               "gpu_speedup": round(cpu_s / batched_s, 1),
+               "sample": (f"oracle on every 8th scene: both tree builds ({ct['elems']} elements, {ct['build']:.2f}s), "
+                          f"traversal of 1 in {args.sample} rays ({ct['rays']}, {ct['trace']:.3f}s), kNN sets + "
+                          f"estimate at 1 in {args.sample} hits ({ct['queries']}, {ct['knn']:.3f}s), all on {ncpu} "
+                          "threads, each part 1 warm-up + median of 5, scaled by elements / rays / hits")}
     knn_ms = float(np.mean([p["knn"] for p in phase]))
     gbs = per_q * hits / (knn_ms * 1e-3) / 1e9
     Q.close()
@@ -327,7 +360,8 @@ def batched_leg(args, torch, api, S, O, ncpu):
         "ms_per_step": round(batched_s * 1e3, 3), "plain_ms_per_step": round(plain_s * 1e3, 3),
         "plain_value": round(hits / plain_s / 1e6, 3), "speedup_vs_plain": round(plain_s / batched_s, 2),
         "phase_ms": {n: round(float(np.mean([p[n] for p in phase])), 3) for n in names},
-        "tuning": tuning, "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 1), "steps": steps}
+        "tuning": tuning, "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 1), "steps": steps,
+        "cpu_baseline": cpu}
 
 
 def adjoint_leg(args, torch, api, S, O, ncpu):
@@ -369,12 +403,31 @@ def adjoint_leg(args, torch, api, S, O, ncpu):
     pp = ph["pos"].cpu().numpy()
     op = O.build(1, pp, max_qr=rmax, root=root)
     qp = q["pos"][:: args.sample].cpu().numpy()
-    _, c, ctr = O.knn(op, pp, qp, K, rmax, nthreads=ncpu)
+    reps = 0 if args.no_cpu else 5
+    (nbr, c, ctr), knn_s = _median_time(lambda: O.knn(op, pp, qp, K, rmax, nthreads=ncpu), reps)
     found = int(np.minimum(c, K).sum())
     fwd_q = knn_bytes(ctr, len(qp), found) / len(qp)
     bwd_q = fwd_q + 12.0 * found / len(qp)
     nq = q["count"]
     fg, bg = fwd_q * nq / (f * 1e-3) / 1e9, bwd_q * nq / (b * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on the same sample: forward = sets + fp32 estimate (all host threads),
+        # backward = sets + fp64 adjoint (one thread), scaled to every query
+        pw, pdr = ph["power"].cpu().numpy(), ph["dir"].cpu().numpy()
+        qh = {k_: q[k_][:: args.sample].cpu().numpy() for k_ in ("nrm", "dir", "infl", "clr")}
+        g = grad[:: args.sample].cpu().numpy()
+        _, rad_s = _median_time(lambda: O.radiance(nbr, c, K, rmax, pp, pw, pdr, qp, qh["nrm"], qh["dir"],
+                                                   qh["infl"], qh["clr"], nthreads=ncpu), reps)
+        _, grad_s = _median_time(lambda: O.radiance_grad(nbr, c, K, rmax, pp, pw, pdr, qp, qh["nrm"], qh["dir"],
+                                                         qh["infl"], qh["clr"], g), reps)
+        scale = nq / len(qp)
+        cf, cb = (knn_s + rad_s) * scale * 1e3, (knn_s + grad_s) * scale * 1e3
+        cpu = {"fwd_ms": round(cf, 1), "bwd_ms": round(cb, 1), "cores": ncpu, "kind": "port",
+               "gpu_speedup_fwd": round(cf / f, 1), "gpu_speedup_bwd": round(cb / b, 1),
+               "sample": (f"oracle on 1 in {args.sample} queries ({len(qp)}), each part 1 warm-up + median of 5: "
+                          f"kNN sets {knn_s:.3f}s and fp32 estimate {rad_s:.3f}s on {ncpu} threads, fp64 adjoint "
+                          f"{grad_s:.3f}s on one thread, scaled to {nq} queries")}
     T.release()
     P.release()
     return {"workload": "small scene + 3 lights, 1M photons, 1920x1080 primary hits, k=50",
@@ -385,7 +438,8 @@ def adjoint_leg(args, torch, api, S, O, ncpu):
             "bwd_roofline": {"kernel": "k_knn_bwd_chunks (mnrt_knn_gather_bwd)", "bound": "hbm",
                              "achieved": round(bg, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(bg / HBM_PEAK_GBS, 5), "alg_bytes_per_query": round(bwd_q, 1),
-                             "sample": f"1 in {args.sample} queries"}}
+                             "sample": f"1 in {args.sample} queries"},
+            "cpu_baseline": cpu}
 
 
 def inverse_leg(args, torch, api, S, O, ncpu):
@@ -440,6 +494,11 @@ def inverse_leg(args, torch, api, S, O, ncpu):
     P.radiance(e_fit, a_fit)
     eh, ah, et = e_fit.cpu().numpy(), a_fit.cpu().numpy(), e_true.cpu().numpy()
     per = {"rad_rel_err_fp64": [], "rad_fp32_bits_equal": [], "power_rel_err_fp64": [], "emission_mean_abs_err": []}
+    # CPU baseline of one Adam step, timed on every 4th scene (1 warm-up + median of 3): the
+    # oracle's kNN sets + fp32 estimate and sets + fp64 adjoint on the check's query sample,
+    # the fp64 reweighting and its adjoint over all of the scene's photons; scaled by queries
+    # and photons to the whole batch
+    cst = dict(q=0.0, nq=0, ph=0.0, nph=0)
     for s in range(NS):
         A = P.scene_arrays(s)
         ls, ts = slice(int(P.loff[s]), int(P.loff[s + 1])), slice(int(P.toff[s]), int(P.toff[s + 1]))
@@ -460,6 +519,32 @@ def inverse_leg(args, torch, api, S, O, ncpu):
         scale = np.abs(o64).max(axis=1) + 1e-30
         per["rad_rel_err_fp64"].append(float(np.max(np.abs(q["rad"].astype(np.float64) - o64).max(axis=1) / scale)))
         per["emission_mean_abs_err"].append(float(np.abs(eh[ls] - et[ls]).mean()))
+        if args.no_cpu or s % 4:
+            continue
+        _, knn_s = _median_time(lambda: O.knn(tree, A["ppos"], q["qpos"], K, A["r_max"], nthreads=ncpu), 3)
+        _, rad_s = _median_time(lambda: O.radiance(nbr, cnt, K, A["r_max"], A["ppos"], A["ppow"], A["pdir"],
+                                                   q["qpos"], q["qnrm"], q["qdir"], q["qinfl"], q["qclr"],
+                                                   nthreads=ncpu), 3)
+        gq = np.ones((len(q["qpos"]), 4), np.float32)
+        (gp, gc), grad_s = _median_time(lambda: O.radiance_grad(nbr, cnt, K, A["r_max"], A["ppos"], A["ppow"],
+                                                                A["pdir"], q["qpos"], q["qnrm"], q["qdir"], q["qinfl"],
+                                                                q["qclr"], gq), 3)
+        _, rw_s = _median_time(lambda: O.reweight(A["base"], lt, tr, eh[ls], ah[ts]), 3)
+        _, rwg_s = _median_time(lambda: O.reweight_grad(A["base"], lt, tr, eh[ls], ah[ts], gp[:, :3]), 3)
+        nqs = len(A["qpos"])
+        cst["q"] += (2 * knn_s + rad_s + grad_s) * nqs / len(q["qpos"])
+        cst["nq"] += nqs
+        cst["ph"] += rw_s + rwg_s
+        cst["nph"] += len(A["ppos"])
+    cpu = None
+    if cst["nq"] > 0:
+        cpu_ms = (cst["q"] * P.num_queries / cst["nq"] + cst["ph"] * int(P.poff[-1]) / cst["nph"]) * 1e3
+        cpu = {"ms_per_step": round(cpu_ms, 1), "cores": ncpu, "kind": "port",
+               "gpu_speedup": round(cpu_ms / (fit_s / max(1, args.inverse_steps) * 1e3), 1),
+               "sample": (f"oracle on every 4th scene ({cst['nq']} queries, {cst['nph']} photons): kNN sets + fp32 "
+                          f"estimate and sets + fp64 adjoint at 1 in {args.sample // 4 or 1} hits (sets and estimate "
+                          f"on {ncpu} threads, adjoint on one), fp64 reweighting and its adjoint over all photons "
+                          "(numpy), each 1 warm-up + median of 3, scaled by queries and photons")}
     check_s = time.time() - t2
     _progress(f"inverse lighting oracle check of {NS} scenes: {check_s:.1f}s")
     steps = max(1, args.inverse_steps)
@@ -479,7 +564,8 @@ def inverse_leg(args, torch, api, S, O, ncpu):
                "all_rad_fp32_bits_equal": all(per["rad_fp32_bits_equal"]),
                "max_power_rel_err_fp64": max(per["power_rel_err_fp64"]),
                "per_scene": {k_: ([round(v, 9) for v in vs] if isinstance(vs[0], float) else vs)
-                             for k_, vs in per.items()}}}
+                             for k_, vs in per.items()}},
+           "cpu_baseline": cpu}
     P.release()
     return out
 
