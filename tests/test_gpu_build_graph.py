@@ -160,6 +160,17 @@ def test_graph_points_equals_batch_and_oracle(orc, cuda):
     torch.cuda.synchronize()
     _same_batch(G, B, "records refreshed")
     G.close()
+    # recorded without records (no powers or directions): the trees alone
+    G2 = api.BuildGraph(N.KIND_POINT, off, dp, root_boxes=boxes, max_query_radius=qr)
+    B2 = api.build_points_batch(dp, off, root_boxes=boxes, max_query_radius=qr)
+    G2.replay()
+    assert G2.status() == 0
+    torch.cuda.synchronize()
+    for s in range(len(sizes)):
+        assert not G2.trees[s].t.d_recs and not G2.trees[s].t.d_qnodes
+        _same(G2.trees[s], B2[s], f"no records, scene {s}")
+    G2.close()
+    B2.release()
     B.release()
 
 
