@@ -427,7 +427,7 @@ struct Active {
 // Large-node stage (LargeNodeStage/ProcessLargeNodes, KDTreeGPU.cpp:247-782; empty space
 // cutting kdtree.cu:467-612 and 1986-2033; median split kdtree.cu:629-682).  A node whose
 // median split would put every element into both children becomes a leaf (the
-// reference loops forever there, F5 in the survey of the reference).
+// reference's level loop never ends there: the node comes back unchanged every level).
 //
 // Level-parallel (the reference's kernels are level-parallel too): the active nodes'
 // elements are cut into chunks; the tight boxes (min / max: exact in any order) and the
