@@ -137,11 +137,22 @@ __device__ __forceinline__ Hit trace_ray(const uint2* __restrict__ nodes, const 
         for (;;) {
             uint2 nd = descend(nodes, st, sp, node, tmin, tmax, o, d, inv);
             uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
+            // the next triangle's record is loaded before the current one is tested
+            const float4* rec = recs + 3ull * begin;
+            float4 na4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ne14 = na4, ne24 = na4;
+            if (count > 0) {
+                na4 = rec[0];
+                ne14 = rec[1];
+                ne24 = rec[2];
+            }
             for (uint32_t r = 0; r < count; ++r) {
-                const float4* rec = recs + 3ull * (begin + r);
-                float4 a4 = rec[0];
-                float4 e14 = rec[1];
-                float4 e24 = rec[2];
+                const float4 a4 = na4, e14 = ne14, e24 = ne24;
+                if (r + 1 < count) {
+                    rec += 3;
+                    na4 = rec[0];
+                    ne14 = rec[1];
+                    ne24 = rec[2];
+                }
                 float lam, u, v;
                 if (ray_tri(xyz(a4), xyz(e14), xyz(e24), o, d, lam, u, v) && lam > eps) {
                     int32_t tri = (int32_t)__float_as_uint(a4.w);
@@ -194,11 +205,25 @@ __device__ __forceinline__ bool trace_any(const uint2* __restrict__ nodes, const
     for (;;) {
         uint2 nd = descend(nodes, st, sp, node, tmin, tmax, o, d, inv);
         uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
+        // the next triangle's record is loaded before the current one is tested
+        const float4* rec = recs + 3ull * begin;
+        float4 na4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ne14 = na4, ne24 = na4;
+        if (count > 0) {
+            na4 = rec[0];
+            ne14 = rec[1];
+            ne24 = rec[2];
+        }
         for (uint32_t r = 0; r < count; ++r) {
-            const float4* rec = recs + 3ull * (begin + r);
+            const float4 a4 = na4, e14 = ne14, e24 = ne24;
+            if (r + 1 < count) {
+                rec += 3;
+                na4 = rec[0];
+                ne14 = rec[1];
+                ne24 = rec[2];
+            }
             float lam, u, v;
-            if (ray_tri(xyz(rec[0]), xyz(rec[1]), xyz(rec[2]), o, d, lam, u, v) && lam > eps &&
-                lam < tlim && lam + slack < dist)
+            if (ray_tri(xyz(a4), xyz(e14), xyz(e24), o, d, lam, u, v) && lam > eps && lam < tlim &&
+                lam + slack < dist)
                 return true;
         }
         if (sp == 0) return false;
