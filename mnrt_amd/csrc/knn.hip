@@ -10,10 +10,12 @@
 //     words are scalar loads) over the query-time node words, in which every subtree of
 //     at most kQueryLeafRefs photons is one leaf; far children go onto a per-wave LDS stack with
 //     their box distance (dev_RangeSearch pushes them by plane distance,
-//     photon_knn.cu:146-171).  Rounding is monotone, so the fp32 box distance never
-//     exceeds the fp32 distance of a photon inside the box and pruning by it is exact;
+//     photon_knn.cu:146-171).  The fp32 box distance (the new axis' squared offset plus
+//     the sum of the other two) can exceed a photon's fp32 distance only by a few
+//     roundings, so pruning against the limit times (1 + 2^-20) drops only nodes none of
+//     whose photons can be taken: exact;
 //   * a leaf's photon records are contiguous, so its photons are tested one per lane
-//     with one coalesced 16-byte load each;
+//     with one coalesced 16-byte buffer load each (bounded at the leaf's end);
 //   * candidates (key = d2 bits << 32 | photon index, so integer order is (d2, index)
 //     order) are appended to a per-wave LDS buffer with ballot + mbcnt compaction; when
 //     k are known or the buffer would overflow, the k best are kept (radix selection of
@@ -42,6 +44,11 @@ namespace mnrt {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kKnnStack = 64;
+// relative margin of the node pruning limit over the photon limit (see knn_query)
+constexpr float kPruneMargin = 1.0f + 0x1p-20f;
+// word 3 of a raw (untyped, stride 0) buffer descriptor on gfx9-family parts: 32-bit data
+// format; reads at or past num_records bytes return zero
+constexpr int kBufferWord3 = 0x00020000;
 
 enum : int { kModeRadii = 0, kModeRadiance = 1, kModeBackward = 2 };
 
@@ -241,6 +248,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     const float4 qp4 = A.pos[q];
     const float qx = uniformf(qp4.x), qy = uniformf(qp4.y), qz = uniformf(qp4.z);
     const f3 qp = mk3(qx, qy, qz);
+    const uint32_t lane32 = lane * 32u;  // this lane's byte offset in a 64-record leaf batch
     const uint32_t k = A.k;
     const float rq = A.r_query ? uniformf(A.r_query[q]) : Tr.r_max;
     const float r2max = A.r_query ? rq * rq : Tr.r2max;
@@ -251,12 +259,32 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
 
     uint32_t cnt = 0;        // candidates in the buffer
     bool full = false;       // the buffer's first k are the k best so far
-    uint64_t thr_key = ~0ull;
-    float thr_d2 = kInf;
+    float thr_d2 = kInf;     // with full: the k best so far lie at d2 <= thr_d2
     int sp = 0;
     uint32_t node = 0;
-    // per-axis distance from the query to the current node's box (0 inside)
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    // Squared per-axis distances from the query to the current node's box (0 inside) and,
+    // per split axis, the sum of the other two: a far child's box distance is
+    // diff^2 + rest[a].  A photon's d2 sums its terms in x, y, z order; the box distance's
+    // other order can round a few units in the last place differently, so the pruning
+    // limit plim carries a relative margin of 2^-20 (far above those roundings): a node is
+    // dropped only when every photon in it lies beyond the limit.
+    float ox2 = 0.0f, oy2 = 0.0f, oz2 = 0.0f;
+    // lanes 0-2 hold the query's x, y, z and the three rest sums: a split axis picks its
+    // value with one lane read
+    const float qlane = lane == 0 ? qx : (lane == 1 ? qy : qz);
+    float rlane = 0.0f;
+    // a photon is taken when d2 < tlim (r2lim, and with full also d2 bits <= thr_d2 bits:
+    // a superset of the keys below the k-th; the exact selections trim it)
+    float tlim = r2lim, plim = r2lim * kPruneMargin;
+    auto set_limits = [&]() {
+        tlim = r2lim;
+        if (full) {
+            const uint32_t tb = __float_as_uint(thr_d2);
+            tlim = fminf(r2lim, __uint_as_float(tb < 0x7F800000u ? tb + 1u : tb));
+        }
+        plim = (full ? fminf(r2lim, thr_d2) : r2lim) * kPruneMargin;
+    };
+    auto set_rest = [&]() { rlane = lane == 0 ? oy2 + oz2 : (lane == 1 ? ox2 + oz2 : ox2 + oy2); };
 
     // sorts the buffer and keeps the first `keep`
     auto sort_keep = [&](uint32_t keep) {
@@ -270,8 +298,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         if (cnt > keep) cnt = keep;
         if (cnt == k) {
             full = true;
-            thr_key = keys[k - 1];
-            thr_d2 = __uint_as_float((uint32_t)(thr_key >> 32));
+            thr_d2 = __uint_as_float((uint32_t)(keys[k - 1] >> 32));
+            set_limits();
         }
     };
     // keeps the k best, unsorted (cnt > k)
@@ -302,8 +330,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         wave_sync();
         cnt = k;
         full = true;
-        thr_key = kth;
         thr_d2 = __uint_as_float((uint32_t)(kth >> 32));
+        set_limits();
     };
     // keeps at least k and at most CAP - 64 candidates under a high-word threshold
     auto window_keep = [&]() {
@@ -338,8 +366,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         wave_sync();
         cnt = upper;
         full = true;
-        thr_key = ((uint64_t)T << 32) | 0xFFFFFFFFull;
         thr_d2 = __uint_as_float(T);
+        set_limits();
     };
     auto select = [&](uint32_t keep) {
         if (kSel == kSelWindow) window_keep();
@@ -354,56 +382,51 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
             nd.x = uniform(nd.x);
             nd.y = uniform(nd.y);
             while (!is_leaf(nd.y)) {
-                uint32_t a = nd.y >> 30;
-                float s = __uint_as_float(nd.x);
-                float qa = comp(qp, a);
-                float diff = qa - s;
-                uint32_t left = node + 1, right = nd.y & MNRT_INDEX_MASK;
-                uint32_t nearc = left, farc = right;
-                if (qa > s) {
-                    nearc = right;
-                    farc = left;
-                }
+                const uint32_t a = nd.y >> 30;
+                const float s = __uint_as_float(nd.x);
+                const float qa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qlane), (int)a));
+                const float rest = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rlane), (int)a));
+                const float diff = qa - s;  // > 0 exactly when qa > s
+                const float d2a = diff * diff;
                 // far child's box distance: the split plane replaces this axis' offset
-                float fx = a == 0 ? diff : ox, fy = a == 1 ? diff : oy, fz = a == 2 ? diff : oz;
-                float bd2 = fx * fx + fy * fy + fz * fz;
-                if (bd2 < r2lim && !(full && bd2 > thr_d2) && sp < kKnnStack) {
+                const float bd2 = d2a + rest;
+                const uint32_t left = node + 1, right = nd.y & MNRT_INDEX_MASK;
+                const bool gr = diff > 0.0f;
+                if (bd2 <= plim && sp < kKnnStack) {
                     if (lane == 0) {
-                        snode[sp] = farc;
-                        soff[sp] = make_float4(fx, fy, fz, bd2);
+                        snode[sp] = gr ? left : right;
+                        soff[sp] = make_float4(a == 0 ? d2a : ox2, a == 1 ? d2a : oy2, a == 2 ? d2a : oz2, bd2);
                     }
                     ++sp;
                 }
-                node = uniform(nearc);
+                node = uniform(gr ? right : left);
                 nd = ld_uniform(Tr.nodes, node);
                 nd.x = uniform(nd.x);
                 nd.y = uniform(nd.y);
             }
             wave_sync();
             const uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
-            const float4* lrec = Tr.recs + 2ull * begin;  // the leaf's records: 32-bit offsets from here
+            // the leaf's records (32 B each, position and photon index first) through a buffer
+            // descriptor that ends with the leaf: lanes past its end read zeros and are masked out
+            const __amdgpu_buffer_rsrc_t lr = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(Tr.recs + 2ull * begin), (short)0, (int)(count * 32u), kBufferWord3);
             for (uint32_t base = 0; base < count; base += 64) {
-                const uint32_t j = base + lane;
-                const bool in = j < count;
-                // lanes past the leaf re-read its last record (no branch around the load)
-                const float4 p = lrec[2u * (in ? j : count - 1u)];
-                const f3 df = sub3(xyz(p), qp);
-                const float d2 = dot3(df, df);
-                uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-                bool take = in && (d2 < r2lim) && (key < thr_key);
-                uint64_t m = ballot(take);
+                const auto w = __builtin_amdgcn_raw_buffer_load_b128(lr, (int)(lane32 + base * 32u), 0, 0);
+                const float dx = __uint_as_float(w[0]) - qx, dy = __uint_as_float(w[1]) - qy,
+                            dz = __uint_as_float(w[2]) - qz;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                uint64_t m = ballot(d2 < tlim) & ballot(lane < count - base);
                 uint32_t nm = (uint32_t)__popcll(m);
                 if (nm == 0) continue;
                 if (cnt + nm > (uint32_t)CAP) {
                     select(k);
-                    take = take && (key < thr_key);
-                    m = ballot(take);
+                    m &= ballot(d2 < tlim);
                     nm = (uint32_t)__popcll(m);
                 }
-                if (take) {
-                    uint32_t slot = cnt + mask_prefix(m);
-                    keys[slot] = key;
-                    kpos[slot] = begin + j;
+                if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+                    const uint32_t slot = cnt + mask_prefix(m);
+                    keys[slot] = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)w[3];
+                    kpos[slot] = begin + base + lane;
                 }
                 cnt += nm;
                 wave_sync();
@@ -413,11 +436,12 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
             while (sp > 0) {
                 --sp;
                 const float4 f = soff[sp];
-                if (full && uniformf(f.w) > thr_d2) continue;
+                if (uniformf(f.w) > plim) continue;
                 node = uniform(snode[sp]);
-                ox = uniformf(f.x);
-                oy = uniformf(f.y);
-                oz = uniformf(f.z);
+                ox2 = uniformf(f.x);
+                oy2 = uniformf(f.y);
+                oz2 = uniformf(f.z);
+                set_rest();
                 found = true;
                 break;
             }
@@ -429,11 +453,12 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         r2lim = r2max;
         cnt = 0;
         full = false;
-        thr_key = ~0ull;
         thr_d2 = kInf;
+        set_limits();
         sp = 0;
         node = 0;
-        ox = oy = oz = 0.0f;
+        ox2 = oy2 = oz2 = 0.0f;
+        rlane = 0.0f;
     }
     if (kSel != kSelSort && cnt > k) radix_keep();
     // the buffer now holds exactly the set; it is sorted only for callers of the lists
