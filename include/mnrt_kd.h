@@ -311,7 +311,8 @@ int mnrt_trace_shadow(const mnrt_tree* tree, const mnrt_light* light, uint32_t n
  * d_io_inv_dist (with d_orig4, the source ray origins): slot += 1 / |position - origin|
  * (the reference's d_ioInvDistAccu), or NULL.  Optional outputs per query: d_out_r2
  * (the r2 used), d_out_irr4 (E), d_out_nbr (n*k photon indices sorted by (d2, index),
- * 0xFFFFFFFF past the count), d_out_count (neighbours found). */
+ * 0xFFFFFFFF past the count), d_out_count (neighbours found).  Photon trees of 2^29 nodes
+ * or more are refused (MNRT_ERR_CAPACITY). */
 int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
                     const float* d_pos4, const float* d_nrm4, const float* d_dir4,
                     const float* d_infl4, const float* d_clr4, const uint32_t* d_idx_sp,

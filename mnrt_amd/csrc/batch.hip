@@ -319,7 +319,7 @@ int fill_scenes(mnrt_batch& b, const mnrt_tree* tri, const mnrt_tree* pm, const 
         if (knn) {
             const mnrt_tree& p = pm[s];
             if (p.kind != MNRT_KIND_POINT || !p.d_nodes || !p.d_recs || p.num_nodes == 0) return MNRT_ERR_STATE;
-            if (p.max_level > MNRT_MAX_LEVEL) return MNRT_ERR_CAPACITY;
+            if (p.max_level > MNRT_MAX_LEVEL || p.num_nodes >= kUniform8MaxIndex) return MNRT_ERR_CAPACITY;
             KnnScene& kq = b.h_knn[s];
             kq.nodes = (const uint2*)(p.d_qnodes ? p.d_qnodes : p.d_nodes);
             kq.recs = (const float4*)p.d_recs;

@@ -75,6 +75,14 @@ __device__ __forceinline__ void wave_bitonic(uint64_t* keys, uint32_t* pos, uint
     }
 }
 
+// x rounded to the nearest integer, ties to even (as __double2ll_rn), for |x| < 2^51: in
+// [2^52, 2^53) doubles are spaced 1 apart, so adding 1.5 * 2^52 rounds x itself, and the
+// sum's bits differ from the constant's by exactly that integer.
+__device__ __forceinline__ long long round_to_i64(double x) {
+    constexpr double kMagic = 0x1.8p52;
+    return (long long)(__double_as_longlong(x + kMagic) - __double_as_longlong(kMagic));
+}
+
 // Exact sum of one float per lane slot (RK slots per lane, unused slots 0) as a 64-bit
 // fixed-point integer: with M the largest |v| (M = f 2^e, f in [0.5, 1)), every term is
 // rounded to a multiple of 2^(e-40) (exact for terms down to 2^-17 M), the integers are
@@ -92,7 +100,7 @@ __device__ __forceinline__ float wave_fixed_sum(const float (&v)[RK]) {
     const int sh = 40 - ex;
     long long acc = 0;
 #pragma unroll
-    for (int t = 0; t < RK; ++t) acc += __double2ll_rn(ldexp((double)v[t], sh));
+    for (int t = 0; t < RK; ++t) acc += round_to_i64(ldexp((double)v[t], sh));
     acc = wave_sum_i64(acc);
     return (float)ldexp((double)acc, -sh);
 }
@@ -303,8 +311,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         }
     };
     // keeps the k best, unsorted (cnt > k)
-    auto radix_keep = [&]() {
-        constexpr int R = CAP / 64;
+    auto radix_keep_slots = [&](auto slots) {
+        constexpr int R = decltype(slots)::value;
         uint64_t key[R];
         uint32_t pos[R];
 #pragma unroll
@@ -319,7 +327,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
 #pragma unroll
         for (int t = 0; t < R; ++t) {
             const bool keepit = key[t] <= kth;
-            const uint64_t m = __ballot(keepit);
+            const uint64_t m = ballot(keepit);
             if (keepit) {
                 const uint32_t slot = base + mask_prefix(m);
                 keys[slot] = key[t];
@@ -332,6 +340,15 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         full = true;
         thr_d2 = __uint_as_float((uint32_t)(kth >> 32));
         set_limits();
+    };
+    // keeps the k best, unsorted (cnt > k); searches only the ceil(cnt / 64) key slots per
+    // lane that hold candidates
+    auto radix_keep = [&]() {
+        constexpr int R = CAP / 64;
+        if (R > 2 && cnt <= 128u) radix_keep_slots(std::integral_constant<int, (R > 2 ? 2 : R)>{});
+        else if (R > 3 && cnt <= 192u) radix_keep_slots(std::integral_constant<int, (R > 3 ? 3 : R)>{});
+        else if (R > 4 && cnt <= 256u) radix_keep_slots(std::integral_constant<int, (R > 4 ? 4 : R)>{});
+        else radix_keep_slots(std::integral_constant<int, R>{});
     };
     // keeps at least k and at most CAP - 64 candidates under a high-word threshold
     auto window_keep = [&]() {
@@ -378,7 +395,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     for (;;) {  // a second pass with r_max only if the warm-start bound held fewer than k
         for (;;) {
             node = uniform(node);
-            uint2 nd = ld_uniform(Tr.nodes, node);
+            uint2 nd = ld_uniform8(Tr.nodes, node);
             nd.x = uniform(nd.x);
             nd.y = uniform(nd.y);
             while (!is_leaf(nd.y)) {
@@ -392,15 +409,19 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
                 const float bd2 = d2a + rest;
                 const uint32_t left = node + 1, right = nd.y & MNRT_INDEX_MASK;
                 const bool gr = diff > 0.0f;
-                if (bd2 <= plim && sp < kKnnStack) {
+                // entries are far children of distinct levels of the current path, so a
+                // tree of max_level <= MNRT_MAX_LEVEL < kKnnStack never fills the stack; the
+                // slot clamp only keeps a malformed tree's writes inside it
+                if (bd2 <= plim) {
                     if (lane == 0) {
-                        snode[sp] = gr ? left : right;
-                        soff[sp] = make_float4(a == 0 ? d2a : ox2, a == 1 ? d2a : oy2, a == 2 ? d2a : oz2, bd2);
+                        const int e = min(sp, kKnnStack - 1);
+                        snode[e] = gr ? left : right;
+                        soff[e] = make_float4(a == 0 ? d2a : ox2, a == 1 ? d2a : oy2, a == 2 ? d2a : oz2, bd2);
                     }
                     ++sp;
                 }
                 node = uniform(gr ? right : left);
-                nd = ld_uniform(Tr.nodes, node);
+                nd = ld_uniform8(Tr.nodes, node);
                 nd.x = uniform(nd.x);
                 nd.y = uniform(nd.y);
             }
@@ -435,9 +456,10 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
             bool found = false;
             while (sp > 0) {
                 --sp;
-                const float4 f = soff[sp];
+                const int e = min(sp, kKnnStack - 1);
+                const float4 f = soff[e];
                 if (uniformf(f.w) > plim) continue;
-                node = uniform(snode[sp]);
+                node = uniform(snode[e]);
                 ox2 = uniformf(f.x);
                 oy2 = uniformf(f.y);
                 oz2 = uniformf(f.z);
@@ -1135,7 +1157,7 @@ static int knn_check(const mnrt_tree* pmap, uint32_t k) {
     if (k == 0 || k > MNRT_KNN_MAX_K) return MNRT_ERR_ARG;
     if (pmap->kind != MNRT_KIND_POINT || !pmap->d_nodes || !pmap->d_recs || pmap->num_nodes == 0)
         return MNRT_ERR_STATE;
-    if (pmap->max_level > MNRT_MAX_LEVEL) return MNRT_ERR_CAPACITY;
+    if (pmap->max_level > MNRT_MAX_LEVEL || pmap->num_nodes >= mnrt::kUniform8MaxIndex) return MNRT_ERR_CAPACITY;
     return MNRT_OK;
 }
 

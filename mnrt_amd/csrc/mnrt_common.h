@@ -214,6 +214,17 @@ __device__ __forceinline__ T ld_uniform(const T* p, uint32_t i) {
     return p[i];
 #endif
 }
+// ld_uniform of an 8-byte node word pair at a 32-bit byte offset: one scalar load with an
+// SGPR offset, no 64-bit address arithmetic.  Valid for i < kUniform8MaxIndex.
+constexpr uint32_t kUniform8MaxIndex = 1u << 29;
+__device__ __forceinline__ uint2 ld_uniform8(const uint2* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __attribute__((address_space(4))) char* c = (const __attribute__((address_space(4))) char*)p;
+    return *(const __attribute__((address_space(4))) uint2*)(c + (i << 3));
+#else
+    return p[i];
+#endif
+}
 __device__ __forceinline__ float uniformf(float v) {
     return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
 }

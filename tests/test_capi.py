@@ -57,6 +57,13 @@ def test_argument_errors_without_device():
     t.kind = N.KIND_POINT
     light.type = N.LIGHT_POINT
     assert L.mnrt_trace_shadow(ctypes.byref(t), ctypes.byref(light), 1, fake, None, 0.0, fake, None) == N.ERR_STATE
+    # photon trees of 2^29 nodes or more (node words addressed by 32-bit byte offsets)
+    t.d_nodes, t.d_recs, t.max_level = fake, fake, 20
+    t.num_nodes = (1 << 29) - 1
+    assert L.mnrt_knn_gather(ctypes.byref(t), 5, 1.0, 0, *([None] * 15)) == N.OK  # n = 0: nothing to do
+    t.num_nodes = 1 << 29
+    assert L.mnrt_knn_gather(ctypes.byref(t), 5, 1.0, 1, *([None] * 15)) == N.ERR_CAPACITY
+    assert L.mnrt_knn_radii(ctypes.byref(t), 5, 1.0, 1, fake, fake, None) == N.ERR_CAPACITY
 
 
 def test_reweight_argument_errors_without_device():

@@ -214,3 +214,18 @@ def test_oct_quotient_by_reciprocal_product():
     ieee = q.astype(np.float32) / np.float32(32767.0)
     prod = (q.astype(np.float64) * (1.0 / 32767.0)).astype(np.float32)
     assert np.array_equal(ieee.view(np.uint32), prod.view(np.uint32))
+
+
+def test_fixed_point_rounding_by_magic_constant():
+    """The kernels' fixed-point terms round x (|x| < 2^41 there) to an int64 by adding
+    1.5 * 2^52 and subtracting the constant's bits; this equals round-half-even (what the
+    oracle's llrint does) for every |x| < 2^51, ties and negative values included."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([
+        rng.uniform(-2.0**41, 2.0**41, 200000),
+        np.arange(-1000, 1000) + 0.5,  # exact ties
+        np.array([0.0, -0.0, 0.49999999999999994, -0.5, 1.5, 2.5, 2.0**50 + 0.5, -(2.0**51) + 1.5]),
+    ])
+    magic = np.float64(1.5 * 2.0**52)
+    got = (x + magic).view(np.int64) - magic.view(np.int64)
+    assert np.array_equal(got, np.rint(x).astype(np.int64))
