@@ -83,25 +83,32 @@ __device__ __forceinline__ long long round_to_i64(double x) {
     return (long long)(__double_as_longlong(x + kMagic) - __double_as_longlong(kMagic));
 }
 
-// Exact sum of one float per lane slot (RK slots per lane, unused slots 0) as a 64-bit
+// Exact sum of one float per lane slot (RK <= 4 slots per lane, unused slots 0) as a 64-bit
 // fixed-point integer: with M the largest |v| (M = f 2^e, f in [0.5, 1)), every term is
 // rounded to a multiple of 2^(e-40) (exact for terms down to 2^-17 M), the integers are
-// added in any order (|sum| < 2^48) and the total is rounded to float once.  Wave-uniform
-// result; the same in every lane.
+// added in any order (|sum| < 2^48) and the total is rounded to float once.  Each term
+// q (|q| <= 2^40) is added as q = hi 2^24 + lo, lo in [0, 2^24): over at most 256 terms both
+// parts' sums fit 32 bits, so the wave adds two 32-bit words instead of one 64-bit word.
+// Finite inputs; wave-uniform result, the same in every lane.
 template <int RK>
 __device__ __forceinline__ float wave_fixed_sum(const float (&v)[RK]) {
-    float m = 0.0f;
+    static_assert(RK <= 4, "32-bit part sums hold at most 256 terms");
+    uint32_t mb = 0;  // max |v| as bits: non-negative floats order as their bits
 #pragma unroll
-    for (int t = 0; t < RK; ++t) m = fmaxf(m, fabsf(v[t]));
-    m = wave_max_f32(m);
+    for (int t = 0; t < RK; ++t) mb = max(mb, __float_as_uint(v[t]) & 0x7FFFFFFFu);
+    const float m = __uint_as_float(wave_max_u32(mb));
     if (!(m > 0.0f)) return 0.0f;
     int ex;
     (void)frexpf(m, &ex);
     const int sh = 40 - ex;
-    long long acc = 0;
+    uint32_t lo = 0, hi = 0;
 #pragma unroll
-    for (int t = 0; t < RK; ++t) acc += round_to_i64(ldexp((double)v[t], sh));
-    acc = wave_sum_i64(acc);
+    for (int t = 0; t < RK; ++t) {
+        const long long q = round_to_i64(ldexp((double)v[t], sh));
+        lo += (uint32_t)q & 0xFFFFFFu;
+        hi += (uint32_t)(int32_t)(q >> 24);
+    }
+    const long long acc = (long long)(int32_t)wave_sum_u32(hi) * (1ll << 24) + (long long)wave_sum_u32(lo);
     return (float)ldexp((double)acc, -sh);
 }
 
@@ -163,13 +170,15 @@ __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&key)[R], uint3
         for (int t = 0; t < R; ++t) lo = max(lo, ((uint32_t)(key[t] >> 32) == hi) ? (uint32_t)key[t] : 0u);
         return ((uint64_t)hi << 32) | wave_max_u32(lo);
     }
+    uint64_t eq[R];  // slots whose high word ties (separate ballots: a ballot of a && b costs two more)
+#pragma unroll
+    for (int t = 0; t < R; ++t) eq[t] = ballot((uint32_t)(key[t] >> 32) == hi);
     uint32_t lo = 0;
     for (int b = 31; b >= 0; --b) {
         const uint32_t probe = lo | (1u << b);
         uint32_t c = 0;
 #pragma unroll
-        for (int t = 0; t < R; ++t)
-            c += (uint32_t)__popcll(ballot((uint32_t)(key[t] >> 32) == hi && (uint32_t)key[t] < probe));
+        for (int t = 0; t < R; ++t) c += (uint32_t)__popcll(eq[t] & ballot((uint32_t)key[t] < probe));
         if (c < need) lo = probe;
     }
     return ((uint64_t)hi << 32) | lo;
@@ -371,9 +380,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
         uint32_t base = 0;
 #pragma unroll
         for (int t = 0; t < R; ++t) {
-            const bool keepit = (uint32_t)(key[t] >> 32) <= T && key[t] != ~0ull;
-            const uint64_t m = __ballot(keepit);
-            if (keepit) {
+            const uint64_t m = ballot((uint32_t)(key[t] >> 32) <= T) & ballot(key[t] != ~0ull);
+            if (__builtin_amdgcn_inverse_ballot_w64(m)) {
                 const uint32_t slot = base + mask_prefix(m);
                 keys[slot] = key[t];
                 kpos[slot] = pos[t];
@@ -598,7 +606,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
                     front = dot3(nq, oct_decode(__float_as_uint(pw.w))) < 0.0f;
                     key = (((keys[i] & 0xFFFFFFFFull) + sink->pbase) << 8) | sink->slot;
                 }
-                const uint64_t m = __ballot(front);
+                const uint64_t m = ballot(front);
                 if (front) sink->pairs[np + mask_prefix(m)] = key;
                 np += (uint32_t)__popcll(m);
             }
@@ -897,7 +905,7 @@ __global__ __launch_bounds__(256) void k_bwd_wmax_batch(uint32_t n, const uint32
         }
     }
     const uint32_t s0 = uniform(s);
-    if (__ballot(s != s0) == 0ull) {
+    if (ballot(s != s0) == 0ull) {
         for (uint32_t o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
         if ((threadIdx.x & 63u) == 0 && m > 0.0f) atomicMax(&wmax_bits[s0], __float_as_uint(m));
     } else if (m > 0.0f) {

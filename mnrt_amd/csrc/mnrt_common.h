@@ -166,6 +166,15 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
     return min(min(row_lane(v, 0), row_lane(v, 1)), min(row_lane(v, 2), row_lane(v, 3)));
 }
 
+// Wrapping 32-bit sum (callers keep the true sum below 2^32).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    v += dpp_mov<kDppPairSwap>(v);
+    v += dpp_mov<kDppQuadSwap>(v);
+    v += dpp_mov<kDppHalfMirror>(v);
+    v += dpp_mov<kDppRowMirror>(v);
+    return (row_lane(v, 0) + row_lane(v, 1)) + (row_lane(v, 2) + row_lane(v, 3));
+}
+
 __device__ __forceinline__ float wave_max_f32(float v) {
     v = fmaxf(v, __uint_as_float(dpp_mov<kDppPairSwap>(__float_as_uint(v))));
     v = fmaxf(v, __uint_as_float(dpp_mov<kDppQuadSwap>(__float_as_uint(v))));
