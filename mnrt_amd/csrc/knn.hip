@@ -246,15 +246,19 @@ struct WarmHint {
 };
 
 // Warm-start bound at q from a query at h with k-th distance sqrt(h.r2): every one of
-// that query's k neighbours lies within sqrt(h.r2) + |h - q| of q (margins of 2^-20 cover
-// the rounding of the fp32 distances; the bound is exclusive, hence nextafter).  r2max
-// when the hint has no distance or the bound is not below it.
+// that query's k neighbours lies within sqrt(h.r2) + |h - q| of q.  Computed in fp32 with
+// the hardware square root; margins of 2^-16 per step are far above its and the fp32
+// distances' roundings (a few units of 2^-24), and the bound is exclusive, hence the next
+// float up.  r2max when the hint has no distance or the bound is not below it.  The search
+// stays exact whatever this returns: a bound holding fewer than k photons is searched again
+// with r2max, and one holding k or more holds the k nearest.
 __device__ __forceinline__ float warm_bound(const WarmHint& h, float qx, float qy, float qz, float r2max) {
     if (!(h.r2 >= 0.0f)) return r2max;
-    const double dx = (double)qx - (double)h.x, dy = (double)qy - (double)h.y, dz = (double)qz - (double)h.z;
-    const double b = (sqrt((double)h.r2) + sqrt(dx * dx + dy * dy + dz * dz)) * (1.0 + 0x1p-20);
-    const double b2 = b * b * (1.0 + 0x1p-20);
-    return b2 < (double)r2max ? nextafterf((float)b2, kInf) : r2max;
+    constexpr float kMargin = 1.0f + 0x1p-16f;
+    const float dx = qx - h.x, dy = qy - h.y, dz = qz - h.z;
+    const float b = (__builtin_amdgcn_sqrtf(h.r2) + __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz)) * kMargin;
+    const float b2 = (b * b) * kMargin;
+    return b2 < r2max ? __uint_as_float(__float_as_uint(b2) + 1u) : r2max;
 }
 
 // One query by one wave; keys / kpos: CAP LDS entries, snode / soff: kKnnStack.
@@ -453,7 +457,7 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
                     nm = (uint32_t)__popcll(m);
                 }
                 if (__builtin_amdgcn_inverse_ballot_w64(m)) {
-                    const uint32_t slot = cnt + mask_prefix(m);
+                    const uint32_t slot = mask_prefix_from(m, cnt);
                     keys[slot] = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)w[3];
                     kpos[slot] = begin + base + lane;
                 }

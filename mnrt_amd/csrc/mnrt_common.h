@@ -247,6 +247,10 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_bal
 __device__ __forceinline__ uint32_t mask_prefix(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
+// base + mask_prefix(m), the addition folded into the count instructions.
+__device__ __forceinline__ uint32_t mask_prefix_from(uint64_t m, uint32_t base) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
+}
 
 // Query-time leaves of photon trees: subtrees with at most this many references (the kNN
 // kernel tests a leaf 64 photons per step; measured on 1M photons, k = 100, with the
