@@ -81,6 +81,7 @@ struct KnnArgs {
     double fix_scale;              // accumulator units per unit gradient (a power of two)
     float4* out_grad_clr;          // dLoss/d(diffuse colour) per query
     const uint32_t* order;         // processing order of the queries (null: 0..n-1)
+    const uint32_t* order_far;     // the order's far-pair count (order_coherent), or null
 };
 
 struct KnnScene {

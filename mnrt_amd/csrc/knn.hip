@@ -243,6 +243,11 @@ struct PairSink {
 struct WarmHint {
     float x, y, z;
     float r2;  // previous query's k-th d2; < 0: none (fewer than k found, or first query)
+    // Queries in spatial-cell order (incoherent hits) lie farther apart than the previous
+    // query's k-th distance more often, so that the bound above is loose: they search first
+    // within d_k(prev) + |prev - q| / 2, a guess, and within the bound only when it holds
+    // fewer than k.  Camera hits in tile order lie close together and use the bound.
+    bool guess;
 };
 
 // Warm-start bound at q from a query at h with k-th distance sqrt(h.r2): every one of
@@ -276,7 +281,15 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     // search radius: r_max, or the warm-start bound when smaller (margins of 2^-20 cover
     // the rounding of the fp32 distances; the bound is exclusive, hence nextafter)
     float r2lim = r2max;
-    if (hint && k > 0) r2lim = warm_bound(*hint, qx, qy, qz, r2max);
+    float r2next = r2max;  // the next pass's limit when this one holds fewer than k
+    if (hint && k > 0) {
+        r2lim = r2next = warm_bound(*hint, qx, qy, qz, r2max);
+        if (hint->guess && hint->r2 >= 0.0f) {
+            const float dx = qx - hint->x, dy = qy - hint->y, dz = qz - hint->z;
+            const float g = __builtin_amdgcn_sqrtf(hint->r2) + 0.5f * __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+            r2lim = fminf(r2next, g * g);
+        }
+    }
 
     uint32_t cnt = 0;        // candidates in the buffer
     bool full = false;       // the buffer's first k are the k best so far
@@ -483,8 +496,9 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
             if (!found) break;
         }
         if (cnt >= k || r2lim == r2max) break;
-        // cannot happen when the bound holds; searched again with r_max to stay exact
-        r2lim = r2max;
+        // a guess that held fewer than k: the bound next; the bound holds k (the r_max pass
+        // after it keeps the search exact whatever the roundings)
+        r2lim = r2lim < r2next ? r2next : r2max;
         cnt = 0;
         full = false;
         thr_d2 = kInf;
@@ -670,7 +684,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     if (j0 >= A.n) return;  // uniform per wave
     const uint32_t j1 = (uint32_t)min<uint64_t>(j0 + kKnnChain, A.n);
     const KnnTree Tr{A.nodes, A.recs, A.r2max, A.r_max};
-    WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
+    WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f, A.order_far && !order_coherent(A.n, A.order_far)};
     for (uint32_t j = (uint32_t)j0; j < j1; ++j) {
         const uint32_t q = A.order ? ld_uniform(A.order, j) : j;
         knn_query<CAP, kMode, kSel>(A, Tr, q, lane, s_key[wave], s_pos[wave], s_snode[wave], s_soff[wave], nullptr,
@@ -803,7 +817,7 @@ __global__ __launch_bounds__(256) void k_knn_bwd_chunks(KnnArgs A, uint32_t chun
     if (lane == 0) s_np[wave] = 0;
     wave_sync();
     const KnnTree Tr{A.nodes, A.recs, A.r2max, A.r_max};
-    WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
+    WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f, A.order_far && !order_coherent(A.n, A.order_far)};
     for (uint32_t j = (uint32_t)j0; j < j1; ++j) {
         const uint32_t q = A.order ? ld_uniform(A.order, j) : j;
         const PairSink sink{pairs, &s_np[wave], s_w[wave], j - (uint32_t)j0, 0u, A.fix_scale};
@@ -1204,7 +1218,7 @@ extern "C" int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, u
     A.out_nbr = d_out_nbr;
     A.out_count = d_out_count;
     void* obuf = nullptr;
-    rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &A.order, &obuf);
+    rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &A.order, &obuf, &A.order_far);
     if (rc != MNRT_OK) return rc;
     float4* tmp = nullptr;
     if (d_idx_sp) {
@@ -1278,7 +1292,7 @@ extern "C" int mnrt_knn_gather_noalloc(const mnrt_tree* pmap, uint32_t k, float 
     A.out_nbr = d_out_nbr;
     A.out_count = d_out_count;
     // scratch layout: [processing order][per-query radiance (4 n)][per-query inverse distance (n)]
-    rc = mnrt::spatial_order_into(pmap->root_min, pmap->root_max, n, d_pos4, st, scr, &A.order);
+    rc = mnrt::spatial_order_into(pmap->root_min, pmap->root_max, n, d_pos4, st, scr, &A.order, &A.order_far);
     if (rc != MNRT_OK) return rc;
     float4* tmp = nullptr;
     if (d_idx_sp) {
@@ -1321,7 +1335,7 @@ extern "C" int mnrt_knn_radii(const mnrt_tree* pmap, uint32_t k, float r_max, ui
     A.out_radius = d_out_radius;
     hipStream_t st = (hipStream_t)stream;
     void* obuf = nullptr;
-    rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &A.order, &obuf);
+    rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &A.order, &obuf, &A.order_far);
     if (rc != MNRT_OK) return rc;
     rc = mnrt::launch_knn<mnrt::kModeRadii>(A, st);
     if (obuf && mnrt::pool_free(obuf, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
@@ -1371,11 +1385,14 @@ extern "C" int mnrt_knn_gather_bwd(const mnrt_tree* pmap, uint32_t k, float r_ma
         rc = MNRT_ERR_ARG;  // non-finite upstream gradient
     }
     const uint32_t* order = nullptr;
+    const uint32_t* order_far = nullptr;
     void* obuf = nullptr;
-    if (rc == MNRT_OK) rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &order, &obuf);
+    if (rc == MNRT_OK)
+        rc = mnrt::spatial_order(pmap->root_min, pmap->root_max, n, d_pos4, st, &order, &obuf, &order_far);
     if (rc == MNRT_OK && scale > 0.0) {
         mnrt::KnnArgs A{};
         A.order = order;
+        A.order_far = order_far;
         A.nodes = (const uint2*)(pmap->d_qnodes ? pmap->d_qnodes : pmap->d_nodes);
         A.recs = (const float4*)pmap->d_recs;
         A.k = k;

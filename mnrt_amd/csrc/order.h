@@ -105,10 +105,12 @@ inline size_t spatial_order_words(uint32_t n) {
 
 // Processing order of n points at d_pos4 in the box lo3 .. hi3, built in buf
 // (spatial_order_words(n) device words): *order = null below kOrderMinItems, else a pointer
-// into buf valid until buf is reused.
+// into buf valid until buf is reused; *far (when given) the device word order_coherent reads
+// (null with no order).
 inline int spatial_order_into(const float* lo3, const float* hi3, uint32_t n, const float* d_pos4, hipStream_t st,
-                              uint32_t* buf, const uint32_t** order) {
+                              uint32_t* buf, const uint32_t** order, const uint32_t** far = nullptr) {
     *order = nullptr;
+    if (far) *far = nullptr;
     if (n < kOrderMinItems) return MNRT_OK;
     uint32_t* cell = buf;
     uint32_t* ord = cell + n;
@@ -136,19 +138,21 @@ inline int spatial_order_into(const float* lo3, const float* hi3, uint32_t n, co
                        (const uint32_t*)far_pairs, cursor, ord);
     MNRT_HIP_CHECK(hipGetLastError());
     *order = ord;
+    if (far) *far = far_pairs;
     return MNRT_OK;
 }
 
 // As spatial_order_into with the buffer from the device pool: *buf = the allocation to
 // return with pool_free after the launch that reads the order (null below kOrderMinItems).
 inline int spatial_order(const float* lo3, const float* hi3, uint32_t n, const float* d_pos4, hipStream_t st,
-                         const uint32_t** order, void** buf) {
+                         const uint32_t** order, void** buf, const uint32_t** far = nullptr) {
     *order = nullptr;
     *buf = nullptr;
+    if (far) *far = nullptr;
     const size_t words = spatial_order_words(n);
     if (words == 0) return MNRT_OK;
     if (pool_alloc(buf, words * 4, st) != hipSuccess) return MNRT_ERR_NOMEM;
-    return spatial_order_into(lo3, hi3, n, d_pos4, st, (uint32_t*)*buf, order);
+    return spatial_order_into(lo3, hi3, n, d_pos4, st, (uint32_t*)*buf, order, far);
 }
 
 }  // namespace
