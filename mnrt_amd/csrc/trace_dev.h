@@ -51,7 +51,7 @@ __host__ __device__ constexpr uint32_t trace_lds_bytes(int lds, uint32_t block) 
 template <int kStack, int kLds>
 struct LaneStack {
     static constexpr int kPriv = kStack > kLds ? kStack - kLds : 1;
-    uint32_t* lds;  // this lane's column: lds_base + lane
+    __attribute__((address_space(3))) uint32_t* lds;  // this lane's column: lds_base + lane
     uint32_t stride;
     uint32_t pnode[kPriv];
     float pmin[kPriv], pmax[kPriv];
@@ -115,6 +115,30 @@ __device__ __forceinline__ uint2 descend(const uint2* __restrict__ nodes, LaneSt
     return nd;
 }
 
+// Calls test(a, e1, e2) on the count triangle records from rec (3 float4 each) until it
+// returns true; returns whether it did.  Two record buffers alternate, so the next record
+// loads while the current one is tested and no registers are copied between tests; the
+// loads are unconditional (past the leaf's end they re-read its last record, which is
+// not tested again), so the wait before a test leaves the next record's loads in flight.
+template <typename F>
+__device__ __forceinline__ bool for_leaf_tris(const float4* rec, uint32_t count, F&& test) {
+    if (count == 0) return false;
+    float4 a0 = rec[0], b0 = rec[1], c0 = rec[2];
+    const uint32_t last = count - 1u;
+    for (uint32_t r = 0;; r += 2) {
+        const float4* p1 = rec + 3u * min(r + 1u, last);
+        const float4 a1 = p1[0], b1 = p1[1], c1 = p1[2];
+        if (test(a0, b0, c0)) return true;
+        if (r + 1 >= count) return false;
+        const float4* p2 = rec + 3u * min(r + 2u, last);
+        a0 = p2[0];
+        b0 = p2[1];
+        c0 = p2[2];
+        if (test(a1, b1, c1)) return true;
+        if (r + 2 >= count) return false;
+    }
+}
+
 // Closest hit.  lds_lane: this lane's column of the workgroup's stack area (kLds > 0),
 // stride = lanes in the workgroup.
 template <int kStack = kTraceStack, int kLds = 0>
@@ -129,33 +153,18 @@ __device__ __forceinline__ Hit trace_ray(const uint2* __restrict__ nodes, const 
         t1 = t1 + eps;
         best_t = t1;
         LaneStack<kStack, kLds> st;
-        st.lds = lds_lane;
+        st.lds = (__attribute__((address_space(3))) uint32_t*)lds_lane;
         st.stride = stride;
         int sp = 0;
         uint32_t node = 0;
         float tmin = t0, tmax = t1;
         for (;;) {
             uint2 nd = descend(nodes, st, sp, node, tmin, tmax, o, d, inv);
-            uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
-            // the next triangle's record is loaded before the current one is tested
-            const float4* rec = recs + 3ull * begin;
-            float4 na4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ne14 = na4, ne24 = na4;
-            if (count > 0) {
-                na4 = rec[0];
-                ne14 = rec[1];
-                ne24 = rec[2];
-            }
-            for (uint32_t r = 0; r < count; ++r) {
-                const float4 a4 = na4, e14 = ne14, e24 = ne24;
-                if (r + 1 < count) {
-                    rec += 3;
-                    na4 = rec[0];
-                    ne14 = rec[1];
-                    ne24 = rec[2];
-                }
+            const uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
+            for_leaf_tris(recs + 3ull * begin, count, [&](const float4& a4, const float4& e14, const float4& e24) {
                 float lam, u, v;
                 if (ray_tri(xyz(a4), xyz(e14), xyz(e24), o, d, lam, u, v) && lam > eps) {
-                    int32_t tri = (int32_t)__float_as_uint(a4.w);
+                    const int32_t tri = (int32_t)__float_as_uint(a4.w);
                     if (lam < best_t || (lam == best_t && best_tri >= 0 && tri < best_tri)) {
                         best_t = lam;
                         best_tri = tri;
@@ -163,7 +172,8 @@ __device__ __forceinline__ Hit trace_ray(const uint2* __restrict__ nodes, const 
                         bv = v;
                     }
                 }
-            }
+                return false;
+            });
             while (sp > 0 && best_tri >= 0 && best_t < st.tmin_of(sp - 1)) --sp;
             if (sp == 0) break;
             --sp;
@@ -197,35 +207,20 @@ __device__ __forceinline__ bool trace_any(const uint2* __restrict__ nodes, const
     if (!ray_box(root.lo, root.hi, o, inv, eps, kInf, t0, t1)) return false;
     const float tlim = fminf(t1, dist) + eps;
     LaneStack<kStack, kLds> st;
-    st.lds = lds_lane;
+    st.lds = (__attribute__((address_space(3))) uint32_t*)lds_lane;
     st.stride = stride;
     int sp = 0;
     uint32_t node = 0;
     float tmin = t0, tmax = tlim;
     for (;;) {
         uint2 nd = descend(nodes, st, sp, node, tmin, tmax, o, d, inv);
-        uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
-        // the next triangle's record is loaded before the current one is tested
-        const float4* rec = recs + 3ull * begin;
-        float4 na4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ne14 = na4, ne24 = na4;
-        if (count > 0) {
-            na4 = rec[0];
-            ne14 = rec[1];
-            ne24 = rec[2];
-        }
-        for (uint32_t r = 0; r < count; ++r) {
-            const float4 a4 = na4, e14 = ne14, e24 = ne24;
-            if (r + 1 < count) {
-                rec += 3;
-                na4 = rec[0];
-                ne14 = rec[1];
-                ne24 = rec[2];
-            }
-            float lam, u, v;
-            if (ray_tri(xyz(a4), xyz(e14), xyz(e24), o, d, lam, u, v) && lam > eps && lam < tlim &&
-                lam + slack < dist)
-                return true;
-        }
+        const uint32_t count = nd.x, begin = nd.y & MNRT_INDEX_MASK;
+        if (for_leaf_tris(recs + 3ull * begin, count, [&](const float4& a4, const float4& e14, const float4& e24) {
+                float lam, u, v;
+                return ray_tri(xyz(a4), xyz(e14), xyz(e24), o, d, lam, u, v) && lam > eps && lam < tlim &&
+                       lam + slack < dist;
+            }))
+            return true;
         if (sp == 0) return false;
         --sp;
         st.get(sp, node, tmin, tmax);
