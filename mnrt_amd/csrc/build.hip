@@ -561,19 +561,29 @@ __global__ __launch_bounds__(256) void k_plan(DN dn_, NodeView A, const uint32_t
     MNRT_GRID_STRIDE(p, n) plan_node<kWrite>(p, A, tlo, thi, ratio, P, base, b0, B, counters);
 }
 
+// Flags per element: left child, right child and, for triangles, straddling the split on
+// the left / right side (fl[2 ne + e], fl[3 ne + e]); one scan over them gives each element
+// side its child rank and each straddling side its slot in the clip queue.
 __global__ __launch_bounds__(256) void k_classify(DN dne, ElemView E, int kind, PlanView P,
                                                   uint32_t* __restrict__ fl) {
     const uint32_t ne = dne.get();
     MNRT_GRID_STRIDE(e, ne) {
         uint32_t p = E.node[e];
-        bool L = false, R = false;
+        bool L = false, R = false, straddle = false;
         if (!P.leaf[p]) {
             float4 lo = E.lo[e];
             float4 hi = (kind == MNRT_KIND_TRI) ? E.hi[e] : lo;
-            classify(kind, E.idx[e], lo, hi, P.axis[p], P.split[p], L, R);
+            const uint32_t axis = P.axis[p];
+            const float s = P.split[p];
+            classify(kind, E.idx[e], lo, hi, axis, s, L, R);
+            straddle = (kind == MNRT_KIND_TRI) && (f4c(lo, axis) < s && s < f4c(hi, axis));
         }
         fl[e] = L ? 1u : 0u;
         fl[ne + e] = R ? 1u : 0u;
+        if (kind == MNRT_KIND_TRI) {
+            fl[2 * ne + e] = (straddle && L) ? 1u : 0u;
+            fl[3 * ne + e] = (straddle && R) ? 1u : 0u;
+        }
     }
 }
 
@@ -699,65 +709,110 @@ __global__ __launch_bounds__(256) void k_leaf_copy(DN dn_, NodeView A, PlanView 
 }
 
 // Scatter each element into its children: small roots append to the small-element
-// buffer, large children to the next level's arrays.  Straddling triangles are clipped
-// to each child box (kernel_PerformSplitClipping, kdtree_tri.cu:389-489).
+// buffer, large children to the next level's arrays.  A straddling triangle's bounds in a
+// child are its clip to the child box (kernel_PerformSplitClipping, kdtree_tri.cu:389-489),
+// which k_clip computes: each straddling element side has a slot in the clip queue from
+// the classification scan (pos[2 ne + side ne + e] - pos[2 ne]), where this writes the
+// side's job (e << 1 | side), or kNone when the side is not scattered, so that the clips
+// run 64 to a wave rather than one pass of a whole wave per straddler.
 __device__ __forceinline__ void scatter_one(uint32_t e, uint32_t n, uint32_t ne, const ElemView& E,
                                             const Geometry& G, const NodeView& A, const PlanView& P,
                                             const ChildView& Cv, const uint32_t* __restrict__ pos,
                                             uint32_t small_elem_base, const ElemView& Sm, uint32_t sm_cap,
-                                            const ElemView& Nx, uint32_t nx_cap, uint32_t* __restrict__ counters) {
-    uint32_t p = E.node[e];
-    if (P.leaf[p] || P.noprog[p]) return;
-    uint32_t idx = E.idx[e];
-    float4 lo = E.lo[e];
-    float4 hi = (G.kind == MNRT_KIND_TRI) ? E.hi[e] : lo;
-    uint32_t axis = P.axis[p];
-    float s = P.split[p];
+                                            const ElemView& Nx, uint32_t nx_cap, uint32_t* __restrict__ counters,
+                                            uint32_t* __restrict__ clipq) {
+    const uint32_t p = E.node[e];
+    if (P.leaf[p]) return;  // no flags: k_classify skips leaves
+    const uint32_t idx = E.idx[e];
+    const float4 lo = E.lo[e];
+    const float4 hi = (G.kind == MNRT_KIND_TRI) ? E.hi[e] : lo;
+    const uint32_t axis = P.axis[p];
+    const float s = P.split[p];
     bool L, R;
     classify(G.kind, idx, lo, hi, axis, s, L, R);
-    bool straddle = (G.kind == MNRT_KIND_TRI) && (f4c(lo, axis) < s && s < f4c(hi, axis));
+    const bool straddle = (G.kind == MNRT_KIND_TRI) && (f4c(lo, axis) < s && s < f4c(hi, axis));
+    const bool noprog = P.noprog[p] != 0u;  // its elements were copied out by k_leaf_copy
     for (uint32_t side = 0; side < 2; ++side) {
         if (!(side ? R : L)) continue;
-        uint32_t c = side * n + p;
-        uint32_t type = Cv.type[c];
-        if (type == BN_DEAD) continue;
-        uint32_t rank = pos[side * ne + e] - Cv.fpos[c];
-        float4 clo = lo, chi = hi;
-        if (straddle) {
-            float4 bmin = P.lo[p], bmax = P.hi[p];
-            if (side == 0) f4set(bmax, axis, s);
-            else f4set(bmin, axis, s);
-            clip_bounds(G, G.base[A.scene[p]] + idx, clo, chi, bmin, bmax);
-        }
-        if (type == BN_SMALL) {
-            uint32_t d = small_elem_base + Cv.soff[c] + rank;
-            if (d >= sm_cap) {
-                atomicOr(&counters[C_ERR], 1u);
-                continue;
+        bool queued = false;
+        const uint32_t c = side * n + p;
+        const uint32_t type = noprog ? (uint32_t)BN_DEAD : Cv.type[c];
+        if (type != BN_DEAD) {
+            const uint32_t rank = pos[side * ne + e] - Cv.fpos[c];
+            if (type == BN_SMALL) {
+                const uint32_t d = small_elem_base + Cv.soff[c] + rank;
+                if (d >= sm_cap) {
+                    atomicOr(&counters[C_ERR], 1u);
+                } else {
+                    Sm.idx[d] = idx;
+                    if (!straddle) {
+                        Sm.lo[d] = lo;
+                        if (G.kind == MNRT_KIND_TRI) Sm.hi[d] = hi;
+                    }
+                    queued = straddle;
+                }
+            } else {
+                const uint32_t d = Cv.loff[c] + rank;
+                if (d >= nx_cap) {
+                    atomicOr(&counters[C_ERR], 2u);
+                } else {
+                    Nx.idx[d] = idx;
+                    Nx.node[d] = Cv.lidx[c];
+                    if (!straddle) {
+                        Nx.lo[d] = lo;
+                        if (G.kind == MNRT_KIND_TRI) Nx.hi[d] = hi;
+                    }
+                    queued = straddle;
+                }
             }
-            Sm.idx[d] = idx;
-            Sm.lo[d] = clo;
-            if (G.kind == MNRT_KIND_TRI) Sm.hi[d] = chi;
-        } else {
-            uint32_t d = Cv.loff[c] + rank;
-            if (d >= nx_cap) {
-                atomicOr(&counters[C_ERR], 2u);
-                continue;
-            }
-            Nx.idx[d] = idx;
-            Nx.lo[d] = clo;
-            if (G.kind == MNRT_KIND_TRI) Nx.hi[d] = chi;
-            Nx.node[d] = Cv.lidx[c];
         }
+        if (straddle) clipq[pos[2 * ne + side * ne + e] - pos[2 * ne]] = queued ? (e << 1 | side) : kNone;
     }
 }
 
 __global__ __launch_bounds__(256) void k_scatter(DN dn_, DN dne, ElemView E, Geometry G, NodeView A, PlanView P,
                                                  ChildView Cv, const uint32_t* __restrict__ pos, DN small_elem_base,
                                                  ElemView Sm, uint32_t sm_cap, ElemView Nx, uint32_t nx_cap,
-                                                 uint32_t* __restrict__ counters) {
+                                                 uint32_t* __restrict__ counters, uint32_t* __restrict__ clipq) {
     const uint32_t n = dn_.get(), ne = dne.get(), seb = small_elem_base.get();
-    MNRT_GRID_STRIDE(e, ne) scatter_one(e, n, ne, E, G, A, P, Cv, pos, seb, Sm, sm_cap, Nx, nx_cap, counters);
+    MNRT_GRID_STRIDE(e, ne)
+    scatter_one(e, n, ne, E, G, A, P, Cv, pos, seb, Sm, sm_cap, Nx, nx_cap, counters, clipq);
+}
+
+// The clip queue of a triangle level (k_scatter): each job clips its triangle to the
+// child box and writes the bounds next to the index k_scatter placed.  A grid-stride
+// launch: the queue holds at most 2 ne jobs, usually a small part of that.
+__global__ __launch_bounds__(256) void k_clip(DN dn_, DN dne, ElemView E, Geometry G, NodeView A, PlanView P,
+                                              ChildView Cv, const uint32_t* __restrict__ pos, DN small_elem_base,
+                                              ElemView Sm, ElemView Nx, const uint32_t* __restrict__ clipq) {
+    const uint32_t n = dn_.get(), ne = dne.get(), seb = small_elem_base.get();
+    const uint32_t nq = pos[4 * ne] - pos[2 * ne];
+    MNRT_GRID_STRIDE(j, nq) {
+        const uint32_t w = clipq[j];
+        if (w == kNone) continue;
+        const uint32_t e = w >> 1, side = w & 1u, p = E.node[e], axis = P.axis[p];
+        const float s = P.split[p];
+        float4 lo = E.lo[e], hi = E.hi[e];
+        float4 bmin = P.lo[p], bmax = P.hi[p];
+        if (side == 0) f4set(bmax, axis, s);
+        else f4set(bmin, axis, s);
+        clip_bounds(G, G.base[A.scene[p]] + E.idx[e], lo, hi, bmin, bmax);
+        const uint32_t c = side * n + p, rank = pos[side * ne + e] - Cv.fpos[c];
+        if (Cv.type[c] == BN_SMALL) {
+            const uint32_t d = seb + Cv.soff[c] + rank;
+            Sm.lo[d] = lo;
+            Sm.hi[d] = hi;
+        } else {
+            const uint32_t d = Cv.loff[c] + rank;
+            Nx.lo[d] = lo;
+            Nx.hi[d] = hi;
+        }
+    }
+}
+
+// Workgroups of k_clip for a level of up to ne elements.
+inline dim3 clip_blocks(uint64_t ne) {
+    return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((2 * ne + 255) / 256, 1024)));
 }
 
 // ---- small-node stage: one wave64 per small root ----------------------------------------
@@ -1672,9 +1727,11 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
             const uint64_t A_ = lv_na_cap, E_ = lv_ne_cap, C_ = E_ / kChunk + A_;
             auto r4 = [](uint64_t k) { return (k + 3) & ~3ull; };
             // the columns handed out below, each rounded to 16 bytes
+            // element flags (left, right; triangles also straddle left / right) and their scan
+            const uint64_t flw = (kind == MNRT_KIND_TRI ? 4 : 2) * E_ + 1;
             const uint64_t words = r4(A_) + 3 * r4(C_) + 2 * r4(3 * A_) + 9 * r4(A_) + 2 * r4(4 * A_) +
-                                   2 * r4(2 * E_ + 1) + r4(22 * A_) +
-                                   std::max<uint64_t>(scan_scratch_words((uint32_t)(2 * E_ + 1)),
+                                   2 * r4(flw) + r4(22 * A_) +
+                                   std::max<uint64_t>(scan_scratch_words((uint32_t)flw),
                                                       kScanSetMax * scan_scratch_words((uint32_t)(2 * A_))) +
                                    16;
             lv_buf = pool.get<uint32_t>(words);
@@ -1704,8 +1761,8 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
             P.cur = take(A_);
             P.leaf = take(A_);
             P.noprog = take(A_);
-            fl = take(2 * E_ + 1);
-            pos = take(2 * E_ + 1);
+            fl = take((kind == MNRT_KIND_TRI ? 4 : 2) * E_ + 1);  // after the scan: the clip queue
+            pos = take((kind == MNRT_KIND_TRI ? 4 : 2) * E_ + 1);
             cbuf = take(22 * A_);
             scr = w;
         }
@@ -1738,7 +1795,8 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         hipLaunchKernelGGL((k_plan<true>), blocks_for(na), dim3(256), 0, st, hn(na), A, tlo, thi,
                            prm->empty_space_ratio, P, (const uint32_t*)P.newcnt, hn(nbn), B, counters);
         hipLaunchKernelGGL(k_classify, blocks_for(ne), dim3(256), 0, st, hn(ne), E, kind, P, fl);
-        MNRT_HIP_CHECK(scan_exclusive(fl, pos, 2 * ne, pos + 2 * ne, scr, st));
+        const uint32_t nfl = (kind == MNRT_KIND_TRI ? 4u : 2u) * ne;
+        MNRT_HIP_CHECK(scan_exclusive(fl, pos, nfl, pos + nfl, scr, st));
         hipLaunchKernelGGL(k_child_plan, blocks_for(2ull * na), dim3(256), 0, st, hn(na), hn(ne), A, P, pos,
                            prm->small_node_max, Cv);
         {
@@ -1752,7 +1810,10 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
                            hn(nse), SV, An, B, counters);
         hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, hn(na), A, P, E, ll_idx, ll_cap, B, counters);
         hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, hn(na), hn(ne), E, G, A, P, Cv, pos, hn(nse),
-                           Sm, se_cap, Nx, nx_cap, counters);
+                           Sm, se_cap, Nx, nx_cap, counters, fl);
+        if (kind == MNRT_KIND_TRI)
+            hipLaunchKernelGGL(k_clip, clip_blocks(ne), dim3(256), 0, st, hn(na), hn(ne), E, G, A, P, Cv,
+                               (const uint32_t*)pos, hn(nse), Sm, Nx, (const uint32_t*)fl);
         MNRT_HIP_CHECK(hipGetLastError());
         int rc = sync_counters();
         if (rc != MNRT_OK) return rc;
@@ -2148,8 +2209,9 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
     P.cur = cv.take<uint32_t>(NA);
     P.leaf = cv.take<uint32_t>(NA);
     P.noprog = cv.take<uint32_t>(NA);
-    uint32_t* fl = cv.take<uint32_t>(2ull * NE + 1);
-    uint32_t* pos = cv.take<uint32_t>(2ull * NE + 1);
+    const uint64_t FLW = (kind == MNRT_KIND_TRI ? 4ull : 2ull) * NE;  // element flags (+ straddle flags)
+    uint32_t* fl = cv.take<uint32_t>(FLW + 1);  // after the scan: the clip queue
+    uint32_t* pos = cv.take<uint32_t>(FLW + 1);
     uint32_t* cbuf = cv.take<uint32_t>(22ull * NA);
     ChildView Cv{};
     {
@@ -2166,7 +2228,7 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
         Cv.soff = cbuf + 9 * m;
         Cv.loff = cbuf + 10 * m;
     }
-    uint32_t* scr = cv.take<uint32_t>(std::max<uint64_t>({(uint64_t)dscan_tiles(NA), (uint64_t)dscan_tiles(2 * NE),
+    uint32_t* scr = cv.take<uint32_t>(std::max<uint64_t>({(uint64_t)dscan_tiles(NA), (uint64_t)dscan_tiles((uint32_t)FLW),
                                                            4ull * dscan_tiles(2 * NA), (uint64_t)dscan_tiles(C.small)}) +
                                       16);
     // small stage
@@ -2270,7 +2332,8 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
         hipLaunchKernelGGL((k_plan<true>), gna, dim3(256), 0, st, na, A, tlo, thi, g.prm.empty_space_ratio, P,
                            (const uint32_t*)P.newcnt, nbn, B, counters);
         hipLaunchKernelGGL(k_classify, gne, dim3(256), 0, st, ne, E, kind, P, fl);
-        MNRT_HIP_CHECK(dscan1(fl, pos, ne2, nullptr, true, scr, st));
+        MNRT_HIP_CHECK(dscan1(fl, pos, kind == MNRT_KIND_TRI ? dn(counters + G_NE, 4 * NE, 2) : ne2, nullptr, true,
+                              scr, st));
         hipLaunchKernelGGL(k_child_plan, gna2, dim3(256), 0, st, na, ne, A, P, pos, g.prm.small_node_max, Cv);
         {
             DScanSet cs{};
@@ -2288,7 +2351,10 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
         hipLaunchKernelGGL(k_child_write, gna2, dim3(256), 0, st, na, A, P, Cv, nsm, nse, SV, An, B, counters);
         hipLaunchKernelGGL(k_leaf_copy, gna, dim3(256), 0, st, na, A, P, E, ll_idx, C.ll, B, counters);
         hipLaunchKernelGGL(k_scatter, gne, dim3(256), 0, st, na, ne, E, G, A, P, Cv, pos, nse, Sm, C.small_elems, Nx,
-                           NE, counters);
+                           NE, counters, fl);
+        if (kind == MNRT_KIND_TRI)
+            hipLaunchKernelGGL(k_clip, clip_blocks(pne), dim3(256), 0, st, na, ne, E, G, A, P, Cv, (const uint32_t*)pos,
+                               nse, Sm, Nx, (const uint32_t*)fl);
         hipLaunchKernelGGL(k_graph_level_end, dim3(1), dim3(64), 0, st, counters, C, l + 1 == g.levels ? 1u : 0u);
         MNRT_HIP_CHECK(hipGetLastError());
     }
