@@ -13,8 +13,9 @@ pytestmark = pytest.mark.gpu
 RTOL64 = 1e-5
 
 
-def _run(orc, cuda, sc, ppos, ppow, pdir, q, k, rmax, brute=True, clr=True):
-    P = orc.build(1, ppos, max_qr=rmax, root=np.concatenate(sc.bounds()))
+def _run(orc, cuda, sc, ppos, ppow, pdir, q, k, rmax, brute=True, clr=True, build_qr=None, root=None):
+    root = np.concatenate(sc.bounds()) if root is None else root
+    P = orc.build(1, ppos, max_qr=rmax if build_qr is None else build_qr, root=root)
     nbr, cnt, _ = orc.knn(P, ppos, q["pos"], k, rmax)
     if brute:
         bn, bc = orc.knn_brute(ppos, q["pos"], k, rmax)
@@ -87,6 +88,33 @@ def test_knn_coincident_photons_tie_by_index(orc, cuda):
     q = hit_queries(orc, sc, o, d)
     q["pos"][:50] = ppos[:50]  # queries exactly on photons (d2 = 0)
     _run(orc, cuda, sc, ppos, ppow, pdir, q, 30, sc.r_max())
+
+
+def test_knn_unbounded_radius_far_queries_and_offset_scene(orc, cuda):
+    """r_max = inf (every photon is a candidate), queries far outside the photon map (large
+    d2: the window and final selections search high exponents), and the scene moved to
+    coordinates near 1e4 (where the fp32 photon distances and the kernel's box distances,
+    summed in another order, round most): sets, radii and radiance equal the oracle's."""
+    sc = S.scene_small(subdiv=2)
+    ppos, ppow, pdir = S.random_surface_photons(sc, 3000, seed=21)
+    o, d, _ = S.camera_rays(32, 32)
+    q = hit_queries(orc, sc, o, d)
+    print("knn: r_max = inf", flush=True)
+    _run(orc, cuda, sc, ppos, ppow, pdir, q, 40, float("inf"), build_qr=sc.r_max())
+    far = dict(q)
+    far["pos"] = q["pos"].copy()
+    far["pos"][:, :3] += np.float32(40.0)
+    print("knn: far queries", flush=True)
+    _run(orc, cuda, sc, ppos, ppow, pdir, far, 40, float("inf"), build_qr=sc.r_max())
+    shift = np.float32(1.0e4)
+    pp = ppos.copy()
+    pp[:, :3] += shift
+    qs = dict(q)
+    qs["pos"] = q["pos"].copy()
+    qs["pos"][:, :3] += shift
+    root = (np.concatenate(sc.bounds()) + shift).astype(np.float32)
+    print("knn: scene near 1e4", flush=True)
+    _run(orc, cuda, sc, pp, ppow, pdir, qs, 40, sc.r_max(), root=root)
 
 
 def test_knn_per_query_radius_and_inverse_distance(orc, cuda):
