@@ -175,16 +175,6 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return (row_lane(v, 0) + row_lane(v, 1)) + (row_lane(v, 2) + row_lane(v, 3));
 }
 
-__device__ __forceinline__ float wave_max_f32(float v) {
-    v = fmaxf(v, __uint_as_float(dpp_mov<kDppPairSwap>(__float_as_uint(v))));
-    v = fmaxf(v, __uint_as_float(dpp_mov<kDppQuadSwap>(__float_as_uint(v))));
-    v = fmaxf(v, __uint_as_float(dpp_mov<kDppHalfMirror>(__float_as_uint(v))));
-    v = fmaxf(v, __uint_as_float(dpp_mov<kDppRowMirror>(__float_as_uint(v))));
-    const uint32_t b = __float_as_uint(v);
-    return fmaxf(fmaxf(__uint_as_float(row_lane(b, 0)), __uint_as_float(row_lane(b, 1))),
-                 fmaxf(__uint_as_float(row_lane(b, 2)), __uint_as_float(row_lane(b, 3))));
-}
-
 template <int kCtrl>
 __device__ __forceinline__ long long dpp_mov64(long long v) {
     const uint32_t lo = dpp_mov<kCtrl>((uint32_t)(unsigned long long)v);
