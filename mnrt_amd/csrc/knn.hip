@@ -278,21 +278,22 @@ __device__ __forceinline__ void knn_query(const KnnArgs& A, const KnnTree& Tr, u
     const uint32_t k = A.k;
     const float rq = A.r_query ? uniformf(A.r_query[q]) : Tr.r_max;
     const float r2max = A.r_query ? rq * rq : Tr.r2max;
-    // search radius: r_max, or the warm-start bound when smaller (margins of 2^-20 cover
-    // the rounding of the fp32 distances; the bound is exclusive, hence nextafter)
+    // search radius: r_max, or the warm-start bound when smaller (warm_bound), and for
+    // queries in spatial-cell order first the guess of WarmHint::guess
     float r2lim = r2max;
     float r2next = r2max;  // the next pass's limit when this one holds fewer than k
     if (hint && k > 0) {
         r2lim = r2next = warm_bound(*hint, qx, qy, qz, r2max);
         if (hint->guess && hint->r2 >= 0.0f) {
             const float dx = qx - hint->x, dy = qy - hint->y, dz = qz - hint->z;
-            const float g = __builtin_amdgcn_sqrtf(hint->r2) + 0.5f * __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+            const float g =
+                __builtin_amdgcn_sqrtf(hint->r2) + 0.5f * __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
             r2lim = fminf(r2next, g * g);
         }
     }
 
     uint32_t cnt = 0;        // candidates in the buffer
-    bool full = false;       // the buffer's first k are the k best so far
+    bool full = false;       // the buffer holds the k best so far (after a window: more)
     float thr_d2 = kInf;     // with full: the k best so far lie at d2 <= thr_d2
     int sp = 0;
     uint32_t node = 0;
