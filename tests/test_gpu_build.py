@@ -70,6 +70,16 @@ def test_build_tri_degenerate(orc, cuda):
     _tri(orc, cuda, sc)
 
 
+def test_build_tri_large_overlapping_triangles(orc, cuda):
+    # triangles spanning most of the scene: nearly every element straddles every split, so
+    # the clip queue holds up to two jobs per element, and no-progress nodes (whose queued
+    # sides are skipped) are common; mixed with small triangles that rarely straddle
+    rng = np.random.default_rng(17)
+    big = rng.uniform(-1.0, 1.0, size=(2500, 3, 3))
+    small = rng.uniform(-1.0, 1.0, size=(4000, 1, 3)) + rng.normal(scale=0.01, size=(4000, 3, 3))
+    _tri(orc, cuda, _scene(np.concatenate([big, small])))
+
+
 def test_build_points_vvh(orc, cuda):
     sc = S.scene_small(subdiv=3)
     ppos, _, _ = S.random_surface_photons(sc, 50000, seed=2)
