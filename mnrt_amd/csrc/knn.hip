@@ -726,7 +726,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
             uint32_t s = uniform(seg_of(slot_begin, ns, (uint32_t)g0));
             uint32_t b = 0, e = 0, m = 0;
             KnnTree Tr{};
-            WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
+            WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f, false};
             auto enter = [&]() {
                 b = ld_uniform(slot_begin, s);
                 e = ld_uniform(slot_begin, s + 1);
@@ -991,7 +991,7 @@ __global__ __launch_bounds__(256) void k_knn_bwd_batch(KnnArgs G, const KnnScene
             uint32_t b = 0, e = 0, m = 0, poff = 0;
             double fs = 0.0;
             KnnTree Tr{};
-            WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f};
+            WarmHint hint{0.0f, 0.0f, 0.0f, -1.0f, false};
             auto enter = [&]() {
                 b = ld_uniform(slot_begin, s);
                 e = ld_uniform(slot_begin, s + 1);
