@@ -132,6 +132,48 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add_set(ScanSet S, uint32_t
     }
 }
 
+// Scans of more than one tile and at most kScanBig elements (the build's per-node scans)
+// run in one workgroup: each thread takes a contiguous run of at most kScanBigPer elements,
+// the run totals are scanned per wave and the wave totals through LDS, so such a scan is one
+// launch instead of three.  in and out may alias (a thread reads each element of its run
+// before writing it, and the runs are disjoint).
+constexpr uint32_t kScanBigBlock = 1024;
+constexpr uint32_t kScanBigPer = 64;
+constexpr uint32_t kScanBig = kScanBigBlock * kScanBigPer;
+
+__device__ __forceinline__ void block_scan_runs(const uint32_t* in, uint32_t n, uint32_t* out,
+                                                uint32_t* __restrict__ total) {
+    __shared__ uint32_t s_wave[kScanBigBlock / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t per = (n + kScanBigBlock - 1) / kScanBigBlock;
+    const uint32_t b = min(tid * per, n), e = min(b + per, n);
+    uint32_t local = 0;
+    for (uint32_t i = b; i < e; ++i) local += in[i];
+    const uint32_t incl = wave_incl_scan(local, lane);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    for (uint32_t w = 0; w < wave; ++w) wave_off += s_wave[w];
+    uint32_t run = wave_off + incl - local;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (tid == kScanBigBlock - 1 && total) *total = wave_off + incl;
+}
+
+__global__ __launch_bounds__(kScanBigBlock) void k_scan_big(const uint32_t* in, uint32_t n, uint32_t* out,
+                                                            uint32_t* __restrict__ total) {
+    block_scan_runs(in, n, out, total);
+}
+
+// blockIdx.x picks the scan of the set
+__global__ __launch_bounds__(kScanBigBlock) void k_scan_big_set(ScanSet S, uint32_t n) {
+    const uint32_t y = blockIdx.x;
+    block_scan_runs(S.in[y], n, S.out[y], S.total[y]);
+}
+
 // Scratch words needed by scan_exclusive for n elements.
 inline size_t scan_scratch_words(uint32_t n) {
     size_t words = 0;
@@ -150,6 +192,10 @@ inline hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, 
     // kernels only (no memset / memcpy nodes), so a captured graph replays it exactly
     if (n == 0) {
         if (total) hipLaunchKernelGGL(k_scan_word, dim3(1), dim3(1), 0, st, total, (const uint32_t*)nullptr);
+        return hipGetLastError();
+    }
+    if (n > kScanTile && n <= kScanBig) {
+        hipLaunchKernelGGL(k_scan_big, dim3(1), dim3(kScanBigBlock), 0, st, in, n, out, total);
         return hipGetLastError();
     }
     uint32_t blocks = (n + kScanTile - 1) / kScanTile;
@@ -171,6 +217,10 @@ inline hipError_t scan_exclusive_set(const ScanSet& S0, int m, uint32_t n, uint3
     if (n == 0) {
         for (int y = 0; y < m; ++y)
             if (S0.total[y]) hipLaunchKernelGGL(k_scan_word, dim3(1), dim3(1), 0, st, S0.total[y], (const uint32_t*)nullptr);
+        return hipGetLastError();
+    }
+    if (n > kScanTile && n <= kScanBig) {
+        hipLaunchKernelGGL(k_scan_big_set, dim3(m), dim3(kScanBigBlock), 0, st, S0, n);
         return hipGetLastError();
     }
     const uint32_t blocks = (n + kScanTile - 1) / kScanTile;

@@ -1,8 +1,9 @@
 """The parallel primitives (mnrt_scan_u32, mnrt_compact_u32; MNCudaPrimitives::Scan /
 ::Compact, MNCudaPrimitives.cpp:160-190) against plain sequential loops, at the sizes where
 a tiled scan can go wrong: empty, one element, around a wave (63 / 64 / 65), around a
-2048-element tile, many tiles (a second scan level) and more than 2048 tiles (a third),
-with sums that wrap modulo 2^32."""
+2048-element tile, the one-workgroup scans of 2049..65536 elements and their upper end,
+many tiles (a second scan level) and more than 2048 tiles (a third), with sums that wrap
+modulo 2^32."""
 
 import numpy as np
 import pytest
@@ -12,7 +13,8 @@ from mnrt_amd import api
 pytestmark = pytest.mark.gpu
 
 TILE = 2048
-SIZES = [0, 1, 63, 64, 65, TILE - 1, TILE, TILE + 1, 64 * TILE + 5, TILE * TILE + 3]
+SIZES = [0, 1, 63, 64, 65, TILE - 1, TILE, TILE + 1, 4 * TILE + 7, 32 * TILE - 1, 32 * TILE, 32 * TILE + 1,
+         64 * TILE + 5, TILE * TILE + 3]
 
 
 def _seq_scan(a, inclusive):
