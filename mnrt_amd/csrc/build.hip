@@ -265,10 +265,20 @@ __global__ __launch_bounds__(256) void k_init_elems(Geometry G, uint32_t n, Elem
     E.node[i] = s;
 }
 
+// Chunks per node, and the node's tight-bound accumulators (3 per node, ordered-int
+// encoding) cleared for k_tight.
 __global__ __launch_bounds__(256) void k_chunk_counts(DN dn_, const uint32_t* __restrict__ cnt,
-                                                      uint32_t* __restrict__ cc) {
+                                                      uint32_t* __restrict__ cc, uint32_t* __restrict__ tlo,
+                                                      uint32_t* __restrict__ thi) {
     const uint32_t n = dn_.get();
-    MNRT_GRID_STRIDE(p, n) cc[p] = (cnt[p] + kChunk - 1) / kChunk;
+    MNRT_GRID_STRIDE(p, n) {
+        cc[p] = (cnt[p] + kChunk - 1) / kChunk;
+#pragma unroll
+        for (uint32_t a = 0; a < 3; ++a) {
+            tlo[3 * p + a] = 0xFFFFFFFFu;
+            thi[3 * p + a] = 0u;
+        }
+    }
 }
 
 // Chunk -> node by binary search over the chunk offsets (kernel_GenerateChunks,
@@ -292,14 +302,6 @@ __global__ __launch_bounds__(256) void k_chunk_fill(DN dn_, uint32_t max_chunks,
         ch_node[c] = p;
         ch_first[c] = first[p] + local * kChunk;
         ch_cnt[c] = min(kChunk, cnt[p] - local * kChunk);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_tight_init(DN dn_, uint32_t* __restrict__ tlo, uint32_t* __restrict__ thi) {
-    const uint32_t n3 = 3u * dn_.get();
-    MNRT_GRID_STRIDE(i, n3) {
-        tlo[i] = 0xFFFFFFFFu;
-        thi[i] = 0u;
     }
 }
 
@@ -680,31 +682,34 @@ __device__ __forceinline__ void child_write_one(uint32_t c, uint32_t n, const No
     }
 }
 
-__global__ __launch_bounds__(256) void k_child_write(DN dn_, NodeView A, PlanView P, ChildView Cv, DN small_base,
-                                                     DN small_elem_base, SmallView S, NodeView Nx, BnView B,
-                                                     uint32_t* __restrict__ counters) {
-    const uint32_t n = dn_.get(), sb = small_base.get(), seb = small_elem_base.get();
-    MNRT_GRID_STRIDE(c, 2 * n) child_write_one(c, n, A, P, Cv, sb, seb, S, Nx, B, counters);
-}
-
 // Elements of large-stage leaves (level cap, no progress) are copied out before the
 // level's element arrays are released.
-__global__ __launch_bounds__(256) void k_leaf_copy(DN dn_, NodeView A, PlanView P, ElemView E,
-                                                   uint32_t* __restrict__ ll_idx, uint32_t ll_cap, BnView B,
-                                                   uint32_t* __restrict__ counters) {
-    const uint32_t n = dn_.get();
-    MNRT_GRID_STRIDE(p, n) {
-        if (!(P.leaf[p] || P.noprog[p])) continue;
-        uint32_t m = A.cnt[p], f = A.first[p];
-        uint32_t start = atomicAdd(&counters[C_LLCOUNT], m);
-        if ((uint64_t)start + m > ll_cap) {
-            atomicOr(&counters[C_ERR], 32u);
-            continue;
-        }
-        for (uint32_t i = 0; i < m; ++i) ll_idx[start + i] = E.idx[f + i];
-        uint32_t cur = P.cur[p];
-        B.aux[cur] = start;
-        B.cnt[cur] = m;
+__device__ __forceinline__ void leaf_copy_one(uint32_t p, const NodeView& A, const PlanView& P, const ElemView& E,
+                                              uint32_t* __restrict__ ll_idx, uint32_t ll_cap, const BnView& B,
+                                              uint32_t* __restrict__ counters) {
+    if (!(P.leaf[p] || P.noprog[p])) return;
+    uint32_t m = A.cnt[p], f = A.first[p];
+    uint32_t start = atomicAdd(&counters[C_LLCOUNT], m);
+    if ((uint64_t)start + m > ll_cap) {
+        atomicOr(&counters[C_ERR], 32u);
+        return;
+    }
+    for (uint32_t i = 0; i < m; ++i) ll_idx[start + i] = E.idx[f + i];
+    uint32_t cur = P.cur[p];
+    B.aux[cur] = start;
+    B.cnt[cur] = m;
+}
+
+// The level's children (2 per node), and with the first n threads the copy of large-stage
+// leaves' elements.
+__global__ __launch_bounds__(256) void k_child_write(DN dn_, NodeView A, PlanView P, ChildView Cv, DN small_base,
+                                                     DN small_elem_base, SmallView S, NodeView Nx, BnView B,
+                                                     uint32_t* __restrict__ counters, ElemView E,
+                                                     uint32_t* __restrict__ ll_idx, uint32_t ll_cap) {
+    const uint32_t n = dn_.get(), sb = small_base.get(), seb = small_elem_base.get();
+    MNRT_GRID_STRIDE(c, 2 * n) {
+        child_write_one(c, n, A, P, Cv, sb, seb, S, Nx, B, counters);
+        if (c < n) leaf_copy_one(c, A, P, E, ll_idx, ll_cap, B, counters);
     }
 }
 
@@ -731,7 +736,7 @@ __device__ __forceinline__ void scatter_one(uint32_t e, uint32_t n, uint32_t ne,
     bool L, R;
     classify(G.kind, idx, lo, hi, axis, s, L, R);
     const bool straddle = (G.kind == MNRT_KIND_TRI) && (f4c(lo, axis) < s && s < f4c(hi, axis));
-    const bool noprog = P.noprog[p] != 0u;  // its elements were copied out by k_leaf_copy
+    const bool noprog = P.noprog[p] != 0u;  // its elements were copied out by k_child_write
     for (uint32_t side = 0; side < 2; ++side) {
         if (!(side ? R : L)) continue;
         bool queued = false;
@@ -1581,11 +1586,10 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         thi = tlo + 3 * S_;
         uint32_t* sc = thi + 3 * S_;
         hipLaunchKernelGGL(k_scene_counts, blocks_for(S_), dim3(256), 0, st, S_, d_base, cnts);
-        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, hn(S_), cnts, cc);
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, hn(S_), cnts, cc, tlo, thi);
         MNRT_HIP_CHECK(scan_exclusive(cc, cc, S_, counters + C_NCHUNKS, sc, st));
         hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, hn(S_), max_chunks, cc, counters,
                            d_base, cnts, chn, chf, chc);
-        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * S_), dim3(256), 0, st, hn(S_), tlo, thi);
         hipLaunchKernelGGL(k_tight, dim3((max_chunks + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st,
                            max_chunks, counters, chn, chf, chc, Ein,
                            kind, tlo, thi);
@@ -1781,11 +1785,10 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
             Cv.soff = cbuf + 9 * m;
             Cv.loff = cbuf + 10 * m;
         }
-        hipLaunchKernelGGL(k_chunk_counts, blocks_for(na), dim3(256), 0, st, hn(na), A.cnt, cc);
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(na), dim3(256), 0, st, hn(na), A.cnt, cc, tlo, thi);
         MNRT_HIP_CHECK(scan_exclusive(cc, cc, na, counters + C_NCHUNKS, scr, st));
         hipLaunchKernelGGL(k_chunk_fill, blocks_for(max_chunks), dim3(256), 0, st, hn(na), max_chunks, cc, counters,
                            A.first, A.cnt, chn, chf, chc);
-        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * na), dim3(256), 0, st, hn(na), tlo, thi);
         hipLaunchKernelGGL(k_tight, dim3((max_chunks + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st,
                            max_chunks, counters, chn, chf, chc, E,
                            kind, tlo, thi);
@@ -1807,8 +1810,7 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
             MNRT_HIP_CHECK(scan_exclusive_set(cs, 4, 2 * na, scr, st));
         }
         hipLaunchKernelGGL(k_child_write, blocks_for(2ull * na), dim3(256), 0, st, hn(na), A, P, Cv, hn(nsmall),
-                           hn(nse), SV, An, B, counters);
-        hipLaunchKernelGGL(k_leaf_copy, blocks_for(na), dim3(256), 0, st, hn(na), A, P, E, ll_idx, ll_cap, B, counters);
+                           hn(nse), SV, An, B, counters, E, ll_idx, ll_cap);
         hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, hn(na), hn(ne), E, G, A, P, Cv, pos, hn(nse),
                            Sm, se_cap, Nx, nx_cap, counters, fl);
         if (kind == MNRT_KIND_TRI)
@@ -2292,11 +2294,10 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
         rthi = rtlo + 3 * S_;
         uint32_t* sc = rthi + 3 * S_;
         hipLaunchKernelGGL(k_scene_counts, blocks_for(S_), dim3(256), 0, st, S_, T.base, cnts);
-        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, hn(S_), cnts, rcc);
+        hipLaunchKernelGGL(k_chunk_counts, blocks_for(S_), dim3(256), 0, st, hn(S_), cnts, rcc, rtlo, rthi);
         MNRT_HIP_CHECK(scan_exclusive(rcc, rcc, S_, counters + C_NCHUNKS, sc, st));
         hipLaunchKernelGGL(k_chunk_fill, blocks_for(ch0), dim3(256), 0, st, hn(S_), ch0, rcc, counters, T.base, cnts,
                            rchn, rchf, rchc);
-        hipLaunchKernelGGL(k_tight_init, blocks_for(3ull * S_), dim3(256), 0, st, hn(S_), rtlo, rthi);
         hipLaunchKernelGGL(k_tight, dim3((ch0 + kTightChunks - 1) / kTightChunks), dim3(kChunk), 0, st, ch0, counters,
                            rchn, rchf, rchc, Ein, kind, rtlo, rthi);
     }
@@ -2320,11 +2321,10 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
         const uint64_t pna = planned(bs.level_na[pl], 256, NA), pne = planned(bs.level_ne[pl], 4096, NE);
         const dim3 gna = blocks_for(pna), gna2 = blocks_for(2 * pna), gne = blocks_for(pne);
         const dim3 gch((uint32_t)std::max<uint64_t>(1, (pne / kChunk + pna + kTightChunks - 1) / kTightChunks));
-        hipLaunchKernelGGL(k_chunk_counts, gna, dim3(256), 0, st, na, A.cnt, cc);
+        hipLaunchKernelGGL(k_chunk_counts, gna, dim3(256), 0, st, na, A.cnt, cc, tlo, thi);
         MNRT_HIP_CHECK(dscan1(cc, cc, na, counters + C_NCHUNKS, false, scr, st));
         hipLaunchKernelGGL(k_chunk_fill, blocks_for(pne / kChunk + pna), dim3(256), 0, st, na, CH, cc, counters,
                            A.first, A.cnt, chn, chf, chc);
-        hipLaunchKernelGGL(k_tight_init, blocks_for(3 * pna), dim3(256), 0, st, na, tlo, thi);
         hipLaunchKernelGGL(k_tight, gch, dim3(kChunk), 0, st, CH, counters, chn, chf, chc, E, kind, tlo, thi);
         hipLaunchKernelGGL((k_plan<false>), gna, dim3(256), 0, st, na, A, tlo, thi, g.prm.empty_space_ratio, P,
                            (const uint32_t*)nullptr, hn(0u), B, counters);
@@ -2348,8 +2348,8 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
             }
             MNRT_HIP_CHECK(dscan(cs, 4, na2, scr, st));
         }
-        hipLaunchKernelGGL(k_child_write, gna2, dim3(256), 0, st, na, A, P, Cv, nsm, nse, SV, An, B, counters);
-        hipLaunchKernelGGL(k_leaf_copy, gna, dim3(256), 0, st, na, A, P, E, ll_idx, C.ll, B, counters);
+        hipLaunchKernelGGL(k_child_write, gna2, dim3(256), 0, st, na, A, P, Cv, nsm, nse, SV, An, B, counters, E,
+                           ll_idx, C.ll);
         hipLaunchKernelGGL(k_scatter, gne, dim3(256), 0, st, na, ne, E, G, A, P, Cv, pos, nse, Sm, C.small_elems, Nx,
                            NE, counters, fl);
         if (kind == MNRT_KIND_TRI)
