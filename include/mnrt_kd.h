@@ -325,7 +325,7 @@ int mnrt_knn_gather(const mnrt_tree* pmap, uint32_t k, float r_max, uint32_t n,
  * reference.  Same arguments, results bit for bit equal to mnrt_knn_gather's, but no
  * hipMalloc / pool allocation: the query processing order and, with d_idx_sp, the
  * per-query values live in one fixed 64 MiB device array of the library (one per device,
- * kept until the process ends), so n is limited to 8,126,329 queries, or 2,321,808 with
+ * kept until the process ends), so n is limited to 7,339,897 queries, or 2,097,113 with
  * d_idx_sp (MNRT_ERR_CAPACITY beyond, nothing launched).  Every call on one device shares
  * that array: calls that overlap on a device (concurrent streams or host threads) race and
  * must be serialised by the caller.  mnrt_device_init looks the array's address up for the

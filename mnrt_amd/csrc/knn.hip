@@ -1158,6 +1158,7 @@ uint32_t* g_scratch_addr[kScratchMaxDevices] = {};
 // Words of scratch mnrt_knn_gather_noalloc needs for n queries: the processing order, and
 // 5 words per query (radiance + inverse distance) when slot indices are given.
 size_t noalloc_words(uint32_t n, bool slots) {
+    if (n >= mnrt::kOrderMaxItems) return SIZE_MAX;  // far beyond the array either way
     return mnrt::spatial_order_words(n) + (slots ? 5ull * n : 0ull);
 }
 

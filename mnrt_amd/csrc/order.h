@@ -21,6 +21,10 @@ namespace {  // one private copy per translation unit
 constexpr uint32_t kOrderBits = 6;  // per axis
 constexpr uint32_t kOrderCells = 1u << (3 * kOrderBits);
 constexpr uint32_t kOrderMinItems = 1u << 15;  // smaller launches keep the caller's order
+constexpr uint32_t kOrderMaxItems = 1u << 29;  // ranks share a word with the histogram copy
+// Histogram copies: one per XCD (accelerator complex die), picked by the die a wave runs on,
+// so a cell's counter is updated from one die's L2 only instead of bouncing between all eight.
+constexpr uint32_t kOrderCopies = 8;
 constexpr uint32_t kFarPerThread = 16;
 
 struct OrderBox {
@@ -63,60 +67,99 @@ __device__ __forceinline__ bool order_coherent(uint32_t n, const uint32_t* far_p
     return 2ull * *far_pairs < (uint64_t)n;
 }
 
-// Cells and their histogram, for incoherent items only.
-__global__ __launch_bounds__(256) void k_order_cells(uint32_t n, const float4* __restrict__ pos, OrderBox B,
-                                                     const uint32_t* __restrict__ far_pairs,
-                                                     uint32_t* __restrict__ cell, uint32_t* __restrict__ hist) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || order_coherent(n, far_pairs)) return;
-    const float4 p = pos[i];
+// Morton cell of a point (NaN coordinates land in cell 0: fmaxf / fminf return the other
+// operand for NaN).
+__device__ __forceinline__ uint32_t order_cell(const float4 p, const OrderBox& B) {
     const float c[3] = {p.x, p.y, p.z};
     uint32_t code = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        // fmaxf / fminf return the other operand for NaN: a NaN coordinate lands in cell 0
         const float t = fminf(fmaxf((c[a] - B.lo[a]) * B.scale[a], 0.0f), (float)((1u << kOrderBits) - 1u));
         code |= spread3((uint32_t)t) << a;
     }
-    cell[i] = code;
-    atomicAdd(&hist[code], 1u);
+    return code;
 }
 
-__global__ __launch_bounds__(256) void k_order_place(uint32_t n, const uint32_t* __restrict__ cell,
+// Die (XCD) the calling wave runs on, 0..kOrderCopies-1.
+__device__ __forceinline__ uint32_t order_copy() {
+    uint32_t x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return x & (kOrderCopies - 1u);
+}
+
+// Histograms of the cells, one per die, for incoherent items only; each item keeps its rank
+// inside its cell's copy (the value its atomic returned) and the copy in bits 29..31, so
+// placing needs no second round of atomics.
+__global__ __launch_bounds__(256) void k_order_cells(uint32_t n, const float4* __restrict__ pos, OrderBox B,
                                                      const uint32_t* __restrict__ far_pairs,
-                                                     uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+                                                     uint32_t* __restrict__ rank, uint32_t* __restrict__ hist) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || order_coherent(n, far_pairs)) return;
+    const uint32_t x = order_copy();
+    rank[i] = atomicAdd(&hist[x * kOrderCells + order_cell(pos[i], B)], 1u) | x << 29;
+}
+
+// Per cell: copy x's count becomes the sum of copies 0..x-1 (copy 0's the cell total, the
+// scan's input; copy 0 itself starts at 0).
+__global__ __launch_bounds__(256) void k_order_sum(uint32_t n, const uint32_t* __restrict__ far_pairs,
+                                                   uint32_t* __restrict__ hist) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= kOrderCells || order_coherent(n, far_pairs)) return;
+    uint32_t s = hist[c];
+#pragma unroll
+    for (uint32_t x = 1; x < kOrderCopies; ++x) {
+        const uint32_t h = hist[x * kOrderCells + c];
+        hist[x * kOrderCells + c] = s;
+        s += h;
+    }
+    hist[c] = s;
+}
+
+// Slot of item i: its cell's start (exclusive scan of the cell totals), the counts of the
+// copies before its own, and its rank in its copy.
+__global__ __launch_bounds__(256) void k_order_place(uint32_t n, const float4* __restrict__ pos, OrderBox B,
+                                                     const uint32_t* __restrict__ rank,
+                                                     const uint32_t* __restrict__ far_pairs,
+                                                     const uint32_t* __restrict__ hist, uint32_t* __restrict__ order) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (order_coherent(n, far_pairs)) order[i] = i;
-    else order[atomicAdd(&cursor[cell[i]], 1u)] = i;
+    if (order_coherent(n, far_pairs)) {
+        order[i] = i;
+        return;
+    }
+    const uint32_t c = order_cell(pos[i], B), rk = rank[i], x = rk >> 29;
+    order[hist[c] + (x ? hist[x * kOrderCells + c] : 0u) + (rk & ((1u << 29) - 1u))] = i;
 }
 
-// Zeroes the cell histogram and the far-pair counter (a kernel rather than memset nodes, so
+// Zeroes the cell histograms and the far-pair counter (a kernel rather than memset nodes, so
 // a caller may capture the sequence in a graph).
 __global__ __launch_bounds__(256) void k_order_clear(uint32_t* __restrict__ hist, uint32_t* __restrict__ far_pairs) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kOrderCells; i += gridDim.x * blockDim.x) hist[i] = 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kOrderCopies * kOrderCells;
+         i += gridDim.x * blockDim.x)
+        hist[i] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) *far_pairs = 0u;
 }
 
-// Device words spatial_order_into needs for n points (0 below kOrderMinItems: no order).
+// Device words spatial_order_into needs for n points (0 outside kOrderMinItems ..
+// kOrderMaxItems - 1: no order).
 inline size_t spatial_order_words(uint32_t n) {
-    return n < kOrderMinItems ? 0 : 2ull * n + 2ull * kOrderCells + 4 + scan_scratch_words(kOrderCells);
+    if (n < kOrderMinItems || n >= kOrderMaxItems) return 0;
+    return 2ull * n + (size_t)kOrderCopies * kOrderCells + 4 + scan_scratch_words(kOrderCells);
 }
 
 // Processing order of n points at d_pos4 in the box lo3 .. hi3, built in buf
-// (spatial_order_words(n) device words): *order = null below kOrderMinItems, else a pointer
+// (spatial_order_words(n) device words): *order = null when spatial_order_words(n) is 0, else a pointer
 // into buf valid until buf is reused; *far (when given) the device word order_coherent reads
 // (null with no order).
 inline int spatial_order_into(const float* lo3, const float* hi3, uint32_t n, const float* d_pos4, hipStream_t st,
                               uint32_t* buf, const uint32_t** order, const uint32_t** far = nullptr) {
     *order = nullptr;
     if (far) *far = nullptr;
-    if (n < kOrderMinItems) return MNRT_OK;
-    uint32_t* cell = buf;
-    uint32_t* ord = cell + n;
-    uint32_t* hist = ord + n;
-    uint32_t* cursor = hist + kOrderCells;
-    uint32_t* far_pairs = cursor + kOrderCells;
+    if (spatial_order_words(n) == 0) return MNRT_OK;
+    uint32_t* rank = buf;
+    uint32_t* ord = rank + n;
+    uint32_t* hist = ord + n;  // kOrderCopies histograms; the first becomes the cell starts
+    uint32_t* far_pairs = hist + kOrderCopies * kOrderCells;
     uint32_t* scr = far_pairs + 4;
     OrderBox B;
     B.diag2 = 0.0f;
@@ -128,14 +171,15 @@ inline int spatial_order_into(const float* lo3, const float* hi3, uint32_t n, co
         B.diag2 += c * c;
     }
     const float4* pos = (const float4*)d_pos4;
-    hipLaunchKernelGGL(k_order_clear, dim3(kOrderCells / 256 / 16), dim3(256), 0, st, hist, far_pairs);
+    hipLaunchKernelGGL(k_order_clear, dim3(kOrderCopies * kOrderCells / 256 / 16), dim3(256), 0, st, hist, far_pairs);
     hipLaunchKernelGGL(k_order_far, dim3((n + 256 * kFarPerThread - 1) / (256 * kFarPerThread)), dim3(256), 0, st, n,
                        pos, B.diag2, far_pairs);
     hipLaunchKernelGGL(k_order_cells, dim3((n + 255) / 256), dim3(256), 0, st, n, pos, B,
-                       (const uint32_t*)far_pairs, cell, hist);
-    MNRT_HIP_CHECK(scan_exclusive(hist, cursor, kOrderCells, nullptr, scr, st));
-    hipLaunchKernelGGL(k_order_place, dim3((n + 255) / 256), dim3(256), 0, st, n, (const uint32_t*)cell,
-                       (const uint32_t*)far_pairs, cursor, ord);
+                       (const uint32_t*)far_pairs, rank, hist);
+    hipLaunchKernelGGL(k_order_sum, dim3(kOrderCells / 256), dim3(256), 0, st, n, (const uint32_t*)far_pairs, hist);
+    MNRT_HIP_CHECK(scan_exclusive(hist, hist, kOrderCells, nullptr, scr, st));
+    hipLaunchKernelGGL(k_order_place, dim3((n + 255) / 256), dim3(256), 0, st, n, pos, B, (const uint32_t*)rank,
+                       (const uint32_t*)far_pairs, (const uint32_t*)hist, ord);
     MNRT_HIP_CHECK(hipGetLastError());
     *order = ord;
     if (far) *far = far_pairs;
@@ -143,7 +187,7 @@ inline int spatial_order_into(const float* lo3, const float* hi3, uint32_t n, co
 }
 
 // As spatial_order_into with the buffer from the device pool: *buf = the allocation to
-// return with pool_free after the launch that reads the order (null below kOrderMinItems).
+// return with pool_free after the launch that reads the order (null with no order).
 inline int spatial_order(const float* lo3, const float* hi3, uint32_t n, const float* d_pos4, hipStream_t st,
                          const uint32_t** order, void** buf, const uint32_t** far = nullptr) {
     *order = nullptr;

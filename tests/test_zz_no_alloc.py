@@ -75,12 +75,12 @@ def test_noalloc_equals_gather_twice(cuda):
     q = {k: qa[k].contiguous() for k in ("pos", "nrm", "dir", "infl", "clr")}
     b = _outs(torch, 16, K, 16)
     p = N.ptr
-    big = 2321809
+    big = 2097114
     rc = L.mnrt_knn_gather_noalloc(P.ref(), K, rmax, big, p(q["pos"]), p(q["nrm"]), p(q["dir"]), p(q["infl"]),
                                    p(q["clr"]), p(q["pos"]), None, None, p(b["rad"]), None, None, None, None, None,
                                    N.stream_handle())
     assert rc == N.ERR_CAPACITY
-    rc = L.mnrt_knn_gather_noalloc(P.ref(), K, rmax, 8126330, p(q["pos"]), p(q["nrm"]), p(q["dir"]), p(q["infl"]),
+    rc = L.mnrt_knn_gather_noalloc(P.ref(), K, rmax, 7339898, p(q["pos"]), p(q["nrm"]), p(q["dir"]), p(q["infl"]),
                                    p(q["clr"]), None, None, None, p(b["rad"]), None, None, None, None, None,
                                    N.stream_handle())
     assert rc == N.ERR_CAPACITY
