@@ -586,6 +586,9 @@ def main():
     ap.add_argument("--build-warmup", type=int, default=10)
     ap.add_argument("--serial", action="store_true",
                     help="build the photon tree after the traces on the same stream (default: concurrently)")
+    ap.add_argument("--host-builds", action="store_true",
+                    help="build both trees with the host-driven level loop every step (default: replay builds "
+                         "recorded at set-up)")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-lighting leg")
@@ -644,11 +647,28 @@ def main():
     stream = torch.cuda.current_stream()
     phases = ["build_tri", "trace_primary", "shade_bounce", "trace_bounce", "build_points", "knn"]
 
-    # The photon tree does not depend on the triangle tree or the rays: unless --serial, a
-    # second host thread builds it on a second stream while this one builds the triangle
-    # tree and traces (the library keeps its build workspace per host thread).
+    # Both trees are rebuilt from their inputs every step.  By default each build replays a
+    # recorded build (mnrt_build_graph_*: the level loop captured once at set-up into a
+    # hipGraph whose kernels read every count on the device, so a replay is one launch call
+    # and no host synchronisation; replays equal host-driven builds word for word).  The
+    # photon tree does not depend on the triangle tree or the rays: unless --serial it is
+    # built on a second stream, concurrently with the triangle tree and the traces.  With
+    # --host-builds both run the host-driven level loop, the photon tree from a second host
+    # thread (the library keeps its build workspace per host thread).
     side = torch.cuda.Stream()
-    worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+    graphs = not args.host_builds
+    worker = None if graphs or args.serial else concurrent.futures.ThreadPoolExecutor(max_workers=1)
+    if graphs:
+        GT = api.BuildGraph(native.KIND_TRI, np.array([0, v0.shape[0]], np.uint32), v0, v1, v2)
+        GP = api.BuildGraph(native.KIND_POINT, np.array([0, ppos.shape[0]], np.uint32), ppos, power=ppow, dirs=pdir,
+                            params=pparams, root_boxes=root[None, :], max_query_radius=np.array([rmax], np.float32))
+        torch.cuda.synchronize()
+
+    def replayed(G, st, what):
+        flags = G.status(st)
+        if flags != 0:
+            raise RuntimeError(f"recorded {what} build outgrew its buffers (flags {flags:#x})")
+        return G.trees[0]
 
     def photon_tree():
         with torch.cuda.stream(side):
@@ -662,8 +682,18 @@ def main():
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
         kev = []
         ev[0].record(stream)
-        pending = None if args.serial else worker.submit(photon_tree)
-        T = api.build_tri(v0, v1, v2)
+        pending = pdone = None
+        if graphs:
+            if not args.serial:
+                side.wait_event(ev[0])  # the last step's queries of the photon tree are done
+                GP.replay(side)
+                pdone = torch.cuda.Event()
+                pdone.record(side)
+            GT.replay(stream)
+            T = replayed(GT, stream, "triangle")
+        else:
+            pending = None if args.serial else worker.submit(photon_tree)
+            T = api.build_tri(v0, v1, v2)
         ev[1].record(stream)
         tri, t, _, _ = api.trace_closest(T, go, gd, eps)
         ev[2].record(stream)
@@ -681,7 +711,13 @@ def main():
         else:
             ev[3].record(stream)
             ev[4].record(stream)
-        if pending is None:
+        if graphs and pdone is not None:
+            stream.wait_event(pdone)
+            P = replayed(GP, side, "photon")
+        elif graphs:
+            GP.replay(stream)
+            P = replayed(GP, stream, "photon")
+        elif pending is None:
             P = api.build_points(ppos, pparams)
             api.attach_photon_records(P, ppos, ppow, pdir)
         else:
@@ -739,8 +775,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene generator, traced photons)",
         "config": {"workload": workload, "triangles": sc.num_tris, "photons": ph["count"], "rays": rays,
+                   "tree_builds": "host-driven level loop" if args.host_builds else
+                   "replays of builds recorded at set-up (hipGraph, counts on the device)",
                    "photon_tree": "after the traces, one stream" if args.serial else
-                   "concurrent with the triangle tree and traces (second host thread and stream)",
+                   "concurrent with the triangle tree and traces (second stream)",
                    "knn_queries": queries, "k": K, "parallelism": "single GPU"},
         "phase_ms": {p: round(v, 3) for p, v in phase_ms.items()},
         "trace_mrays_per_s": round(rays / (trace_ms * 1e-3) / 1e6, 2),

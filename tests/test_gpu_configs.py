@@ -76,6 +76,36 @@ def test_large_trees_and_photons_equal_oracle(orc, large):
     print(f"trees: {X['T'].num_nodes} / {X['P'].num_nodes} nodes, {oph['count']} photons equal", flush=True)
 
 
+def test_large_recorded_builds_equal_host_builds(large):
+    # the benchmark's step rebuilds both trees by replaying builds recorded once
+    # (mnrt_build_graph_*): at full size every replay equals the host-driven trees, which the
+    # test above holds to the oracle, word for word (nodes, references, root box, leaf-ordered
+    # records, query-time nodes)
+    X = large
+    sc, root, rmax, ph = X["sc"], X["root"], X["rmax"], X["ph"]
+    v = [api.dev(a) for a in (sc.v0, sc.v1, sc.v2)]
+    GT = api.BuildGraph(N.KIND_TRI, np.array([0, sc.num_tris], np.uint32), *v)
+    GP = api.BuildGraph(N.KIND_POINT, np.array([0, ph["count"]], np.uint32), ph["pos"], power=ph["power"],
+                        dirs=ph["dir"], params=api.default_params(1, root, rmax), root_boxes=root[None, :],
+                        max_query_radius=np.array([rmax], np.float32))
+    for rep in range(3):
+        GT.replay()
+        GP.replay()
+        assert GT.status() == 0 and GP.status() == 0
+        for G, H, what in ((GT, X["T"], "triangles"), (GP, X["P"], "photons")):
+            a = G.trees[0]
+            assert a.num_nodes == H.num_nodes and a.num_refs == H.num_refs, what
+            assert a.max_level == H.max_level and a.t.num_leaves == H.t.num_leaves, what
+            assert np.array_equal(a.root.view(np.uint32), H.root.view(np.uint32)), what
+            assert np.array_equal(a.nodes_host(), H.nodes_host()), what
+            assert np.array_equal(a.refs_host(), H.refs_host()), what
+            assert np.array_equal(a.recs_host().view(np.uint32), H.recs_host().view(np.uint32)), what
+            assert np.array_equal(a.qnodes_host(), H.qnodes_host()), what
+        print(f"replay {rep}: recorded builds equal the host-driven trees", flush=True)
+    GT.close()
+    GP.close()
+
+
 def test_large_hits_equal_oracle(orc, large):
     X = large
     sc = X["sc"]
