@@ -437,6 +437,16 @@ int mnrt_photons_reweight_bwd(uint32_t np, const float* d_base_pow4, const uint3
 int mnrt_scan_u32(const uint32_t* d_in, uint32_t n, int inclusive, uint32_t* d_out, uint32_t* d_total,
                   void* stream);
 
+/* Exclusive add-scan of a length that lives on the device (no reference counterpart: the
+ * reference's scans take a host count, MNCudaPrimitives.cpp:160-177): n = min(*d_count,
+ * capacity), read by the kernels, so the call can be recorded into a graph and replayed
+ * after the count changes; d_out[0..n) = the scan of d_in[0..n) (d_out may equal d_in),
+ * *d_total (device word, may be null) = the sum, and d_out[n] = the sum too when end_total
+ * is nonzero (d_out then holds capacity + 1 words).  The recorded builds scan their
+ * per-level counts with it. */
+int mnrt_scan_u32_counted(const uint32_t* d_in, const uint32_t* d_count, uint32_t capacity,
+                          uint32_t* d_out, uint32_t* d_total, int end_total, void* stream);
+
 /* MNCudaPrimitives::Compact (MNCudaPrimitives.h:187-211, MNCudaPrimitives.cpp:180-190):
  * the words d_in[i] with d_is_valid[i] != 0, in input order, to d_out; *d_out_count (device
  * word) = their number.  Not in place (d_out != d_in). */

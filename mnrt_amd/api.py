@@ -572,6 +572,21 @@ def scan_u32(a, inclusive=False, out=None):
     return out, tot
 
 
+def scan_u32_counted(a, count: int, capacity: int, end_total=False, out=None):
+    """Exclusive add-scan of the first min(count, capacity) words of a, the count handed over
+    in a device word (mnrt_scan_u32_counted): (scan, total); out (default: capacity + 1 words
+    of -1) may be a itself."""
+    torch = _torch()
+    a = a.contiguous()
+    out = out if out is not None else torch.full((capacity + 1,), -1, dtype=torch.int32, device="cuda")
+    tot = torch.empty(1, dtype=torch.int32, device="cuda")
+    cnt = torch.from_numpy(np.array([count], np.uint32).view(np.int32)).cuda()
+    N.check(N.load().mnrt_scan_u32_counted(N.ptr(a) if a.numel() else None, N.ptr(cnt), capacity, N.ptr(out),
+                                           N.ptr(tot), 1 if end_total else 0, N.stream_handle()),
+            "mnrt_scan_u32_counted")
+    return out, tot
+
+
 def compact_u32(a, valid):
     """The words of a where valid != 0, in order (mnrt_compact_u32): (compacted, count)."""
     torch = _torch()

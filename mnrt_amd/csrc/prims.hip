@@ -93,6 +93,23 @@ extern "C" int mnrt_scan_u32(const uint32_t* d_in, uint32_t n, int inclusive, ui
     return rc;
 }
 
+extern "C" int mnrt_scan_u32_counted(const uint32_t* d_in, const uint32_t* d_count, uint32_t capacity,
+                                     uint32_t* d_out, uint32_t* d_total, int end_total, void* stream) {
+    using namespace mnrt;
+    if (!d_count || (!d_out && (end_total || capacity > 0)) || (capacity > 0 && !d_in)) return MNRT_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    void* buf = nullptr;
+    MNRT_HIP_CHECK(pool_alloc(&buf, (size_t)dscan_tiles(capacity) * 4, st));
+    DScanSet S{};
+    S.in[0] = d_in;
+    S.out[0] = d_out;
+    S.total[0] = d_total;
+    S.end_total = end_total ? 1u : 0u;
+    int rc = dscan(S, 1, dn(d_count, capacity), (uint32_t*)buf, st) == hipSuccess ? MNRT_OK : MNRT_ERR_HIP;
+    if (pool_free(buf, st) != hipSuccess && rc == MNRT_OK) rc = MNRT_ERR_HIP;
+    return rc;
+}
+
 extern "C" int mnrt_compact_u32(const uint32_t* d_in, const uint32_t* d_is_valid, uint32_t n, uint32_t* d_out,
                                 uint32_t* d_out_count, void* stream) {
     using namespace mnrt;

@@ -141,8 +141,9 @@ constexpr uint32_t kScanBigBlock = 1024;
 constexpr uint32_t kScanBigPer = 64;
 constexpr uint32_t kScanBig = kScanBigBlock * kScanBigPer;
 
-__device__ __forceinline__ void block_scan_runs(const uint32_t* in, uint32_t n, uint32_t* out,
-                                                uint32_t* __restrict__ total) {
+// Returns the total in thread kScanBigBlock - 1 (and writes it to *total when given).
+__device__ __forceinline__ uint32_t block_scan_runs(const uint32_t* in, uint32_t n, uint32_t* out,
+                                                    uint32_t* __restrict__ total) {
     __shared__ uint32_t s_wave[kScanBigBlock / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t per = (n + kScanBigBlock - 1) / kScanBigBlock;
@@ -161,6 +162,7 @@ __device__ __forceinline__ void block_scan_runs(const uint32_t* in, uint32_t n, 
         run += v;
     }
     if (tid == kScanBigBlock - 1 && total) *total = wave_off + incl;
+    return wave_off + incl;
 }
 
 __global__ __launch_bounds__(kScanBigBlock) void k_scan_big(const uint32_t* in, uint32_t n, uint32_t* out,
@@ -245,8 +247,11 @@ inline hipError_t scan_exclusive_set(const ScanSet& S0, int m, uint32_t n, uint3
 }
 
 // ---- scans of a device count: n = DN::get(), read by every kernel of the scan, so the
-// three launches below can be recorded once and replayed on other sizes (up to the
-// capacity n.v).  Blocks stride over the tiles; one workgroup per scan scans the tile sums.
+// launches below can be recorded once and replayed on other sizes (up to the capacity n.v).
+// A capacity of at most kScanBig is scanned by one workgroup per scan (one launch);
+// larger ones reduce each 2048-element tile to its total (tiles_cap words per scan), then
+// scan each tile from the sum of the totals before it (two launches; blocks stride over the
+// tiles and carry that sum from one of their tiles to the next).
 struct DScanSet {
     const uint32_t* in[kScanSetMax];
     uint32_t* out[kScanSetMax];
@@ -256,16 +261,66 @@ struct DScanSet {
     uint32_t end_total;            // 1: the total is also written to out[n]
 };
 
-__global__ __launch_bounds__(kScanBlock) void k_dscan_tiles(DScanSet S, DN dn_) {
+// blockIdx.x picks the scan of the set; n <= kScanBig.
+__global__ __launch_bounds__(kScanBigBlock) void k_dscan_one(DScanSet S, DN dn_) {
+    const uint32_t n = dn_.get();
+    const uint32_t y = blockIdx.x;
+    const uint32_t t = block_scan_runs(S.in[y], n, S.out[y], S.total[y]);
+    if (S.end_total && threadIdx.x == kScanBigBlock - 1) S.out[y][n] = t;
+}
+
+// Sum of one value per thread over the block, returned to every thread.  s_red: at least
+// kScanBlock / 64 words; the block is synchronised on return.
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_red) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+#pragma unroll
+    for (uint32_t o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) s_red[wave] = v;
+    __syncthreads();
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanBlock / 64; ++w) r += s_red[w];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_dscan_reduce(DScanSet S, DN dn_) {
+    __shared__ uint32_t s_red[kScanBlock / 64];
+    const uint32_t n = dn_.get();
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    const uint32_t y = blockIdx.y;
+    const uint32_t* in = S.in[y];
+    uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {  // uniform per block
+        const size_t base = (size_t)t * kScanTile + (size_t)threadIdx.x * kScanPerThread;
+        uint32_t local = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kScanPerThread; ++j) {
+            const size_t i = base + j;
+            local += (i < n) ? in[i] : 0u;
+        }
+        const uint32_t sum = block_sum(local, s_red);
+        if (threadIdx.x == 0) sums[t] = sum;
+    }
+}
+
+// in and out may alias: each thread reads its elements before writing them.
+__global__ __launch_bounds__(kScanBlock) void k_dscan_scan(DScanSet S, DN dn_) {
     __shared__ uint32_t s_wave[kScanBlock / 64];
+    __shared__ uint32_t s_red[kScanBlock / 64];
     const uint32_t n = dn_.get();
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
     const uint32_t y = blockIdx.y;
     const uint32_t* in = S.in[y];
     uint32_t* out = S.out[y];
-    uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
+    const uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t prefix = 0, done = 0;  // prefix = sum of sums[0 .. done)
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {  // uniform per block
+        uint32_t part = 0;
+        for (uint32_t k = done + tid; k < t; k += kScanBlock) part += sums[k];
+        prefix += block_sum(part, s_red);
+        done = t;
         const size_t base = (size_t)t * kScanTile + (size_t)tid * kScanPerThread;
         uint32_t v[kScanPerThread];
         uint32_t local = 0;
@@ -280,62 +335,22 @@ __global__ __launch_bounds__(kScanBlock) void k_dscan_tiles(DScanSet S, DN dn_) 
         __syncthreads();
         uint32_t wave_off = 0;
         for (uint32_t w = 0; w < wave; ++w) wave_off += s_wave[w];
-        uint32_t run = wave_off + incl - local;
+        uint32_t run = prefix + wave_off + incl - local;
 #pragma unroll
         for (uint32_t j = 0; j < kScanPerThread; ++j) {
             const size_t i = base + j;
             if (i < n) out[i] = run;
             run += v[j];
         }
-        if (tid == kScanBlock - 1) sums[t] = wave_off + incl;
+        if (t == tiles - 1 && tid == kScanBlock - 1) {
+            if (S.total[y]) *S.total[y] = run;
+            if (S.end_total) out[n] = run;
+        }
         __syncthreads();  // s_wave is rewritten by the next tile
     }
-}
-
-// One workgroup per scan (blockIdx.x): exclusive scan of the tile sums in place, in
-// rounds of 1024, and the total.
-__global__ __launch_bounds__(1024) void k_dscan_sums(DScanSet S, DN dn_) {
-    __shared__ uint32_t s_wave[1024 / 64];
-    const uint32_t n = dn_.get();
-    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
-    const uint32_t y = blockIdx.x;
-    uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < tiles; b += 1024) {
-        const uint32_t i = b + tid;
-        const uint32_t v = i < tiles ? sums[i] : 0u;
-        const uint32_t incl = wave_incl_scan(v, lane);
-        if (lane == 63) s_wave[wave] = incl;
-        __syncthreads();
-        uint32_t wave_off = 0, round = 0;
-        for (uint32_t w = 0; w < 1024 / 64; ++w) {
-            if (w < wave) wave_off += s_wave[w];
-            round += s_wave[w];
-        }
-        if (i < tiles) sums[i] = carry + wave_off + incl - v;
-        carry += round;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        if (S.total[y]) *S.total[y] = carry;
-        if (S.end_total) S.out[y][n] = carry;
-    }
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_dscan_add(DScanSet S, DN dn_) {
-    const uint32_t n = dn_.get();
-    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
-    const uint32_t y = blockIdx.y;
-    uint32_t* out = S.out[y];
-    const uint32_t* sums = S.sums + (size_t)y * S.tiles_cap;
-    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        const uint32_t add = sums[t];
-        if (add == 0u) continue;
-        for (uint32_t j = threadIdx.x; j < kScanTile; j += kScanBlock) {
-            const size_t i = (size_t)t * kScanTile + j;
-            if (i < n) out[i] += add;
-        }
+    if (tiles == 0 && blockIdx.x == 0 && tid == 0) {
+        if (S.total[y]) *S.total[y] = 0u;
+        if (S.end_total) out[0] = 0u;
     }
 }
 
@@ -350,10 +365,13 @@ inline hipError_t dscan(const DScanSet& S0, int m, DN n, uint32_t* scratch, hipS
     DScanSet S = S0;
     S.sums = scratch;
     S.tiles_cap = dscan_tiles(n.v);
+    if (n.v <= kScanBig) {
+        hipLaunchKernelGGL(k_dscan_one, dim3(m), dim3(kScanBigBlock), 0, st, S, n);
+        return hipGetLastError();
+    }
     const uint32_t grid = std::min<uint32_t>(S.tiles_cap, 1024u);
-    hipLaunchKernelGGL(k_dscan_tiles, dim3(grid, m), dim3(kScanBlock), 0, st, S, n);
-    hipLaunchKernelGGL(k_dscan_sums, dim3(m), dim3(1024), 0, st, S, n);
-    hipLaunchKernelGGL(k_dscan_add, dim3(grid, m), dim3(kScanBlock), 0, st, S, n);
+    hipLaunchKernelGGL(k_dscan_reduce, dim3(grid, m), dim3(kScanBlock), 0, st, S, n);
+    hipLaunchKernelGGL(k_dscan_scan, dim3(grid, m), dim3(kScanBlock), 0, st, S, n);
     return hipGetLastError();
 }
 

@@ -209,6 +209,7 @@ def load():
     L.mnrt_photons_spherical_to_dir.argtypes = [u32, vp, vp, vp, vp]
     L.mnrt_photons_dir_to_spherical.argtypes = [u32, vp, vp, vp, vp]
     L.mnrt_scan_u32.argtypes = [vp, u32, ctypes.c_int, vp, vp, vp]
+    L.mnrt_scan_u32_counted.argtypes = [vp, vp, u32, vp, vp, ctypes.c_int, vp]
     L.mnrt_compact_u32.argtypes = [vp, vp, u32, vp, vp, vp]
     L.mnrt_shading_points.argtypes = [u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(u32), vp]

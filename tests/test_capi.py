@@ -91,6 +91,10 @@ def test_primitive_argument_errors_without_device():
     fake = 256
     assert L.mnrt_scan_u32(None, 5, 0, fake, None, None) == N.ERR_ARG
     assert L.mnrt_scan_u32(fake, 5, 1, None, None, None) == N.ERR_ARG
+    assert L.mnrt_scan_u32_counted(fake, None, 5, fake, None, 0, None) == N.ERR_ARG  # no count word
+    assert L.mnrt_scan_u32_counted(None, fake, 5, fake, None, 0, None) == N.ERR_ARG
+    assert L.mnrt_scan_u32_counted(fake, fake, 5, None, None, 0, None) == N.ERR_ARG
+    assert L.mnrt_scan_u32_counted(None, fake, 0, None, None, 1, None) == N.ERR_ARG  # end total, no output
     assert L.mnrt_compact_u32(fake, fake, 5, fake, None, None) == N.ERR_ARG
     assert L.mnrt_compact_u32(fake, fake, 5, fake, fake, None) == N.ERR_ARG  # in place
     assert L.mnrt_compact_u32(fake, None, 5, 512, fake, None) == N.ERR_ARG

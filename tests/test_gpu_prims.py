@@ -1,4 +1,4 @@
-"""The parallel primitives (mnrt_scan_u32, mnrt_compact_u32; MNCudaPrimitives::Scan /
+"""The parallel primitives (mnrt_scan_u32, mnrt_scan_u32_counted, mnrt_compact_u32; MNCudaPrimitives::Scan /
 ::Compact, MNCudaPrimitives.cpp:160-190) against plain sequential loops, at the sizes where
 a tiled scan can go wrong: empty, one element, around a wave (63 / 64 / 65), around a
 2048-element tile, the one-workgroup scans of 2049..65536 elements and their upper end,
@@ -73,3 +73,39 @@ def test_compact_vs_sequential_loop(cuda, n):
         assert c == len(want), (n, density)
         assert np.array_equal(out[:c].cpu().numpy().view(np.uint32), want), (n, density)
     print(f"compact n={n}: four densities equal", flush=True)
+
+
+# (capacity, device count): the one-workgroup form (capacity <= 65536) and the two-launch
+# form above it, counts of 0, 1, a tile and its neighbours, counts far below the capacity
+# (blocks with no tile), more tiles than blocks (1024), and counts beyond the capacity
+COUNTED = [(0, 0), (1, 0), (1, 1), (TILE, TILE), (TILE + 1, 1), (65536, 65536), (65536, 2049), (65537, 65537),
+           (65537, 0), (65537, 2048), (65537, 2049), (64 * TILE + 5, 64 * TILE + 5), (TILE * TILE + 3, TILE * TILE + 3),
+           (TILE * TILE + 3, 1500 * TILE + 17), (5000, 0xFFFFFFFF), (70000, 70001)]
+
+
+@pytest.mark.parametrize("cap,count", COUNTED)
+def test_counted_scan_vs_sequential_loop(cuda, cap, count):
+    torch = cuda
+    n = min(count, cap)
+    rng = np.random.default_rng(cap + 7 * n)
+    a = rng.integers(0, 1 << 32, cap, dtype=np.uint64).astype(np.uint32) if cap < 70000 else \
+        rng.integers(0, 1 << 20, cap, dtype=np.uint64).astype(np.uint32)
+    if n > 3:
+        a[n - 3:n] = 0xFFFFFFF0
+    d = torch.from_numpy(a.view(np.int32)).cuda()
+    want, tot = _seq_scan(a[:n], False)
+    for end_total in (False, True):
+        got, gt = api.scan_u32_counted(d, count, cap, end_total)
+        g = got.cpu().numpy().view(np.uint32)
+        assert np.array_equal(g[:n], want), (cap, count, end_total)
+        assert int(gt.cpu().numpy().view(np.uint32)[0]) == tot
+        rest = g[n + 1:] if end_total else g[n:]
+        assert np.all(rest == 0xFFFFFFFF), (cap, count, end_total, "past the count")
+        if end_total:
+            assert int(g[n]) == tot
+        inplace = torch.cat([d, torch.full((1,), -1, dtype=torch.int32, device="cuda")])
+        api.scan_u32_counted(inplace, count, cap, end_total, out=inplace)
+        g = inplace.cpu().numpy().view(np.uint32)
+        assert np.array_equal(g[:n], want), (cap, count, end_total, "in place")
+        assert np.array_equal(g[n + 1 if end_total else n:cap], a[n + 1 if end_total else n:cap])
+    print(f"counted scan cap={cap} count={count}: equal, nothing written past the count", flush=True)
