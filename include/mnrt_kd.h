@@ -43,7 +43,7 @@ extern "C" {
 
 /* Raised whenever a declaration or struct layout here changes (2: mnrt_batch_io gained the
  * adjoint buffers). */
-#define MNRT_ABI_VERSION 4
+#define MNRT_ABI_VERSION 5
 
 enum mnrt_status_code {
     MNRT_OK = 0,

@@ -136,7 +136,7 @@ class BatchIO(ctypes.Structure):
 PHASE_TRACE, PHASE_SHADE, PHASE_KNN, PHASE_KNN_BWD = 1, 2, 4, 8
 
 # MNRT_ABI_VERSION of the header these declarations mirror
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 # status codes (the MNRT_OK / MNRT_ERR_* values of mnrt_kd.h)
