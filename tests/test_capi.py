@@ -20,7 +20,7 @@ def test_status_strings():
     L = N.load()
     assert L.mnrt_status_string(0) == b"ok"
     assert L.mnrt_status_string(-1) == b"invalid argument"
-    assert L.mnrt_abi_version() == N.ABI_VERSION == 4
+    assert L.mnrt_abi_version() == N.ABI_VERSION == 5
 
 
 def test_argument_errors_without_device():
