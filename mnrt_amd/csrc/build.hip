@@ -1211,13 +1211,19 @@ struct LayoutView {
 // Build nodes bucketed by level (the order inside a level does not matter).  Counts go
 // through a per-workgroup LDS histogram, so each workgroup adds to a level's global
 // counter once.
-__global__ __launch_bounds__(256) void k_level_hist(DN dnbn, BnView B, uint32_t* __restrict__ hist) {
+// top: the deepest level the layout launches cover; a node below it sets err_bit in *err.
+__global__ __launch_bounds__(256) void k_level_hist(DN dnbn, BnView B, uint32_t* __restrict__ hist, uint32_t top,
+                                                    uint32_t* __restrict__ err, uint32_t err_bit) {
     __shared__ uint32_t s_hist[64];
     if (threadIdx.x < 64) s_hist[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t nbn = dnbn.get();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nbn && B.type[i] < BN_PENDING) atomicAdd(&s_hist[B.level[i] & 63u], 1u);
+    if (i < nbn && B.type[i] < BN_PENDING) {
+        const uint32_t lvl = B.level[i] & 63u;
+        if (lvl > top) atomicOr(err, err_bit);
+        atomicAdd(&s_hist[lvl], 1u);
+    }
     __syncthreads();
     if (threadIdx.x < 64 && s_hist[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
 }
@@ -1888,7 +1894,8 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         uint32_t* list = cursor + 64;
         uint32_t* sc = list + nbn;
         MNRT_HIP_CHECK(hipMemsetAsync(hist, 0, 64 * 4, st));
-        hipLaunchKernelGGL(k_level_hist, blocks_for(nbn), dim3(256), 0, st, hn(nbn), B, hist);
+        hipLaunchKernelGGL(k_level_hist, blocks_for(nbn), dim3(256), 0, st, hn(nbn), B, hist, 63u,
+                           (uint32_t*)nullptr, 0u);
         MNRT_HIP_CHECK(scan_exclusive(hist, off, 64, off + 64, sc, st));
         MNRT_HIP_CHECK(hipMemcpyAsync(cursor, off, 64 * 4, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_level_scatter, blocks_for(nbn), dim3(256), 0, st, hn(nbn), B, cursor, list);
@@ -2006,6 +2013,9 @@ enum : uint32_t { G_NA = C_COUNT, G_NE, G_NBN, G_NSMALL, G_NSE, G_WORDS };
 constexpr uint32_t kErrLevelRoom = 64u;    // a level's nodes / elements outgrew the fixed buffers
 constexpr uint32_t kErrLevels = 128u;      // active nodes left after the last recorded level
 constexpr uint32_t kErrOutputRoom = 256u;  // the trees outgrew the output arrays
+constexpr uint32_t kErrLayoutDepth = 512u; // a large-stage node deeper than the layout launches
+// Levels the recorded layout covers beyond the first build's deepest large-stage node.
+constexpr uint32_t kLayoutSpareLevels = 6u;
 
 struct GraphCaps {
     uint32_t na;           // node arrays of a level (hold twice the active nodes: the next level)
@@ -2378,12 +2388,15 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
         uint32_t* list = cursor + 64;
         uint32_t* sc = list + C.bn;
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, hist, 64u, 0u);
-        hipLaunchKernelGGL(k_level_hist, blocks_for(C.bn), dim3(256), 0, st, nbn, B, hist);
+        // the layout launches cover the first build's depth and a few levels more; a deeper
+        // replay sets kErrLayoutDepth here, before the layout, so nothing is emitted
+        const uint32_t top = std::min<uint32_t>({bs.maxlvl + kLayoutSpareLevels, (uint32_t)MNRT_MAX_LEVEL, 63u});
+        hipLaunchKernelGGL(k_level_hist, blocks_for(C.bn), dim3(256), 0, st, nbn, B, hist, top, counters + C_ERR,
+                           kErrLayoutDepth);
         MNRT_HIP_CHECK(scan_exclusive(hist, off, 64, off + 64, sc, st));
         hipLaunchKernelGGL(k_graph_copy, dim3(1), dim3(64), 0, st, cursor, (const uint32_t*)off, 64u);
         hipLaunchKernelGGL(k_level_scatter, blocks_for(C.bn), dim3(256), 0, st, nbn, B, cursor, list);
         LayoutSrc Ls{off, list, O.nsize, O.rsize, O.mlvl};
-        const uint32_t top = std::min<uint32_t>(MNRT_MAX_LEVEL, 63);
         const dim3 lgrid(std::min<uint32_t>(blocks_for(C.bn).x, 512u));
         for (int lvl = (int)top; lvl >= 0; --lvl)
             hipLaunchKernelGGL(k_layout_up, lgrid, dim3(256), 0, st, (uint32_t)lvl, Ls, B, Lv);

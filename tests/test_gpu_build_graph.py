@@ -280,3 +280,44 @@ def test_graph_reports_outgrown_buffers(cuda):
     _same_batch(G, B, "small triangles again")
     G.close()
     B.release()
+
+
+def test_graph_reports_deeper_tree(cuda):
+    """Recorded on uniformly scattered points, with room to spare in every buffer, then
+    replayed on Cantor dust (sums of cube corners shrinking 4x per step: each middle split
+    halves the points, as for the uniform points, but leaves an empty slab that is cut off,
+    so the tree is many levels deeper than the recorded layout covers): status reports the
+    depth bit (512) instead of a partly laid out tree, and a later replay on the first
+    input gives its trees again."""
+    torch = cuda
+    rng = np.random.default_rng(5)
+    n, ns = 4000, 2
+    uni = rng.uniform(-1, 1, (n * ns, 3))
+    geo = np.zeros((n * ns, 3))
+    for k in range(8):
+        geo += rng.integers(0, 2, (n * ns, 3)) * 0.25 ** k
+    geo = geo / geo.max() * 1.8 - 0.9 + rng.uniform(0, 1e-4, (n * ns, 3))
+    w = lambda a: np.concatenate([a, np.zeros((len(a), 1))], 1).astype(np.float32)
+    off = (np.arange(ns + 1) * n).astype(np.uint32)
+    boxes = np.tile(np.array([-1, -1, -1, 1, 1, 1], np.float32), (ns, 1))
+    qr = np.full(ns, 0.01, np.float32)
+    d = api.dev(w(uni))
+    G = api.BuildGraph(N.KIND_POINT, off, d, root_boxes=boxes, max_query_radius=qr, margin=16.0)
+    B = api.build_points_batch(d, off, root_boxes=boxes, max_query_radius=qr)
+    torch.cuda.synchronize()
+    _same_batch(G, B, "uniform points")
+    Bg = api.build_points_batch(api.dev(w(geo)), off, root_boxes=boxes, max_query_radius=qr)
+    print(f"max level: uniform {max(B[s].max_level for s in range(ns))}, "
+          f"Cantor dust {max(Bg[s].max_level for s in range(ns))}", flush=True)
+    d.copy_(torch.from_numpy(w(geo)).cuda())
+    G.replay()
+    flags = G.status()
+    print(f"flags after the deeper replay: {flags:#x}", flush=True)
+    assert flags & 512
+    d.copy_(torch.from_numpy(w(uni)).cuda())
+    G.replay()
+    assert G.status() == 0
+    _same_batch(G, B, "uniform points again")
+    G.close()
+    B.release()
+    Bg.release()
