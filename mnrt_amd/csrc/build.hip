@@ -164,9 +164,27 @@ __device__ __forceinline__ void classify(int kind, uint32_t idx, const float4& l
 
 // ---- split clipping (dev_ClipSegment / dev_ClipTriToBounds, kdtree_tri.cu:190-316) ----
 
-struct PList {
-    int n;
-    f3 p[12];
+// Vertex lists of the clipping in LDS (a per-lane array indexed by the running vertex
+// count would live in scratch memory): per lane two lists of up to kClipVerts vertices,
+// coordinate-major with the lane innermost, so the lanes of a wave touch consecutive banks.
+// k_clip runs kClipBlock lanes per workgroup.
+constexpr int kClipBlock = 64;
+constexpr int kClipVerts = 12;  // a triangle clipped by six planes has at most 9
+
+struct ClipLists {
+    __attribute__((address_space(3))) float* col;  // this lane's column
+    int n[2];
+    __device__ __forceinline__ float get(int l, int v, int c) const {
+        return col[((l * kClipVerts + v) * 3 + c) * kClipBlock];
+    }
+    __device__ __forceinline__ f3 vert(int l, int v) const { return mk3(get(l, v, 0), get(l, v, 1), get(l, v, 2)); }
+    __device__ __forceinline__ void push(int l, f3 p) {
+        const int v = n[l]++;
+        if (v >= kClipVerts) return;  // (cannot happen: at most 9 vertices)
+        col[((l * kClipVerts + v) * 3 + 0) * kClipBlock] = p.x;
+        col[((l * kClipVerts + v) * 3 + 1) * kClipBlock] = p.y;
+        col[((l * kClipVerts + v) * 3 + 2) * kClipBlock] = p.z;
+    }
 };
 
 __device__ __forceinline__ float f3c(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
@@ -176,50 +194,56 @@ __device__ __forceinline__ void f3set(f3& v, int a, float x) {
     else v.z = x;
 }
 
-__device__ void clip_segments(const PList& in, int axis, float val, float sense, PList& out) {
-    out.n = 0;
-    for (int i = 0; i < in.n; ++i) {
-        f3 a = in.p[i];
-        f3 b = in.p[(i + 1 == in.n) ? 0 : i + 1];
+// Sutherland-Hodgman step: list `in` clipped by the plane x[axis] = val, keeping the side
+// where (x[axis] - val) * sense >= 0, into list `out`.
+__device__ void clip_segments(ClipLists& L, int in, int axis, float val, float sense, int out) {
+    L.n[out] = 0;
+    const int n = min(L.n[in], kClipVerts);
+    for (int i = 0; i < n; ++i) {
+        f3 a = L.vert(in, i);
+        f3 b = L.vert(in, (i + 1 == n) ? 0 : i + 1);
         bool ins = (f3c(a, axis) - val) * sense >= 0.0f;
         bool ine = (f3c(b, axis) - val) * sense >= 0.0f;
         if (ins != ine) {
             float t = (val - f3c(a, axis)) / (f3c(b, axis) - f3c(a, axis));
             f3 h = mk3(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z));
             f3set(h, axis, val);
-            out.p[out.n++] = h;
+            L.push(out, h);
         }
-        if (ine) out.p[out.n++] = b;
+        if (ine) L.push(out, b);
     }
 }
 
 // Clipped bounds of a straddling triangle inside a child box, padded outward by 2^-20 of
 // the largest magnitude involved, clamped to the child box and to the previous bounds.
+// col: this lane's LDS column (2 * kClipVerts * 3 words at stride kClipBlock).
 __device__ void clip_bounds(const Geometry& G, uint32_t idx, float4& lo, float4& hi, const float4& bmin,
-                            const float4& bmax) {
-    PList a, b;
-    a.n = 3;
-    a.p[0] = xyz(G.v0[idx]);
-    a.p[1] = xyz(G.v1[idx]);
-    a.p[2] = xyz(G.v2[idx]);
+                            const float4& bmax, __attribute__((address_space(3))) float* col) {
+    ClipLists L;
+    L.col = col;
+    L.n[0] = L.n[1] = 0;
+    L.push(0, xyz(G.v0[idx]));
+    L.push(0, xyz(G.v1[idx]));
+    L.push(0, xyz(G.v2[idx]));
     for (int ax = 0; ax < 3; ++ax) {
         if (f4c(lo, ax) != f4c(hi, ax)) {
-            clip_segments(a, ax, f4c(bmin, ax), 1.0f, b);
-            clip_segments(b, ax, f4c(bmax, ax), -1.0f, a);
+            clip_segments(L, 0, ax, f4c(bmin, ax), 1.0f, 1);
+            clip_segments(L, 1, ax, f4c(bmax, ax), -1.0f, 0);
         }
     }
+    const int na = min(L.n[0], kClipVerts);
     for (int ax = 0; ax < 3; ++ax) {
         float l, h;
         float bl = f4c(bmin, ax), bh = f4c(bmax, ax);
-        if (a.n == 0) {
+        if (na == 0) {
             l = f4c(lo, ax);
             h = f4c(hi, ax);
         } else {
             l = kInf;
             h = -kInf;
-            for (int i = 0; i < a.n; ++i) {
-                l = fminf(f3c(a.p[i], ax), l);
-                h = fmaxf(f3c(a.p[i], ax), h);
+            for (int i = 0; i < na; ++i) {
+                l = fminf(L.get(0, i, ax), l);
+                h = fmaxf(L.get(0, i, ax), h);
             }
             float m = fmaxf(fmaxf(fabsf(l), fabsf(h)), fmaxf(fabsf(bl), fabsf(bh)));
             float pad = m * (1.0f / 1048576.0f);
@@ -787,9 +811,12 @@ __global__ __launch_bounds__(256) void k_scatter(DN dn_, DN dne, ElemView E, Geo
 // The clip queue of a triangle level (k_scatter): each job clips its triangle to the
 // child box and writes the bounds next to the index k_scatter placed.  A grid-stride
 // launch: the queue holds at most 2 ne jobs, usually a small part of that.
-__global__ __launch_bounds__(256) void k_clip(DN dn_, DN dne, ElemView E, Geometry G, NodeView A, PlanView P,
-                                              ChildView Cv, const uint32_t* __restrict__ pos, DN small_elem_base,
-                                              ElemView Sm, ElemView Nx, const uint32_t* __restrict__ clipq) {
+__global__ __launch_bounds__(kClipBlock) void k_clip(DN dn_, DN dne, ElemView E, Geometry G, NodeView A, PlanView P,
+                                                     ChildView Cv, const uint32_t* __restrict__ pos,
+                                                     DN small_elem_base, ElemView Sm, ElemView Nx,
+                                                     const uint32_t* __restrict__ clipq) {
+    __shared__ float s_lists[2 * kClipVerts * 3 * kClipBlock];
+    __attribute__((address_space(3))) float* col = (__attribute__((address_space(3))) float*)(s_lists + threadIdx.x);
     const uint32_t n = dn_.get(), ne = dne.get(), seb = small_elem_base.get();
     const uint32_t nq = pos[4 * ne] - pos[2 * ne];
     MNRT_GRID_STRIDE(j, nq) {
@@ -801,7 +828,7 @@ __global__ __launch_bounds__(256) void k_clip(DN dn_, DN dne, ElemView E, Geomet
         float4 bmin = P.lo[p], bmax = P.hi[p];
         if (side == 0) f4set(bmax, axis, s);
         else f4set(bmin, axis, s);
-        clip_bounds(G, G.base[A.scene[p]] + E.idx[e], lo, hi, bmin, bmax);
+        clip_bounds(G, G.base[A.scene[p]] + E.idx[e], lo, hi, bmin, bmax, col);
         const uint32_t c = side * n + p, rank = pos[side * ne + e] - Cv.fpos[c];
         if (Cv.type[c] == BN_SMALL) {
             const uint32_t d = seb + Cv.soff[c] + rank;
@@ -815,9 +842,9 @@ __global__ __launch_bounds__(256) void k_clip(DN dn_, DN dne, ElemView E, Geomet
     }
 }
 
-// Workgroups of k_clip for a level of up to ne elements.
+// Workgroups of k_clip (kClipBlock lanes) for a level of up to ne elements.
 inline dim3 clip_blocks(uint64_t ne) {
-    return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((2 * ne + 255) / 256, 1024)));
+    return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((2 * ne + kClipBlock - 1) / kClipBlock, 4096)));
 }
 
 // ---- small-node stage: one wave64 per small root ----------------------------------------
@@ -1820,7 +1847,7 @@ int build_batch_impl(int kind, const float* d_v0, const float* d_v1, const float
         hipLaunchKernelGGL(k_scatter, blocks_for(ne), dim3(256), 0, st, hn(na), hn(ne), E, G, A, P, Cv, pos, hn(nse),
                            Sm, se_cap, Nx, nx_cap, counters, fl);
         if (kind == MNRT_KIND_TRI)
-            hipLaunchKernelGGL(k_clip, clip_blocks(ne), dim3(256), 0, st, hn(na), hn(ne), E, G, A, P, Cv,
+            hipLaunchKernelGGL(k_clip, clip_blocks(ne), dim3(kClipBlock), 0, st, hn(na), hn(ne), E, G, A, P, Cv,
                                (const uint32_t*)pos, hn(nse), Sm, Nx, (const uint32_t*)fl);
         MNRT_HIP_CHECK(hipGetLastError());
         int rc = sync_counters();
@@ -2363,7 +2390,7 @@ int graph_record(mnrt_build_graph& g, char* base, hipStream_t st) {
         hipLaunchKernelGGL(k_scatter, gne, dim3(256), 0, st, na, ne, E, G, A, P, Cv, pos, nse, Sm, C.small_elems, Nx,
                            NE, counters, fl);
         if (kind == MNRT_KIND_TRI)
-            hipLaunchKernelGGL(k_clip, clip_blocks(pne), dim3(256), 0, st, na, ne, E, G, A, P, Cv, (const uint32_t*)pos,
+            hipLaunchKernelGGL(k_clip, clip_blocks(pne), dim3(kClipBlock), 0, st, na, ne, E, G, A, P, Cv, (const uint32_t*)pos,
                                nse, Sm, Nx, (const uint32_t*)fl);
         hipLaunchKernelGGL(k_graph_level_end, dim3(1), dim3(64), 0, st, counters, C, l + 1 == g.levels ? 1u : 0u);
         MNRT_HIP_CHECK(hipGetLastError());
