@@ -917,11 +917,8 @@ template <int kMode, int kKind>
 __global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallView S, ElemView Sm,
                                                             const float* __restrict__ scene_qr, SmallOut O) {
     constexpr int kind = kKind;
-    constexpr int kCandMax = (kKind == MNRT_KIND_TRI) ? 384 : 192;
-    __shared__ float s_cpos[kSmallWaves][kCandMax];
-    __shared__ uint64_t s_mL[kSmallWaves][kCandMax];
-    __shared__ uint64_t s_mR[kSmallWaves][kCandMax];
-    __shared__ uint8_t s_cax[kSmallWaves][kCandMax];
+    // candidate slots of an element: (isMax, axis), slot = isMax * 3 + axis
+    constexpr int kSlots = (kKind == MNRT_KIND_TRI) ? 6 : 3;
     __shared__ SmallStackEnt s_stack[kSmallWaves][kSmallStack];
 
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
@@ -929,10 +926,6 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallVie
     if (r >= dnsmall.get()) return;  // uniform per wave
     if (kMode == kSmallEmit && O.st_ovf && O.st_ovf[r] == 0u) return;
     if (kMode == kSmallEmit && O.counters[C_ERR] != 0u) return;  // failed build: nothing is emitted
-    float* cpos = s_cpos[wave];
-    uint64_t* mL = s_mL[wave];
-    uint64_t* mR = s_mR[wave];
-    uint8_t* cax = s_cax[wave];
     SmallStackEnt* stack = s_stack[wave];
 
     const uint32_t first = S.first[r], n = S.cnt[r];
@@ -952,14 +945,14 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallVie
     // the plane, the comparisons' lane masks combine into the candidate's masks (element
     // i itself, and elements lying in the plane, by the rules below), and lane i keeps
     // them for its own candidate (writelane); each lane writes its candidates at the end
-    // of the axis.
+    // of the axis.  Lane i keeps the masks of its element's candidates in registers (slot
+    // isMax * 3 + axis: candidate c = slot * n + i).
     //   triangles: L = min < s, R = s < max; the candidate's own element goes right of
     //              its minimum and left of its maximum; a flat element in the plane goes
     //              left if plane_goes_left, else right;
     //   points:    L = p < s, R = s < p; a point in the plane goes left if
     //              plane_goes_left, else right.
-    const uint32_t per = 3 * n;
-    const uint32_t ncand = (kind == MNRT_KIND_TRI) ? 2 * per : per;
+    uint64_t cmL[kSlots], cmR[kSlots];
     const uint64_t vmask = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
     const uint64_t leftm = __builtin_amdgcn_ballot_w64(mleft) & vmask;
     for (uint32_t ax = 0; ax < 3; ++ax) {
@@ -999,18 +992,15 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallVie
                 }
             }
         }
-        if (valid) {
-            const uint32_t c0 = ax * n + lane;
-            cpos[c0] = mn;
-            cax[c0] = (uint8_t)ax;
-            mL[c0] = ((uint64_t)l0hi << 32) | l0lo;
-            mR[c0] = ((uint64_t)r0hi << 32) | r0lo;
-            if (kind == MNRT_KIND_TRI) {
-                const uint32_t c1 = per + c0;
-                cpos[c1] = mx;
-                cax[c1] = (uint8_t)ax;
-                mL[c1] = ((uint64_t)l1hi << 32) | l1lo;
-                mR[c1] = ((uint64_t)r1hi << 32) | r1lo;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {  // registers by constant index: the axis as selects
+            if (ax == (uint32_t)k) {
+                cmL[k] = ((uint64_t)l0hi << 32) | l0lo;
+                cmR[k] = ((uint64_t)r0hi << 32) | r0lo;
+                if (kind == MNRT_KIND_TRI) {
+                    cmL[3 + k] = ((uint64_t)l1hi << 32) | l1lo;
+                    cmR[3 + k] = ((uint64_t)r1hi << 32) | r1lo;
+                }
             }
         }
     }
@@ -1078,22 +1068,23 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallVie
                 float area = 2.0f * (ext[0] * ext[1] + ext[0] * ext[2] + ext[1] * ext[2]);
                 denom = 1.0f / area;
             }
-            // the candidate's axis comes from the table and the node's bounds on it by
-            // selects: indexing e.lo / e.hi with a lane-varying axis, or deriving the axis
-            // from c by division, costs more than the cost evaluation itself
-            for (uint32_t c = lane; c < ncand; c += 64) {
-                const uint32_t a = cax[c];
-                const float s = cpos[c];
-                const float lo_a = a == 0 ? e.lo[0] : (a == 1 ? e.lo[1] : e.lo[2]);
-                const float hi_a = a == 0 ? e.hi[0] : (a == 1 ? e.hi[1] : e.hi[2]);
+            // lane i evaluates its element's candidates, slot by slot (the axis a constant:
+            // the node's bounds and extents on it need no selects); key = slot * 64 + i
+            // orders the candidates as c = slot * n + i does
+#pragma unroll
+            for (int slot = 0; slot < kSlots; ++slot) {
+                if (!valid) break;
+                const int a = slot % 3;
+                const float s = slot < 3 ? f4c(mlo, a) : f4c(mhi, a);
+                const float lo_a = e.lo[a], hi_a = e.hi[a];
                 if (!(lo_a < s && s < hi_a)) continue;
-                const uint64_t bl = mL[c] & e.mask, br = mR[c] & e.mask;
+                const uint64_t bl = cmL[slot] & e.mask, br = cmR[slot] & e.mask;
                 const uint32_t nl = (uint32_t)__popc((uint32_t)bl) + (uint32_t)__popc((uint32_t)(bl >> 32));
                 const uint32_t nr = (uint32_t)__popc((uint32_t)br) + (uint32_t)__popc((uint32_t)(br >> 32));
                 if (nl == 0 || nr == 0) continue;
                 // the other two axes' extents, (a + 1) % 3 and (a + 2) % 3
-                const float x1 = a == 0 ? ext[1] : (a == 1 ? ext[2] : ext[0]);
-                const float x2 = a == 0 ? ext[2] : (a == 1 ? ext[0] : ext[1]);
+                const float x1 = ext[(a + 1) % 3];
+                const float x2 = ext[(a + 2) % 3];
                 float cost;
                 if (kind == MNRT_KIND_POINT) {
                     float vl = (x1 + qr2) * (s - lo_a + qr2) * (x2 + qr2);
@@ -1106,7 +1097,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallVie
                 }
                 if (cost < best_cost) {
                     best_cost = cost;
-                    best = c;
+                    best = (uint32_t)slot * 64u + lane;
                 }
             }
             // lowest cost, ties to the lower candidate (an order-independent minimum)
@@ -1151,22 +1142,40 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_small(DN dnsmall, SmallVie
             rbase += cnt;
             continue;
         }
-        const uint32_t a = cax[best];
-        const float s = cpos[best];
+        // the chosen candidate: slot and element lane from the key, its plane and masks
+        // read from that lane
+        const uint32_t bslot = __builtin_amdgcn_readfirstlane(best) >> 6, blane = __builtin_amdgcn_readfirstlane(best) & 63u;
+        const uint32_t a = bslot % 3u;
+        float sv = 0.0f;
+        uint64_t bmL = 0, bmR = 0;
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) {
+            if (bslot == (uint32_t)k) {
+                sv = k < 3 ? f4c(mlo, k % 3) : f4c(mhi, k % 3);
+                bmL = cmL[k];
+                bmR = cmR[k];
+            }
+        }
+        // (readlane returns int: each half goes through uint32_t, or the low half would be
+        // sign-extended into the high one)
+        auto lane_u32 = [&](uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane(v, (int)blane); };
+        const float s = __uint_as_float(lane_u32(__float_as_uint(sv)));
+        const uint64_t selL = ((uint64_t)lane_u32((uint32_t)(bmL >> 32)) << 32) | (uint64_t)lane_u32((uint32_t)bmL);
+        const uint64_t selR = ((uint64_t)lane_u32((uint32_t)(bmR >> 32)) << 32) | (uint64_t)lane_u32((uint32_t)bmR);
         if (writing && lane == 0) {
             nodes[2 * addr] = __float_as_uint(s);
             nodes[2 * addr + 1] = a << 30;
         }
         if (lane == 0) {
             SmallStackEnt R = e, L = e;
-            R.mask = mR[best] & e.mask;
+            R.mask = selR & e.mask;
             for (uint32_t k = 0; k < 3; ++k) {  // the plane bounds the split axis (selects, not indexing)
                 R.lo[k] = (a == k) ? s : e.lo[k];
                 L.hi[k] = (a == k) ? s : e.hi[k];
             }
             R.lvl_ax = (e.level() + 1) | (a << 30);
             R.patch = addr;
-            L.mask = mL[best] & e.mask;
+            L.mask = selL & e.mask;
             L.lvl_ax = e.level() + 1;
             L.patch = kNone;
             stack[sp] = R;
