@@ -171,7 +171,10 @@ int mnrt_tree_attach_photon_records_batch(mnrt_tree* trees, uint32_t num_scenes,
  * only when their sizes change: mnrt_build_graph_status waits for the stream, returns in
  * *h_flags 0 when the last replay fit its buffers (and then refreshes out[], if given,
  * with its sizes, root boxes and addresses), nonzero when it did not (the trees are then
- * invalid until a replay that fits; create a new graph for the new sizes). */
+ * invalid until a replay that fits; create a new graph for the new sizes).  Bits: 64 a
+ * level outgrew the level buffers, 128 more levels than recorded, 256 the trees outgrew
+ * the output arrays, 512 the large-node stage went more than 6 levels deeper than the
+ * first build (the recorded layout's depth); 1..32 as the host-driven build's errors. */
 typedef struct mnrt_build_graph mnrt_build_graph;
 typedef struct mnrt_build_graph_desc {
     int kind;                         /* MNRT_KIND_TRI or MNRT_KIND_POINT */
