@@ -25,6 +25,8 @@ constexpr uint32_t kOrderMaxItems = 1u << 29;  // ranks share a word with the hi
 // Histogram copies: one per XCD (accelerator complex die), picked by the die a wave runs on,
 // so a cell's counter is updated from one die's L2 only instead of bouncing between all eight.
 constexpr uint32_t kOrderCopies = 8;
+// 16 items per lane: with 4 (4x the workgroups) the workgroups' atomics on the one counter
+// word serialise, 17 -> 27 us per 2M items
 constexpr uint32_t kFarPerThread = 16;
 
 struct OrderBox {
